@@ -130,6 +130,8 @@ def library() -> ctypes.CDLL:
         lib.oracle_laplace_scale.argtypes = (ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double)
         lib.oracle_laplace_shrinkage.restype = ctypes.c_double
         lib.oracle_laplace_shrinkage.argtypes = (ctypes.c_double,)
+        lib.oracle_mmse_centre.restype = ctypes.c_double
+        lib.oracle_mmse_centre.argtypes = (ctypes.c_int16, ctypes.c_double, ctypes.c_double)
         lib.oracle_sum_rows.restype = ctypes.c_double
         lib.oracle_sum_rows.argtypes = (dp, ctypes.c_uint64, ctypes.c_uint64)
         _library = lib
@@ -372,6 +374,10 @@ def laplace_scale(zeros: int, others: int, total: int, step: float) -> float:
 
 def laplace_shrinkage(rho: float) -> float:
     return float(library().oracle_laplace_shrinkage(rho))
+
+
+def mmse_centre(level: int, step: float, scale: float) -> float:
+    return float(library().oracle_mmse_centre(level, step, scale))
 
 
 def sum_rows(terms: np.ndarray) -> float:

@@ -219,6 +219,11 @@ static double mmse_centre(int16_t level, double step, double shrink) {
   return q == 0.0 ? 0.0 : signed_size;
 }
 
+double oracle_mmse_centre(int16_t level, double step, double scale) {
+  const double rho = scale > 0.0 ? step / scale : INFINITY;
+  return mmse_centre(level, step, oracle_laplace_shrinkage(rho));
+}
+
 /* ------------------------------------------------------------------------------------
  * The model of a component (docs/math.md:129-146, 377-492; cpp/src/model.cpp:126-262) */
 

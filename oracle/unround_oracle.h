@@ -120,6 +120,8 @@ void oracle_dct_forward(const double in[64], double out[64]);
 void oracle_dct_inverse(const double in[64], double out[64]);
 double oracle_laplace_scale(uint64_t zeros, uint64_t others, uint64_t sum, double step);
 double oracle_laplace_shrinkage(double rho);
+/* The MMSE centre of an AC level of step Q at the Laplace scale given. */
+double oracle_mmse_centre(int16_t level, double step, double scale);
 double oracle_sum_rows(const double* terms, uint64_t rows, uint64_t columns);
 
 #ifdef __cplusplus
