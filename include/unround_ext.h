@@ -1,0 +1,46 @@
+/* Additions of the MI355X edition to the C interface of JPEG-Unround (include/unround.h).
+ *
+ * These entry points have no counterpart in the reference's C ABI: they expose what its
+ * library API has (a start to resume from) or what this GPU edition adds (kernel timings,
+ * batches of pictures). The reference's own declarations are unchanged, and so is
+ * UNROUND_ABI_VERSION.
+ */
+#ifndef UNROUND_EXT_H
+#define UNROUND_EXT_H
+
+#include "unround.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A state to start the solver from (the reference's library start,
+ * cpp/include/unround/pdhg.hpp:128-133 and rust/unround/src/pdhg.rs:354-363, which its C
+ * ABI does not expose): any pointer may be null. coefficients holds one pointer per
+ * component (its rows x columns x 64 coefficients, clipped to their intervals) or is
+ * null; p and w are [C][2][H][W] and r [C][3][H][W] binary64 fields of the canvas (p and
+ * r are projected onto their balls; w defaults to the gradient of x). */
+typedef struct unround_start {
+  const double* const* coefficients;
+  const double* p;
+  const double* w;
+  const double* r;
+} unround_start;
+
+/* unround_solve from a given state. */
+unround_status unround_solve_start(const unround_input* input, const unround_settings* settings,
+                                   const unround_start* start, unround_observer observer, void* user,
+                                   unround_result** result, char* message, size_t message_size);
+
+/* The durations of the solve's kernels, where the environment variable UNROUND_TIMING was
+ * 1 during the solve (HIP events on the solve's stream): values[0] the seconds of the
+ * primal kernels and [1] their launches, [2] and [3] those of the dual kernels, [4] the
+ * seconds of the iterations (records excluded) and [5] the iterations; at most count
+ * values are written. Returns 1 where timings were taken, 0 otherwise. */
+int32_t unround_result_kernel_times(const unround_result* result, double* values, size_t count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,492 @@
+// The C interface (include/unround.h): settings from options, the solve of components
+// given as arrays on the GPU, and the result's accessors (rust/capi/src/lib.rs:99-117,
+// 459-694; rust/unround/src/decode.rs:269-358). No C++ exception crosses it: each
+// function that computes catches them and returns UNROUND_ERROR_INTERNAL.
+#include "unround.h"
+#include "unround_ext.h"
+
+#include <array>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "model.hpp"
+#include "settings.hpp"
+#include "solver.hpp"
+
+struct unround_settings {
+  unround::Settings settings;
+};
+
+struct unround_result {
+  unround::FrameModel frame;
+  unround::Settings settings;
+  unround::SolveOutput out;
+  bool solver = false;
+  std::vector<double> planes;
+  std::vector<double> picture;
+  uint64_t picture_channels = 1;
+  std::vector<uint64_t> record_iterations;
+  std::array<std::vector<double>, 5> record_values;
+  // The problem's arrays, made once on demand: lower, upper, centres, steps, weights.
+  mutable std::mutex made_guard;
+  mutable std::vector<std::array<std::vector<double>, 5>> problems;
+  mutable std::vector<bool> made;
+  std::string warning;
+};
+
+namespace {
+
+const char* const kVersion = "unround 0.1.0 (MI355X, HIP gfx950, C interface)";
+
+// The message NUL-terminated, cut at a UTF-8 boundary (rust/capi/src/lib.rs:84-96).
+void write_message(char* buffer, size_t size, const std::string& text) {
+  if (buffer == nullptr || size == 0) return;
+  size_t end = std::min(text.size(), size - 1);
+  while (end > 0 && end < text.size() && (static_cast<unsigned char>(text[end]) & 0xC0u) == 0x80u) --end;
+  std::memcpy(buffer, text.data(), end);
+  buffer[end] = '\0';
+}
+
+template <typename Body>
+unround_status guarded(char* message, size_t size, Body body) {
+  try {
+    std::string text;
+    const unround_status status = body(text);
+    if (status != UNROUND_OK) write_message(message, size, text);
+    return status;
+  } catch (const std::exception& error) {
+    write_message(message, size, std::string("an internal error of JPEG-Unround: ") + error.what());
+    return UNROUND_ERROR_INTERNAL;
+  } catch (...) {
+    write_message(message, size, "an internal error of JPEG-Unround");
+    return UNROUND_ERROR_INTERNAL;
+  }
+}
+
+// JFIF's RGB of Y, Cb and Cr (docs/math.md:1014-1024): the doubles nearest to 701/500,
+// 443/250, 25251/73375 and 209599/293500.
+constexpr double kCr = 1.402;
+constexpr double kCb = 1.772;
+const double kGb = 25251.0 / 73375.0;
+const double kGr = 209599.0 / 293500.0;
+
+void rgb_of(const double* planes, size_t count, double* picture) {
+  const double* y = planes;
+  const double* cb = planes + count;
+  const double* cr = planes + 2 * count;
+  for (size_t k = 0; k < count; ++k) {
+    const double b = cb[k] - 128.0, r = cr[k] - 128.0;
+    picture[3 * k + 0] = y[k] + kCr * r;
+    picture[3 * k + 1] = (y[k] - kGb * b) - kGr * r;
+    picture[3 * k + 2] = y[k] + kCb * b;
+  }
+}
+
+// The canvas cut to the picture, and the picture of the colour space
+// (rust/unround/src/decode.rs:306-358).
+void assemble(unround_result& result) {
+  const unround::FrameModel& f = result.frame;
+  const size_t C = f.channels(), h = f.height, w = f.width;
+  result.planes.resize(C * h * w);
+  for (size_t c = 0; c < C; ++c)
+    for (size_t i = 0; i < h; ++i)
+      std::memcpy(&result.planes[(c * h + i) * w], &result.out.canvas[(c * f.rows + i) * f.columns], sizeof(double) * w);
+  if (f.color_space == UNROUND_COLOR_YCBCR) {
+    result.picture.resize(3 * h * w);
+    rgb_of(result.planes.data(), h * w, result.picture.data());
+    result.picture_channels = 3;
+  } else if (C == 1) {
+    result.picture = result.planes;
+    result.picture_channels = 1;
+  } else {
+    result.picture.resize(C * h * w);
+    for (size_t k = 0; k < h * w; ++k)
+      for (size_t c = 0; c < C; ++c) result.picture[C * k + c] = result.planes[c * h * w + k];
+    result.picture_channels = C;
+  }
+  result.record_iterations.clear();
+  for (auto& values : result.record_values) values.clear();
+  for (const unround::Record& r : result.out.records) {
+    result.record_iterations.push_back(r.iteration);
+    result.record_values[0].push_back(r.seconds);
+    result.record_values[1].push_back(r.primal);
+    result.record_values[2].push_back(r.dual);
+    result.record_values[3].push_back(r.theta);
+    result.record_values[4].push_back(r.partial);
+  }
+  result.problems.assign(C, {});
+  result.made.assign(C, false);
+}
+
+unround_status reconstruct(const unround_input* input, const unround_settings* settings, const unround_start* start,
+                           unround_observer observer, void* user, unround_result** result, std::string& message) {
+  if (input == nullptr || settings == nullptr || result == nullptr) {
+    message = "the input, the settings and the result are not null pointers";
+    return UNROUND_ERROR_ARGUMENT;
+  }
+  if (input->color_space < 1 || input->color_space > 3) {
+    message = "a color space of 1, 2 or 3, not " + std::to_string(input->color_space);
+    return UNROUND_ERROR_ARGUMENT;
+  }
+  if (input->components == nullptr || input->count == 0) {
+    message = "components are given";
+    return UNROUND_ERROR_ARGUMENT;
+  }
+  const unround::Settings& s = settings->settings;
+  if (s.method == unround::Method::subgradient) {
+    message = "the subgradient method is not part of this library: use --method tv, tgv or mmse";
+    return UNROUND_ERROR_UNSUPPORTED;
+  }
+  std::vector<unround::ComponentInput> inputs;
+  for (uint32_t c = 0; c < input->count; ++c) {
+    const unround_component& given = input->components[c];
+    if (given.levels == nullptr) {
+      message = "a component's levels are a null pointer";
+      return UNROUND_ERROR_ARGUMENT;
+    }
+    if (given.h_samp_factor < 0 || given.v_samp_factor < 0) {
+      message = "a sampling factor below 0";
+      return UNROUND_ERROR_ARGUMENT;
+    }
+    unround::ComponentInput in;
+    in.levels = given.levels;
+    in.rows = given.rows;
+    in.columns = given.columns;
+    in.h = given.h_samp_factor;
+    in.v = given.v_samp_factor;
+    std::memcpy(in.table.data(), given.quant_table, sizeof(uint16_t) * 64);
+    inputs.push_back(in);
+  }
+  auto made = std::make_unique<unround_result>();
+  int32_t status = UNROUND_OK;
+  if (!unround::make_frame(input->height, input->width, input->color_space, inputs, s, made->frame, message, status)) {
+    return status;
+  }
+  unround::Plan plan;
+  const bool solving = s.method == unround::Method::tv || s.method == unround::Method::tgv;
+  if (solving && !unround::make_plan(s, made->frame.channels(), plan, message)) return UNROUND_ERROR_OPTIONS;
+  if (!solving) plan.gamma.assign(made->frame.channels(), 1.0);
+  unround::StartState state;
+  if (start != nullptr) {
+    if (start->coefficients != nullptr)
+      for (uint32_t c = 0; c < input->count; ++c) state.coefficients.push_back(start->coefficients[c]);
+    state.p = start->p;
+    state.w = start->w;
+    state.r = start->r;
+    if (s.method == unround::Method::tv && (state.w != nullptr || state.r != nullptr)) {
+      message = "a start of TV has no fields w and r";
+      return UNROUND_ERROR_OPTIONS;
+    }
+  }
+  const unround::FrameModel& frame = made->frame;
+  const double samples = static_cast<double>(frame.channels()) * frame.rows * frame.columns;
+  unround::Observer watch;
+  if (observer != nullptr) {
+    watch = [observer, user, &frame, samples](const unround::Record& record, const double* canvas) {
+      unround_record raw{};
+      raw.iteration = record.iteration;
+      raw.gap = (record.primal - record.dual) / samples;
+      raw.primal = record.primal;
+      raw.dual = record.dual;
+      raw.channels = frame.channels();
+      raw.height = frame.rows;
+      raw.width = frame.columns;
+      raw.canvas = canvas;
+      return observer(user, &raw) != 0;
+    };
+  }
+  const char* timing = std::getenv("UNROUND_TIMING");
+  const bool timed = timing != nullptr && timing[0] == '1';
+  if (!unround::solve_on_gpu(frame, s, plan, start != nullptr ? &state : nullptr, watch, timed, made->out, message,
+                             status)) {
+    return status;
+  }
+  made->settings = s;
+  made->solver = solving;
+  assemble(*made);
+  *result = made.release();
+  return UNROUND_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t unround_abi_version(void) { return UNROUND_ABI_VERSION; }
+
+const char* unround_version(void) { return kVersion; }
+
+unround_settings* unround_settings_new(size_t count, const char* const* options, char* message, size_t message_size) {
+  unround_settings* made = nullptr;
+  guarded(message, message_size, [&](std::string& text) -> unround_status {
+    if (count > 0 && options == nullptr) {
+      text = "the strings are a null pointer";
+      return UNROUND_ERROR_ARGUMENT;
+    }
+    std::vector<std::string> arguments;
+    for (size_t k = 0; k < count; ++k) {
+      if (options[k] == nullptr) {
+        text = "a string is a null pointer";
+        return UNROUND_ERROR_ARGUMENT;
+      }
+      arguments.emplace_back(options[k]);
+    }
+    unround::Settings settings;
+    if (!unround::parse_settings(arguments, settings, text)) return UNROUND_ERROR_OPTIONS;
+    made = new unround_settings{std::move(settings)};
+    return UNROUND_OK;
+  });
+  return made;
+}
+
+void unround_settings_free(unround_settings* settings) { delete settings; }
+
+size_t unround_settings_options(const unround_settings* settings, char* buffer, size_t size) {
+  if (settings == nullptr) return 0;
+  try {
+    const std::string text = unround::options_text(settings->settings);
+    write_message(buffer, size, text);
+    return text.size();
+  } catch (...) {
+    return 0;
+  }
+}
+
+unround_status unround_decode(const uint8_t*, size_t, const unround_settings*, unround_observer, void*,
+                              unround_result** result, char* message, size_t message_size) {
+  if (result != nullptr) *result = nullptr;
+  write_message(message, message_size,
+                "reading JPEG and PNG files is not part of this library: give the components as arrays to "
+                "unround_solve");
+  return UNROUND_ERROR_UNSUPPORTED;
+}
+
+unround_status unround_solve(const unround_input* input, const unround_settings* settings, unround_observer observer,
+                             void* user, unround_result** result, char* message, size_t message_size) {
+  return guarded(message, message_size, [&](std::string& text) {
+    return reconstruct(input, settings, nullptr, observer, user, result, text);
+  });
+}
+
+unround_status unround_solve_start(const unround_input* input, const unround_settings* settings,
+                                   const unround_start* start, unround_observer observer, void* user,
+                                   unround_result** result, char* message, size_t message_size) {
+  return guarded(message, message_size, [&](std::string& text) {
+    return reconstruct(input, settings, start, observer, user, result, text);
+  });
+}
+
+unround_status unround_project(const uint8_t*, size_t, const unround_settings*, const double*, uint64_t, uint64_t,
+                               uint64_t, unround_result** result, char* message, size_t message_size) {
+  if (result != nullptr) *result = nullptr;
+  write_message(message, message_size, "projecting a picture onto a file's intervals needs a JPEG reader, "
+                                       "which is not part of this library");
+  return UNROUND_ERROR_UNSUPPORTED;
+}
+
+void unround_result_free(unround_result* result) { delete result; }
+
+const double* unround_result_picture(const unround_result* result, uint64_t* height, uint64_t* width,
+                                     uint64_t* channels) {
+  if (result == nullptr) return nullptr;
+  if (height != nullptr) *height = result->frame.height;
+  if (width != nullptr) *width = result->frame.width;
+  if (channels != nullptr) *channels = result->picture_channels;
+  return result->picture.data();
+}
+
+const double* unround_result_planes(const unround_result* result, uint64_t* count) {
+  if (result == nullptr) return nullptr;
+  if (count != nullptr) *count = result->planes.size();
+  return result->planes.data();
+}
+
+int32_t unround_result_color_space(const unround_result* result) {
+  return result == nullptr ? 0 : result->frame.color_space;
+}
+
+const double* unround_result_canvas(const unround_result* result, uint64_t* channels, uint64_t* height,
+                                    uint64_t* width) {
+  if (result == nullptr) return nullptr;
+  if (channels != nullptr) *channels = result->frame.channels();
+  if (height != nullptr) *height = result->frame.rows;
+  if (width != nullptr) *width = result->frame.columns;
+  return result->out.canvas.data();
+}
+
+uint64_t unround_result_components(const unround_result* result) {
+  return result == nullptr ? 0 : result->frame.channels();
+}
+
+const double* unround_result_coefficients(const unround_result* result, uint64_t component, uint64_t* count) {
+  if (result == nullptr || component >= result->frame.channels()) return nullptr;
+  if (count != nullptr) *count = result->out.coefficients[component].size();
+  return result->out.coefficients[component].data();
+}
+
+const double* unround_result_problem(const unround_result* result, uint64_t component, int32_t which,
+                                     uint64_t* count) {
+  if (result == nullptr || component >= result->frame.channels() || which < 0 || which > 4) return nullptr;
+  try {
+    const std::lock_guard<std::mutex> lock(result->made_guard);
+    if (!result->made[component]) {
+      const unround::ComponentModel& k = result->frame.components[component];
+      auto& arrays = result->problems[component];
+      const size_t entries = 64 * k.blocks();
+      for (int a = 0; a < 3; ++a) arrays[a].resize(entries);
+      for (size_t b = 0; b < k.blocks(); ++b) {
+        for (int q = 0; q < 64; ++q) {
+          double own_lower, own_upper;
+          k.coefficient(b, q, arrays[0][64 * b + q], arrays[1][64 * b + q], arrays[2][64 * b + q], own_lower,
+                        own_upper);
+        }
+      }
+      arrays[3].assign(k.steps.begin(), k.steps.end());
+      arrays[4].assign(k.weights.begin(), k.weights.end());
+      result->made[component] = true;
+    }
+    const std::vector<double>& values = result->problems[component][which];
+    if (count != nullptr) *count = values.size();
+    return values.data();
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+unround_status unround_result_component(const unround_result* result, uint64_t component, uint64_t* rows,
+                                        uint64_t* columns, uint64_t* down, uint64_t* across) {
+  if (result == nullptr || component >= result->frame.channels()) return UNROUND_ERROR_ARGUMENT;
+  const unround::ComponentModel& k = result->frame.components[component];
+  if (rows != nullptr) *rows = k.rows;
+  if (columns != nullptr) *columns = k.columns;
+  if (down != nullptr) *down = k.down;
+  if (across != nullptr) *across = k.across;
+  return UNROUND_OK;
+}
+
+int32_t unround_result_solver(const unround_result* result, uint64_t* iterations, int32_t* stop) {
+  if (result == nullptr || !result->solver) return 0;
+  if (iterations != nullptr) *iterations = result->out.iterations;
+  if (stop != nullptr) *stop = result->out.stop;
+  return 1;
+}
+
+const uint64_t* unround_result_record_iterations(const unround_result* result, uint64_t* count) {
+  if (result == nullptr) return nullptr;
+  if (count != nullptr) *count = result->record_iterations.size();
+  return result->record_iterations.data();
+}
+
+const double* unround_result_record_values(const unround_result* result, int32_t which, uint64_t* count) {
+  if (result == nullptr || which < 0 || which > 4) return nullptr;
+  if (count != nullptr) *count = result->record_values[which].size();
+  return result->record_values[which].data();
+}
+
+int32_t unround_result_alpha_solver(const unround_result*, uint64_t*, int32_t*) { return 0; }
+
+const uint64_t* unround_result_alpha_record_iterations(const unround_result*, uint64_t* count) {
+  if (count != nullptr) *count = 0;
+  return nullptr;
+}
+
+const double* unround_result_alpha_record_values(const unround_result*, int32_t, uint64_t* count) {
+  if (count != nullptr) *count = 0;
+  return nullptr;
+}
+
+const uint8_t* unround_result_icc_profile(const unround_result*, uint64_t* size) {
+  if (size != nullptr) *size = 0;
+  return nullptr;
+}
+
+int32_t unround_result_exif_orientation(const unround_result*) { return 0; }
+
+int32_t unround_result_format(const unround_result* result) { return result == nullptr ? 0 : UNROUND_FORMAT_JPEG; }
+
+int32_t unround_result_bits(const unround_result* result) { return result == nullptr ? 0 : 8; }
+
+const char* unround_result_warning(const unround_result* result) {
+  return result == nullptr ? nullptr : result->warning.c_str();
+}
+
+const double* unround_result_excess(const unround_result*, uint64_t, uint64_t* count) {
+  if (count != nullptr) *count = 0;
+  return nullptr;
+}
+
+int32_t unround_result_kernel_times(const unround_result* result, double* values, size_t count) {
+  if (result == nullptr || values == nullptr) return 0;
+  const unround::KernelTimes& t = result->out.times;
+  const double all[6] = {t.primal_seconds, static_cast<double>(t.primal_launches), t.dual_seconds,
+                         static_cast<double>(t.dual_launches), t.iterations_seconds, static_cast<double>(t.iterations)};
+  for (size_t k = 0; k < count && k < 6; ++k) values[k] = all[k];
+  return t.iterations > 0 ? 1 : 0;
+}
+
+namespace {
+unround_status unsupported(char* message, size_t size, const char* what) {
+  write_message(message, size, std::string(what) + " is not part of this library");
+  return UNROUND_ERROR_UNSUPPORTED;
+}
+}  // namespace
+
+unround_status unround_tiff(const double*, uint64_t, uint64_t, uint64_t, int32_t, const uint8_t*, uint64_t,
+                            uint8_t** bytes, uint64_t* size, char* message, size_t message_size) {
+  if (bytes != nullptr) *bytes = nullptr;
+  if (size != nullptr) *size = 0;
+  return unsupported(message, message_size, "writing TIFF");
+}
+
+unround_status unround_png(const double*, uint64_t, uint64_t, uint64_t, int32_t, int32_t, int32_t, const uint8_t*,
+                           uint64_t, uint8_t** bytes, uint64_t* size, char* message, size_t message_size) {
+  if (bytes != nullptr) *bytes = nullptr;
+  if (size != nullptr) *size = 0;
+  return unsupported(message, message_size, "writing PNG");
+}
+
+unround_status unround_pnm(const double*, uint64_t, uint64_t, uint64_t, int32_t, int32_t, uint8_t** bytes,
+                           uint64_t* size, char* message, size_t message_size) {
+  if (bytes != nullptr) *bytes = nullptr;
+  if (size != nullptr) *size = 0;
+  return unsupported(message, message_size, "writing PNM");
+}
+
+void unround_bytes_free(uint8_t* bytes, uint64_t) { std::free(bytes); }
+
+unround_status unround_orient(const double*, uint64_t, uint64_t, uint64_t, int32_t, double*, uint64_t*, uint64_t*,
+                              char* message, size_t message_size) {
+  return unsupported(message, message_size, "turning a picture upright");
+}
+
+unround_status unround_to_rgb(const double* planes, uint64_t height, uint64_t width, double* picture) {
+  if (planes == nullptr || picture == nullptr) return UNROUND_ERROR_ARGUMENT;
+  rgb_of(planes, static_cast<size_t>(height * width), picture);
+  return UNROUND_OK;
+}
+
+unround_status unround_to_ycbcr(const double* picture, uint64_t height, uint64_t width, double* planes) {
+  if (planes == nullptr || picture == nullptr) return UNROUND_ERROR_ARGUMENT;
+  // Y = (kR R + kG G) + kB B, Cb = 128 + (250/443)(B - Y), Cr = 128 + (500/701)(R - Y)
+  // (docs/math.md:1035-1040).
+  const size_t count = static_cast<size_t>(height * width);
+  const double kCbInverse = 250.0 / 443.0, kCrInverse = 500.0 / 701.0;
+  for (size_t k = 0; k < count; ++k) {
+    const double r = picture[3 * k], g = picture[3 * k + 1], b = picture[3 * k + 2];
+    const double y = (0.299 * r + 0.587 * g) + 0.114 * b;
+    planes[k] = y;
+    planes[count + k] = 128.0 + kCbInverse * (b - y);
+    planes[2 * count + k] = 128.0 + kCrInverse * (r - y);
+  }
+  return UNROUND_OK;
+}
+
+int32_t unround_main(size_t, const char* const*) { return 2; }
+
+}  // extern "C"

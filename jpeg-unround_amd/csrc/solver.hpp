@@ -1,0 +1,65 @@
+// The GPU solve of a frame: the primal-dual iterations of docs/math.md 5 on HIP kernels,
+// their records (docs/math.md 6) reduced in binary64 on the device, and the final binary64
+// projection of the last canvas onto the quantization constraint set.
+#pragma once
+
+#include <cstdint>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "model.hpp"
+#include "settings.hpp"
+
+namespace unround {
+
+// A record of the solver (cpp/include/unround/results.hpp:17-24): NaN where not taken.
+struct Record {
+  uint64_t iteration = 0;
+  double seconds = 0.0;
+  double primal = 0.0;
+  double dual = 0.0;
+  double theta = 0.0;
+  double partial = 0.0;
+};
+
+// A state to start from (cpp/include/unround/pdhg.hpp:128-133): null pointers for the
+// default. coefficients has one pointer per component (its blocks x 64), p and w are
+// [C][2][H][W], r [C][3][H][W], in binary64.
+struct StartState {
+  std::vector<const double*> coefficients;
+  const double* p = nullptr;
+  const double* w = nullptr;
+  const double* r = nullptr;
+};
+
+// Called at each record after the first with the tilde point's canvas (C x H x W
+// binary64); true stops the solver.
+using Observer = std::function<bool(const Record&, const double* canvas)>;
+
+// How long the kernels of the iterations took, measured with HIP events on the solve's
+// stream where timing is asked for.
+struct KernelTimes {
+  double primal_seconds = 0.0;
+  uint64_t primal_launches = 0;
+  double dual_seconds = 0.0;
+  uint64_t dual_launches = 0;
+  double iterations_seconds = 0.0;  // the iterations alone, records excluded
+  uint64_t iterations = 0;
+};
+
+struct SolveOutput {
+  std::vector<double> canvas;                     // C x H x W, the final projection
+  std::vector<std::vector<double>> coefficients;  // per component, blocks x 64
+  std::vector<Record> records;
+  uint64_t iterations = 0;
+  int32_t stop = 0;  // 0 no solver, 1 converged, 2 iterations, 3 observer
+  double tau = 0.0, sigma = 0.0;
+  KernelTimes times;
+};
+
+// Solves on the current HIP device. false with a message (and a status) on failure.
+bool solve_on_gpu(const FrameModel& frame, const Settings& settings, const Plan& plan, const StartState* start,
+                  const Observer& observer, bool timing, SolveOutput& out, std::string& message, int32_t& status);
+
+}  // namespace unround

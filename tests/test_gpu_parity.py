@@ -1,0 +1,101 @@
+"""The GPU's binary32 path against the oracle (binary64 CPU) on the same seeded synthetic
+inputs, at sizes the oracle finishes in seconds: 8-bit outputs within 1 level, differing on
+at most 0.1 % of the pixels; PSNR against the uncompressed picture within 0.01 dB; every
+coefficient of the result requantizing to its level, bit for bit; the records within
+binary32's reach of the oracle's. Run with -s to see the measured distances."""
+
+import numpy as np
+import pytest
+
+import helpers
+import unround
+from unround import native, synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def check(image, original, options, precision="single", ratio_limit=1e-3, psnr_limit=0.01):
+    print(f"parity {image.height}x{image.width} space {int(image.color_space)} {' '.join(options)} {precision}",
+          flush=True)
+    expected = helpers.oracle_solve(image, options)
+    decoded = native.solve(image, options + ["--precision", precision])
+    C, H, W = expected.canvas.shape
+    assert decoded.canvas.shape == (C, H, W)
+    h, w = int(image.height), int(image.width)
+    ours = decoded.planes if C == 1 else decoded.planes
+    theirs = expected.canvas[:, :h, :w]
+    a, b = helpers.rounded(ours), helpers.rounded(theirs)
+    differing = float(np.mean(a != b))
+    largest = int(np.max(np.abs(a - b)))
+    worst = float(np.max(np.abs(ours - theirs)))
+    print(f"  max |diff| {worst:.3g}, 8-bit differing {differing:.5f}, largest {largest}")
+    assert largest <= 1
+    assert differing <= ratio_limit
+    if original is not None:
+        if C == 1:
+            ours_psnr, theirs_psnr = helpers.psnr(ours[0], original), helpers.psnr(theirs[0], original)
+        else:
+            ours_rgb = unround.to_rgb(ours)
+            theirs_rgb = unround.to_rgb(theirs)
+            ours_psnr, theirs_psnr = helpers.psnr(ours_rgb, original), helpers.psnr(theirs_rgb, original)
+        print(f"  PSNR {ours_psnr:.4f} vs {theirs_psnr:.4f}")
+        assert abs(ours_psnr - theirs_psnr) < psnr_limit
+    for c, problem in enumerate(decoded.problems):
+        rows, columns = decoded.coefficients[c].shape[:2]
+        levels = helpers.requantized(decoded.canvas[c], image, c, problem.ratio)
+        assert np.array_equal(levels, helpers.kept_levels(image, c, rows, columns)), f"component {c} requantizes"
+        assert np.all(decoded.coefficients[c] > problem.lower) and np.all(decoded.coefficients[c] < problem.upper)
+    history = decoded.result.history
+    assert list(history.iterations) == list(expected.record_iterations)
+    relative = np.abs(history.primal - expected.primal) / np.abs(expected.primal)
+    print(f"  records: primal rel {np.max(relative):.3g}")
+    assert np.max(relative) < 1e-4
+    return decoded, expected
+
+
+def test_tv_grey_256_q50_200_iterations():
+    picture = synthetic.grey_picture(256, 256, 0)
+    image = synthetic.encode_grey(picture, 50)
+    check(image, picture, ["--method", "tv", "--iterations", "200", "--tolerance", "0"])
+
+
+def test_tgv_grey_odd_size():
+    picture = synthetic.grey_picture(120, 200, 3)
+    image = synthetic.encode_grey(picture, 40)
+    check(image, picture, ["--method", "tgv", "--iterations", "150", "--tolerance", "0"])
+
+
+def test_tgv_colour420_free_samples():
+    # 136 x 200 is not a multiple of 16: samples beyond the blocks are free.
+    picture = synthetic.colour_picture(136, 200, 5)
+    image = synthetic.encode_colour420(picture, 75)
+    check(image, picture, ["--method", "tgv", "--iterations", "100", "--tolerance", "0"])
+
+
+def test_tv_colour420_apart_weights():
+    picture = synthetic.colour_picture(96, 128, 6)
+    image = synthetic.encode_colour420(picture, 60)
+    check(image, picture, ["--method", "tv", "--channels", "apart", "--channel-weights", "1,0.5,0.75", "--iterations",
+                           "80", "--tolerance", "0"])
+
+
+def test_tgv_grey_partial_gap_and_model_options():
+    picture = synthetic.grey_picture(64, 96, 8)
+    image = synthetic.encode_grey(picture, 30)
+    options = ["--method", "tgv", "--iterations", "60", "--tolerance", "0", "--partial-radius", "20", "--slack",
+               "0.5", "--slack-cost", "3", "--centres", "mmse", "--flat-weight", "0.3"]
+    decoded, expected = check(image, picture, options)
+    ours = decoded.result.history.partial_gap
+    assert np.max(np.abs(ours - expected.partial) / np.abs(expected.partial)) < 1e-4
+
+
+def test_tolerance_stop_matches():
+    picture = synthetic.grey_picture(64, 64, 9)
+    image = synthetic.encode_grey(picture, 50)
+    options = ["--method", "tv", "--iterations", "2000"]
+    expected = helpers.oracle_solve(image, options)
+    decoded = native.solve(image, options)
+    assert expected.stop == 1
+    assert decoded.result.stop == native.Stop.CONVERGED
+    # binary32 may reach the tolerance a record apart from binary64.
+    assert abs(decoded.result.iterations - expected.iterations) <= 10
