@@ -40,12 +40,11 @@ def block_dct(plane: np.ndarray) -> np.ndarray:
     return np.einsum("km,abmn,un->abku", basis, blocks, basis)
 
 
-def requantized(canvas: np.ndarray, image, index: int, cells: tuple[int, int]) -> np.ndarray:
-    """The levels of a component of a canvas: its cells' means' DCT less the level shift,
-    over the steps, rounded to the nearest integer."""
+def requantized(canvas: np.ndarray, image, index: int, cells: tuple[int, int], rows: int, columns: int) -> np.ndarray:
+    """The levels of the first rows x columns blocks of a component of a canvas: its cells'
+    means' DCT less the level shift, over the steps, rounded to the nearest integer."""
     component = image.components[index]
     down, across = cells
-    rows, columns = component.coefficients.shape[:2]
     plane = canvas[: rows * 8 * down, : columns * 8 * across]
     if down * across > 1:
         plane = plane.reshape(rows * 8, down, columns * 8, across).mean(axis=(1, 3))

@@ -40,11 +40,14 @@ def check(image, original, options, precision="single", ratio_limit=1e-3, psnr_l
             ours_psnr, theirs_psnr = helpers.psnr(ours_rgb, original), helpers.psnr(theirs_rgb, original)
         print(f"  PSNR {ours_psnr:.4f} vs {theirs_psnr:.4f}")
         assert abs(ours_psnr - theirs_psnr) < psnr_limit
+    widened = "--slack" in options
     for c, problem in enumerate(decoded.problems):
         rows, columns = decoded.coefficients[c].shape[:2]
-        levels = helpers.requantized(decoded.canvas[c], image, c, problem.ratio)
-        assert np.array_equal(levels, helpers.kept_levels(image, c, rows, columns)), f"component {c} requantizes"
         assert np.all(decoded.coefficients[c] > problem.lower) and np.all(decoded.coefficients[c] < problem.upper)
+        if widened:
+            continue  # a slack widens the intervals beyond the file's own: no requantization
+        levels = helpers.requantized(decoded.canvas[c], image, c, problem.ratio, rows, columns)
+        assert np.array_equal(levels, helpers.kept_levels(image, c, rows, columns)), f"component {c} requantizes"
     history = decoded.result.history
     assert list(history.iterations) == list(expected.record_iterations)
     relative = np.abs(history.primal - expected.primal) / np.abs(expected.primal)
