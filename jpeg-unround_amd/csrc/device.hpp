@@ -1,0 +1,297 @@
+// Device-side pieces shared by the kernels: the DCT basis in constant memory, a
+// component's constants, the ends, proximal map, conjugate and value of a coefficient,
+// the 8-point transforms, the finite differences at a sample, and the storage of the dual
+// fields (binary32, binary64 or bfloat16).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace unround {
+namespace device {
+
+extern __constant__ double basis64[64];
+extern __constant__ float basis32[64];
+
+template <typename T>
+__device__ __forceinline__ T basis(int k, int n);
+template <>
+__device__ __forceinline__ double basis<double>(int k, int n) {
+  return basis64[8 * k + n];
+}
+template <>
+__device__ __forceinline__ float basis<float>(int k, int n) {
+  return basis32[8 * k + n];
+}
+
+// A bfloat16 in memory: the high half of a binary32.
+struct bf16 {
+  uint16_t bits;
+};
+
+// Values of the storage type S as the compute type, and back, rounding to nearest even.
+__device__ __forceinline__ float widen(float v) { return v; }
+__device__ __forceinline__ double widen(double v) { return v; }
+__device__ __forceinline__ float widen(bf16 v) { return __uint_as_float(static_cast<uint32_t>(v.bits) << 16); }
+template <typename S>
+__device__ __forceinline__ S narrow(float v);
+template <>
+__device__ __forceinline__ float narrow<float>(float v) {
+  return v;
+}
+template <>
+__device__ __forceinline__ bf16 narrow<bf16>(float v) {
+  const uint32_t bits = __float_as_uint(v);
+  const uint32_t rounding = 0x7fffu + ((bits >> 16) & 1u);
+  return bf16{static_cast<uint16_t>((bits + rounding) >> 16)};
+}
+template <typename S>
+__device__ __forceinline__ S narrow(double v);
+template <>
+__device__ __forceinline__ double narrow<double>(double v) {
+  return v;
+}
+
+// What the kernels know of a component: its levels and flat marks, its blocks and cells,
+// and its per-frequency constants (cpp/include/unround/model.hpp:78-164).
+struct Component {
+  const int16_t* levels;
+  const uint8_t* flat;
+  int rows, columns;  // blocks
+  int down, across;   // the cell
+  int tiles_y, tiles_x;
+  int mmse, costed, free_samples;
+  double slack;
+  double steps[64];
+  double weights[64];
+  double flat_weights[64];
+  double shrink[64];
+  double costs[64];
+};
+
+// The ends, centre and file's own ends of coefficient k of block b, in binary64, each
+// step rounded apart as the reference rounds it (cpp/src/model.cpp:44-48, 242-262).
+__device__ inline void ends_of(const Component& K, int16_t q, int k, double& lower, double& upper, double& centre,
+                               double& own_lower, double& own_upper) {
+#pragma clang fp contract(off)
+  const double level = static_cast<double>(q);
+  const double shift = k == 0 ? 1024.0 : 0.0;
+  const double step = K.steps[k];
+  lower = ((level - 0.5) - K.slack) * step + shift;
+  upper = ((level + 0.5) + K.slack) * step + shift;
+  own_lower = (level - 0.5) * step + shift;
+  own_upper = (level + 0.5) * step + shift;
+  centre = level * step + shift;
+  if (K.mmse && k > 0) {
+    const double size = (fabs(level) - K.shrink[k]) * step;
+    centre = q == 0 ? 0.0 : (q < 0 ? -size : size);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T clamp_to(T value, T low, T high) {
+  const T up = value < low ? low : value;
+  return high < up ? high : up;
+}
+
+// The proximal map of one coefficient of level q (docs/math.md:457-491;
+// cpp/src/model.cpp:62-75): the quadratic's minimizer, clipped to the interval.
+template <typename T>
+__device__ __forceinline__ T prox_of(const Component& K, int16_t q, bool flat, int k, T e, T step) {
+  double lower, upper, centre, own_lower, own_upper;
+  ends_of(K, q, k, lower, upper, centre, own_lower, own_upper);
+  const T weight = static_cast<T>(flat ? K.flat_weights[k] : K.weights[k]);
+  const T t = step * weight;
+  const T r = T(1) / (T(1) + t);
+  const T z = (e + t * static_cast<T>(centre)) * r;
+  if (!K.costed) return clamp_to(z, static_cast<T>(lower), static_cast<T>(upper));
+  const T h = step * static_cast<T>(K.costs[k]) * r;
+  const T high_own = static_cast<T>(own_upper), low_own = static_cast<T>(own_lower);
+  const T above = high_own > z - h ? high_own : z - h;
+  const T below = low_own < z + h ? low_own : z + h;
+  const T inside = z < low_own ? below : z;
+  return clamp_to(z > high_own ? above : inside, static_cast<T>(lower), static_cast<T>(upper));
+}
+
+// g*(s) of one coefficient, in binary64 (cpp/src/model.cpp:80-109).
+__device__ inline double conjugate_of(const Component& K, int16_t q, bool flat, int k, double s) {
+  double low, high, centre, own_low, own_high;
+  ends_of(K, q, k, low, high, centre, own_low, own_high);
+  const double m = flat ? K.flat_weights[k] : K.weights[k];
+  if (!K.costed) {
+    const double best = clamp_to(centre + s / m, low, high);
+    const double away = best - centre;
+    const double weighted = s * best - 0.5 * m * (away * away);
+    const double support = fmax(s * low, s * high);
+    return m > 0.0 ? weighted : support;
+  }
+  const double lambda = K.costs[k];
+  const double free_at = centre + s / m;
+  const double upward = clamp_to(centre + (s - lambda) / m, own_high, high);
+  const double downward = clamp_to(centre + (s + lambda) / m, low, own_low);
+  const double within = free_at < own_low ? downward : free_at;
+  const double weighted = free_at > own_high ? upward : within;
+  const double lower_end = s < -lambda ? low : own_low;
+  const double upper_end = s > lambda ? high : own_high;
+  const double flat_end = s > 0.0 ? upper_end : lower_end;
+  const double best = m > 0.0 ? weighted : flat_end;
+  const double away = best - centre;
+  const double beyond = fmax(fmax(best - own_high, own_low - best), 0.0);
+  return s * best - 0.5 * m * (away * away) - lambda * beyond;
+}
+
+// G's term of one coefficient (cpp/src/model.cpp:411-436).
+__device__ inline double value_of(const Component& K, int16_t q, bool flat, int k, double c) {
+  double low, high, centre, own_low, own_high;
+  ends_of(K, q, k, low, high, centre, own_low, own_high);
+  const double m = flat ? K.flat_weights[k] : K.weights[k];
+  const double away = c - centre;
+  double term = 0.5 * m * (away * away);
+  if (K.costed) term = term + K.costs[k] * fmax(fmax(c - own_high, own_low - c), 0.0);
+  return term;
+}
+
+// ---------------------------------------------------------------------------------------
+// 8-point transforms by their even and odd halves (docs/math.md:108-122): each output a
+// sum of four products from its first term on.
+
+// y = C x of eight values in registers.
+template <typename T>
+__device__ __forceinline__ void forward8(const T (&x)[8], T (&y)[8]) {
+  T s[4], d[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    s[j] = x[j] + x[7 - j];
+    d[j] = x[j] - x[7 - j];
+  }
+#pragma unroll
+  for (int k = 0; k < 8; k += 2) {
+    y[k] = basis<T>(k, 0) * s[0] + basis<T>(k, 1) * s[1] + basis<T>(k, 2) * s[2] + basis<T>(k, 3) * s[3];
+    y[k + 1] =
+        basis<T>(k + 1, 0) * d[0] + basis<T>(k + 1, 1) * d[1] + basis<T>(k + 1, 2) * d[2] + basis<T>(k + 1, 3) * d[3];
+  }
+}
+
+// x = C^T y of eight values in registers: even and odd parts, their sum and difference.
+template <typename T>
+__device__ __forceinline__ void inverse8(const T (&y)[8], T (&x)[8]) {
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    const T even = basis<T>(0, n) * y[0] + basis<T>(2, n) * y[2] + basis<T>(4, n) * y[4] + basis<T>(6, n) * y[6];
+    const T odd = basis<T>(1, n) * y[1] + basis<T>(3, n) * y[3] + basis<T>(5, n) * y[5] + basis<T>(7, n) * y[7];
+    x[n] = even + odd;
+    x[7 - n] = even - odd;
+  }
+}
+
+// One lane per entry (m, n) of a block in LDS (block[8 row + column]).
+template <typename T>
+__device__ __forceinline__ T forward_down(const T* block, int m, int n) {
+  const bool odd = m & 1;
+  T a[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const T top = block[8 * j + n], bottom = block[8 * (7 - j) + n];
+    a[j] = odd ? top - bottom : top + bottom;
+  }
+  return basis<T>(m, 0) * a[0] + basis<T>(m, 1) * a[1] + basis<T>(m, 2) * a[2] + basis<T>(m, 3) * a[3];
+}
+
+template <typename T>
+__device__ __forceinline__ T forward_along(const T* block, int m, int n) {
+  const bool odd = n & 1;
+  T a[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const T left = block[8 * m + j], right = block[8 * m + 7 - j];
+    a[j] = odd ? left - right : left + right;
+  }
+  return basis<T>(n, 0) * a[0] + basis<T>(n, 1) * a[1] + basis<T>(n, 2) * a[2] + basis<T>(n, 3) * a[3];
+}
+
+template <typename T>
+__device__ __forceinline__ T inverse_along(const T* block, int m, int n) {
+  const int near = n < 4 ? n : 7 - n;
+  const T* row = block + 8 * m;
+  const T even = basis<T>(0, near) * row[0] + basis<T>(2, near) * row[2] + basis<T>(4, near) * row[4] +
+                 basis<T>(6, near) * row[6];
+  const T odd = basis<T>(1, near) * row[1] + basis<T>(3, near) * row[3] + basis<T>(5, near) * row[5] +
+                basis<T>(7, near) * row[7];
+  return n < 4 ? even + odd : even - odd;
+}
+
+template <typename T>
+__device__ __forceinline__ T inverse_down(const T* block, int m, int n) {
+  const int near = m < 4 ? m : 7 - m;
+  const T even = basis<T>(0, near) * block[n] + basis<T>(2, near) * block[16 + n] + basis<T>(4, near) * block[32 + n] +
+                 basis<T>(6, near) * block[48 + n];
+  const T odd = basis<T>(1, near) * block[8 + n] + basis<T>(3, near) * block[24 + n] +
+                basis<T>(5, near) * block[40 + n] + basis<T>(7, near) * block[56 + n];
+  return m < 4 ? even + odd : even - odd;
+}
+
+// ---------------------------------------------------------------------------------------
+// Differences (docs/math.md:320-371; cpp/src/operators.cpp:49-198) at one sample, of a
+// field stored as S and taken as R.
+
+template <typename R, typename S>
+__device__ __forceinline__ R at(const S* f, int i, int j, int W) {
+  return static_cast<R>(widen(f[static_cast<size_t>(i) * W + j]));
+}
+
+// delta_x f at (i, j): f[i, j] (not in the last column) - f[i, j - 1] (not in the first).
+template <typename R, typename S>
+__device__ __forceinline__ R backward_across(const S* f, int i, int j, int W) {
+  if (W == 1) return R(0);
+  if (j == 0) return at<R>(f, i, 0, W);
+  if (j == W - 1) return -at<R>(f, i, j - 1, W);
+  return at<R>(f, i, j, W) - at<R>(f, i, j - 1, W);
+}
+
+template <typename R, typename S>
+__device__ __forceinline__ R backward_down(const S* f, int i, int j, int H, int W) {
+  const bool first = i == 0, last = i == H - 1;
+  if (!first && !last) return at<R>(f, i, j, W) - at<R>(f, i - 1, j, W);
+  if (!last) return at<R>(f, i, j, W);
+  if (!first) return -at<R>(f, i - 1, j, W);
+  return R(0);
+}
+
+template <typename R, typename S>
+__device__ __forceinline__ R forward_across(const S* f, int i, int j, int W) {
+  return j + 1 < W ? at<R>(f, i, j + 1, W) - at<R>(f, i, j, W) : R(0);
+}
+
+template <typename R, typename S>
+__device__ __forceinline__ R forward_down(const S* f, int i, int j, int H, int W) {
+  return i + 1 < H ? at<R>(f, i + 1, j, W) - at<R>(f, i, j, W) : R(0);
+}
+
+// div p = delta_y p_y + delta_x p_x.
+template <typename R, typename S>
+__device__ __forceinline__ R divergence_at(const S* px, const S* py, int i, int j, int H, int W) {
+  R value = backward_down<R>(py, i, j, H, W);
+  value += backward_across<R>(px, i, j, W);
+  return value;
+}
+
+// div_2 r = (d_x r_xx + d_y r_xy, d_x r_xy + d_y r_yy) at one sample.
+template <typename R, typename S>
+__device__ __forceinline__ void tensor_divergence_at(const S* rxx, const S* ryy, const S* rxy, int i, int j, int H,
+                                                     int W, R& dx, R& dy) {
+  dx = forward_across<R>(rxx, i, j, W);
+  dx += forward_down<R>(rxy, i, j, H, W);
+  dy = forward_across<R>(rxy, i, j, W);
+  dy += forward_down<R>(ryy, i, j, H, W);
+}
+
+// radius / |p| beyond the ball, 1 within (cpp/include/unround/operators.hpp:112-115).
+template <typename T>
+__device__ __forceinline__ T ball_scale(T square, T radius) {
+  const T norm = sqrt(square);
+  return norm > radius ? radius / norm : T(1);
+}
+
+}  // namespace device
+}  // namespace unround

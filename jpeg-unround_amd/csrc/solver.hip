@@ -1,22 +1,26 @@
 // HIP kernels of the TV/TGV primal-dual iteration for gfx950, and the host driver.
 //
 // Layout in HBM: every field is planes of the canvas, row-major, pitch = canvas width:
-// x[C][H][W], p[C][2][H][W] (x then y), w[C][2][H][W], r[C][3][H][W] (xx, yy, xy), plus
-// the extrapolations x- and w- that the primal kernel hands to the dual kernel, and the
-// tilde point kept for records. Levels stay int16 (rows x columns x 64 per component):
-// the kernels make the ends, centres and weights of each coefficient from its level and
-// 64 per-frequency constants, so the data term costs 2 bytes a coefficient to read.
+// x[C][H][W], p[C][2][H][W] (x then y), w[C][2][H][W], r[C][3][H][W] (xx, yy, xy). x and
+// w are stored as the iterate's type T (binary32 or binary64); p and r as S (T, or
+// bfloat16 in the mixed precision). Levels stay int16 (rows x columns x 64 per
+// component): the kernels make the ends, centres and weights of each coefficient from
+// its level and 64 per-frequency constants, so the data term costs 2 bytes a coefficient.
 //
 // One iteration (docs/math.md:599-611) is two kernels:
-//  * k_primal, one 64-lane wavefront per 8x8 block of a component (its cells of
-//    canvas samples): v = x + tau gamma div p at every sample, the means of the cells, the
-//    block DCT in LDS, the proximal map of each coefficient, the inverse DCT, the canvas
-//    moved by the change; and for TGV w~ = w + tau gamma (p + div_2 r). It writes
-//    x- = 2x~ - x and w- = 2w~ - w, and relaxes x and w in place (each is read only at
-//    its own sample there).
-//  * k_dual, one lane per pixel and every channel: the steps of p and r from the
-//    differences of x- and w-, their projections onto the balls (coupled over the
-//    channels or apart), and the relaxation of p and r in place.
+//  * prox_kernel, per component: a wavefront takes a strip of 8 blocks (64 columns of the
+//    component) and 8 rows, one lane per column. Each lane makes v = x + tau gamma div p
+//    down its column (the means of its cells where a sample stands for several), the DCT
+//    down the column in registers; the lanes swap through LDS so that each holds a row of
+//    a block, which it transforms, maps (the proximal map of each coefficient) and
+//    transforms back; they swap again and transform back down the columns. It writes
+//    x- = 2 x~ - x and relaxes x in place (each lane reads x only at its own samples).
+//    It reads x, p and the levels, and writes x- and x.
+//  * dual_kernel, one lane per pixel and every channel: TGV's w~ = w + tau gamma (p +
+//    div_2 r) at the pixel and at its neighbours left and above (for E w-), the steps of p
+//    and r from the differences of x- and w-, their projections onto the balls (coupled
+//    over the channels or apart), and the relaxation of w, p and r into the other of two
+//    buffers (the neighbours read the buffers as they were).
 // Records reduce in binary64 on the device, each work-group's terms in a fixed tree and
 // the work-groups' partial sums in a fixed order, so that a run repeats to the last bit.
 #include <hip/hip_runtime.h>
@@ -24,21 +28,29 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
-#include <mutex>
-#include <set>
 #include <cmath>
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
+#include "device.hpp"
 #include "solver.hpp"
 
 namespace unround {
+namespace device {
+__constant__ double basis64[64];
+__constant__ float basis32[64];
+}  // namespace device
 
 namespace {
+
+using device::bf16;
+using device::Component;
 
 struct GpuError : std::runtime_error {
   using std::runtime_error::runtime_error;
@@ -52,373 +64,237 @@ struct GpuError : std::runtime_error {
     }                                                                                                     \
   } while (0)
 
-__constant__ double basis64[64];
-__constant__ float basis32[64];
-
-template <typename T>
-__device__ __forceinline__ T basis(int k, int n);
-template <>
-__device__ __forceinline__ double basis<double>(int k, int n) {
-  return basis64[8 * k + n];
-}
-template <>
-__device__ __forceinline__ float basis<float>(int k, int n) {
-  return basis32[8 * k + n];
-}
-
-// What the kernels know of a component: its levels and flat marks, its blocks and cells,
-// and its per-frequency constants (cpp/include/unround/model.hpp:78-164).
-struct DeviceComponent {
-  const int16_t* levels;
-  const uint8_t* flat;
-  int rows, columns;  // blocks
-  int down, across;   // the cell
-  int tiles_y, tiles_x;
-  int mmse, costed, free_samples;
-  double slack;
-  double steps[64];
-  double weights[64];
-  double flat_weights[64];
-  double shrink[64];
-  double costs[64];
-};
-
-// The fields of the method on the device, [channel][entry][H][W].
-template <typename T>
+// The fields of the method on the device, [channel][entry][H][W]: the state (w, p and r
+// in two buffers each, the iteration reading one and writing the other), the
+// extrapolation x-, the tilde point kept for records, and the coefficients zeta.
+template <typename T, typename S>
 struct Fields {
   T* x;
   T* xbar;
   T* tx;
-  T* p;
-  T* tp;
+  S* p;
+  S* p_next;
+  S* tp;
   T* w;
-  T* wbar;
+  T* w_next;
   T* tw;
-  T* r;
-  T* tr;
+  S* r;
+  S* r_next;
+  S* tr;
   T* zeta[3];
 };
 
-// The ends, centre and file's own ends of coefficient k of block b, in binary64, each
-// step rounded apart as the reference rounds it (cpp/src/model.cpp:44-48, 242-262).
-__device__ void ends_of(const DeviceComponent& K, int b, int k, double& lower, double& upper, double& centre,
-                        double& own_lower, double& own_upper) {
-#pragma clang fp contract(off)
-  const int16_t q = K.levels[64 * b + k];
-  const double level = static_cast<double>(q);
-  const double shift = k == 0 ? 1024.0 : 0.0;
-  const double step = K.steps[k];
-  lower = ((level - 0.5) - K.slack) * step + shift;
-  upper = ((level + 0.5) + K.slack) * step + shift;
-  own_lower = (level - 0.5) * step + shift;
-  own_upper = (level + 0.5) * step + shift;
-  centre = level * step + shift;
-  if (K.mmse && k > 0) {
-    const double size = (fabs(level) - K.shrink[k]) * step;
-    centre = q == 0 ? 0.0 : (q < 0 ? -size : size);
-  }
-}
-
+// The steps of an iteration.
 template <typename T>
-__device__ __forceinline__ T clamp_to(T value, T low, T high) {
-  const T up = value < low ? low : value;
-  return high < up ? high : up;
-}
-
-// The proximal map of one coefficient (docs/math.md:457-491; cpp/src/model.cpp:62-75).
-template <typename T>
-__device__ T prox_of(const DeviceComponent& K, int b, int k, T e, T step) {
-  double lower, upper, centre, own_lower, own_upper;
-  ends_of(K, b, k, lower, upper, centre, own_lower, own_upper);
-  const T weight = static_cast<T>(K.flat[b] ? K.flat_weights[k] : K.weights[k]);
-  const T t = step * weight;
-  const T r = T(1) / (T(1) + t);
-  const T z = (e + t * static_cast<T>(centre)) * r;
-  if (!K.costed) return clamp_to(z, static_cast<T>(lower), static_cast<T>(upper));
-  const T h = step * static_cast<T>(K.costs[k]) * r;
-  const T high_own = static_cast<T>(own_upper), low_own = static_cast<T>(own_lower);
-  const T above = high_own > z - h ? high_own : z - h;
-  const T below = low_own < z + h ? low_own : z + h;
-  const T inside = z < low_own ? below : z;
-  return clamp_to(z > high_own ? above : inside, static_cast<T>(lower), static_cast<T>(upper));
-}
-
-// g*(s) of one coefficient, in binary64 (cpp/src/model.cpp:80-109).
-__device__ double conjugate_of(const DeviceComponent& K, int b, int k, double s) {
-  double low, high, centre, own_low, own_high;
-  ends_of(K, b, k, low, high, centre, own_low, own_high);
-  const double m = K.flat[b] ? K.flat_weights[k] : K.weights[k];
-  if (!K.costed) {
-    const double best = clamp_to(centre + s / m, low, high);
-    const double away = best - centre;
-    const double weighted = s * best - 0.5 * m * (away * away);
-    const double support = fmax(s * low, s * high);
-    return m > 0.0 ? weighted : support;
-  }
-  const double lambda = K.costs[k];
-  const double free_at = centre + s / m;
-  const double upward = clamp_to(centre + (s - lambda) / m, own_high, high);
-  const double downward = clamp_to(centre + (s + lambda) / m, low, own_low);
-  const double within = free_at < own_low ? downward : free_at;
-  const double weighted = free_at > own_high ? upward : within;
-  const double lower_end = s < -lambda ? low : own_low;
-  const double upper_end = s > lambda ? high : own_high;
-  const double flat = s > 0.0 ? upper_end : lower_end;
-  const double best = m > 0.0 ? weighted : flat;
-  const double away = best - centre;
-  const double beyond = fmax(fmax(best - own_high, own_low - best), 0.0);
-  return s * best - 0.5 * m * (away * away) - lambda * beyond;
-}
-
-// G's term of one coefficient (cpp/src/model.cpp:411-436).
-__device__ double value_of(const DeviceComponent& K, int b, int k, double c) {
-  double low, high, centre, own_low, own_high;
-  ends_of(K, b, k, low, high, centre, own_low, own_high);
-  const double m = K.flat[b] ? K.flat_weights[k] : K.weights[k];
-  const double away = c - centre;
-  double term = 0.5 * m * (away * away);
-  if (K.costed) term = term + K.costs[k] * fmax(fmax(c - own_high, own_low - c), 0.0);
-  return term;
-}
-
-// ---------------------------------------------------------------------------------------
-// The 8x8 DCT of a block in LDS, one lane per entry (m, n) = (lane / 8, lane % 8): the
-// 8-point transforms by their even and odd halves (docs/math.md:108-122), forward down
-// the columns then along the rows, inverse along the rows then down the columns.
-
-// Output (m, n) of the forward transform down column n of block[8 row + column].
-template <typename T>
-__device__ __forceinline__ T forward_down(const T* block, int m, int n) {
-  const bool odd = m & 1;
-  T a[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const T top = block[8 * j + n], bottom = block[8 * (7 - j) + n];
-    a[j] = odd ? top - bottom : top + bottom;
-  }
-  return basis<T>(m, 0) * a[0] + basis<T>(m, 1) * a[1] + basis<T>(m, 2) * a[2] + basis<T>(m, 3) * a[3];
-}
-
-// Output (m, n) of the forward transform along row m.
-template <typename T>
-__device__ __forceinline__ T forward_along(const T* block, int m, int n) {
-  const bool odd = n & 1;
-  T a[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const T left = block[8 * m + j], right = block[8 * m + 7 - j];
-    a[j] = odd ? left - right : left + right;
-  }
-  return basis<T>(n, 0) * a[0] + basis<T>(n, 1) * a[1] + basis<T>(n, 2) * a[2] + basis<T>(n, 3) * a[3];
-}
-
-// Output n of the inverse transform along row m: even part e_n' and odd part o_n' of
-// n' = min(n, 7 - n), their sum below 4 and difference above.
-template <typename T>
-__device__ __forceinline__ T inverse_along(const T* block, int m, int n) {
-  const int near = n < 4 ? n : 7 - n;
-  const T* row = block + 8 * m;
-  const T even = basis<T>(0, near) * row[0] + basis<T>(2, near) * row[2] + basis<T>(4, near) * row[4] +
-                 basis<T>(6, near) * row[6];
-  const T odd = basis<T>(1, near) * row[1] + basis<T>(3, near) * row[3] + basis<T>(5, near) * row[5] +
-                basis<T>(7, near) * row[7];
-  return n < 4 ? even + odd : even - odd;
-}
-
-// Output m of the inverse transform down column n.
-template <typename T>
-__device__ __forceinline__ T inverse_down(const T* block, int m, int n) {
-  const int near = m < 4 ? m : 7 - m;
-  const T even = basis<T>(0, near) * block[n] + basis<T>(2, near) * block[16 + n] + basis<T>(4, near) * block[32 + n] +
-                 basis<T>(6, near) * block[48 + n];
-  const T odd = basis<T>(1, near) * block[8 + n] + basis<T>(3, near) * block[24 + n] +
-                basis<T>(5, near) * block[40 + n] + basis<T>(7, near) * block[56 + n];
-  return m < 4 ? even + odd : even - odd;
-}
-
-// ---------------------------------------------------------------------------------------
-// Differences (docs/math.md:320-371; cpp/src/operators.cpp:49-198), at one sample.
-
-template <typename T>
-__device__ __forceinline__ T load(const T* f, int i, int j, int W) {
-  return f[static_cast<size_t>(i) * W + j];
-}
-
-// delta_x f at (i, j): f[i, j] (not in the last column) - f[i, j - 1] (not in the first).
-template <typename T, typename R = T>
-__device__ __forceinline__ R backward_across(const T* f, int i, int j, int W) {
-  if (W == 1) return R(0);
-  if (j == 0) return static_cast<R>(load(f, i, 0, W));
-  if (j == W - 1) return -static_cast<R>(load(f, i, j - 1, W));
-  return static_cast<R>(load(f, i, j, W)) - static_cast<R>(load(f, i, j - 1, W));
-}
-
-template <typename T, typename R = T>
-__device__ __forceinline__ R backward_down(const T* f, int i, int j, int H, int W) {
-  const bool first = i == 0, last = i == H - 1;
-  if (!first && !last) return static_cast<R>(load(f, i, j, W)) - static_cast<R>(load(f, i - 1, j, W));
-  if (!last) return static_cast<R>(load(f, i, j, W));
-  if (!first) return -static_cast<R>(load(f, i - 1, j, W));
-  return R(0);
-}
-
-template <typename T, typename R = T>
-__device__ __forceinline__ R forward_across(const T* f, int i, int j, int W) {
-  return j + 1 < W ? static_cast<R>(load(f, i, j + 1, W)) - static_cast<R>(load(f, i, j, W)) : R(0);
-}
-
-template <typename T, typename R = T>
-__device__ __forceinline__ R forward_down(const T* f, int i, int j, int H, int W) {
-  return i + 1 < H ? static_cast<R>(load(f, i + 1, j, W)) - static_cast<R>(load(f, i, j, W)) : R(0);
-}
-
-// div p = delta_y p_y + delta_x p_x.
-template <typename T, typename R = T>
-__device__ __forceinline__ R divergence_at(const T* px, const T* py, int i, int j, int H, int W) {
-  R value = backward_down<T, R>(py, i, j, H, W);
-  value += backward_across<T, R>(px, i, j, W);
-  return value;
-}
-
-template <typename T>
-__device__ __forceinline__ T ball_scale(T square, T radius) {
-  const T norm = sqrt(square);
-  return norm > radius ? radius / norm : T(1);
-}
+struct Steps {
+  T descent[3];  // tau gamma_c
+  T ascent[3];   // sigma gamma_c
+  T radius;      // alpha (TV) or alpha1
+  T radius0;     // alpha0
+  T rho;
+  T rest;  // 1 - rho
+};
 
 constexpr int kMaxCell = 4;
 
 // ---------------------------------------------------------------------------------------
-// The primal step of one component (primal_dual.hpp:525-558; frames.cpp:490-548).
-template <typename T, bool TGV>
-__global__ __launch_bounds__(256) void primal_kernel(const DeviceComponent* __restrict__ component, Fields<T> F,
-                                                     int c, int H, int W, T descent, T step, T rho, T rest,
-                                                     int keep) {
-  __shared__ T first[4][64];
-  __shared__ T second[4][64];
-  const DeviceComponent& K = *component;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane >> 3, n = lane & 7;
-  const int tile = blockIdx.x * 4 + wave;
-  const bool active = tile < K.tiles_y * K.tiles_x;
-  const int ty = active ? tile / K.tiles_x : 0, tx = active ? tile % K.tiles_x : 0;
-  const bool inside = active && ty < K.rows && tx < K.columns;
-  const int dv = K.down, dh = K.across, cells = dv * dh;
+// The proximal step of one component (primal_dual.hpp:536-546; frames.cpp:490-548).
+template <typename T, typename S, bool CELLS>
+__global__ __launch_bounds__(256) void prox_kernel(const Component* __restrict__ component, Fields<T, S> F, int c,
+                                                   int H, int W, T descent, T step, T rho, T rest, int keep) {
+  __shared__ T forward_room[4][8][65];
+  __shared__ T inverse_room[4][8][65];
+  const Component& K = *component;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int strips = (K.tiles_x + 7) >> 3;
+  const int strip = blockIdx.x % strips;
+  const int ty = (blockIdx.x / strips) * 4 + wave;  // the wave's row of tiles (blocks)
+  const int tx = strip * 8 + (lane >> 3);           // the tile of the lane's column
+  const int n = lane & 7;
+  const bool live = ty < K.tiles_y && tx < K.tiles_x;
+  const int dv = CELLS ? K.down : 1, dh = CELLS ? K.across : 1;
   const size_t N = static_cast<size_t>(H) * W;
   T* x = F.x + c * N;
-  const T* px = F.p + (2 * c) * N;
-  const T* py = F.p + (2 * c + 1) * N;
-  T xs[kMaxCell * kMaxCell], vs[kMaxCell * kMaxCell];
-  // v at each sample of the lane's cell, and their sum: rows of a column first, then
-  // the columns (cpp/src/frames.cpp:323-339).
-  T sum = T(0);
+  const S* px = F.p + (2 * c) * N;
+  const S* py = F.p + (2 * c + 1) * N;
+  // v down the lane's column of the block: of each sample, or each cell's mean (its rows
+  // of a column added first, then the columns; cpp/src/frames.cpp:323-339).
+  T column[8], old[8];
 #pragma unroll
-  for (int b = 0; b < kMaxCell; ++b) {
-    T column = T(0);
-#pragma unroll
-    for (int a = 0; a < kMaxCell; ++a) {
-      if (active && a < dv && b < dh) {
-        const int i = (8 * ty + m) * dv + a, j = (8 * tx + n) * dh + b;
-        const T here = load(x, i, j, W);
-        const T v = here + descent * divergence_at(px, py, i, j, H, W);
-        xs[kMaxCell * a + b] = here;
-        vs[kMaxCell * a + b] = v;
-        column = a == 0 ? v : column + v;
+  for (int m = 0; m < 8; ++m) {
+    T value = T(0), here = T(0);
+    if (live) {
+      if constexpr (!CELLS) {
+        const int i = 8 * ty + m, j = 8 * tx + n;
+        here = x[static_cast<size_t>(i) * W + j];
+        value = here + descent * device::divergence_at<T>(px, py, i, j, H, W);
+      } else {
+        T sum = T(0);
+        for (int b = 0; b < dh; ++b) {
+          T partial = T(0);
+          for (int a = 0; a < dv; ++a) {
+            const int i = (8 * ty + m) * dv + a, j = (8 * tx + n) * dh + b;
+            const T v = x[static_cast<size_t>(i) * W + j] + descent * device::divergence_at<T>(px, py, i, j, H, W);
+            partial = a == 0 ? v : partial + v;
+          }
+          sum = b == 0 ? partial : sum + partial;
+        }
+        value = sum / static_cast<T>(dv * dh);
       }
     }
-    if (b < dh) sum = b == 0 ? column : sum + column;
+    column[m] = value;
+    old[m] = here;
   }
-  const T mean = cells > 1 ? sum / static_cast<T>(cells) : sum;
-  first[wave][lane] = mean;
+  T y[8];
+  device::forward8(column, y);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) forward_room[wave][k][lane] = y[k];
   __syncthreads();
-  second[wave][lane] = forward_down(first[wave], m, n);
+  // A lane now takes row k of block b of the strip.
+  const int b = lane >> 3, k = lane & 7;
+  const int bx = strip * 8 + b;
+  const bool inside = ty < K.rows && bx < K.columns;
+  T row[8], coefficient[8], zeta[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) row[u] = forward_room[wave][k][8 * b + u];
+  device::forward8(row, coefficient);
+  if (inside) {
+    const size_t block = static_cast<size_t>(ty) * K.columns + bx;
+    const int16_t* levels = K.levels + 64 * block + 8 * k;
+    const bool flat = K.flat[block] != 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) zeta[u] = device::prox_of(K, levels[u], flat, 8 * k + u, coefficient[u], step);
+    if (keep) {
+      T* kept = F.zeta[c] + 64 * block + 8 * k;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) kept[u] = zeta[u];
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) zeta[u] = T(0);
+  }
+  T back_row[8];
+  device::inverse8(zeta, back_row);
+#pragma unroll
+  for (int v = 0; v < 8; ++v) inverse_room[wave][k][8 * b + v] = back_row[v];
   __syncthreads();
-  const T e = forward_along(second[wave], m, n);
-  const int b = ty * K.columns + tx;
-  const T zeta = inside ? prox_of(K, b, lane, e, step) : T(0);
-  if (keep && inside) F.zeta[c][64 * static_cast<size_t>(b) + lane] = zeta;
-  first[wave][lane] = zeta;
-  __syncthreads();
-  second[wave][lane] = inverse_along(first[wave], m, n);
-  __syncthreads();
-  const T back = inverse_down(second[wave], m, n);
-  const T change = back - mean;
+  T back_column[8], back[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) back_column[m] = inverse_room[wave][m][lane];
+  device::inverse8(back_column, back);
+  if (!live) return;
+  const bool own_inside = ty < K.rows && tx < K.columns;
   T* xbar = F.xbar + c * N;
-  T* tx_field = F.tx + c * N;
+  T* tilde_x = F.tx + c * N;
 #pragma unroll
-  for (int a = 0; a < kMaxCell; ++a) {
-#pragma unroll
-    for (int bb = 0; bb < kMaxCell; ++bb) {
-      if (!(active && a < dv && bb < dh)) continue;
-      const int i = (8 * ty + m) * dv + a, j = (8 * tx + n) * dh + bb;
-      const T v = vs[kMaxCell * a + bb], old = xs[kMaxCell * a + bb];
-      const T tilde = inside ? (cells > 1 ? v + change : back) : v;
-      const size_t at = static_cast<size_t>(i) * W + j;
-      xbar[at] = T(2) * tilde - old;
-      x[at] = rho * tilde + rest * old;
-      if (keep) tx_field[at] = tilde;
-      if constexpr (TGV) {
-        // w~ = w + tau gamma (p + div_2 r), w- = 2 w~ - w, w relaxed.
-        const T* rxx = F.r + (3 * c) * N;
-        const T* ryy = F.r + (3 * c + 1) * N;
-        const T* rxy = F.r + (3 * c + 2) * N;
-        T d2x = forward_across(rxx, i, j, W);
-        d2x += forward_down(rxy, i, j, H, W);
-        T d2y = forward_across(rxy, i, j, W);
-        d2y += forward_down(ryy, i, j, H, W);
-        T* wx = F.w + (2 * c) * N;
-        T* wy = F.w + (2 * c + 1) * N;
-        const T wx_old = wx[at], wy_old = wy[at];
-        const T twx = wx_old + descent * (px[at] + d2x);
-        const T twy = wy_old + descent * (py[at] + d2y);
-        F.wbar[(2 * c) * N + at] = T(2) * twx - wx_old;
-        F.wbar[(2 * c + 1) * N + at] = T(2) * twy - wy_old;
-        wx[at] = rho * twx + rest * wx_old;
-        wy[at] = rho * twy + rest * wy_old;
-        if (keep) {
-          F.tw[(2 * c) * N + at] = twx;
-          F.tw[(2 * c + 1) * N + at] = twy;
+  for (int m = 0; m < 8; ++m) {
+    if constexpr (!CELLS) {
+      const size_t at = static_cast<size_t>(8 * ty + m) * W + 8 * tx + n;
+      const T tilde = own_inside ? back[m] : column[m];
+      xbar[at] = T(2) * tilde - old[m];
+      x[at] = rho * tilde + rest * old[m];
+      if (keep) tilde_x[at] = tilde;
+    } else {
+      // x~ = v + the change of the cell (the inverse DCT of zeta less the mean), v beyond
+      // the blocks.
+      const T change = back[m] - column[m];
+      for (int bb = 0; bb < dh; ++bb) {
+        for (int a = 0; a < dv; ++a) {
+          const int i = (8 * ty + m) * dv + a, j = (8 * tx + n) * dh + bb;
+          const size_t at = static_cast<size_t>(i) * W + j;
+          const T here = x[at];
+          const T v = here + descent * device::divergence_at<T>(px, py, i, j, H, W);
+          const T tilde = own_inside ? v + change : v;
+          xbar[at] = T(2) * tilde - here;
+          x[at] = rho * tilde + rest * here;
+          if (keep) tilde_x[at] = tilde;
         }
       }
     }
   }
 }
 
-// The dual steps of every channel at one pixel (primal_dual.hpp:572-735).
-template <typename T, bool TGV, bool COUPLED>
-__global__ __launch_bounds__(256) void dual_kernel(Fields<T> F, int C, int H, int W, T a0, T a1, T a2, T radius,
-                                                   T radius0, T rho, T rest, int keep) {
+// w~ = w + tau gamma (p + div_2 r) at (i, j) of a channel's fields, and w there.
+template <typename T, typename S>
+__device__ __forceinline__ void w_step_at(const T* wx, const T* wy, const S* px, const S* py, const S* rxx,
+                                          const S* ryy, const S* rxy, int i, int j, int H, int W, T descent, T& twx,
+                                          T& twy, T& ox, T& oy) {
+  T dx, dy;
+  device::tensor_divergence_at<T>(rxx, ryy, rxy, i, j, H, W, dx, dy);
+  ox = device::at<T>(wx, i, j, W);
+  oy = device::at<T>(wy, i, j, W);
+  twx = ox + descent * (device::at<T>(px, i, j, W) + dx);
+  twy = oy + descent * (device::at<T>(py, i, j, W) + dy);
+}
+
+// The dual step of every channel at one pixel, with TGV's step of w (primal_dual.hpp:547-557,
+// 572-735): from the buffers of the iteration's start into the other ones.
+template <typename T, typename S, int C, bool TGV, bool COUPLED>
+__global__ __launch_bounds__(256) void dual_kernel(Fields<T, S> F, int H, int W, Steps<T> steps, int keep) {
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
   const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (i >= H || j >= W) return;
   const size_t N = static_cast<size_t>(H) * W;
   const size_t at = static_cast<size_t>(i) * W + j;
-  const T ascents[3] = {a0, a1, a2};
-  T step_p[3][2], step_r[3][3];
+  T step_p[C][2], step_r[C][3], tilde_w[C][2], old_w[C][2], old_p[C][2], old_r[C][3];
   T square = T(0), tensor_square = T(0);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    if (c >= C) break;
-    const T a = ascents[c];
+  for (int c = 0; c < C; ++c) {
+    const T a = steps.ascent[c];
     const T* xbar = F.xbar + c * N;
-    const T g1 = forward_across(xbar, i, j, W);
-    const T g2 = forward_down(xbar, i, j, H, W);
-    const T pxv = F.p[(2 * c) * N + at], pyv = F.p[(2 * c + 1) * N + at];
+    const T g1 = device::forward_across<T>(xbar, i, j, W);
+    const T g2 = device::forward_down<T>(xbar, i, j, H, W);
+    const S* px = F.p + (2 * c) * N;
+    const S* py = F.p + (2 * c + 1) * N;
+    old_p[c][0] = device::widen(px[at]);
+    old_p[c][1] = device::widen(py[at]);
     if constexpr (TGV) {
-      const T* bwx = F.wbar + (2 * c) * N;
-      const T* bwy = F.wbar + (2 * c + 1) * N;
-      step_p[c][0] = pxv + a * (g1 - bwx[at]);
-      step_p[c][1] = pyv + a * (g2 - bwy[at]);
-      const T exx = backward_across(bwx, i, j, W);
-      const T eyy = backward_down(bwy, i, j, H, W);
-      T exy = backward_down(bwx, i, j, H, W);
-      exy += backward_across(bwy, i, j, W);
+      const T* wx = F.w + (2 * c) * N;
+      const T* wy = F.w + (2 * c + 1) * N;
+      const S* rxx = F.r + (3 * c) * N;
+      const S* ryy = F.r + (3 * c + 1) * N;
+      const S* rxy = F.r + (3 * c + 2) * N;
+      const T descent = steps.descent[c];
+      // w- at the pixel, at its left and above: 2 w~ - w.
+      T twx, twy, ox, oy;
+      w_step_at<T>(wx, wy, px, py, rxx, ryy, rxy, i, j, H, W, descent, twx, twy, ox, oy);
+      tilde_w[c][0] = twx;
+      tilde_w[c][1] = twy;
+      old_w[c][0] = ox;
+      old_w[c][1] = oy;
+      const T bx = T(2) * twx - ox, by = T(2) * twy - oy;
+      T left_x = T(0), left_y = T(0), up_x = T(0), up_y = T(0);
+      if (j > 0) {
+        w_step_at<T>(wx, wy, px, py, rxx, ryy, rxy, i, j - 1, H, W, descent, twx, twy, ox, oy);
+        left_x = T(2) * twx - ox;
+        left_y = T(2) * twy - oy;
+      }
+      if (i > 0) {
+        w_step_at<T>(wx, wy, px, py, rxx, ryy, rxy, i - 1, j, H, W, descent, twx, twy, ox, oy);
+        up_x = T(2) * twx - ox;
+        up_y = T(2) * twy - oy;
+      }
+      // E w- (docs/math.md:355): backward differences, the last column's and row's own
+      // term left out.
+      const bool first_row = i == 0, last_row = i == H - 1;
+      const T across_x = W == 1 ? T(0) : (j == 0 ? bx : (j == W - 1 ? -left_x : bx - left_x));
+      const T across_y = W == 1 ? T(0) : (j == 0 ? by : (j == W - 1 ? -left_y : by - left_y));
+      const T down_x = (!first_row && !last_row) ? bx - up_x : (!last_row ? bx : (!first_row ? -up_x : T(0)));
+      const T down_y = (!first_row && !last_row) ? by - up_y : (!last_row ? by : (!first_row ? -up_y : T(0)));
+      T exy = down_x;
+      exy += across_y;
       exy = T(0.5) * exy;
-      step_r[c][0] = F.r[(3 * c) * N + at] + a * exx;
-      step_r[c][1] = F.r[(3 * c + 1) * N + at] + a * eyy;
-      step_r[c][2] = F.r[(3 * c + 2) * N + at] + a * exy;
+      step_p[c][0] = old_p[c][0] + a * (g1 - bx);
+      step_p[c][1] = old_p[c][1] + a * (g2 - by);
+      old_r[c][0] = device::widen(rxx[at]);
+      old_r[c][1] = device::widen(ryy[at]);
+      old_r[c][2] = device::widen(rxy[at]);
+      step_r[c][0] = old_r[c][0] + a * across_x;
+      step_r[c][1] = old_r[c][1] + a * down_y;
+      step_r[c][2] = old_r[c][2] + a * exy;
     } else {
-      step_p[c][0] = pxv + a * g1;
-      step_p[c][1] = pyv + a * g2;
+      step_p[c][0] = old_p[c][0] + a * g1;
+      step_p[c][1] = old_p[c][1] + a * g2;
     }
     if constexpr (COUPLED) {
       square += step_p[c][0] * step_p[c][0] + step_p[c][1] * step_p[c][1];
@@ -428,35 +304,38 @@ __global__ __launch_bounds__(256) void dual_kernel(Fields<T> F, int C, int H, in
       }
     }
   }
-  const T shared_scale = COUPLED ? ball_scale(square, radius) : T(1);
-  const T shared_tensor_scale = (COUPLED && TGV) ? ball_scale(tensor_square, radius0) : T(1);
+  const T rho = steps.rho, rest = steps.rest;
+  const T shared_scale = COUPLED ? device::ball_scale(square, steps.radius) : T(1);
+  const T shared_tensor_scale = (COUPLED && TGV) ? device::ball_scale(tensor_square, steps.radius0) : T(1);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    if (c >= C) break;
+  for (int c = 0; c < C; ++c) {
     T scale = shared_scale;
     if constexpr (!COUPLED) {
-      scale = ball_scale(step_p[c][0] * step_p[c][0] + step_p[c][1] * step_p[c][1], radius);
+      scale = device::ball_scale(step_p[c][0] * step_p[c][0] + step_p[c][1] * step_p[c][1], steps.radius);
     }
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
       const T projected = step_p[c][e] * scale;
-      T* p = F.p + (2 * c + e) * N;
-      if (keep) F.tp[(2 * c + e) * N + at] = projected;
-      p[at] = rho * projected + rest * p[at];
+      if (keep) F.tp[(2 * c + e) * N + at] = device::narrow<S>(projected);
+      F.p_next[(2 * c + e) * N + at] = device::narrow<S>(rho * projected + rest * old_p[c][e]);
     }
     if constexpr (TGV) {
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        if (keep) F.tw[(2 * c + e) * N + at] = tilde_w[c][e];
+        F.w_next[(2 * c + e) * N + at] = rho * tilde_w[c][e] + rest * old_w[c][e];
+      }
       T tensor_scale = shared_tensor_scale;
       if constexpr (!COUPLED) {
-        tensor_scale = ball_scale(step_r[c][0] * step_r[c][0] + step_r[c][1] * step_r[c][1] +
-                                      T(2) * (step_r[c][2] * step_r[c][2]),
-                                  radius0);
+        tensor_scale = device::ball_scale(step_r[c][0] * step_r[c][0] + step_r[c][1] * step_r[c][1] +
+                                              T(2) * (step_r[c][2] * step_r[c][2]),
+                                          steps.radius0);
       }
 #pragma unroll
       for (int e = 0; e < 3; ++e) {
         const T projected = step_r[c][e] * tensor_scale;
-        T* r = F.r + (3 * c + e) * N;
-        if (keep) F.tr[(3 * c + e) * N + at] = projected;
-        r[at] = rho * projected + rest * r[at];
+        if (keep) F.tr[(3 * c + e) * N + at] = device::narrow<S>(projected);
+        F.r_next[(3 * c + e) * N + at] = device::narrow<S>(rho * projected + rest * old_r[c][e]);
       }
     }
   }
@@ -465,31 +344,31 @@ __global__ __launch_bounds__(256) void dual_kernel(Fields<T> F, int C, int H, in
 // ---------------------------------------------------------------------------------------
 // The start (primal_dual.hpp:401-432; frames.cpp:569-593): x = the canvas of the
 // coefficients given (or the centres) clipped to their intervals, 128 beyond the blocks.
-template <typename T>
-__global__ __launch_bounds__(256) void start_kernel(const DeviceComponent* __restrict__ component, Fields<T> F, int c,
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void start_kernel(const Component* __restrict__ component, Fields<T, S> F, int c,
                                                     int H, int W, const double* __restrict__ given) {
   __shared__ double first[4][64];
   __shared__ double second[4][64];
-  const DeviceComponent& K = *component;
+  const Component& K = *component;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane >> 3, n = lane & 7;
   const int tile = blockIdx.x * 4 + wave;
   const bool active = tile < K.tiles_y * K.tiles_x;
   const int ty = active ? tile / K.tiles_x : 0, tx = active ? tile % K.tiles_x : 0;
   const bool inside = active && ty < K.rows && tx < K.columns;
-  const int b = ty * K.columns + tx;
+  const size_t block = static_cast<size_t>(ty) * K.columns + tx;
   double zeta = 0.0;
   if (inside) {
     double lower, upper, centre, own_lower, own_upper;
-    ends_of(K, b, lane, lower, upper, centre, own_lower, own_upper);
-    const double from = given != nullptr ? given[64 * static_cast<size_t>(b) + lane] : centre;
-    zeta = clamp_to(from, lower, upper);
-    F.zeta[c][64 * static_cast<size_t>(b) + lane] = static_cast<T>(zeta);
+    device::ends_of(K, K.levels[64 * block + lane], lane, lower, upper, centre, own_lower, own_upper);
+    const double from = given != nullptr ? given[64 * block + lane] : centre;
+    zeta = device::clamp_to(from, lower, upper);
+    F.zeta[c][64 * block + lane] = static_cast<T>(zeta);
   }
   first[wave][lane] = zeta;
   __syncthreads();
-  second[wave][lane] = inverse_along(first[wave], m, n);
+  second[wave][lane] = device::inverse_along(first[wave], m, n);
   __syncthreads();
-  const double back = inside ? inverse_down(second[wave], m, n) : 128.0;
+  const double back = inside ? device::inverse_down(second[wave], m, n) : 128.0;
   const size_t N = static_cast<size_t>(H) * W;
   for (int a = 0; a < K.down; ++a) {
     for (int bb = 0; bb < K.across; ++bb) {
@@ -502,62 +381,66 @@ __global__ __launch_bounds__(256) void start_kernel(const DeviceComponent* __res
   }
 }
 
-// w = grad x (the default start of TGV's w), or the given fields converted.
-template <typename T>
-__global__ void gradient_kernel(Fields<T> F, int C, int H, int W) {
+// w = grad x, the default start of TGV's w.
+template <typename T, typename S>
+__global__ void gradient_kernel(Fields<T, S> F, int C, int H, int W) {
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
   const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (i >= H || j >= W) return;
   const size_t N = static_cast<size_t>(H) * W, at = static_cast<size_t>(i) * W + j;
   for (int c = 0; c < C; ++c) {
     const T* x = F.x + c * N;
-    F.w[(2 * c) * N + at] = forward_across(x, i, j, W);
-    F.w[(2 * c + 1) * N + at] = forward_down(x, i, j, H, W);
+    F.w[(2 * c) * N + at] = device::forward_across<T>(x, i, j, W);
+    F.w[(2 * c + 1) * N + at] = device::forward_down<T>(x, i, j, H, W);
   }
 }
 
-template <typename T>
-__global__ void convert_kernel(const double* __restrict__ from, T* __restrict__ to, size_t count) {
+template <typename S>
+__global__ void convert_kernel(const double* __restrict__ from, S* __restrict__ to, size_t count) {
+  using Wide = decltype(device::widen(S{}));
   const size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k < count) to[k] = static_cast<T>(from[k]);
+  if (k < count) to[k] = device::narrow<S>(static_cast<Wide>(from[k]));
 }
 
-// The duals of a start projected onto their balls (operators.cpp:378-425), and the
-// tilde point made the start.
-template <typename T>
-__global__ void project_start_kernel(Fields<T> F, int C, int H, int W, int tgv, int coupled, T radius, T radius0) {
+// The duals of a start projected onto their balls (operators.cpp:378-425), and the tilde
+// point made the start.
+template <typename T, typename S>
+__global__ void project_start_kernel(Fields<T, S> F, int C, int H, int W, int tgv, int coupled, T radius, T radius0) {
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
   const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (i >= H || j >= W) return;
   const size_t N = static_cast<size_t>(H) * W, at = static_cast<size_t>(i) * W + j;
+  const auto p_of = [&](int c, int e) { return static_cast<T>(device::widen(F.p[(2 * c + e) * N + at])); };
+  const auto r_of = [&](int c, int e) { return static_cast<T>(device::widen(F.r[(3 * c + e) * N + at])); };
   T square = T(0), tensor_square = T(0);
   for (int c = 0; c < C && coupled; ++c) {
-    const T a = F.p[(2 * c) * N + at], b = F.p[(2 * c + 1) * N + at];
+    const T a = p_of(c, 0), b = p_of(c, 1);
     square += a * a + b * b;
-    if (tgv) {
-      const T xx = F.r[(3 * c) * N + at], yy = F.r[(3 * c + 1) * N + at], xy = F.r[(3 * c + 2) * N + at];
-      tensor_square += xx * xx + yy * yy + T(2) * (xy * xy);
-    }
+    if (tgv) tensor_square += r_of(c, 0) * r_of(c, 0) + r_of(c, 1) * r_of(c, 1) + T(2) * (r_of(c, 2) * r_of(c, 2));
   }
   for (int c = 0; c < C; ++c) {
-    T scale = ball_scale(square, radius), tensor_scale = ball_scale(tensor_square, radius0);
+    T scale = device::ball_scale(square, radius), tensor_scale = device::ball_scale(tensor_square, radius0);
     if (!coupled) {
-      const T a = F.p[(2 * c) * N + at], b = F.p[(2 * c + 1) * N + at];
-      scale = ball_scale(a * a + b * b, radius);
+      const T a = p_of(c, 0), b = p_of(c, 1);
+      scale = device::ball_scale(a * a + b * b, radius);
       if (tgv) {
-        const T xx = F.r[(3 * c) * N + at], yy = F.r[(3 * c + 1) * N + at], xy = F.r[(3 * c + 2) * N + at];
-        tensor_scale = ball_scale(xx * xx + yy * yy + T(2) * (xy * xy), radius0);
+        tensor_scale = device::ball_scale(
+            r_of(c, 0) * r_of(c, 0) + r_of(c, 1) * r_of(c, 1) + T(2) * (r_of(c, 2) * r_of(c, 2)), radius0);
       }
     }
+    T projected_p[2];
+    for (int e = 0; e < 2; ++e) projected_p[e] = p_of(c, e) * scale;
     for (int e = 0; e < 2; ++e) {
-      const T value = F.p[(2 * c + e) * N + at] * scale;
+      const S value = device::narrow<S>(projected_p[e]);
       F.p[(2 * c + e) * N + at] = value;
       F.tp[(2 * c + e) * N + at] = value;
     }
     if (tgv) {
       for (int e = 0; e < 2; ++e) F.tw[(2 * c + e) * N + at] = F.w[(2 * c + e) * N + at];
+      T projected[3];
+      for (int e = 0; e < 3; ++e) projected[e] = r_of(c, e) * tensor_scale;
       for (int e = 0; e < 3; ++e) {
-        const T value = F.r[(3 * c + e) * N + at] * tensor_scale;
+        const S value = device::narrow<S>(projected[e]);
         F.r[(3 * c + e) * N + at] = value;
         F.tr[(3 * c + e) * N + at] = value;
       }
@@ -586,12 +469,12 @@ __device__ double group_reduce(double value, double* room) {
 }
 
 // The norms of the tilde point's pixels: the first order (TV's gradient, or TGV's
-// grad x~ - w~), TGV's |E w~|_F, the largest norm of div_2 r~, and the partial gap's
-// sum over the channels of gamma |p~ + div_2 r~|; each work-group's sums and largest into
+// grad x~ - w~), TGV's |E w~|_F, the largest norm of div_2 r~, and the partial gap's sum
+// over the channels of gamma |p~ + div_2 r~|; each work-group's sums and largest into
 // partials[4 group + 0..3].
-template <typename T, bool TGV>
-__global__ __launch_bounds__(256) void norms_kernel(Fields<T> F, int C, int H, int W, double g0, double g1, double g2,
-                                                    int coupled, double* __restrict__ partials) {
+template <typename T, typename S, bool TGV>
+__global__ __launch_bounds__(256) void norms_kernel(Fields<T, S> F, int C, int H, int W, double g0, double g1,
+                                                    double g2, int coupled, double* __restrict__ partials) {
   __shared__ double room[256];
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
   const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -604,8 +487,8 @@ __global__ __launch_bounds__(256) void norms_kernel(Fields<T> F, int C, int H, i
     double squares = 0.0;
     for (int c = 0; c < C; ++c) {
       const T* x = F.tx + c * N;
-      double a = forward_across<T, double>(x, i, j, W);
-      double b = forward_down<T, double>(x, i, j, H, W);
+      double a = device::forward_across<double>(x, i, j, W);
+      double b = device::forward_down<double>(x, i, j, H, W);
       if (TGV) {
         a = a + -1.0 * static_cast<double>(F.tw[(2 * c) * N + at]);
         b = b + -1.0 * static_cast<double>(F.tw[(2 * c + 1) * N + at]);
@@ -622,10 +505,10 @@ __global__ __launch_bounds__(256) void norms_kernel(Fields<T> F, int C, int H, i
       for (int c = 0; c < C; ++c) {
         const T* wx = F.tw + (2 * c) * N;
         const T* wy = F.tw + (2 * c + 1) * N;
-        const double xx = backward_across<T, double>(wx, i, j, W);
-        const double yy = backward_down<T, double>(wy, i, j, H, W);
-        double xy = backward_down<T, double>(wx, i, j, H, W);
-        xy += backward_across<T, double>(wy, i, j, W);
+        const double xx = device::backward_across<double>(wx, i, j, W);
+        const double yy = device::backward_down<double>(wy, i, j, H, W);
+        double xy = device::backward_down<double>(wx, i, j, H, W);
+        xy += device::backward_across<double>(wy, i, j, W);
         xy = 0.5 * xy;
         squares += (coupled ? gammas[c] * gammas[c] : 1.0) * (xx * xx + yy * yy + 2.0 * (xy * xy));
         if (!coupled) {
@@ -636,20 +519,16 @@ __global__ __launch_bounds__(256) void norms_kernel(Fields<T> F, int C, int H, i
       if (coupled) second = sqrt(squares);
       squares = 0.0;
       for (int c = 0; c < C; ++c) {
-        const T* rxx = F.tr + (3 * c) * N;
-        const T* ryy = F.tr + (3 * c + 1) * N;
-        const T* rxy = F.tr + (3 * c + 2) * N;
-        double dx = forward_across<T, double>(rxx, i, j, W);
-        dx += forward_down<T, double>(rxy, i, j, H, W);
-        double dy = forward_across<T, double>(rxy, i, j, W);
-        dy += forward_down<T, double>(ryy, i, j, H, W);
+        double dx, dy;
+        device::tensor_divergence_at<double>(F.tr + (3 * c) * N, F.tr + (3 * c + 1) * N, F.tr + (3 * c + 2) * N, i, j,
+                                             H, W, dx, dy);
         squares += 1.0 * (dx * dx + dy * dy);
         if (!coupled) {
           most = fmax(most, sqrt(squares));
           squares = 0.0;
         }
-        const double ra = static_cast<double>(F.tp[(2 * c) * N + at]) + 1.0 * dx;
-        const double rb = static_cast<double>(F.tp[(2 * c + 1) * N + at]) + 1.0 * dy;
+        const double ra = static_cast<double>(device::widen(F.tp[(2 * c) * N + at])) + 1.0 * dx;
+        const double rb = static_cast<double>(device::widen(F.tp[(2 * c + 1) * N + at])) + 1.0 * dy;
         residual += gammas[c] * sqrt(1.0 * (ra * ra + rb * rb));
       }
       if (coupled) most = sqrt(squares);
@@ -670,42 +549,33 @@ __global__ __launch_bounds__(256) void norms_kernel(Fields<T> F, int C, int H, i
 
 // G's terms at the kept coefficients of one component, a work-group's sum each.
 template <typename T>
-__global__ __launch_bounds__(256) void value_kernel(const DeviceComponent* __restrict__ component, const T* zeta,
+__global__ __launch_bounds__(256) void value_kernel(const Component* __restrict__ component, const T* zeta,
                                                     double* __restrict__ partials) {
   __shared__ double room[256];
-  const DeviceComponent& K = *component;
+  const Component& K = *component;
   const size_t index = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
   const size_t count = static_cast<size_t>(K.rows) * K.columns * 64;
   double term = 0.0;
   if (index < count) {
-    const int b = static_cast<int>(index / 64), k = static_cast<int>(index % 64);
-    term = value_of(K, b, k, static_cast<double>(zeta[index]));
+    const size_t block = index / 64;
+    const int k = static_cast<int>(index % 64);
+    term = device::value_of(K, K.levels[index], K.flat[block] != 0, k, static_cast<double>(zeta[index]));
   }
   const double total = group_reduce<false>(term, room);
   if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
-// div_2 r at one sample, in binary64.
-template <typename T>
-__device__ void tensor_divergence_at(const T* rxx, const T* ryy, const T* rxy, int i, int j, int H, int W, double& dx,
-                                     double& dy) {
-  dx = forward_across<T, double>(rxx, i, j, W);
-  dx += forward_down<T, double>(rxy, i, j, H, W);
-  dy = forward_across<T, double>(rxy, i, j, W);
-  dy += forward_down<T, double>(ryy, i, j, H, W);
-}
-
 // G_c* at xi = gamma div p~ (from_r = 0) or xi = gamma div(-theta div_2 r~) (from_r = 1),
 // with the free samples' terms (frames.cpp:616-681): a work-group's sum into partials.
-template <typename T>
-__global__ __launch_bounds__(256) void conjugate_kernel(const DeviceComponent* __restrict__ component, Fields<T> F,
-                                                        int c, int H, int W, double gamma, int from_r,
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void conjugate_kernel(const Component* __restrict__ component, Fields<T, S> F, int c,
+                                                        int H, int W, double gamma, int from_r,
                                                         const double* __restrict__ most, double alpha1, double radius,
                                                         double* __restrict__ partials) {
   __shared__ double first[4][64];
   __shared__ double second[4][64];
   __shared__ double room[256];
-  const DeviceComponent& K = *component;
+  const Component& K = *component;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane >> 3, n = lane & 7;
   const int tile = blockIdx.x * 4 + wave;
   const bool active = tile < K.tiles_y * K.tiles_x;
@@ -715,11 +585,11 @@ __global__ __launch_bounds__(256) void conjugate_kernel(const DeviceComponent* _
   const size_t N = static_cast<size_t>(H) * W;
   double theta = 1.0;
   if (from_r) theta = *most > alpha1 ? alpha1 / *most : 1.0;
-  const T* px = F.tp + (2 * c) * N;
-  const T* py = F.tp + (2 * c + 1) * N;
-  const T* rxx = F.tr + (3 * c) * N;
-  const T* ryy = F.tr + (3 * c + 1) * N;
-  const T* rxy = F.tr + (3 * c + 2) * N;
+  const S* px = F.tp + (2 * c) * N;
+  const S* py = F.tp + (2 * c + 1) * N;
+  const S* rxx = F.tr + (3 * c) * N;
+  const S* ryy = F.tr + (3 * c + 1) * N;
+  const S* rxy = F.tr + (3 * c + 2) * N;
   double xis[kMaxCell * kMaxCell];
   double sum = 0.0;
 #pragma unroll
@@ -731,32 +601,25 @@ __global__ __launch_bounds__(256) void conjugate_kernel(const DeviceComponent* _
         const int i = (8 * ty + m) * dv + a, j = (8 * tx + n) * dh + b;
         double div = 0.0;
         if (!from_r) {
-          div = divergence_at<T, double>(px, py, i, j, H, W);
+          div = device::divergence_at<double>(px, py, i, j, H, W);
         } else {
           // g = -theta div_2 r~, then div g = delta_y g_y + delta_x g_x.
-          double dx, dy, gy_here = 0.0, gy_up = 0.0, gx_here = 0.0, gx_left = 0.0;
-          tensor_divergence_at(rxx, ryy, rxy, i, j, H, W, dx, dy);
-          gx_here = -theta * dx;
-          gy_here = -theta * dy;
+          double dx, dy, gy_up = 0.0, gx_left = 0.0;
+          device::tensor_divergence_at<double>(rxx, ryy, rxy, i, j, H, W, dx, dy);
+          const double gx_here = -theta * dx, gy_here = -theta * dy;
           if (i > 0) {
-            tensor_divergence_at(rxx, ryy, rxy, i - 1, j, H, W, dx, dy);
+            device::tensor_divergence_at<double>(rxx, ryy, rxy, i - 1, j, H, W, dx, dy);
             gy_up = -theta * dy;
           }
           if (j > 0) {
-            tensor_divergence_at(rxx, ryy, rxy, i, j - 1, H, W, dx, dy);
+            device::tensor_divergence_at<double>(rxx, ryy, rxy, i, j - 1, H, W, dx, dy);
             gx_left = -theta * dx;
           }
           const bool top = i == 0, bottom = i == H - 1;
-          double down_part;
-          if (!top && !bottom) down_part = gy_here - gy_up;
-          else if (!bottom) down_part = gy_here;
-          else if (!top) down_part = -gy_up;
-          else down_part = 0.0;
-          double across_part;
-          if (W == 1) across_part = 0.0;
-          else if (j == 0) across_part = gx_here;
-          else if (j == W - 1) across_part = -gx_left;
-          else across_part = gx_here - gx_left;
+          const double down_part =
+              (!top && !bottom) ? gy_here - gy_up : (!bottom ? gy_here : (!top ? -gy_up : 0.0));
+          const double across_part =
+              W == 1 ? 0.0 : (j == 0 ? gx_here : (j == W - 1 ? -gx_left : gx_here - gx_left));
           div = down_part;
           div += across_part;
         }
@@ -769,10 +632,11 @@ __global__ __launch_bounds__(256) void conjugate_kernel(const DeviceComponent* _
   }
   first[wave][lane] = sum;
   __syncthreads();
-  second[wave][lane] = forward_down(first[wave], m, n);
+  second[wave][lane] = device::forward_down(first[wave], m, n);
   __syncthreads();
-  const double s = forward_along(second[wave], m, n);
-  double term = inside ? conjugate_of(K, ty * K.columns + tx, lane, s) : 0.0;
+  const double s = device::forward_along(second[wave], m, n);
+  const size_t block = static_cast<size_t>(ty) * K.columns + tx;
+  double term = inside ? device::conjugate_of(K, K.levels[64 * block + lane], K.flat[block] != 0, lane, s) : 0.0;
   if (K.free_samples && active) {
     // <zeta, m> + R |zeta|_1 of the samples: within the blocks R |xi - mean of its cell|,
     // beyond them 128 xi + R |xi| (docs/math.md:911-949).
@@ -808,12 +672,12 @@ __global__ __launch_bounds__(256) void reduce_kernel(const double* __restrict__ 
 // cells) clipped to their intervals narrowed by `margin`, and the canvas moved by the
 // change, so that the result's block DCT requantizes to the file's levels.
 template <typename T>
-__global__ __launch_bounds__(256) void final_kernel(const DeviceComponent* __restrict__ component, const T* tx_field,
-                                                    int W, double margin, double* __restrict__ canvas,
+__global__ __launch_bounds__(256) void final_kernel(const Component* __restrict__ component, const T* tx_field, int W,
+                                                    double margin, double* __restrict__ canvas,
                                                     double* __restrict__ coefficients) {
   __shared__ double first[4][64];
   __shared__ double second[4][64];
-  const DeviceComponent& K = *component;
+  const Component& K = *component;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane >> 3, n = lane & 7;
   const int tile = blockIdx.x * 4 + wave;
   const bool active = tile < K.tiles_y * K.tiles_x;
@@ -829,7 +693,7 @@ __global__ __launch_bounds__(256) void final_kernel(const DeviceComponent* __res
     for (int a = 0; a < kMaxCell; ++a) {
       if (active && a < dv && b < dh) {
         const int i = (8 * ty + m) * dv + a, j = (8 * tx + n) * dh + b;
-        const double v = static_cast<double>(load(tx_field, i, j, W));
+        const double v = static_cast<double>(tx_field[static_cast<size_t>(i) * W + j]);
         vs[kMaxCell * a + b] = v;
         column = a == 0 ? v : column + v;
       }
@@ -839,22 +703,22 @@ __global__ __launch_bounds__(256) void final_kernel(const DeviceComponent* __res
   const double mean = cells > 1 ? sum / static_cast<double>(cells) : sum;
   first[wave][lane] = mean;
   __syncthreads();
-  second[wave][lane] = forward_down(first[wave], m, n);
+  second[wave][lane] = device::forward_down(first[wave], m, n);
   __syncthreads();
-  const double e = forward_along(second[wave], m, n);
+  const double e = device::forward_along(second[wave], m, n);
   double zeta = 0.0;
   if (inside) {
     double lower, upper, centre, own_lower, own_upper;
-    const int b = ty * K.columns + tx;
-    ends_of(K, b, lane, lower, upper, centre, own_lower, own_upper);
-    zeta = clamp_to(e, lower + margin, upper - margin);
-    coefficients[64 * static_cast<size_t>(b) + lane] = zeta;
+    const size_t block = static_cast<size_t>(ty) * K.columns + tx;
+    device::ends_of(K, K.levels[64 * block + lane], lane, lower, upper, centre, own_lower, own_upper);
+    zeta = device::clamp_to(e, lower + margin, upper - margin);
+    coefficients[64 * block + lane] = zeta;
   }
   first[wave][lane] = zeta;
   __syncthreads();
-  second[wave][lane] = inverse_along(first[wave], m, n);
+  second[wave][lane] = device::inverse_along(first[wave], m, n);
   __syncthreads();
-  const double back = inverse_down(second[wave], m, n);
+  const double back = device::inverse_down(second[wave], m, n);
   for (int a = 0; a < dv; ++a) {
     for (int bb = 0; bb < dh; ++bb) {
       if (!active) continue;
@@ -922,12 +786,12 @@ void upload_basis() {
   const std::array<double, 64>& b = dct_basis();
   float single[64];
   for (int k = 0; k < 64; ++k) single[k] = static_cast<float>(b[k]);
-  HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(basis64), b.data(), sizeof(double) * 64));
-  HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(basis32), single, sizeof(float) * 64));
+  HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(device::basis64), b.data(), sizeof(double) * 64));
+  HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(device::basis32), single, sizeof(float) * 64));
   devices.insert(device);
 }
 
-template <typename T>
+template <typename T, typename S>
 class Solver {
  public:
   Solver(const FrameModel& frame, const Settings& settings, const Plan& plan, bool timing)
@@ -943,6 +807,7 @@ class Solver {
     tgv_ = settings_.method == Method::tgv;
     const bool solving = settings_.method == Method::tv || settings_.method == Method::tgv;
     coupled_ = settings_.coupled && C_ > 1;
+    if (C_ != 1 && C_ != 3) throw GpuError("a frame of 1 or 3 channels");
     hipStream_t s = stream_.stream;
     allocate();
     upload_components();
@@ -959,7 +824,7 @@ class Solver {
         HIP_CHECK(hipMemcpyAsync(given[c].pointer, coefficients, bytes, hipMemcpyHostToDevice, s));
         device_given = given[c].as<double>();
       }
-      start_kernel<T><<<tile_groups(c), 256, 0, s>>>(component(c), fields_, c, H_, W_, device_given);
+      start_kernel<T, S><<<(tiles(c) + 3) / 4, 256, 0, s>>>(component(c), fields_, c, H_, W_, device_given);
       HIP_CHECK(hipGetLastError());
     }
     const dim3 pixels((W_ + 63) / 64, (H_ + 3) / 4);
@@ -969,14 +834,14 @@ class Solver {
         if (start != nullptr && start->w != nullptr) {
           upload_field(start->w, fields_.w, 2);
         } else {
-          gradient_kernel<T><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_);
+          gradient_kernel<T, S><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_);
           HIP_CHECK(hipGetLastError());
         }
         upload_field(start != nullptr ? start->r : nullptr, fields_.r, 3);
       }
       const T radius = static_cast<T>(tgv_ ? settings_.alpha1 : settings_.alpha);
-      project_start_kernel<T><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_, tgv_, settings_.coupled ? 1 : 0, radius,
-                                                     static_cast<T>(settings_.alpha0));
+      project_start_kernel<T, S><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_, tgv_, settings_.coupled ? 1 : 0, radius,
+                                                        static_cast<T>(settings_.alpha0));
       HIP_CHECK(hipGetLastError());
     }
     out.tau = plan_.tau;
@@ -1016,8 +881,8 @@ class Solver {
         const uint64_t next = iteration + 1;
         const bool keep = (every > 0 && next % every == 0) || next == most;
         if (timing_) {
-          // The events of every launch, read once the iterations between two records
-          // are done, so that timing adds no wait between launches.
+          // The events of every launch, read once the iterations between two records are
+          // done, so that timing adds no wait between launches.
           launches_.emplace_back();
           HIP_CHECK(hipEventRecord(launches_.back().begin, s));
         }
@@ -1038,48 +903,71 @@ class Solver {
   }
 
  private:
-  int tile_groups(int c) const {
+  int tiles(int c) const {
     const ComponentModel& k = frame_.components[c];
-    const int tiles = static_cast<int>((frame_.rows / (8 * k.down)) * (frame_.columns / (8 * k.across)));
-    return (tiles + 3) / 4;
+    return static_cast<int>((frame_.rows / (8 * k.down)) * (frame_.columns / (8 * k.across)));
   }
 
-  const DeviceComponent* component(int c) const { return components_.as<DeviceComponent>() + c; }
+  // The prox kernel's work-groups: strips of 8 tiles across, 4 rows of tiles each.
+  int prox_groups(int c) const {
+    const ComponentModel& k = frame_.components[c];
+    const int tiles_y = static_cast<int>(frame_.rows / (8 * k.down));
+    const int tiles_x = static_cast<int>(frame_.columns / (8 * k.across));
+    return ((tiles_x + 7) / 8) * ((tiles_y + 3) / 4);
+  }
+
+  const Component* component(int c) const { return components_.as<Component>() + c; }
 
   void allocate() {
     const size_t C = static_cast<size_t>(C_);
-    size_t count = 3 * C * N_ + 2 * 2 * C * N_;  // x, xbar, tx, p, tp
-    if (tgv_) count += 3 * 2 * C * N_ + 2 * 3 * C * N_;  // w, wbar, tw, r, tr
+    size_t t_count = 3 * C * N_;      // x, xbar, tx
+    size_t s_count = 3 * 2 * C * N_;  // p, p_next, tp
+    if (tgv_) {
+      t_count += 3 * 2 * C * N_;  // w, w_next, tw
+      s_count += 3 * 3 * C * N_;  // r, r_next, tr
+    }
     size_t zeta_count = 0;
     for (const ComponentModel& k : frame_.components) zeta_count += 64 * k.blocks();
-    state_ = DeviceBuffer(sizeof(T) * (count + zeta_count));
-    HIP_CHECK(hipMemsetAsync(state_.pointer, 0, sizeof(T) * (count + zeta_count), stream_.stream));
+    const size_t t_bytes = sizeof(T) * (t_count + zeta_count);
+    const size_t s_bytes = sizeof(S) * s_count;
+    state_ = DeviceBuffer(t_bytes + s_bytes + 256);
+    HIP_CHECK(hipMemsetAsync(state_.pointer, 0, t_bytes + s_bytes + 256, stream_.stream));
     T* at = state_.as<T>();
     const auto take = [&at](size_t n) {
       T* here = at;
       at += n;
       return here;
     };
-    fields_ = Fields<T>{};
+    fields_ = Fields<T, S>{};
     fields_.x = take(C * N_);
     fields_.xbar = take(C * N_);
     fields_.tx = take(C * N_);
-    fields_.p = take(2 * C * N_);
-    fields_.tp = take(2 * C * N_);
     if (tgv_) {
       fields_.w = take(2 * C * N_);
-      fields_.wbar = take(2 * C * N_);
+      fields_.w_next = take(2 * C * N_);
       fields_.tw = take(2 * C * N_);
-      fields_.r = take(3 * C * N_);
-      fields_.tr = take(3 * C * N_);
     }
     for (int c = 0; c < C_; ++c) fields_.zeta[c] = take(64 * frame_.components[c].blocks());
-    // Partials of the records: the largest count of work-groups any of their kernels has.
+    S* dual = reinterpret_cast<S*>((reinterpret_cast<uintptr_t>(at) + 255) & ~static_cast<uintptr_t>(255));
+    const auto take_dual = [&dual](size_t n) {
+      S* here = dual;
+      dual += n;
+      return here;
+    };
+    fields_.p = take_dual(2 * C * N_);
+    fields_.p_next = take_dual(2 * C * N_);
+    fields_.tp = take_dual(2 * C * N_);
+    if (tgv_) {
+      fields_.r = take_dual(3 * C * N_);
+      fields_.r_next = take_dual(3 * C * N_);
+      fields_.tr = take_dual(3 * C * N_);
+    }
+    // Partials of the records: as many as the work-groups of their kernels.
     const size_t pixel_groups = static_cast<size_t>((W_ + 63) / 64) * ((H_ + 3) / 4);
     size_t value_groups = 0, conjugate_groups = 0;
     for (int c = 0; c < C_; ++c) {
       value_groups += (64 * frame_.components[c].blocks() + 255) / 256;
-      conjugate_groups += static_cast<size_t>(tile_groups(c));
+      conjugate_groups += static_cast<size_t>((tiles(c) + 3) / 4);
     }
     norm_partials_count_ = pixel_groups;
     value_partials_count_ = value_groups;
@@ -1089,18 +977,21 @@ class Solver {
   }
 
   void upload_components() {
-    std::vector<DeviceComponent> host(C_);
+    std::vector<Component> host(C_);
     levels_.clear();
     flats_.clear();
     for (int c = 0; c < C_; ++c) {
       const ComponentModel& k = frame_.components[c];
+      if (k.down > static_cast<uint32_t>(kMaxCell) || k.across > static_cast<uint32_t>(kMaxCell)) {
+        throw GpuError("a cell larger than 4 x 4");
+      }
       levels_.emplace_back(sizeof(int16_t) * std::max<size_t>(1, k.levels.size()));
       flats_.emplace_back(std::max<size_t>(1, k.flat.size()));
       HIP_CHECK(hipMemcpyAsync(levels_.back().pointer, k.levels.data(), sizeof(int16_t) * k.levels.size(),
                                hipMemcpyHostToDevice, stream_.stream));
       HIP_CHECK(hipMemcpyAsync(flats_.back().pointer, k.flat.data(), k.flat.size(), hipMemcpyHostToDevice,
                                stream_.stream));
-      DeviceComponent& d = host[c];
+      Component& d = host[c];
       d.levels = levels_.back().as<int16_t>();
       d.flat = flats_.back().as<uint8_t>();
       d.rows = static_cast<int>(k.rows);
@@ -1120,26 +1011,29 @@ class Solver {
         d.shrink[q] = k.shrink[q];
         d.costs[q] = k.costs[q];
       }
-      if (k.down > static_cast<uint32_t>(kMaxCell) || k.across > static_cast<uint32_t>(kMaxCell)) {
-        throw GpuError("a cell larger than 4 x 4");
-      }
     }
-    components_ = DeviceBuffer(sizeof(DeviceComponent) * C_);
-    HIP_CHECK(hipMemcpyAsync(components_.pointer, host.data(), sizeof(DeviceComponent) * C_, hipMemcpyHostToDevice,
+    components_ = DeviceBuffer(sizeof(Component) * C_);
+    HIP_CHECK(hipMemcpyAsync(components_.pointer, host.data(), sizeof(Component) * C_, hipMemcpyHostToDevice,
                              stream_.stream));
     HIP_CHECK(hipStreamSynchronize(stream_.stream));
   }
 
   // A field of the start, converted from binary64, or left at 0.
-  void upload_field(const double* given, T* field, int entries) {
+  template <typename F>
+  void upload_field(const double* given, F* field, int entries) {
     if (given == nullptr) return;
     const size_t count = static_cast<size_t>(entries) * C_ * N_;
     DeviceBuffer staging(sizeof(double) * count);
     HIP_CHECK(hipMemcpyAsync(staging.pointer, given, sizeof(double) * count, hipMemcpyHostToDevice, stream_.stream));
-    convert_kernel<T><<<static_cast<unsigned>((count + 255) / 256), 256, 0, stream_.stream>>>(staging.as<double>(),
+    convert_kernel<F><<<static_cast<unsigned>((count + 255) / 256), 256, 0, stream_.stream>>>(staging.as<double>(),
                                                                                              field, count);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(stream_.stream));
+  }
+
+  template <int C, bool TGV, bool COUPLED>
+  void launch_dual(const dim3& grid, const Steps<T>& steps, int keep) {
+    dual_kernel<T, S, C, TGV, COUPLED><<<grid, 256, 0, stream_.stream>>>(fields_, H_, W_, steps, keep);
   }
 
   void iterate(bool keep) {
@@ -1155,37 +1049,40 @@ class Solver {
       const T descent = static_cast<T>(plan_.tau * plan_.gamma[c]);
       const T step = static_cast<T>(plan_.tau / static_cast<double>(k.cells()));
       if (timing_) HIP_CHECK(hipEventRecord(primal_events_[first_event + c].begin, s));
-      if (tgv_) {
-        primal_kernel<T, true><<<tile_groups(c), 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho,
-                                                              rest, keep ? 1 : 0);
-      } else {
-        primal_kernel<T, false><<<tile_groups(c), 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho,
+      if (k.cells() > 1) {
+        prox_kernel<T, S, true><<<prox_groups(c), 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho,
                                                                rest, keep ? 1 : 0);
+      } else {
+        prox_kernel<T, S, false><<<prox_groups(c), 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho,
+                                                                rest, keep ? 1 : 0);
       }
       HIP_CHECK(hipGetLastError());
       if (timing_) HIP_CHECK(hipEventRecord(primal_events_[first_event + c].end, s));
     }
     const dim3 pixels((W_ + 63) / 64, (H_ + 3) / 4);
-    T a[3] = {T(0), T(0), T(0)};
-    for (int c = 0; c < C_; ++c) a[c] = static_cast<T>(plan_.sigma * plan_.gamma[c]);
-    const T radius = static_cast<T>(tgv_ ? settings_.alpha1 : settings_.alpha);
-    const T radius0 = static_cast<T>(settings_.alpha0);
+    Steps<T> steps{};
+    for (int c = 0; c < C_; ++c) {
+      steps.descent[c] = static_cast<T>(plan_.tau * plan_.gamma[c]);
+      steps.ascent[c] = static_cast<T>(plan_.sigma * plan_.gamma[c]);
+    }
+    steps.radius = static_cast<T>(tgv_ ? settings_.alpha1 : settings_.alpha);
+    steps.radius0 = static_cast<T>(settings_.alpha0);
+    steps.rho = rho;
+    steps.rest = rest;
+    const int k = keep ? 1 : 0;
     if (timing_) HIP_CHECK(hipEventRecord(dual_events_.back().begin, s));
-    if (tgv_ && coupled_) {
-      dual_kernel<T, true, true><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_, a[0], a[1], a[2], radius, radius0, rho,
-                                                        rest, keep ? 1 : 0);
-    } else if (tgv_) {
-      dual_kernel<T, true, false><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_, a[0], a[1], a[2], radius, radius0, rho,
-                                                         rest, keep ? 1 : 0);
+    if (C_ == 1) {
+      tgv_ ? launch_dual<1, true, false>(pixels, steps, k) : launch_dual<1, false, false>(pixels, steps, k);
     } else if (coupled_) {
-      dual_kernel<T, false, true><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_, a[0], a[1], a[2], radius, radius0, rho,
-                                                         rest, keep ? 1 : 0);
+      tgv_ ? launch_dual<3, true, true>(pixels, steps, k) : launch_dual<3, false, true>(pixels, steps, k);
     } else {
-      dual_kernel<T, false, false><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_, a[0], a[1], a[2], radius, radius0,
-                                                          rho, rest, keep ? 1 : 0);
+      tgv_ ? launch_dual<3, true, false>(pixels, steps, k) : launch_dual<3, false, false>(pixels, steps, k);
     }
     HIP_CHECK(hipGetLastError());
     if (timing_) HIP_CHECK(hipEventRecord(dual_events_.back().end, s));
+    std::swap(fields_.p, fields_.p_next);
+    std::swap(fields_.w, fields_.w_next);
+    std::swap(fields_.r, fields_.r_next);
   }
 
   // The durations of the launches since the last collection, then their events freed.
@@ -1224,12 +1121,11 @@ class Solver {
     double* results = record_out_.as<double>();
     const dim3 pixels((W_ + 63) / 64, (H_ + 3) / 4);
     const double g[3] = {plan_.gamma[0], C_ > 1 ? plan_.gamma[1] : 0.0, C_ > 2 ? plan_.gamma[2] : 0.0};
+    const int coupled = settings_.coupled ? 1 : 0;
     if (tgv_) {
-      norms_kernel<T, true><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_, g[0], g[1], g[2], settings_.coupled ? 1 : 0,
-                                                   norm_partials);
+      norms_kernel<T, S, true><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_, g[0], g[1], g[2], coupled, norm_partials);
     } else {
-      norms_kernel<T, false><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_, g[0], g[1], g[2], settings_.coupled ? 1 : 0,
-                                                    norm_partials);
+      norms_kernel<T, S, false><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_, g[0], g[1], g[2], coupled, norm_partials);
     }
     HIP_CHECK(hipGetLastError());
     // results[0..3]: first, second, most, residual.
@@ -1249,9 +1145,9 @@ class Solver {
     const auto conjugates = [&](int from_r, double* into, double* result) {
       size_t at = 0;
       for (int c = 0; c < C_; ++c) {
-        const int groups = tile_groups(c);
-        conjugate_kernel<T><<<groups, 256, 0, s>>>(component(c), fields_, c, H_, W_, plan_.gamma[c], from_r,
-                                                   results + 2, settings_.alpha1, plan_.free_radius, into + at);
+        const int groups = (tiles(c) + 3) / 4;
+        conjugate_kernel<T, S><<<groups, 256, 0, s>>>(component(c), fields_, c, H_, W_, plan_.gamma[c], from_r,
+                                                      results + 2, settings_.alpha1, plan_.free_radius, into + at);
         HIP_CHECK(hipGetLastError());
         at += static_cast<size_t>(groups);
       }
@@ -1298,8 +1194,9 @@ class Solver {
     out.coefficients.assign(C_, {});
     for (int c = 0; c < C_; ++c) {
       coefficients.emplace_back(sizeof(double) * std::max<size_t>(1, 64 * frame_.components[c].blocks()));
-      final_kernel<T><<<tile_groups(c), 256, 0, s>>>(component(c), fields_.tx + c * N_, W_, margin,
-                                                     canvas.as<double>() + c * N_, coefficients.back().as<double>());
+      final_kernel<T><<<(tiles(c) + 3) / 4, 256, 0, s>>>(component(c), fields_.tx + c * N_, W_, margin,
+                                                         canvas.as<double>() + c * N_,
+                                                         coefficients.back().as<double>());
       HIP_CHECK(hipGetLastError());
     }
     out.canvas.resize(static_cast<size_t>(C_) * N_);
@@ -1323,7 +1220,7 @@ class Solver {
   StreamHolder stream_;
   DeviceBuffer state_, components_, partials_, record_out_;
   std::vector<DeviceBuffer> levels_, flats_;
-  Fields<T> fields_{};
+  Fields<T, S> fields_{};
   size_t norm_partials_count_ = 0, value_partials_count_ = 0, conjugate_partials_count_ = 0;
   std::deque<EventPair> primal_events_, dual_events_, launches_;
 };
@@ -1340,10 +1237,13 @@ bool solve_on_gpu(const FrameModel& frame, const Settings& settings, const Plan&
       return false;
     }
     if (settings.precision == Precision::dual) {
-      Solver<double> solver(frame, settings, plan, timing);
+      Solver<double, double> solver(frame, settings, plan, timing);
+      solver.run(start, observer, out);
+    } else if (settings.precision == Precision::mixed) {
+      Solver<float, bf16> solver(frame, settings, plan, timing);
       solver.run(start, observer, out);
     } else {
-      Solver<float> solver(frame, settings, plan, timing);
+      Solver<float, float> solver(frame, settings, plan, timing);
       solver.run(start, observer, out);
     }
     return true;

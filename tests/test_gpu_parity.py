@@ -102,3 +102,30 @@ def test_tolerance_stop_matches():
     assert decoded.result.stop == native.Stop.CONVERGED
     # binary32 may reach the tolerance a record apart from binary64.
     assert abs(decoded.result.iterations - expected.iterations) <= 10
+
+
+def test_mixed_precision_bf16_duals_error_against_binary64():
+    # p and r stored in bfloat16 (8-bit significands), binary32 arithmetic: the error
+    # against the binary64 oracle is reported; the requantization stays exact.
+    picture = synthetic.grey_picture(256, 256, 11)
+    image = synthetic.encode_grey(picture, 40)
+    options = ["--method", "tgv", "--iterations", "200", "--tolerance", "0"]
+    expected = helpers.oracle_solve(image, options)
+    decoded = native.solve(image, options + ["--precision", "mixed"])
+    worst = float(np.max(np.abs(decoded.planes - expected.canvas[:, :256, :256])))
+    ours, theirs = helpers.psnr(decoded.planes[0], picture), helpers.psnr(expected.canvas[0], picture)
+    print(f"  mixed: max |diff| {worst:.3g}, PSNR {ours:.4f} vs {theirs:.4f}")
+    assert worst < 0.5
+    assert abs(ours - theirs) < 0.01
+    levels = helpers.requantized(decoded.canvas[0], image, 0, (1, 1), 32, 32)
+    assert np.array_equal(levels, helpers.kept_levels(image, 0, 32, 32))
+
+
+def test_runs_repeat_to_the_last_bit():
+    picture = synthetic.colour_picture(64, 80, 12)
+    image = synthetic.encode_colour420(picture, 50)
+    options = ["--method", "tgv", "--iterations", "40", "--tolerance", "0"]
+    first, second = native.solve(image, options), native.solve(image, options)
+    assert np.array_equal(first.canvas, second.canvas)
+    assert np.array_equal(first.result.history.primal, second.result.history.primal)
+    assert np.array_equal(first.result.history.dual, second.result.history.dual)
