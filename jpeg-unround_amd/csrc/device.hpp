@@ -293,5 +293,58 @@ __device__ __forceinline__ T ball_scale(T square, T radius) {
   return norm > radius ? radius / norm : T(1);
 }
 
+// ---------------------------------------------------------------------------------------
+// Planes of one kind of field: `count` planes of N samples each from `base`, a sample
+// addressed by its index in its plane.
+//
+// BufferPlanes reads and writes through one buffer resource: the plane's byte offset is a
+// scalar and the sample's a 32-bit lane offset, so that a row of every plane of the kind
+// shares one address register (64-bit addresses per plane cost two registers each). The
+// planes must span less than 4 GiB. WidePlanes takes 64-bit addresses for larger spans.
+constexpr int kRawBufferWord3 = 0x00020000;  // a raw (untyped) buffer resource on gfx9
+
+template <typename X>
+struct BufferPlanes {
+  __amdgpu_buffer_rsrc_t resource;
+  uint32_t plane_bytes;
+  __device__ __forceinline__ BufferPlanes(X* base, int count, size_t N)
+      : resource(__builtin_amdgcn_make_buffer_rsrc(
+            const_cast<void*>(static_cast<const void*>(base)), 0,
+            static_cast<int>(static_cast<uint32_t>(static_cast<size_t>(count) * N * sizeof(X))), kRawBufferWord3)),
+        plane_bytes(static_cast<uint32_t>(N * sizeof(X))) {}
+  __device__ __forceinline__ X load(int plane, uint32_t index) const {
+    const int offset = static_cast<int>(index * sizeof(X));
+    const int scalar = static_cast<int>(static_cast<uint32_t>(plane) * plane_bytes);
+    if constexpr (sizeof(X) == 2) {
+      return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b16(resource, offset, scalar, 0));
+    } else if constexpr (sizeof(X) == 4) {
+      return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b32(resource, offset, scalar, 0));
+    } else {
+      return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b64(resource, offset, scalar, 0));
+    }
+  }
+  __device__ __forceinline__ void store(int plane, uint32_t index, X value) const {
+    const int offset = static_cast<int>(index * sizeof(X));
+    const int scalar = static_cast<int>(static_cast<uint32_t>(plane) * plane_bytes);
+    if constexpr (sizeof(X) == 2) {
+      __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, value), resource, offset, scalar, 0);
+    } else if constexpr (sizeof(X) == 4) {
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, value), resource, offset, scalar, 0);
+    } else {
+      using Pair = unsigned int __attribute__((ext_vector_type(2)));
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(Pair, value), resource, offset, scalar, 0);
+    }
+  }
+};
+
+template <typename X>
+struct WidePlanes {
+  X* base;
+  size_t N;
+  __device__ __forceinline__ WidePlanes(X* base_, int, size_t N_) : base(base_), N(N_) {}
+  __device__ __forceinline__ X load(int plane, uint32_t index) const { return base[plane * N + index]; }
+  __device__ __forceinline__ void store(int plane, uint32_t index, X value) const { base[plane * N + index] = value; }
+};
+
 }  // namespace device
 }  // namespace unround

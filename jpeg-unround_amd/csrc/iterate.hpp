@@ -1,0 +1,451 @@
+// The kernels of one iteration (included by solver.hip, inside its anonymous namespace,
+// after Fields, Steps and kMaxCell).
+//
+// Both kernels give each lane of a wavefront one column of a 64-column strip and walk
+// down rows, so that every load of a row is one coalesced 256-byte stretch (binary32).
+// A value of the neighbouring column comes from the neighbouring lane (a shuffle); of the
+// row above, from the lane's own registers; only the lanes at a strip's two edges read a
+// column beyond it from memory. Each field is then read about once per iteration.
+
+// The value of the lane on the left (lane 0: its own) and on the right (lane 63: its own).
+template <typename T>
+__device__ __forceinline__ T from_left(T value) {
+  return __shfl_up(value, 1, 64);
+}
+template <typename T>
+__device__ __forceinline__ T from_right(T value) {
+  return __shfl_down(value, 1, 64);
+}
+
+// delta_x f at column j from f there and at j - 1 (docs/math.md:337).
+template <typename T>
+__device__ __forceinline__ T backward_across_of(T here, T left, int j, int W) {
+  if (W == 1) return T(0);
+  if (j == 0) return here;
+  if (j == W - 1) return -left;
+  return here - left;
+}
+
+// delta_y f at row i from f there and at i - 1.
+template <typename T>
+__device__ __forceinline__ T backward_down_of(T here, T up, int i, int H) {
+  const bool first = i == 0, last = i == H - 1;
+  if (!first && !last) return here - up;
+  if (!last) return here;
+  if (!first) return -up;
+  return T(0);
+}
+
+// ---------------------------------------------------------------------------------------
+// The proximal step of one component whose cells are DV x DH samples (1 or 2 each)
+// (primal_dual.hpp:536-546; frames.cpp:490-548). A wavefront takes 8 rows of blocks of a
+// strip 64 component columns wide, a lane per column; the cell's samples of a lane stay
+// in its registers from the descent to the write-back.
+template <typename T, typename S, int DV, int DH>
+__global__ __launch_bounds__(256) void prox_kernel(const Component* __restrict__ component, Fields<T, S> F, int c,
+                                                   int H, int W, T descent, T step, T rho, T rest, int keep) {
+  __shared__ T forward_room[4][8][65];
+  __shared__ T inverse_room[4][8][65];
+  const Component& K = *component;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int strips = (K.tiles_x + 7) >> 3;
+  const int strip = blockIdx.x % strips;
+  const int ty = (blockIdx.x / strips) * 4 + wave;  // the wave's row of tiles (blocks)
+  const int tx = strip * 8 + (lane >> 3);           // the tile of the lane's column
+  const int n = lane & 7;
+  const bool live = ty < K.tiles_y && tx < K.tiles_x;
+  const size_t N = static_cast<size_t>(H) * W;
+  T* x = F.x + c * N;
+  const S* px = F.p + (2 * c) * N;
+  const S* py = F.p + (2 * c + 1) * N;
+  const int first_column = (8 * tx + n) * DH;  // the lane's first canvas column
+  // p_y of the canvas row above, per column of the cell.
+  T up[DH];
+#pragma unroll
+  for (int b = 0; b < DH; ++b) {
+    const int i = 8 * ty * DV - 1;
+    up[b] = (live && i >= 0) ? device::widen(py[static_cast<size_t>(i) * W + first_column + b]) : T(0);
+  }
+  T vs[8][DV * DH], xs[8][DV * DH], column[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    T partial[DH];
+#pragma unroll
+    for (int a = 0; a < DV; ++a) {
+      const int i = (8 * ty + m) * DV + a;
+      T here[DH], across[DH], down[DH];
+#pragma unroll
+      for (int b = 0; b < DH; ++b) {
+        const size_t at = static_cast<size_t>(i) * W + first_column + b;
+        here[b] = live ? x[at] : T(0);
+        across[b] = live ? device::widen(px[at]) : T(0);
+        down[b] = live ? device::widen(py[at]) : T(0);
+      }
+      // p_x of the column on the left: the lane's own previous column, or the previous
+      // lane's last one, or beyond the strip, memory.
+      T left = from_left(across[DH - 1]);
+      if (lane == 0 && live && first_column > 0) {
+        left = device::widen(px[static_cast<size_t>(i) * W + first_column - 1]);
+      }
+#pragma unroll
+      for (int b = 0; b < DH; ++b) {
+        const int j = first_column + b;
+        const T before = b == 0 ? left : across[b - 1];
+        T div = backward_down_of(down[b], up[b], i, H);
+        div += backward_across_of(across[b], before, j, W);
+        const T v = here[b] + descent * div;
+        up[b] = down[b];
+        vs[m][a * DH + b] = v;
+        xs[m][a * DH + b] = here[b];
+        // The cell's sum: the rows of a column first, then the columns.
+        partial[b] = a == 0 ? v : partial[b] + v;
+      }
+    }
+    T sum = partial[0];
+#pragma unroll
+    for (int b = 1; b < DH; ++b) sum += partial[b];
+    column[m] = DV * DH > 1 ? sum / static_cast<T>(DV * DH) : sum;
+  }
+  T y[8];
+  device::forward8(column, y);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) forward_room[wave][k][lane] = y[k];
+  __syncthreads();
+  // A lane now takes row k of block b of the strip.
+  const int b = lane >> 3, k = lane & 7;
+  const int bx = strip * 8 + b;
+  const bool inside = ty < K.rows && bx < K.columns;
+  T row[8], coefficient[8], zeta[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) row[u] = forward_room[wave][k][8 * b + u];
+  device::forward8(row, coefficient);
+  if (inside) {
+    const size_t block = static_cast<size_t>(ty) * K.columns + bx;
+    const int16_t* levels = K.levels + 64 * block + 8 * k;
+    const bool flat = K.flat[block] != 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) zeta[u] = device::prox_of(K, levels[u], flat, 8 * k + u, coefficient[u], step);
+    if (keep) {
+      T* kept = F.zeta[c] + 64 * block + 8 * k;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) kept[u] = zeta[u];
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) zeta[u] = T(0);
+  }
+  T back_row[8];
+  device::inverse8(zeta, back_row);
+#pragma unroll
+  for (int v = 0; v < 8; ++v) inverse_room[wave][k][8 * b + v] = back_row[v];
+  __syncthreads();
+  T back_column[8], back[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) back_column[m] = inverse_room[wave][m][lane];
+  device::inverse8(back_column, back);
+  if (!live) return;
+  const bool own_inside = ty < K.rows && tx < K.columns;
+  T* xbar = F.xbar + c * N;
+  T* tilde_x = F.tx + c * N;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    // x~: the inverse DCT of zeta (one sample to a cell), or v moved by the change of its
+    // cell (the inverse DCT less the mean); v beyond the blocks.
+    const T change = back[m] - column[m];
+#pragma unroll
+    for (int a = 0; a < DV; ++a) {
+#pragma unroll
+      for (int bb = 0; bb < DH; ++bb) {
+        const size_t at = static_cast<size_t>((8 * ty + m) * DV + a) * W + first_column + bb;
+        const T v = vs[m][a * DH + bb], here = xs[m][a * DH + bb];
+        const T tilde = own_inside ? (DV * DH > 1 ? v + change : back[m]) : v;
+        xbar[at] = T(2) * tilde - here;
+        x[at] = rho * tilde + rest * here;
+        if (keep) tilde_x[at] = tilde;
+      }
+    }
+  }
+}
+
+// The same for any cell up to 4 x 4, its samples read again for the write-back.
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void prox_any_cell_kernel(const Component* __restrict__ component, Fields<T, S> F,
+                                                            int c, int H, int W, T descent, T step, T rho, T rest,
+                                                            int keep) {
+  __shared__ T forward_room[4][8][65];
+  __shared__ T inverse_room[4][8][65];
+  const Component& K = *component;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int strips = (K.tiles_x + 7) >> 3;
+  const int strip = blockIdx.x % strips;
+  const int ty = (blockIdx.x / strips) * 4 + wave;
+  const int tx = strip * 8 + (lane >> 3);
+  const int n = lane & 7;
+  const bool live = ty < K.tiles_y && tx < K.tiles_x;
+  const int dv = K.down, dh = K.across;
+  const size_t N = static_cast<size_t>(H) * W;
+  T* x = F.x + c * N;
+  const S* px = F.p + (2 * c) * N;
+  const S* py = F.p + (2 * c + 1) * N;
+  T column[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    T value = T(0);
+    if (live) {
+      T sum = T(0);
+      for (int bb = 0; bb < dh; ++bb) {
+        T partial = T(0);
+        for (int a = 0; a < dv; ++a) {
+          const int i = (8 * ty + m) * dv + a, j = (8 * tx + n) * dh + bb;
+          const T v = x[static_cast<size_t>(i) * W + j] + descent * device::divergence_at<T>(px, py, i, j, H, W);
+          partial = a == 0 ? v : partial + v;
+        }
+        sum = bb == 0 ? partial : sum + partial;
+      }
+      value = sum / static_cast<T>(dv * dh);
+    }
+    column[m] = value;
+  }
+  T y[8];
+  device::forward8(column, y);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) forward_room[wave][k][lane] = y[k];
+  __syncthreads();
+  const int b = lane >> 3, k = lane & 7;
+  const int bx = strip * 8 + b;
+  const bool inside = ty < K.rows && bx < K.columns;
+  T row[8], coefficient[8], zeta[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) row[u] = forward_room[wave][k][8 * b + u];
+  device::forward8(row, coefficient);
+  if (inside) {
+    const size_t block = static_cast<size_t>(ty) * K.columns + bx;
+    const int16_t* levels = K.levels + 64 * block + 8 * k;
+    const bool flat = K.flat[block] != 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) zeta[u] = device::prox_of(K, levels[u], flat, 8 * k + u, coefficient[u], step);
+    if (keep) {
+      T* kept = F.zeta[c] + 64 * block + 8 * k;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) kept[u] = zeta[u];
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) zeta[u] = T(0);
+  }
+  T back_row[8];
+  device::inverse8(zeta, back_row);
+#pragma unroll
+  for (int v = 0; v < 8; ++v) inverse_room[wave][k][8 * b + v] = back_row[v];
+  __syncthreads();
+  T back_column[8], back[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) back_column[m] = inverse_room[wave][m][lane];
+  device::inverse8(back_column, back);
+  if (!live) return;
+  const bool own_inside = ty < K.rows && tx < K.columns;
+  T* xbar = F.xbar + c * N;
+  T* tilde_x = F.tx + c * N;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const T change = back[m] - column[m];
+    for (int bb = 0; bb < dh; ++bb) {
+      for (int a = 0; a < dv; ++a) {
+        const int i = (8 * ty + m) * dv + a, j = (8 * tx + n) * dh + bb;
+        const size_t at = static_cast<size_t>(i) * W + j;
+        const T here = x[at];
+        const T v = here + descent * device::divergence_at<T>(px, py, i, j, H, W);
+        const T tilde = own_inside ? (dv * dh > 1 ? v + change : back[m]) : v;
+        xbar[at] = T(2) * tilde - here;
+        x[at] = rho * tilde + rest * here;
+        if (keep) tilde_x[at] = tilde;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// The dual step of every channel, with TGV's step of w (primal_dual.hpp:547-557, 572-735),
+// from the buffers of the iteration's start into the other ones.
+//
+// A wavefront walks down `rows` rows of a strip; its lanes take 64 consecutive columns
+// from the one before the strip to the one after it, and only lanes 1..62 write: lane 0
+// makes w- of the column on the strip's left and lane 63 reads x- and r of the column on
+// its right, so that every neighbour comes from a lane shuffle and every lane runs the
+// same loads. Per lane and channel the wavefront carries x- and r of the row and the next
+// one and w- of the row above, and it loads the row after next while it works on the row.
+constexpr int kStripColumns = 62;
+
+template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes>
+__global__ __launch_bounds__(256) void dual_kernel(Fields<T, S> F, int H, int W, int rows, Steps<T> steps, int keep) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int strip = blockIdx.x * 4 + wave;
+  if (strip * kStripColumns >= W) return;  // the whole wavefront beyond the canvas
+  const int j = strip * kStripColumns + lane - 1;
+  const bool writes = lane >= 1 && lane <= kStripColumns && j < W;
+  const bool right = j + 1 < W;
+  const uint32_t column = static_cast<uint32_t>(min(max(j, 0), W - 1));
+  const int first = blockIdx.y * rows;
+  const int end = min(first + rows, H);
+  const size_t N = static_cast<size_t>(H) * W;
+  const Planes<T> xbar(F.xbar, C, N);
+  const Planes<S> p(F.p, 2 * C, N), p_next(F.p_next, 2 * C, N), tp(F.tp, 2 * C, N);
+  const Planes<T> w(F.w, 2 * C, N), w_next(F.w_next, 2 * C, N), tw(F.tw, 2 * C, N);
+  const Planes<S> r(F.r, 3 * C, N), r_next(F.r_next, 3 * C, N), tr(F.tr, 3 * C, N);
+  // The index of the lane's sample in row i, i clamped into the canvas (a clamped load
+  // only feeds differences that the canvas' edges take as zero).
+  const auto index = [&](int i) { return static_cast<uint32_t>(min(i, H - 1)) * static_cast<uint32_t>(W) + column; };
+  // w~ and w- at a sample from w, p and r there, r of the row below and of the column on
+  // the right (through the lane on the right): w~ = w + tau (p + div_2 r).
+  const auto w_step = [&](const T old[2], const T q[2], const T here[3], const T next[3], bool below, T descent,
+                          T tilde[2], T bar[2]) {
+    const T right_xx = from_right(here[0]), right_xy = from_right(here[2]);
+    const T dx = (right ? right_xx - here[0] : T(0)) + (below ? next[2] - here[2] : T(0));
+    const T dy = (right ? right_xy - here[2] : T(0)) + (below ? next[1] - here[1] : T(0));
+    tilde[0] = old[0] + descent * (q[0] + dx);
+    tilde[1] = old[1] + descent * (q[1] + dy);
+    bar[0] = T(2) * tilde[0] - old[0];
+    bar[1] = T(2) * tilde[1] - old[1];
+  };
+
+  // The rows of the start: x- and r of the first two rows, p and w of the first, and w-
+  // of the row above.
+  T bar[C], bar_next[C], r_here[C][3], r_below[C][3], p_here[C][2], w_here[C][2], w_up[C][2];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    bar[c] = xbar.load(c, index(first));
+    bar_next[c] = xbar.load(c, index(first + 1));
+#pragma unroll
+    for (int e = 0; e < 2; ++e) p_here[c][e] = device::widen(p.load(2 * c + e, index(first)));
+    if constexpr (TGV) {
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        r_here[c][e] = device::widen(r.load(3 * c + e, index(first)));
+        r_below[c][e] = device::widen(r.load(3 * c + e, index(first + 1)));
+      }
+#pragma unroll
+      for (int e = 0; e < 2; ++e) w_here[c][e] = w.load(2 * c + e, index(first));
+      w_up[c][0] = w_up[c][1] = T(0);
+      if (first > 0) {
+        T r_up[3], p_up[2], old_up[2], tilde_up[2];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) r_up[e] = device::widen(r.load(3 * c + e, index(first - 1)));
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          p_up[e] = device::widen(p.load(2 * c + e, index(first - 1)));
+          old_up[e] = w.load(2 * c + e, index(first - 1));
+        }
+        w_step(old_up, p_up, r_up, r_here[c], true, steps.descent[c], tilde_up, w_up[c]);
+      }
+    }
+  }
+
+#pragma unroll 1
+  for (int i = first; i < end; ++i) {
+    const bool below = i + 1 < H;
+    // The loads of the next iteration: x- and r two rows down, p and w one row down.
+    T bar_ahead[C], r_ahead[C][3], p_ahead[C][2], w_ahead[C][2];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      bar_ahead[c] = xbar.load(c, index(i + 2));
+#pragma unroll
+      for (int e = 0; e < 2; ++e) p_ahead[c][e] = device::widen(p.load(2 * c + e, index(i + 1)));
+      if constexpr (TGV) {
+#pragma unroll
+        for (int e = 0; e < 3; ++e) r_ahead[c][e] = device::widen(r.load(3 * c + e, index(i + 2)));
+#pragma unroll
+        for (int e = 0; e < 2; ++e) w_ahead[c][e] = w.load(2 * c + e, index(i + 1));
+      }
+    }
+
+    T step_p[C][2], step_r[C][3], tilde_w[C][2], bar_w[C][2];
+    T square = T(0), tensor_square = T(0);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const T a = steps.ascent[c];
+      const T bar_right = from_right(bar[c]);
+      const T g1 = right ? bar_right - bar[c] : T(0);
+      const T g2 = below ? bar_next[c] - bar[c] : T(0);
+      if constexpr (TGV) {
+        w_step(w_here[c], p_here[c], r_here[c], r_below[c], below, steps.descent[c], tilde_w[c], bar_w[c]);
+        // E w- (docs/math.md:355): backward differences, w- of the column on the left
+        // from the lane on the left.
+        const T left_x = from_left(bar_w[c][0]), left_y = from_left(bar_w[c][1]);
+        const T across_x = backward_across_of(bar_w[c][0], left_x, j, W);
+        const T down_y = backward_down_of(bar_w[c][1], w_up[c][1], i, H);
+        const T exy =
+            T(0.5) * (backward_down_of(bar_w[c][0], w_up[c][0], i, H) + backward_across_of(bar_w[c][1], left_y, j, W));
+        step_p[c][0] = p_here[c][0] + a * (g1 - bar_w[c][0]);
+        step_p[c][1] = p_here[c][1] + a * (g2 - bar_w[c][1]);
+        step_r[c][0] = r_here[c][0] + a * across_x;
+        step_r[c][1] = r_here[c][1] + a * down_y;
+        step_r[c][2] = r_here[c][2] + a * exy;
+      } else {
+        step_p[c][0] = p_here[c][0] + a * g1;
+        step_p[c][1] = p_here[c][1] + a * g2;
+      }
+      if constexpr (COUPLED) {
+        square += step_p[c][0] * step_p[c][0] + step_p[c][1] * step_p[c][1];
+        if constexpr (TGV) {
+          tensor_square +=
+              step_r[c][0] * step_r[c][0] + step_r[c][1] * step_r[c][1] + T(2) * (step_r[c][2] * step_r[c][2]);
+        }
+      }
+    }
+    const T rho = steps.rho, rest = steps.rest;
+    const T shared_scale = COUPLED ? device::ball_scale(square, steps.radius) : T(1);
+    const T shared_tensor_scale = (COUPLED && TGV) ? device::ball_scale(tensor_square, steps.radius0) : T(1);
+    const uint32_t at = index(i);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      if (writes) {
+        T scale = shared_scale;
+        if constexpr (!COUPLED) {
+          scale = device::ball_scale(step_p[c][0] * step_p[c][0] + step_p[c][1] * step_p[c][1], steps.radius);
+        }
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const T projected = step_p[c][e] * scale;
+          if (keep) tp.store(2 * c + e, at, device::narrow<S>(projected));
+          p_next.store(2 * c + e, at, device::narrow<S>(rho * projected + rest * p_here[c][e]));
+        }
+        if constexpr (TGV) {
+#pragma unroll
+          for (int e = 0; e < 2; ++e) {
+            if (keep) tw.store(2 * c + e, at, tilde_w[c][e]);
+            w_next.store(2 * c + e, at, rho * tilde_w[c][e] + rest * w_here[c][e]);
+          }
+          T tensor_scale = shared_tensor_scale;
+          if constexpr (!COUPLED) {
+            tensor_scale = device::ball_scale(step_r[c][0] * step_r[c][0] + step_r[c][1] * step_r[c][1] +
+                                                  T(2) * (step_r[c][2] * step_r[c][2]),
+                                              steps.radius0);
+          }
+#pragma unroll
+          for (int e = 0; e < 3; ++e) {
+            const T projected = step_r[c][e] * tensor_scale;
+            if (keep) tr.store(3 * c + e, at, device::narrow<S>(projected));
+            r_next.store(3 * c + e, at, device::narrow<S>(rho * projected + rest * r_here[c][e]));
+          }
+        }
+      }
+      // The next row's values become the current ones.
+      bar[c] = bar_next[c];
+      bar_next[c] = bar_ahead[c];
+#pragma unroll
+      for (int e = 0; e < 2; ++e) p_here[c][e] = p_ahead[c][e];
+      if constexpr (TGV) {
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          r_here[c][e] = r_below[c][e];
+          r_below[c][e] = r_ahead[c][e];
+        }
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          w_up[c][e] = bar_w[c][e];
+          w_here[c][e] = w_ahead[c][e];
+        }
+      }
+    }
+  }
+}

@@ -29,6 +29,7 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -97,249 +98,7 @@ struct Steps {
 
 constexpr int kMaxCell = 4;
 
-// ---------------------------------------------------------------------------------------
-// The proximal step of one component (primal_dual.hpp:536-546; frames.cpp:490-548).
-template <typename T, typename S, bool CELLS>
-__global__ __launch_bounds__(256) void prox_kernel(const Component* __restrict__ component, Fields<T, S> F, int c,
-                                                   int H, int W, T descent, T step, T rho, T rest, int keep) {
-  __shared__ T forward_room[4][8][65];
-  __shared__ T inverse_room[4][8][65];
-  const Component& K = *component;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int strips = (K.tiles_x + 7) >> 3;
-  const int strip = blockIdx.x % strips;
-  const int ty = (blockIdx.x / strips) * 4 + wave;  // the wave's row of tiles (blocks)
-  const int tx = strip * 8 + (lane >> 3);           // the tile of the lane's column
-  const int n = lane & 7;
-  const bool live = ty < K.tiles_y && tx < K.tiles_x;
-  const int dv = CELLS ? K.down : 1, dh = CELLS ? K.across : 1;
-  const size_t N = static_cast<size_t>(H) * W;
-  T* x = F.x + c * N;
-  const S* px = F.p + (2 * c) * N;
-  const S* py = F.p + (2 * c + 1) * N;
-  // v down the lane's column of the block: of each sample, or each cell's mean (its rows
-  // of a column added first, then the columns; cpp/src/frames.cpp:323-339).
-  T column[8], old[8];
-#pragma unroll
-  for (int m = 0; m < 8; ++m) {
-    T value = T(0), here = T(0);
-    if (live) {
-      if constexpr (!CELLS) {
-        const int i = 8 * ty + m, j = 8 * tx + n;
-        here = x[static_cast<size_t>(i) * W + j];
-        value = here + descent * device::divergence_at<T>(px, py, i, j, H, W);
-      } else {
-        T sum = T(0);
-        for (int b = 0; b < dh; ++b) {
-          T partial = T(0);
-          for (int a = 0; a < dv; ++a) {
-            const int i = (8 * ty + m) * dv + a, j = (8 * tx + n) * dh + b;
-            const T v = x[static_cast<size_t>(i) * W + j] + descent * device::divergence_at<T>(px, py, i, j, H, W);
-            partial = a == 0 ? v : partial + v;
-          }
-          sum = b == 0 ? partial : sum + partial;
-        }
-        value = sum / static_cast<T>(dv * dh);
-      }
-    }
-    column[m] = value;
-    old[m] = here;
-  }
-  T y[8];
-  device::forward8(column, y);
-#pragma unroll
-  for (int k = 0; k < 8; ++k) forward_room[wave][k][lane] = y[k];
-  __syncthreads();
-  // A lane now takes row k of block b of the strip.
-  const int b = lane >> 3, k = lane & 7;
-  const int bx = strip * 8 + b;
-  const bool inside = ty < K.rows && bx < K.columns;
-  T row[8], coefficient[8], zeta[8];
-#pragma unroll
-  for (int u = 0; u < 8; ++u) row[u] = forward_room[wave][k][8 * b + u];
-  device::forward8(row, coefficient);
-  if (inside) {
-    const size_t block = static_cast<size_t>(ty) * K.columns + bx;
-    const int16_t* levels = K.levels + 64 * block + 8 * k;
-    const bool flat = K.flat[block] != 0;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) zeta[u] = device::prox_of(K, levels[u], flat, 8 * k + u, coefficient[u], step);
-    if (keep) {
-      T* kept = F.zeta[c] + 64 * block + 8 * k;
-#pragma unroll
-      for (int u = 0; u < 8; ++u) kept[u] = zeta[u];
-    }
-  } else {
-#pragma unroll
-    for (int u = 0; u < 8; ++u) zeta[u] = T(0);
-  }
-  T back_row[8];
-  device::inverse8(zeta, back_row);
-#pragma unroll
-  for (int v = 0; v < 8; ++v) inverse_room[wave][k][8 * b + v] = back_row[v];
-  __syncthreads();
-  T back_column[8], back[8];
-#pragma unroll
-  for (int m = 0; m < 8; ++m) back_column[m] = inverse_room[wave][m][lane];
-  device::inverse8(back_column, back);
-  if (!live) return;
-  const bool own_inside = ty < K.rows && tx < K.columns;
-  T* xbar = F.xbar + c * N;
-  T* tilde_x = F.tx + c * N;
-#pragma unroll
-  for (int m = 0; m < 8; ++m) {
-    if constexpr (!CELLS) {
-      const size_t at = static_cast<size_t>(8 * ty + m) * W + 8 * tx + n;
-      const T tilde = own_inside ? back[m] : column[m];
-      xbar[at] = T(2) * tilde - old[m];
-      x[at] = rho * tilde + rest * old[m];
-      if (keep) tilde_x[at] = tilde;
-    } else {
-      // x~ = v + the change of the cell (the inverse DCT of zeta less the mean), v beyond
-      // the blocks.
-      const T change = back[m] - column[m];
-      for (int bb = 0; bb < dh; ++bb) {
-        for (int a = 0; a < dv; ++a) {
-          const int i = (8 * ty + m) * dv + a, j = (8 * tx + n) * dh + bb;
-          const size_t at = static_cast<size_t>(i) * W + j;
-          const T here = x[at];
-          const T v = here + descent * device::divergence_at<T>(px, py, i, j, H, W);
-          const T tilde = own_inside ? v + change : v;
-          xbar[at] = T(2) * tilde - here;
-          x[at] = rho * tilde + rest * here;
-          if (keep) tilde_x[at] = tilde;
-        }
-      }
-    }
-  }
-}
-
-// w~ = w + tau gamma (p + div_2 r) at (i, j) of a channel's fields, and w there.
-template <typename T, typename S>
-__device__ __forceinline__ void w_step_at(const T* wx, const T* wy, const S* px, const S* py, const S* rxx,
-                                          const S* ryy, const S* rxy, int i, int j, int H, int W, T descent, T& twx,
-                                          T& twy, T& ox, T& oy) {
-  T dx, dy;
-  device::tensor_divergence_at<T>(rxx, ryy, rxy, i, j, H, W, dx, dy);
-  ox = device::at<T>(wx, i, j, W);
-  oy = device::at<T>(wy, i, j, W);
-  twx = ox + descent * (device::at<T>(px, i, j, W) + dx);
-  twy = oy + descent * (device::at<T>(py, i, j, W) + dy);
-}
-
-// The dual step of every channel at one pixel, with TGV's step of w (primal_dual.hpp:547-557,
-// 572-735): from the buffers of the iteration's start into the other ones.
-template <typename T, typename S, int C, bool TGV, bool COUPLED>
-__global__ __launch_bounds__(256) void dual_kernel(Fields<T, S> F, int H, int W, Steps<T> steps, int keep) {
-  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (i >= H || j >= W) return;
-  const size_t N = static_cast<size_t>(H) * W;
-  const size_t at = static_cast<size_t>(i) * W + j;
-  T step_p[C][2], step_r[C][3], tilde_w[C][2], old_w[C][2], old_p[C][2], old_r[C][3];
-  T square = T(0), tensor_square = T(0);
-#pragma unroll
-  for (int c = 0; c < C; ++c) {
-    const T a = steps.ascent[c];
-    const T* xbar = F.xbar + c * N;
-    const T g1 = device::forward_across<T>(xbar, i, j, W);
-    const T g2 = device::forward_down<T>(xbar, i, j, H, W);
-    const S* px = F.p + (2 * c) * N;
-    const S* py = F.p + (2 * c + 1) * N;
-    old_p[c][0] = device::widen(px[at]);
-    old_p[c][1] = device::widen(py[at]);
-    if constexpr (TGV) {
-      const T* wx = F.w + (2 * c) * N;
-      const T* wy = F.w + (2 * c + 1) * N;
-      const S* rxx = F.r + (3 * c) * N;
-      const S* ryy = F.r + (3 * c + 1) * N;
-      const S* rxy = F.r + (3 * c + 2) * N;
-      const T descent = steps.descent[c];
-      // w- at the pixel, at its left and above: 2 w~ - w.
-      T twx, twy, ox, oy;
-      w_step_at<T>(wx, wy, px, py, rxx, ryy, rxy, i, j, H, W, descent, twx, twy, ox, oy);
-      tilde_w[c][0] = twx;
-      tilde_w[c][1] = twy;
-      old_w[c][0] = ox;
-      old_w[c][1] = oy;
-      const T bx = T(2) * twx - ox, by = T(2) * twy - oy;
-      T left_x = T(0), left_y = T(0), up_x = T(0), up_y = T(0);
-      if (j > 0) {
-        w_step_at<T>(wx, wy, px, py, rxx, ryy, rxy, i, j - 1, H, W, descent, twx, twy, ox, oy);
-        left_x = T(2) * twx - ox;
-        left_y = T(2) * twy - oy;
-      }
-      if (i > 0) {
-        w_step_at<T>(wx, wy, px, py, rxx, ryy, rxy, i - 1, j, H, W, descent, twx, twy, ox, oy);
-        up_x = T(2) * twx - ox;
-        up_y = T(2) * twy - oy;
-      }
-      // E w- (docs/math.md:355): backward differences, the last column's and row's own
-      // term left out.
-      const bool first_row = i == 0, last_row = i == H - 1;
-      const T across_x = W == 1 ? T(0) : (j == 0 ? bx : (j == W - 1 ? -left_x : bx - left_x));
-      const T across_y = W == 1 ? T(0) : (j == 0 ? by : (j == W - 1 ? -left_y : by - left_y));
-      const T down_x = (!first_row && !last_row) ? bx - up_x : (!last_row ? bx : (!first_row ? -up_x : T(0)));
-      const T down_y = (!first_row && !last_row) ? by - up_y : (!last_row ? by : (!first_row ? -up_y : T(0)));
-      T exy = down_x;
-      exy += across_y;
-      exy = T(0.5) * exy;
-      step_p[c][0] = old_p[c][0] + a * (g1 - bx);
-      step_p[c][1] = old_p[c][1] + a * (g2 - by);
-      old_r[c][0] = device::widen(rxx[at]);
-      old_r[c][1] = device::widen(ryy[at]);
-      old_r[c][2] = device::widen(rxy[at]);
-      step_r[c][0] = old_r[c][0] + a * across_x;
-      step_r[c][1] = old_r[c][1] + a * down_y;
-      step_r[c][2] = old_r[c][2] + a * exy;
-    } else {
-      step_p[c][0] = old_p[c][0] + a * g1;
-      step_p[c][1] = old_p[c][1] + a * g2;
-    }
-    if constexpr (COUPLED) {
-      square += step_p[c][0] * step_p[c][0] + step_p[c][1] * step_p[c][1];
-      if constexpr (TGV) {
-        tensor_square +=
-            step_r[c][0] * step_r[c][0] + step_r[c][1] * step_r[c][1] + T(2) * (step_r[c][2] * step_r[c][2]);
-      }
-    }
-  }
-  const T rho = steps.rho, rest = steps.rest;
-  const T shared_scale = COUPLED ? device::ball_scale(square, steps.radius) : T(1);
-  const T shared_tensor_scale = (COUPLED && TGV) ? device::ball_scale(tensor_square, steps.radius0) : T(1);
-#pragma unroll
-  for (int c = 0; c < C; ++c) {
-    T scale = shared_scale;
-    if constexpr (!COUPLED) {
-      scale = device::ball_scale(step_p[c][0] * step_p[c][0] + step_p[c][1] * step_p[c][1], steps.radius);
-    }
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const T projected = step_p[c][e] * scale;
-      if (keep) F.tp[(2 * c + e) * N + at] = device::narrow<S>(projected);
-      F.p_next[(2 * c + e) * N + at] = device::narrow<S>(rho * projected + rest * old_p[c][e]);
-    }
-    if constexpr (TGV) {
-#pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        if (keep) F.tw[(2 * c + e) * N + at] = tilde_w[c][e];
-        F.w_next[(2 * c + e) * N + at] = rho * tilde_w[c][e] + rest * old_w[c][e];
-      }
-      T tensor_scale = shared_tensor_scale;
-      if constexpr (!COUPLED) {
-        tensor_scale = device::ball_scale(step_r[c][0] * step_r[c][0] + step_r[c][1] * step_r[c][1] +
-                                              T(2) * (step_r[c][2] * step_r[c][2]),
-                                          steps.radius0);
-      }
-#pragma unroll
-      for (int e = 0; e < 3; ++e) {
-        const T projected = step_r[c][e] * tensor_scale;
-        if (keep) F.tr[(3 * c + e) * N + at] = device::narrow<S>(projected);
-        F.r_next[(3 * c + e) * N + at] = device::narrow<S>(rho * projected + rest * old_r[c][e]);
-      }
-    }
-  }
-}
+#include "iterate.hpp"
 
 // ---------------------------------------------------------------------------------------
 // The start (primal_dual.hpp:401-432; frames.cpp:569-593): x = the canvas of the
@@ -808,6 +567,7 @@ class Solver {
     const bool solving = settings_.method == Method::tv || settings_.method == Method::tgv;
     coupled_ = settings_.coupled && C_ > 1;
     if (C_ != 1 && C_ != 3) throw GpuError("a frame of 1 or 3 channels");
+    if (const char* rows = std::getenv("UNROUND_DUAL_ROWS")) dual_rows_ = std::max(1, std::atoi(rows));
     hipStream_t s = stream_.stream;
     allocate();
     upload_components();
@@ -1031,9 +791,24 @@ class Solver {
     HIP_CHECK(hipStreamSynchronize(stream_.stream));
   }
 
+  // The dual step addresses each kind of field through one buffer resource when its planes
+  // span less than 4 GiB (w: 2C planes of T, r: 3C planes of S), else with 64-bit addresses.
+  bool narrow_spans() const {
+    const size_t limit = size_t{1} << 32;
+    return 2 * C_ * N_ * sizeof(T) < limit && 3 * C_ * N_ * sizeof(S) < limit;
+  }
+
   template <int C, bool TGV, bool COUPLED>
-  void launch_dual(const dim3& grid, const Steps<T>& steps, int keep) {
-    dual_kernel<T, S, C, TGV, COUPLED><<<grid, 256, 0, stream_.stream>>>(fields_, H_, W_, steps, keep);
+  void launch_dual(const Steps<T>& steps, int keep) {
+    const int strips = (W_ + kStripColumns - 1) / kStripColumns;
+    const dim3 grid((strips + 3) / 4, (H_ + dual_rows_ - 1) / dual_rows_);
+    if (narrow_spans()) {
+      dual_kernel<T, S, C, TGV, COUPLED, device::BufferPlanes>
+          <<<grid, 256, 0, stream_.stream>>>(fields_, H_, W_, dual_rows_, steps, keep);
+    } else {
+      dual_kernel<T, S, C, TGV, COUPLED, device::WidePlanes>
+          <<<grid, 256, 0, stream_.stream>>>(fields_, H_, W_, dual_rows_, steps, keep);
+    }
   }
 
   void iterate(bool keep) {
@@ -1049,17 +824,22 @@ class Solver {
       const T descent = static_cast<T>(plan_.tau * plan_.gamma[c]);
       const T step = static_cast<T>(plan_.tau / static_cast<double>(k.cells()));
       if (timing_) HIP_CHECK(hipEventRecord(primal_events_[first_event + c].begin, s));
-      if (k.cells() > 1) {
-        prox_kernel<T, S, true><<<prox_groups(c), 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho,
-                                                               rest, keep ? 1 : 0);
+      const int groups = prox_groups(c), live = keep ? 1 : 0;
+      if (k.down == 1 && k.across == 1) {
+        prox_kernel<T, S, 1, 1><<<groups, 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho, rest, live);
+      } else if (k.down == 2 && k.across == 2) {
+        prox_kernel<T, S, 2, 2><<<groups, 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho, rest, live);
+      } else if (k.down == 1 && k.across == 2) {
+        prox_kernel<T, S, 1, 2><<<groups, 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho, rest, live);
+      } else if (k.down == 2 && k.across == 1) {
+        prox_kernel<T, S, 2, 1><<<groups, 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho, rest, live);
       } else {
-        prox_kernel<T, S, false><<<prox_groups(c), 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho,
-                                                                rest, keep ? 1 : 0);
+        prox_any_cell_kernel<T, S><<<groups, 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho, rest,
+                                                          live);
       }
       HIP_CHECK(hipGetLastError());
       if (timing_) HIP_CHECK(hipEventRecord(primal_events_[first_event + c].end, s));
     }
-    const dim3 pixels((W_ + 63) / 64, (H_ + 3) / 4);
     Steps<T> steps{};
     for (int c = 0; c < C_; ++c) {
       steps.descent[c] = static_cast<T>(plan_.tau * plan_.gamma[c]);
@@ -1072,11 +852,11 @@ class Solver {
     const int k = keep ? 1 : 0;
     if (timing_) HIP_CHECK(hipEventRecord(dual_events_.back().begin, s));
     if (C_ == 1) {
-      tgv_ ? launch_dual<1, true, false>(pixels, steps, k) : launch_dual<1, false, false>(pixels, steps, k);
+      tgv_ ? launch_dual<1, true, false>(steps, k) : launch_dual<1, false, false>(steps, k);
     } else if (coupled_) {
-      tgv_ ? launch_dual<3, true, true>(pixels, steps, k) : launch_dual<3, false, true>(pixels, steps, k);
+      tgv_ ? launch_dual<3, true, true>(steps, k) : launch_dual<3, false, true>(steps, k);
     } else {
-      tgv_ ? launch_dual<3, true, false>(pixels, steps, k) : launch_dual<3, false, false>(pixels, steps, k);
+      tgv_ ? launch_dual<3, true, false>(steps, k) : launch_dual<3, false, false>(steps, k);
     }
     HIP_CHECK(hipGetLastError());
     if (timing_) HIP_CHECK(hipEventRecord(dual_events_.back().end, s));
@@ -1223,6 +1003,8 @@ class Solver {
   Fields<T, S> fields_{};
   size_t norm_partials_count_ = 0, value_partials_count_ = 0, conjugate_partials_count_ = 0;
   std::deque<EventPair> primal_events_, dual_events_, launches_;
+  // Rows a wavefront of the dual kernel walks down.
+  int dual_rows_ = 16;
 };
 
 }  // namespace
