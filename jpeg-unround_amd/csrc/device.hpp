@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 namespace unround {
@@ -70,9 +71,44 @@ struct Component {
   double costs[64];
 };
 
+// A component's per-frequency constants and scalars copied into LDS by a workgroup, so that
+// lanes that each take other frequencies read them without a trip to memory.
+struct StagedComponent {
+  double steps[64];
+  double weights[64];
+  double flat_weights[64];
+  double shrink[64];
+  double costs[64];
+  double slack;
+  int mmse, costed;
+};
+static_assert(offsetof(Component, costs) == offsetof(Component, steps) + 4 * 64 * sizeof(double));
+static_assert(offsetof(StagedComponent, costs) == offsetof(StagedComponent, steps) + 4 * 64 * sizeof(double));
+
+// The workgroup's threads copy the constants of K into `into`; a barrier must follow
+// before any thread reads them.
+__device__ __forceinline__ void stage(const Component& K, StagedComponent& into) {
+  const double* from = K.steps;
+  double* to = into.steps;
+  for (int e = threadIdx.x; e < 5 * 64; e += blockDim.x) to[e] = from[e];
+  if (threadIdx.x == 0) {
+    into.slack = K.slack;
+    into.mmse = K.mmse;
+    into.costed = K.costed;
+  }
+}
+
+// A pointer into device memory seen as one into the global address space, so that loads
+// through it are global loads rather than generic ones.
+template <typename X>
+__device__ __forceinline__ const __attribute__((address_space(1))) X* global_view(const X* p) {
+  return (const __attribute__((address_space(1))) X*)(p);
+}
+
 // The ends, centre and file's own ends of coefficient k of block b, in binary64, each
 // step rounded apart as the reference rounds it (cpp/src/model.cpp:44-48, 242-262).
-__device__ inline void ends_of(const Component& K, int16_t q, int k, double& lower, double& upper, double& centre,
+template <typename Constants>
+__device__ inline void ends_of(const Constants& K, int16_t q, int k, double& lower, double& upper, double& centre,
                                double& own_lower, double& own_upper) {
 #pragma clang fp contract(off)
   const double level = static_cast<double>(q);
@@ -97,8 +133,8 @@ __device__ __forceinline__ T clamp_to(T value, T low, T high) {
 
 // The proximal map of one coefficient of level q (docs/math.md:457-491;
 // cpp/src/model.cpp:62-75): the quadratic's minimizer, clipped to the interval.
-template <typename T>
-__device__ __forceinline__ T prox_of(const Component& K, int16_t q, bool flat, int k, T e, T step) {
+template <typename Constants, typename T>
+__device__ __forceinline__ T prox_of(const Constants& K, int16_t q, bool flat, int k, T e, T step) {
   double lower, upper, centre, own_lower, own_upper;
   ends_of(K, q, k, lower, upper, centre, own_lower, own_upper);
   const T weight = static_cast<T>(flat ? K.flat_weights[k] : K.weights[k]);

@@ -36,6 +36,30 @@ __device__ __forceinline__ T backward_down_of(T here, T up, int i, int H) {
   return T(0);
 }
 
+// The levels of a lane's row of a block in the coefficient phase, row k of block (ty, bx)
+// (one 16-byte load), and the block's flat mark; zeros beyond the blocks.
+struct LevelRow {
+  int16_t q[8];
+  bool flat;
+};
+__device__ __forceinline__ LevelRow level_row(const Component& K, int ty, int bx, int k, bool inside) {
+  LevelRow row{};
+  if (inside) {
+    const size_t block = static_cast<size_t>(ty) * K.columns + bx;
+    using Words = unsigned int __attribute__((ext_vector_type(4)));
+    const Words words =
+        *reinterpret_cast<const __attribute__((address_space(1))) Words*>(device::global_view(K.levels) + 64 * block + 8 * k);
+    const uint32_t pairs[4] = {words.x, words.y, words.z, words.w};
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      row.q[2 * m] = static_cast<int16_t>(pairs[m] & 0xffffu);
+      row.q[2 * m + 1] = static_cast<int16_t>(pairs[m] >> 16);
+    }
+    row.flat = device::global_view(K.flat)[block] != 0;
+  }
+  return row;
+}
+
 // ---------------------------------------------------------------------------------------
 // The proximal step of one component whose cells are DV x DH samples (1 or 2 each)
 // (primal_dual.hpp:536-546; frames.cpp:490-548). A wavefront takes 8 rows of blocks of a
@@ -46,6 +70,7 @@ __global__ __launch_bounds__(256) void prox_kernel(const Component* __restrict__
                                                    int H, int W, T descent, T step, T rho, T rest, int keep) {
   __shared__ T forward_room[4][8][65];
   __shared__ T inverse_room[4][8][65];
+  __shared__ device::StagedComponent staged;
   const Component& K = *component;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int strips = (K.tiles_x + 7) >> 3;
@@ -54,6 +79,13 @@ __global__ __launch_bounds__(256) void prox_kernel(const Component* __restrict__
   const int tx = strip * 8 + (lane >> 3);           // the tile of the lane's column
   const int n = lane & 7;
   const bool live = ty < K.tiles_y && tx < K.tiles_x;
+  // In the coefficient phase a lane takes row k of block b of the strip: its levels are
+  // loaded first, with the constants, so that they arrive with the samples.
+  const int b = lane >> 3, k = lane & 7;
+  const int bx = strip * 8 + b;
+  const bool inside = ty < K.rows && bx < K.columns;
+  device::stage(K, staged);
+  const LevelRow own = level_row(K, ty, bx, k, inside);
   const size_t N = static_cast<size_t>(H) * W;
   T* x = F.x + c * N;
   const S* px = F.p + (2 * c) * N;
@@ -112,19 +144,14 @@ __global__ __launch_bounds__(256) void prox_kernel(const Component* __restrict__
   for (int k = 0; k < 8; ++k) forward_room[wave][k][lane] = y[k];
   __syncthreads();
   // A lane now takes row k of block b of the strip.
-  const int b = lane >> 3, k = lane & 7;
-  const int bx = strip * 8 + b;
-  const bool inside = ty < K.rows && bx < K.columns;
   T row[8], coefficient[8], zeta[8];
 #pragma unroll
   for (int u = 0; u < 8; ++u) row[u] = forward_room[wave][k][8 * b + u];
   device::forward8(row, coefficient);
   if (inside) {
     const size_t block = static_cast<size_t>(ty) * K.columns + bx;
-    const int16_t* levels = K.levels + 64 * block + 8 * k;
-    const bool flat = K.flat[block] != 0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) zeta[u] = device::prox_of(K, levels[u], flat, 8 * k + u, coefficient[u], step);
+    for (int u = 0; u < 8; ++u) zeta[u] = device::prox_of(staged, own.q[u], own.flat, 8 * k + u, coefficient[u], step);
     if (keep) {
       T* kept = F.zeta[c] + 64 * block + 8 * k;
 #pragma unroll
@@ -174,6 +201,7 @@ __global__ __launch_bounds__(256) void prox_any_cell_kernel(const Component* __r
                                                             int keep) {
   __shared__ T forward_room[4][8][65];
   __shared__ T inverse_room[4][8][65];
+  __shared__ device::StagedComponent staged;
   const Component& K = *component;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int strips = (K.tiles_x + 7) >> 3;
@@ -182,6 +210,11 @@ __global__ __launch_bounds__(256) void prox_any_cell_kernel(const Component* __r
   const int tx = strip * 8 + (lane >> 3);
   const int n = lane & 7;
   const bool live = ty < K.tiles_y && tx < K.tiles_x;
+  const int b = lane >> 3, k = lane & 7;
+  const int bx = strip * 8 + b;
+  const bool inside = ty < K.rows && bx < K.columns;
+  device::stage(K, staged);
+  const LevelRow own = level_row(K, ty, bx, k, inside);
   const int dv = K.down, dh = K.across;
   const size_t N = static_cast<size_t>(H) * W;
   T* x = F.x + c * N;
@@ -211,19 +244,14 @@ __global__ __launch_bounds__(256) void prox_any_cell_kernel(const Component* __r
 #pragma unroll
   for (int k = 0; k < 8; ++k) forward_room[wave][k][lane] = y[k];
   __syncthreads();
-  const int b = lane >> 3, k = lane & 7;
-  const int bx = strip * 8 + b;
-  const bool inside = ty < K.rows && bx < K.columns;
   T row[8], coefficient[8], zeta[8];
 #pragma unroll
   for (int u = 0; u < 8; ++u) row[u] = forward_room[wave][k][8 * b + u];
   device::forward8(row, coefficient);
   if (inside) {
     const size_t block = static_cast<size_t>(ty) * K.columns + bx;
-    const int16_t* levels = K.levels + 64 * block + 8 * k;
-    const bool flat = K.flat[block] != 0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) zeta[u] = device::prox_of(K, levels[u], flat, 8 * k + u, coefficient[u], step);
+    for (int u = 0; u < 8; ++u) zeta[u] = device::prox_of(staged, own.q[u], own.flat, 8 * k + u, coefficient[u], step);
     if (keep) {
       T* kept = F.zeta[c] + 64 * block + 8 * k;
 #pragma unroll
@@ -268,23 +296,85 @@ __global__ __launch_bounds__(256) void prox_any_cell_kernel(const Component* __r
 // The dual step of every channel, with TGV's step of w (primal_dual.hpp:547-557, 572-735),
 // from the buffers of the iteration's start into the other ones.
 //
-// A wavefront walks down `rows` rows of a strip; its lanes take 64 consecutive columns
-// from the one before the strip to the one after it, and only lanes 1..62 write: lane 0
-// makes w- of the column on the strip's left and lane 63 reads x- and r of the column on
-// its right, so that every neighbour comes from a lane shuffle and every lane runs the
-// same loads. Per lane and channel the wavefront carries x- and r of the row and the next
-// one and w- of the row above, and it loads the row after next while it works on the row.
-constexpr int kStripColumns = 62;
+// A wavefront walks down `rows` rows of a strip of 64 columns, a lane per column, so that
+// each row of each field is one aligned 256-byte load. Per lane and channel it carries x-
+// and r of the row and the next one and w- of the row above, and it loads the row after
+// next while it works on the row. A neighbour in the strip comes from the neighbouring lane;
+// the few samples beyond the strip's two edges (x- and r on the right for lane 63; w, p
+// and r on the left, from which lane 0 makes w- of the column on its left) come from one
+// gather per row, a lane per sample, read back as scalars.
+constexpr int kStripColumns = 64;
+
+// The value of `v` in lane `source` (a constant), as a scalar.
+template <typename T>
+__device__ __forceinline__ T lane_value(T v, int source) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), source));
+  } else {
+    struct Halves {
+      int low, high;
+    };
+    Halves h = __builtin_bit_cast(Halves, v);
+    h.low = __builtin_amdgcn_readlane(h.low, source);
+    h.high = __builtin_amdgcn_readlane(h.high, source);
+    return __builtin_bit_cast(T, h);
+  }
+}
+
+// The gather of a strip's edge samples for one row. Lanes 3c + e of the T gather hold w_x,
+// w_y on the left (e = 0, 1) and x- on the right (e = 2) of channel c; lanes 9c + e of the
+// S gather hold p_x, p_y, r_xx, r_yy, r_xy on the left (e = 0..4), r_yy and r_xy on the
+// left one row down (e = 5, 6), and r_xx, r_xy on the right (e = 7, 8).
+template <typename T, typename S, int C, bool TGV>
+struct EdgeGather {
+  const T* t_source = nullptr;
+  const S* s_source = nullptr;
+  int t_down = 0, s_down = 0;
+  size_t t_column = 0, s_column = 0;
+
+  __device__ EdgeGather(const Fields<T, S>& F, int lane, int H, int W, int j0) {
+    const size_t N = static_cast<size_t>(H) * W;
+    const size_t left = static_cast<size_t>(max(j0 - 1, 0));
+    const size_t right = static_cast<size_t>(min(j0 + kStripColumns, W - 1));
+    if (lane < 3 * C) {
+      const int c = lane / 3, e = lane % 3;
+      if (e == 2) {
+        t_source = F.xbar + c * N;
+        t_column = right;
+      } else if (TGV) {
+        t_source = F.w + (2 * c + e) * N;
+        t_column = left;
+      }
+    }
+    if (TGV && lane < 9 * C) {
+      const int c = lane / 9, e = lane % 9;
+      // The entry of p (e < 2) or r for each e, two bits each: 0, 1, 0, 1, 2, 1, 2, 0, 2.
+      const int entry = (0x22644 >> (2 * e)) & 3;
+      s_source = (e < 2 ? F.p + (2 * c) * N : F.r + (3 * c) * N) + entry * N;
+      s_down = (e == 5 || e == 6) ? 1 : 0;
+      s_column = e < 7 ? left : right;
+    }
+  }
+
+  // This lane's sample for row i (rows clamped into the canvas), or 0.
+  __device__ __forceinline__ void load(int i, int H, int W, T& t, T& s) const {
+    t = T(0);
+    s = T(0);
+    if (t_source != nullptr) t = t_source[static_cast<size_t>(min(i + t_down, H - 1)) * W + t_column];
+    if (s_source != nullptr) s = device::widen(s_source[static_cast<size_t>(min(i + s_down, H - 1)) * W + s_column]);
+  }
+};
 
 template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes>
 __global__ __launch_bounds__(256) void dual_kernel(Fields<T, S> F, int H, int W, int rows, Steps<T> steps, int keep) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int strip = blockIdx.x * 4 + wave;
-  if (strip * kStripColumns >= W) return;  // the whole wavefront beyond the canvas
-  const int j = strip * kStripColumns + lane - 1;
-  const bool writes = lane >= 1 && lane <= kStripColumns && j < W;
+  const int j0 = strip * kStripColumns;
+  if (j0 >= W) return;  // the whole wavefront beyond the canvas
+  const int j = j0 + lane;
+  const bool writes = j < W;
   const bool right = j + 1 < W;
-  const uint32_t column = static_cast<uint32_t>(min(max(j, 0), W - 1));
+  const uint32_t column = static_cast<uint32_t>(min(j, W - 1));
   const int first = blockIdx.y * rows;
   const int end = min(first + rows, H);
   const size_t N = static_cast<size_t>(H) * W;
@@ -292,25 +382,28 @@ __global__ __launch_bounds__(256) void dual_kernel(Fields<T, S> F, int H, int W,
   const Planes<S> p(F.p, 2 * C, N), p_next(F.p_next, 2 * C, N), tp(F.tp, 2 * C, N);
   const Planes<T> w(F.w, 2 * C, N), w_next(F.w_next, 2 * C, N), tw(F.tw, 2 * C, N);
   const Planes<S> r(F.r, 3 * C, N), r_next(F.r_next, 3 * C, N), tr(F.tr, 3 * C, N);
+  const EdgeGather<T, S, C, TGV> edges(F, lane, H, W, j0);
   // The index of the lane's sample in row i, i clamped into the canvas (a clamped load
   // only feeds differences that the canvas' edges take as zero).
   const auto index = [&](int i) { return static_cast<uint32_t>(min(i, H - 1)) * static_cast<uint32_t>(W) + column; };
-  // w~ and w- at a sample from w, p and r there, r of the row below and of the column on
-  // the right (through the lane on the right): w~ = w + tau (p + div_2 r).
-  const auto w_step = [&](const T old[2], const T q[2], const T here[3], const T next[3], bool below, T descent,
-                          T tilde[2], T bar[2]) {
-    const T right_xx = from_right(here[0]), right_xy = from_right(here[2]);
-    const T dx = (right ? right_xx - here[0] : T(0)) + (below ? next[2] - here[2] : T(0));
-    const T dy = (right ? right_xy - here[2] : T(0)) + (below ? next[1] - here[1] : T(0));
+  // w~ and w- at a sample from w, p and r there, r of the row below and r_xx, r_xy of the
+  // column on the right: w~ = w + tau (p + div_2 r).
+  const auto w_step = [&](const T old[2], const T q[2], const T here[3], const T next[3], T right_xx, T right_xy,
+                          bool across, bool below, T descent, T tilde[2], T bar[2]) {
+    const T dx = (across ? right_xx - here[0] : T(0)) + (below ? next[2] - here[2] : T(0));
+    const T dy = (across ? right_xy - here[2] : T(0)) + (below ? next[1] - here[1] : T(0));
     tilde[0] = old[0] + descent * (q[0] + dx);
     tilde[1] = old[1] + descent * (q[1] + dy);
     bar[0] = T(2) * tilde[0] - old[0];
     bar[1] = T(2) * tilde[1] - old[1];
   };
 
-  // The rows of the start: x- and r of the first two rows, p and w of the first, and w-
-  // of the row above.
+  // The rows of the start: x- and r of the first two rows, p and w of the first, the edge
+  // samples of the first row, and w- of the row above.
   T bar[C], bar_next[C], r_here[C][3], r_below[C][3], p_here[C][2], w_here[C][2], w_up[C][2];
+  T edge_t, edge_s, up_t, up_s;
+  edges.load(first, H, W, edge_t, edge_s);
+  if (TGV && first > 0) edges.load(first - 1, H, W, up_t, up_s);
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     bar[c] = xbar.load(c, index(first));
@@ -335,7 +428,10 @@ __global__ __launch_bounds__(256) void dual_kernel(Fields<T, S> F, int H, int W,
           p_up[e] = device::widen(p.load(2 * c + e, index(first - 1)));
           old_up[e] = w.load(2 * c + e, index(first - 1));
         }
-        w_step(old_up, p_up, r_up, r_here[c], true, steps.descent[c], tilde_up, w_up[c]);
+        const T edge_xx = lane_value(up_s, 9 * c + 7), edge_xy = lane_value(up_s, 9 * c + 8);
+        const T shifted_xx = from_right(r_up[0]), shifted_xy = from_right(r_up[2]);
+        const T right_xx = lane == 63 ? edge_xx : shifted_xx, right_xy = lane == 63 ? edge_xy : shifted_xy;
+        w_step(old_up, p_up, r_up, r_here[c], right_xx, right_xy, right, true, steps.descent[c], tilde_up, w_up[c]);
       }
     }
   }
@@ -343,8 +439,10 @@ __global__ __launch_bounds__(256) void dual_kernel(Fields<T, S> F, int H, int W,
 #pragma unroll 1
   for (int i = first; i < end; ++i) {
     const bool below = i + 1 < H;
-    // The loads of the next iteration: x- and r two rows down, p and w one row down.
-    T bar_ahead[C], r_ahead[C][3], p_ahead[C][2], w_ahead[C][2];
+    // The loads of the next iteration: x- and r two rows down, p and w one row down, and the
+    // edge samples one row down.
+    T bar_ahead[C], r_ahead[C][3], p_ahead[C][2], w_ahead[C][2], edge_t_ahead, edge_s_ahead;
+    edges.load(i + 1, H, W, edge_t_ahead, edge_s_ahead);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       bar_ahead[c] = xbar.load(c, index(i + 2));
@@ -363,14 +461,29 @@ __global__ __launch_bounds__(256) void dual_kernel(Fields<T, S> F, int H, int W,
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const T a = steps.ascent[c];
-      const T bar_right = from_right(bar[c]);
+      const T edge_bar = lane_value(edge_t, 3 * c + 2), shifted_bar = from_right(bar[c]);
+      const T bar_right = lane == 63 ? edge_bar : shifted_bar;
       const T g1 = right ? bar_right - bar[c] : T(0);
       const T g2 = below ? bar_next[c] - bar[c] : T(0);
       if constexpr (TGV) {
-        w_step(w_here[c], p_here[c], r_here[c], r_below[c], below, steps.descent[c], tilde_w[c], bar_w[c]);
-        // E w- (docs/math.md:355): backward differences, w- of the column on the left
-        // from the lane on the left.
-        const T left_x = from_left(bar_w[c][0]), left_y = from_left(bar_w[c][1]);
+        const T edge_xx = lane_value(edge_s, 9 * c + 7), edge_xy = lane_value(edge_s, 9 * c + 8);
+        const T shifted_xx = from_right(r_here[c][0]), shifted_xy = from_right(r_here[c][2]);
+        const T right_xx = lane == 63 ? edge_xx : shifted_xx, right_xy = lane == 63 ? edge_xy : shifted_xy;
+        w_step(w_here[c], p_here[c], r_here[c], r_below[c], right_xx, right_xy, right, below, steps.descent[c],
+               tilde_w[c], bar_w[c]);
+        // w- of the column on the left: the lane on the left's, or for lane 0, made from the
+        // edge samples and its own r (every lane makes it; lane 0 keeps it).
+        const T old_left[2] = {lane_value(edge_t, 3 * c), lane_value(edge_t, 3 * c + 1)};
+        const T p_left[2] = {lane_value(edge_s, 9 * c), lane_value(edge_s, 9 * c + 1)};
+        const T r_left[3] = {lane_value(edge_s, 9 * c + 2), lane_value(edge_s, 9 * c + 3),
+                             lane_value(edge_s, 9 * c + 4)};
+        const T r_left_below[3] = {T(0), lane_value(edge_s, 9 * c + 5), lane_value(edge_s, 9 * c + 6)};
+        T tilde_left[2], bar_left[2];
+        w_step(old_left, p_left, r_left, r_left_below, r_here[c][0], r_here[c][2], true, below, steps.descent[c],
+               tilde_left, bar_left);
+        const T shifted_x = from_left(bar_w[c][0]), shifted_y = from_left(bar_w[c][1]);
+        const T left_x = lane == 0 ? bar_left[0] : shifted_x, left_y = lane == 0 ? bar_left[1] : shifted_y;
+        // E w- (docs/math.md:355): backward differences.
         const T across_x = backward_across_of(bar_w[c][0], left_x, j, W);
         const T down_y = backward_down_of(bar_w[c][1], w_up[c][1], i, H);
         const T exy =
@@ -447,5 +560,7 @@ __global__ __launch_bounds__(256) void dual_kernel(Fields<T, S> F, int H, int W,
         }
       }
     }
+    edge_t = edge_t_ahead;
+    edge_s = edge_s_ahead;
   }
 }
