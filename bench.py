@@ -8,15 +8,23 @@ channels). The default workload is the 2048 x 2048 RGB picture of seed 0, JFIF Y
 binary32. W warm-up iterations run in a solve of their own; then one solve of exactly K
 iterations is timed, its records only at its end (--record-every 0 --tolerance 0),
 bracketed by torch.cuda.synchronize(). value counts the samples of every channel times K
-over the iterations' own time (HIP events on the solve's stream around each iteration,
-records and setup excluded, as the reference's throughput does); ms_per_step is that time
-per iteration, and the wall time of the whole timed solve is printed to stderr.
+over the iterations' own time (one pair of HIP events on the solve's stream around the K
+iterations, records and setup excluded, as the reference's throughput does); ms_per_step
+is that time per iteration, and the wall time of the whole timed solve is printed to
+stderr.
 
-roofline: the iteration's kernels read and write HBM; the algorithmic bytes of a sample
-and iteration are 2 x the bytes of the state's fields (x, w, p, r) + 2 x the coefficients
-per sample (each int16 level read once), and achieved = those bytes of one iteration over
-its measured duration, against MI355X's 8 TB/s. cpu_baseline: the CPU oracle (binary64,
-one thread) on a bounded sample of the same kind of workload, timed on this host.
+roofline: of the dominant kernel, the dual step (dual_kernel, one launch per iteration
+over every channel), timed launch by launch with HIP events in a second solve of
+min(K, 100) iterations. Its algorithmic bytes per channel-sample are one read of x-, p, w
+and r and one write of p, w and r (60 B in binary32 TGV, 40 B with bfloat16 p and r);
+achieved = those bytes of one launch over its mean duration, against MI355X's 8 TB/s.
+traffic: the HBM bytes of one launch from rocprofv3 counters (FETCH_SIZE + WRITE_SIZE,
+scaled by a copy kernel's reading), where profiles/traffic.json holds them for this
+workload. iteration_achieved is the same for a whole iteration (2 x the state's bytes +
+2 x the coefficients per sample, each int16 level read once).
+
+cpu_baseline: the CPU oracle (binary64, one thread) on a bounded sample of the same kind
+of workload, timed on this host.
 """
 
 from __future__ import annotations
@@ -101,6 +109,27 @@ def cpu_baseline(method: str, budget_seconds: float = 12.0) -> dict:
     }
 
 
+def dual_bytes(method: str, precision: str) -> float:
+    """Algorithmic bytes of the dual kernel per channel-sample: it reads x-, p (and w, r) once
+    and writes p (and w, r) once."""
+    iterate = 8 if precision == "double" else 4
+    dual = 2 if precision == "mixed" else iterate
+    if method == "tv":
+        return iterate + 2 * dual + 2 * dual
+    return (iterate + 2 * dual + 2 * iterate + 3 * dual) + (2 * dual + 2 * iterate + 3 * dual)
+
+
+def measured_traffic(kernel: str, workload_name: str, precision: str) -> float | None:
+    """HBM bytes per launch of `kernel` from the committed counter summary, if it covers this case."""
+    path = ROOT / "profiles" / "traffic.json"
+    if not path.exists():
+        return None
+    for entry in json.loads(path.read_text()).get("entries", []):
+        if (entry.get("kernel"), entry.get("config"), entry.get("precision")) == (kernel, workload_name, precision):
+            return float(entry["bytes_per_launch"])
+    return None
+
+
 def main() -> None:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -124,26 +153,38 @@ def main() -> None:
     base = ["--method", method, "--tolerance", "0", "--precision", args.precision]
     if args.warmup > 0:
         native.solve(image, base + ["--iterations", str(args.warmup), "--record-every", "0"])
+    # The timed solve: one pair of events around its iterations, nothing between launches.
     os.environ["UNROUND_TIMING"] = "1"
     torch.cuda.synchronize()
     began = time.perf_counter()
     decoded = native.solve(image, base + ["--iterations", str(args.steps), "--record-every", "0"])
     torch.cuda.synchronize()
     wall = time.perf_counter() - began
-    os.environ.pop("UNROUND_TIMING", None)
     times = decoded.kernel_times
     assert times is not None and times.iterations == args.steps, "the library took no timings"
+    # The kernels' own durations: events around every launch, in a solve of their own.
+    kernel_steps = min(args.steps, 100)
+    os.environ["UNROUND_TIMING"] = "2"
+    launches = native.solve(image, base + ["--iterations", str(kernel_steps), "--record-every", "0"]).kernel_times
+    os.environ.pop("UNROUND_TIMING", None)
+    assert launches is not None and launches.dual_launches == kernel_steps, "the library took no kernel timings"
+
     channels, rows, columns = decoded.canvas.shape
     samples = channels * rows * columns
     seconds = times.iterations_seconds
     value = samples * args.steps / seconds / 1e6
     coefficients = sum(int(np.prod(c.shape)) for c in decoded.coefficients)
     per_sample = 2 * field_bytes(method, args.precision) + 2 * coefficients / samples
-    achieved = per_sample * samples / (seconds / args.steps) / 1e9
+    iteration_achieved = per_sample * samples / (seconds / args.steps) / 1e9
+    dual_launch = launches.dual_seconds / launches.dual_launches
+    primal_launch = launches.primal_seconds / max(launches.primal_launches, 1)
+    dual_launch_bytes = dual_bytes(method, args.precision) * samples
+    achieved = dual_launch_bytes / dual_launch / 1e9
     log(f"wall {wall:.3f} s for the timed solve ({args.steps} iterations + setup + final projection); "
-        f"iterations {seconds:.4f} s; primal kernels {times.primal_seconds / max(times.primal_launches, 1) * 1e6:.1f} us "
-        f"x {times.primal_launches}, dual {times.dual_seconds / max(times.dual_launches, 1) * 1e6:.1f} us "
-        f"x {times.dual_launches}")
+        f"iterations {seconds:.4f} s; per launch (solve of {kernel_steps} iterations timed launch by launch): "
+        f"primal kernels {primal_launch * 1e6:.1f} us x {launches.primal_launches}, dual kernel "
+        f"{dual_launch * 1e6:.1f} us x {launches.dual_launches}; whole iterations {iteration_achieved:.1f} GB/s "
+        f"of {per_sample:.1f} algorithmic bytes per sample")
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -167,13 +208,16 @@ def main() -> None:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "one iteration (primal_kernel per component + dual_kernel)",
-            "bytes_per_sample_iteration": round(per_sample, 3),
+            "kernel": "dual_kernel",
+            "bytes_per_launch": int(dual_launch_bytes),
+            "launch_us": round(dual_launch * 1e6, 2),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic("dual_kernel", args.config, args.precision),
+            "iteration_achieved": round(iteration_achieved, 1),
+            "iteration_frac": round(iteration_achieved / HBM_PEAK_GBS, 4),
         },
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(method),
     }

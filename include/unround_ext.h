@@ -33,10 +33,13 @@ unround_status unround_solve_start(const unround_input* input, const unround_set
                                    unround_result** result, char* message, size_t message_size);
 
 /* The durations of the solve's kernels, where the environment variable UNROUND_TIMING was
- * 1 during the solve (HIP events on the solve's stream): values[0] the seconds of the
+ * set during the solve (HIP events on the solve's stream): values[0] the seconds of the
  * primal kernels and [1] their launches, [2] and [3] those of the dual kernels, [4] the
  * seconds of the iterations (records excluded) and [5] the iterations; at most count
- * values are written. Returns 1 where timings were taken, 0 otherwise. */
+ * values are written. UNROUND_TIMING=1 brackets each run of iterations between two
+ * records with one pair of events (launches [1] and [3] are then 0); UNROUND_TIMING=2
+ * brackets every iteration and every launch. Returns 1 where timings were taken, 0
+ * otherwise. */
 int32_t unround_result_kernel_times(const unround_result* result, double* values, size_t count);
 
 #ifdef __cplusplus

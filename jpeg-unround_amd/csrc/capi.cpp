@@ -201,8 +201,12 @@ unround_status reconstruct(const unround_input* input, const unround_settings* s
       return observer(user, &raw) != 0;
     };
   }
+  // UNROUND_TIMING=1 times the runs of iterations between records; 2 also every launch.
   const char* timing = std::getenv("UNROUND_TIMING");
-  const bool timed = timing != nullptr && timing[0] == '1';
+  const unround::Timing timed = timing == nullptr                ? unround::Timing::none
+                                : std::string(timing) == "2"     ? unround::Timing::kernels
+                                : std::string(timing) == "1"     ? unround::Timing::iterations
+                                                                 : unround::Timing::none;
   if (!unround::solve_on_gpu(frame, s, plan, start != nullptr ? &state : nullptr, watch, timed, made->out, message,
                              status)) {
     return status;

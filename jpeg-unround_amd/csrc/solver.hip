@@ -523,6 +523,7 @@ struct StreamHolder {
 
 struct EventPair {
   hipEvent_t begin = nullptr, end = nullptr;
+  uint64_t iterations = 0;  // the iterations between the two events
   EventPair() {
     HIP_CHECK(hipEventCreate(&begin));
     HIP_CHECK(hipEventCreate(&end));
@@ -553,7 +554,7 @@ void upload_basis() {
 template <typename T, typename S>
 class Solver {
  public:
-  Solver(const FrameModel& frame, const Settings& settings, const Plan& plan, bool timing)
+  Solver(const FrameModel& frame, const Settings& settings, const Plan& plan, Timing timing)
       : frame_(frame), settings_(settings), plan_(plan), timing_(timing) {}
 
   void run(const StartState* start, const Observer& observer, SolveOutput& out) {
@@ -567,6 +568,7 @@ class Solver {
     const bool solving = settings_.method == Method::tv || settings_.method == Method::tgv;
     coupled_ = settings_.coupled && C_ > 1;
     if (C_ != 1 && C_ != 3) throw GpuError("a frame of 1 or 3 channels");
+    dual_rows_ = dual_rows_for(H_, W_);
     if (const char* rows = std::getenv("UNROUND_DUAL_ROWS")) dual_rows_ = std::max(1, std::atoi(rows));
     hipStream_t s = stream_.stream;
     allocate();
@@ -637,24 +639,30 @@ class Solver {
         }
       };
       if (every > 0 || most == 0) take(0);
+      bool segment_open = false;
       while (!stopped && iteration < most) {
         const uint64_t next = iteration + 1;
         const bool keep = (every > 0 && next % every == 0) || next == most;
-        if (timing_) {
-          // The events of every launch, read once the iterations between two records are
-          // done, so that timing adds no wait between launches.
+        // The events are read once the iterations between two records are done, so that
+        // timing adds no wait between launches.
+        if (timing_ == Timing::kernels || (timing_ == Timing::iterations && !segment_open)) {
           launches_.emplace_back();
           HIP_CHECK(hipEventRecord(launches_.back().begin, s));
+          segment_open = true;
         }
         iterate(keep);
-        if (timing_) HIP_CHECK(hipEventRecord(launches_.back().end, s));
+        if (timing_ != Timing::none) launches_.back().iterations += 1;
+        if (timing_ == Timing::kernels) HIP_CHECK(hipEventRecord(launches_.back().end, s));
         iteration = next;
         if ((every > 0 && iteration % every == 0) || iteration == most) {
-          if (timing_) collect_kernel_times(out.times);
+          if (timing_ == Timing::iterations) HIP_CHECK(hipEventRecord(launches_.back().end, s));
+          segment_open = false;
+          if (timing_ != Timing::none) collect_kernel_times(out.times);
           take(iteration);
         }
       }
-      if (timing_) collect_kernel_times(out.times);
+      if (timing_ == Timing::iterations && segment_open) HIP_CHECK(hipEventRecord(launches_.back().end, s));
+      if (timing_ != Timing::none) collect_kernel_times(out.times);
       if (stop == 0) stop = 2;
     }
     out.iterations = iteration;
@@ -791,6 +799,16 @@ class Solver {
     HIP_CHECK(hipStreamSynchronize(stream_.stream));
   }
 
+  // Rows a wavefront of the dual kernel walks down: long enough to spread its first rows'
+  // extra loads thin, short enough for about 2048 wavefronts (two per SIMD), which measured
+  // best at 2048 x 2048; a power of two from 8 to 64.
+  static int dual_rows_for(int H, int W) {
+    const long strips = (W + kStripColumns - 1) / kStripColumns;
+    int rows = 8;
+    while (rows < 64 && strips * ((H + 2 * rows - 1) / (2 * rows)) >= 2048) rows *= 2;
+    return rows;
+  }
+
   // The dual step addresses each kind of field through one buffer resource when its planes
   // span less than 4 GiB (w: 2C planes of T, r: 3C planes of S), else with 64-bit addresses.
   bool narrow_spans() const {
@@ -814,16 +832,17 @@ class Solver {
   void iterate(bool keep) {
     hipStream_t s = stream_.stream;
     const T rho = static_cast<T>(plan_.relaxation), rest = static_cast<T>(1.0 - plan_.relaxation);
-    if (timing_) {
+    const bool per_kernel = timing_ == Timing::kernels;
+    if (per_kernel) {
       for (int c = 0; c < C_; ++c) primal_events_.emplace_back();
       dual_events_.emplace_back();
     }
-    const size_t first_event = timing_ ? primal_events_.size() - static_cast<size_t>(C_) : 0;
+    const size_t first_event = per_kernel ? primal_events_.size() - static_cast<size_t>(C_) : 0;
     for (int c = 0; c < C_; ++c) {
       const ComponentModel& k = frame_.components[c];
       const T descent = static_cast<T>(plan_.tau * plan_.gamma[c]);
       const T step = static_cast<T>(plan_.tau / static_cast<double>(k.cells()));
-      if (timing_) HIP_CHECK(hipEventRecord(primal_events_[first_event + c].begin, s));
+      if (per_kernel) HIP_CHECK(hipEventRecord(primal_events_[first_event + c].begin, s));
       const int groups = prox_groups(c), live = keep ? 1 : 0;
       if (k.down == 1 && k.across == 1) {
         prox_kernel<T, S, 1, 1><<<groups, 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho, rest, live);
@@ -838,7 +857,7 @@ class Solver {
                                                           live);
       }
       HIP_CHECK(hipGetLastError());
-      if (timing_) HIP_CHECK(hipEventRecord(primal_events_[first_event + c].end, s));
+      if (per_kernel) HIP_CHECK(hipEventRecord(primal_events_[first_event + c].end, s));
     }
     Steps<T> steps{};
     for (int c = 0; c < C_; ++c) {
@@ -850,7 +869,7 @@ class Solver {
     steps.rho = rho;
     steps.rest = rest;
     const int k = keep ? 1 : 0;
-    if (timing_) HIP_CHECK(hipEventRecord(dual_events_.back().begin, s));
+    if (per_kernel) HIP_CHECK(hipEventRecord(dual_events_.back().begin, s));
     if (C_ == 1) {
       tgv_ ? launch_dual<1, true, false>(steps, k) : launch_dual<1, false, false>(steps, k);
     } else if (coupled_) {
@@ -859,7 +878,7 @@ class Solver {
       tgv_ ? launch_dual<3, true, false>(steps, k) : launch_dual<3, false, false>(steps, k);
     }
     HIP_CHECK(hipGetLastError());
-    if (timing_) HIP_CHECK(hipEventRecord(dual_events_.back().end, s));
+    if (per_kernel) HIP_CHECK(hipEventRecord(dual_events_.back().end, s));
     std::swap(fields_.p, fields_.p_next);
     std::swap(fields_.w, fields_.w_next);
     std::swap(fields_.r, fields_.r_next);
@@ -883,7 +902,7 @@ class Solver {
     }
     for (const EventPair& pair : launches_) {
       times.iterations_seconds += elapsed(pair);
-      times.iterations += 1;
+      times.iterations += pair.iterations;
     }
     primal_events_.clear();
     dual_events_.clear();
@@ -993,7 +1012,7 @@ class Solver {
   const FrameModel& frame_;
   const Settings& settings_;
   const Plan& plan_;
-  bool timing_;
+  Timing timing_;
   int C_ = 0, H_ = 0, W_ = 0;
   size_t N_ = 0;
   bool tgv_ = false, coupled_ = false;
@@ -1004,13 +1023,13 @@ class Solver {
   size_t norm_partials_count_ = 0, value_partials_count_ = 0, conjugate_partials_count_ = 0;
   std::deque<EventPair> primal_events_, dual_events_, launches_;
   // Rows a wavefront of the dual kernel walks down.
-  int dual_rows_ = 16;
+  int dual_rows_ = 32;
 };
 
 }  // namespace
 
 bool solve_on_gpu(const FrameModel& frame, const Settings& settings, const Plan& plan, const StartState* start,
-                  const Observer& observer, bool timing, SolveOutput& out, std::string& message, int32_t& status) {
+                  const Observer& observer, Timing timing, SolveOutput& out, std::string& message, int32_t& status) {
   try {
     int devices = 0;
     if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) {

@@ -38,7 +38,11 @@ struct StartState {
 using Observer = std::function<bool(const Record&, const double* canvas)>;
 
 // How long the kernels of the iterations took, measured with HIP events on the solve's
-// stream where timing is asked for.
+// stream where timing is asked for: Timing::iterations brackets each run of iterations
+// between two records with one pair of events (nothing between the launches);
+// Timing::kernels brackets every iteration and every kernel launch.
+enum class Timing { none = 0, iterations = 1, kernels = 2 };
+
 struct KernelTimes {
   double primal_seconds = 0.0;
   uint64_t primal_launches = 0;
@@ -60,6 +64,6 @@ struct SolveOutput {
 
 // Solves on the current HIP device. false with a message (and a status) on failure.
 bool solve_on_gpu(const FrameModel& frame, const Settings& settings, const Plan& plan, const StartState* start,
-                  const Observer& observer, bool timing, SolveOutput& out, std::string& message, int32_t& status);
+                  const Observer& observer, Timing timing, SolveOutput& out, std::string& message, int32_t& status);
 
 }  // namespace unround

@@ -378,7 +378,8 @@ class Solution:
 
 @dataclass(frozen=True, slots=True)
 class KernelTimes:
-    """HIP-event durations of a solve's kernels (UNROUND_TIMING=1 during the solve)."""
+    """HIP-event durations of a solve's kernels: UNROUND_TIMING=1 during the solve times the
+    runs of iterations between records only, UNROUND_TIMING=2 also every launch."""
 
     primal_seconds: float
     primal_launches: int
