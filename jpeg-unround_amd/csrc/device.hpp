@@ -71,31 +71,42 @@ struct Component {
   double costs[64];
 };
 
-// A component's per-frequency constants and scalars copied into LDS by a workgroup, so that
-// lanes that each take other frequencies read them without a trip to memory.
-struct StagedComponent {
+// A component's per-frequency constants copied into LDS by a work-group, so that lanes
+// that each take other frequencies read them without a trip to memory; Staged reads them
+// there, with the component's scalars, as a Component is read.
+struct StagedArrays {
   double steps[64];
   double weights[64];
   double flat_weights[64];
   double shrink[64];
   double costs[64];
+};
+static_assert(offsetof(Component, costs) == offsetof(Component, steps) + 4 * 64 * sizeof(double));
+static_assert(offsetof(StagedArrays, costs) == offsetof(StagedArrays, steps) + 4 * 64 * sizeof(double));
+
+struct Staged {
+  const double* steps;
+  const double* weights;
+  const double* flat_weights;
+  const double* shrink;
+  const double* costs;
   double slack;
   int mmse, costed;
 };
-static_assert(offsetof(Component, costs) == offsetof(Component, steps) + 4 * 64 * sizeof(double));
-static_assert(offsetof(StagedComponent, costs) == offsetof(StagedComponent, steps) + 4 * 64 * sizeof(double));
 
-// The workgroup's threads copy the constants of K into `into`; a barrier must follow
-// before any thread reads them.
-__device__ __forceinline__ void stage(const Component& K, StagedComponent& into) {
+// The work-group's 256 threads copy the constants of K into `into` (two loads each at
+// most, with no branch); a barrier must follow before any thread reads them.
+__device__ __forceinline__ Staged stage(const Component& K, StagedArrays& into) {
+  static_assert(5 * 64 <= 2 * 256);
   const double* from = K.steps;
   double* to = into.steps;
-  for (int e = threadIdx.x; e < 5 * 64; e += blockDim.x) to[e] = from[e];
-  if (threadIdx.x == 0) {
-    into.slack = K.slack;
-    into.mmse = K.mmse;
-    into.costed = K.costed;
-  }
+  const int e = threadIdx.x;
+  const bool second = e + 256 < 5 * 64;
+  const int other = second ? e + 256 : e;
+  const double a = from[e], b = from[other];
+  to[e] = a;
+  to[other] = second ? b : a;
+  return Staged{into.steps, into.weights, into.flat_weights, into.shrink, into.costs, K.slack, K.mmse, K.costed};
 }
 
 // A pointer into device memory seen as one into the global address space, so that loads
@@ -105,24 +116,28 @@ __device__ __forceinline__ const __attribute__((address_space(1))) X* global_vie
   return (const __attribute__((address_space(1))) X*)(p);
 }
 
-// The ends, centre and file's own ends of coefficient k of block b, in binary64, each
-// step rounded apart as the reference rounds it (cpp/src/model.cpp:44-48, 242-262).
-template <typename Constants>
-__device__ inline void ends_of(const Constants& K, int16_t q, int k, double& lower, double& upper, double& centre,
-                               double& own_lower, double& own_upper) {
+// The ends, centre and file's own ends of coefficient k of level q, in binary64, each
+// step rounded apart as the reference rounds it (cpp/src/model.cpp:44-48, 242-262), from
+// the frequency's step and MMSE shrinkage.
+__device__ __forceinline__ void ends_from(int16_t q, int k, double step, double shrink, double slack, bool mmse,
+                                          double& lower, double& upper, double& centre, double& own_lower,
+                                          double& own_upper) {
 #pragma clang fp contract(off)
   const double level = static_cast<double>(q);
   const double shift = k == 0 ? 1024.0 : 0.0;
-  const double step = K.steps[k];
-  lower = ((level - 0.5) - K.slack) * step + shift;
-  upper = ((level + 0.5) + K.slack) * step + shift;
+  lower = ((level - 0.5) - slack) * step + shift;
+  upper = ((level + 0.5) + slack) * step + shift;
   own_lower = (level - 0.5) * step + shift;
   own_upper = (level + 0.5) * step + shift;
-  centre = level * step + shift;
-  if (K.mmse && k > 0) {
-    const double size = (fabs(level) - K.shrink[k]) * step;
-    centre = q == 0 ? 0.0 : (q < 0 ? -size : size);
-  }
+  const double size = (fabs(level) - shrink) * step;
+  const double shrunk = q == 0 ? 0.0 : (q < 0 ? -size : size);
+  centre = (mmse && k > 0) ? shrunk : level * step + shift;
+}
+
+template <typename Constants>
+__device__ inline void ends_of(const Constants& K, int16_t q, int k, double& lower, double& upper, double& centre,
+                               double& own_lower, double& own_upper) {
+  ends_from(q, k, K.steps[k], K.shrink[k], K.slack, K.mmse != 0, lower, upper, centre, own_lower, own_upper);
 }
 
 template <typename T>
@@ -132,22 +147,28 @@ __device__ __forceinline__ T clamp_to(T value, T low, T high) {
 }
 
 // The proximal map of one coefficient of level q (docs/math.md:457-491;
-// cpp/src/model.cpp:62-75): the quadratic's minimizer, clipped to the interval.
+// cpp/src/model.cpp:62-75): the quadratic's minimizer, clipped to the interval. The
+// frequency's constants are read first and every case is computed and selected, so that
+// a row of coefficients runs without branches.
 template <typename Constants, typename T>
 __device__ __forceinline__ T prox_of(const Constants& K, int16_t q, bool flat, int k, T e, T step) {
+  const double frequency_step = K.steps[k];
+  const double weight_of = (flat ? K.flat_weights : K.weights)[k];
+  const double shrink = K.shrink[k];
+  const double cost = K.costs[k];
   double lower, upper, centre, own_lower, own_upper;
-  ends_of(K, q, k, lower, upper, centre, own_lower, own_upper);
-  const T weight = static_cast<T>(flat ? K.flat_weights[k] : K.weights[k]);
+  ends_from(q, k, frequency_step, shrink, K.slack, K.mmse != 0, lower, upper, centre, own_lower, own_upper);
+  const T weight = static_cast<T>(weight_of);
   const T t = step * weight;
   const T r = T(1) / (T(1) + t);
   const T z = (e + t * static_cast<T>(centre)) * r;
-  if (!K.costed) return clamp_to(z, static_cast<T>(lower), static_cast<T>(upper));
-  const T h = step * static_cast<T>(K.costs[k]) * r;
+  const T h = step * static_cast<T>(cost) * r;
   const T high_own = static_cast<T>(own_upper), low_own = static_cast<T>(own_lower);
   const T above = high_own > z - h ? high_own : z - h;
   const T below = low_own < z + h ? low_own : z + h;
   const T inside = z < low_own ? below : z;
-  return clamp_to(z > high_own ? above : inside, static_cast<T>(lower), static_cast<T>(upper));
+  const T costed = z > high_own ? above : inside;
+  return clamp_to(K.costed ? costed : z, static_cast<T>(lower), static_cast<T>(upper));
 }
 
 // g*(s) of one coefficient, in binary64 (cpp/src/model.cpp:80-109).
@@ -329,6 +350,17 @@ __device__ __forceinline__ T ball_scale(T square, T radius) {
   return norm > radius ? radius / norm : T(1);
 }
 
+// The hardware hands work-group g to XCD g mod 8 (each XCD with its own L2). The order in
+// which a kernel takes its tiles, ordered(g), gives each XCD one contiguous run of them
+// (XCD x the x-th eighth), so that tiles next to each other, which read each other's edge
+// samples, share an L2. A bijection of [0, total).
+__device__ __forceinline__ int ordered(int g, int total) {
+  constexpr int kXcds = 8;
+  const int xcd = g % kXcds, slot = g / kXcds;
+  const int q = total / kXcds, r = total % kXcds;
+  return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
 // ---------------------------------------------------------------------------------------
 // Planes of one kind of field: `count` planes of N samples each from `base`, a sample
 // addressed by its index in its plane.
@@ -341,6 +373,9 @@ constexpr int kRawBufferWord3 = 0x00020000;  // a raw (untyped) buffer resource 
 
 template <typename X>
 struct BufferPlanes {
+  // A byte offset beyond the resource where its planes span at most 2 GiB: a load there
+  // returns zero and a store there is dropped, by the hardware's range check.
+  static constexpr uint32_t kOutside = 0x80000000u;
   __amdgpu_buffer_rsrc_t resource;
   uint32_t plane_bytes;
   __device__ __forceinline__ BufferPlanes(X* base, int count, size_t N)
@@ -348,8 +383,22 @@ struct BufferPlanes {
             const_cast<void*>(static_cast<const void*>(base)), 0,
             static_cast<int>(static_cast<uint32_t>(static_cast<size_t>(count) * N * sizeof(X))), kRawBufferWord3)),
         plane_bytes(static_cast<uint32_t>(N * sizeof(X))) {}
+  // The sample, or zero where `valid` is false (spans of at most 2 GiB), with no branch.
+  __device__ __forceinline__ X load_if(bool valid, int plane, uint32_t index) const {
+    return load_at(plane, valid ? index * static_cast<uint32_t>(sizeof(X)) : kOutside);
+  }
+  // The store where `valid`, dropped elsewhere (spans of at most 2 GiB), with no branch.
+  __device__ __forceinline__ void store_if(bool valid, int plane, uint32_t index, X value) const {
+    store_at(plane, valid ? index * static_cast<uint32_t>(sizeof(X)) : kOutside, value);
+  }
   __device__ __forceinline__ X load(int plane, uint32_t index) const {
-    const int offset = static_cast<int>(index * sizeof(X));
+    return load_at(plane, index * static_cast<uint32_t>(sizeof(X)));
+  }
+  __device__ __forceinline__ void store(int plane, uint32_t index, X value) const {
+    store_at(plane, index * static_cast<uint32_t>(sizeof(X)), value);
+  }
+  __device__ __forceinline__ X load_at(int plane, uint32_t bytes) const {
+    const int offset = static_cast<int>(bytes);
     const int scalar = static_cast<int>(static_cast<uint32_t>(plane) * plane_bytes);
     if constexpr (sizeof(X) == 2) {
       return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b16(resource, offset, scalar, 0));
@@ -359,8 +408,8 @@ struct BufferPlanes {
       return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b64(resource, offset, scalar, 0));
     }
   }
-  __device__ __forceinline__ void store(int plane, uint32_t index, X value) const {
-    const int offset = static_cast<int>(index * sizeof(X));
+  __device__ __forceinline__ void store_at(int plane, uint32_t bytes, X value) const {
+    const int offset = static_cast<int>(bytes);
     const int scalar = static_cast<int>(static_cast<uint32_t>(plane) * plane_bytes);
     if constexpr (sizeof(X) == 2) {
       __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, value), resource, offset, scalar, 0);
@@ -380,6 +429,12 @@ struct WidePlanes {
   __device__ __forceinline__ WidePlanes(X* base_, int, size_t N_) : base(base_), N(N_) {}
   __device__ __forceinline__ X load(int plane, uint32_t index) const { return base[plane * N + index]; }
   __device__ __forceinline__ void store(int plane, uint32_t index, X value) const { base[plane * N + index] = value; }
+  __device__ __forceinline__ X load_if(bool valid, int plane, uint32_t index) const {
+    return valid ? base[plane * N + index] : X{};
+  }
+  __device__ __forceinline__ void store_if(bool valid, int plane, uint32_t index, X value) const {
+    if (valid) base[plane * N + index] = value;
+  }
 };
 
 }  // namespace device

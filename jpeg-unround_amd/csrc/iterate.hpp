@@ -43,20 +43,21 @@ struct LevelRow {
   bool flat;
 };
 __device__ __forceinline__ LevelRow level_row(const Component& K, int ty, int bx, int k, bool inside) {
-  LevelRow row{};
-  if (inside) {
-    const size_t block = static_cast<size_t>(ty) * K.columns + bx;
-    using Words = unsigned int __attribute__((ext_vector_type(4)));
-    const Words words =
-        *reinterpret_cast<const __attribute__((address_space(1))) Words*>(device::global_view(K.levels) + 64 * block + 8 * k);
-    const uint32_t pairs[4] = {words.x, words.y, words.z, words.w};
+  // Beyond the blocks the lane reads block 0 and keeps zeros, so that no branch parts the
+  // loads from the others.
+  const size_t block = static_cast<size_t>(inside ? ty : 0) * K.columns + (inside ? bx : 0);
+  using Words = unsigned int __attribute__((ext_vector_type(4)));
+  const Words words =
+      *reinterpret_cast<const __attribute__((address_space(1))) Words*>(device::global_view(K.levels) + 64 * block + 8 * k);
+  const uint8_t flat = device::global_view(K.flat)[block];
+  const uint32_t pairs[4] = {words.x, words.y, words.z, words.w};
+  LevelRow row;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      row.q[2 * m] = static_cast<int16_t>(pairs[m] & 0xffffu);
-      row.q[2 * m + 1] = static_cast<int16_t>(pairs[m] >> 16);
-    }
-    row.flat = device::global_view(K.flat)[block] != 0;
+  for (int m = 0; m < 4; ++m) {
+    row.q[2 * m] = inside ? static_cast<int16_t>(pairs[m] & 0xffffu) : int16_t{0};
+    row.q[2 * m + 1] = inside ? static_cast<int16_t>(pairs[m] >> 16) : int16_t{0};
   }
+  row.flat = inside && flat != 0;
   return row;
 }
 
@@ -64,78 +65,84 @@ __device__ __forceinline__ LevelRow level_row(const Component& K, int ty, int bx
 // The proximal step of one component whose cells are DV x DH samples (1 or 2 each)
 // (primal_dual.hpp:536-546; frames.cpp:490-548). A wavefront takes 8 rows of blocks of a
 // strip 64 component columns wide, a lane per column; the cell's samples of a lane stay
-// in its registers from the descent to the write-back.
-template <typename T, typename S, int DV, int DH>
+// in its registers from the descent to the write-back. Every load and store is issued by
+// every lane, a lane beyond the tiles addressing outside its resource (zero read, store
+// dropped), so that the loads of all rows are in flight together.
+template <typename T, typename S, int DV, int DH, template <typename> class Planes>
 __global__ __launch_bounds__(256) void prox_kernel(const Component* __restrict__ component, Fields<T, S> F, int c,
                                                    int H, int W, T descent, T step, T rho, T rest, int keep) {
   __shared__ T forward_room[4][8][65];
   __shared__ T inverse_room[4][8][65];
-  __shared__ device::StagedComponent staged;
+  __shared__ device::StagedArrays staged_arrays;
   const Component& K = *component;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int strips = (K.tiles_x + 7) >> 3;
-  const int strip = blockIdx.x % strips;
-  const int ty = (blockIdx.x / strips) * 4 + wave;  // the wave's row of tiles (blocks)
-  const int tx = strip * 8 + (lane >> 3);           // the tile of the lane's column
-  const int n = lane & 7;
-  const bool live = ty < K.tiles_y && tx < K.tiles_x;
-  // In the coefficient phase a lane takes row k of block b of the strip: its levels are
-  // loaded first, with the constants, so that they arrive with the samples.
-  const int b = lane >> 3, k = lane & 7;
-  const int bx = strip * 8 + b;
-  const bool inside = ty < K.rows && bx < K.columns;
-  device::stage(K, staged);
-  const LevelRow own = level_row(K, ty, bx, k, inside);
+  const int g = device::ordered(blockIdx.x, gridDim.x);  // a run of work-groups on each XCD
+  const int strip = g % strips;
+  const int ty = (g / strips) * 4 + wave;  // the wave's row of tiles (blocks)
+  const int b = lane >> 3, n = lane & 7;
+  const int tx = strip * 8 + b;  // the tile of the lane's column
+  const bool live = (ty < K.tiles_y) & (tx < K.tiles_x);
   const size_t N = static_cast<size_t>(H) * W;
-  T* x = F.x + c * N;
-  const S* px = F.p + (2 * c) * N;
-  const S* py = F.p + (2 * c + 1) * N;
+  const Planes<T> x(F.x + c * N, 1, N), xbar(F.xbar + c * N, 1, N), tilde_x(F.tx + c * N, 1, N);
+  const Planes<S> p(F.p + (2 * c) * N, 2, N);  // p_x, p_y
   const int first_column = (8 * tx + n) * DH;  // the lane's first canvas column
-  // p_y of the canvas row above, per column of the cell.
-  T up[DH];
+  const int top = 8 * ty * DV;                 // the wave's first canvas row
+  const auto at = [&](int i, int j) { return static_cast<uint32_t>(i) * static_cast<uint32_t>(W) + j; };
+
+  // The loads: p_y of the canvas row above, and per canvas row x, p_x and p_y at the lane's
+  // columns and p_x of the column on its left (lane 0 only; the others take their left
+  // lane's).
+  T up[DH], here[8][DV][DH], across[8][DV][DH], down[8][DV][DH], edge[8][DV];
 #pragma unroll
-  for (int b = 0; b < DH; ++b) {
-    const int i = 8 * ty * DV - 1;
-    up[b] = (live && i >= 0) ? device::widen(py[static_cast<size_t>(i) * W + first_column + b]) : T(0);
+  for (int bb = 0; bb < DH; ++bb) up[bb] = device::widen(p.load_if(live && top > 0, 1, at(top - 1, first_column + bb)));
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+#pragma unroll
+    for (int a = 0; a < DV; ++a) {
+      const int i = top + m * DV + a;
+#pragma unroll
+      for (int bb = 0; bb < DH; ++bb) {
+        here[m][a][bb] = x.load_if(live, 0, at(i, first_column + bb));
+        across[m][a][bb] = device::widen(p.load_if(live, 0, at(i, first_column + bb)));
+        down[m][a][bb] = device::widen(p.load_if(live, 1, at(i, first_column + bb)));
+      }
+      edge[m][a] = device::widen(p.load_if(live && lane == 0 && first_column > 0, 0, at(i, first_column - 1)));
+    }
   }
-  T vs[8][DV * DH], xs[8][DV * DH], column[8];
+  // In the coefficient phase a lane takes row n of block b of the strip: its levels and the
+  // constants, loaded while the samples are in flight.
+  const bool inside = (ty < K.rows) & (tx < K.columns);
+  const LevelRow own = level_row(K, ty, tx, n, inside);
+  const device::Staged staged = device::stage(K, staged_arrays);
+
+  T vs[8][DV * DH], column[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     T partial[DH];
 #pragma unroll
     for (int a = 0; a < DV; ++a) {
-      const int i = (8 * ty + m) * DV + a;
-      T here[DH], across[DH], down[DH];
+      const int i = top + m * DV + a;
+      // p_x of the column on the left: the lane's own previous column, or the lane on the
+      // left's last one, or beyond the strip, the edge sample.
+      const T shifted = from_left(across[m][a][DH - 1]);
+      const T left = lane == 0 ? edge[m][a] : shifted;
 #pragma unroll
-      for (int b = 0; b < DH; ++b) {
-        const size_t at = static_cast<size_t>(i) * W + first_column + b;
-        here[b] = live ? x[at] : T(0);
-        across[b] = live ? device::widen(px[at]) : T(0);
-        down[b] = live ? device::widen(py[at]) : T(0);
-      }
-      // p_x of the column on the left: the lane's own previous column, or the previous
-      // lane's last one, or beyond the strip, memory.
-      T left = from_left(across[DH - 1]);
-      if (lane == 0 && live && first_column > 0) {
-        left = device::widen(px[static_cast<size_t>(i) * W + first_column - 1]);
-      }
-#pragma unroll
-      for (int b = 0; b < DH; ++b) {
-        const int j = first_column + b;
-        const T before = b == 0 ? left : across[b - 1];
-        T div = backward_down_of(down[b], up[b], i, H);
-        div += backward_across_of(across[b], before, j, W);
-        const T v = here[b] + descent * div;
-        up[b] = down[b];
-        vs[m][a * DH + b] = v;
-        xs[m][a * DH + b] = here[b];
+      for (int bb = 0; bb < DH; ++bb) {
+        const int j = first_column + bb;
+        const T before = bb == 0 ? left : across[m][a][bb - 1];
+        T div = backward_down_of(down[m][a][bb], up[bb], i, H);
+        div += backward_across_of(across[m][a][bb], before, j, W);
+        const T v = here[m][a][bb] + descent * div;
+        up[bb] = down[m][a][bb];
+        vs[m][a * DH + bb] = v;
         // The cell's sum: the rows of a column first, then the columns.
-        partial[b] = a == 0 ? v : partial[b] + v;
+        partial[bb] = a == 0 ? v : partial[bb] + v;
       }
     }
     T sum = partial[0];
 #pragma unroll
-    for (int b = 1; b < DH; ++b) sum += partial[b];
+    for (int bb = 1; bb < DH; ++bb) sum += partial[bb];
     column[m] = DV * DH > 1 ? sum / static_cast<T>(DV * DH) : sum;
   }
   T y[8];
@@ -143,37 +150,33 @@ __global__ __launch_bounds__(256) void prox_kernel(const Component* __restrict__
 #pragma unroll
   for (int k = 0; k < 8; ++k) forward_room[wave][k][lane] = y[k];
   __syncthreads();
-  // A lane now takes row k of block b of the strip.
+  // A lane now takes row n of block b of the strip.
   T row[8], coefficient[8], zeta[8];
 #pragma unroll
-  for (int u = 0; u < 8; ++u) row[u] = forward_room[wave][k][8 * b + u];
+  for (int u = 0; u < 8; ++u) row[u] = forward_room[wave][n][8 * b + u];
   device::forward8(row, coefficient);
-  if (inside) {
-    const size_t block = static_cast<size_t>(ty) * K.columns + bx;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) zeta[u] = device::prox_of(staged, own.q[u], own.flat, 8 * k + u, coefficient[u], step);
-    if (keep) {
-      T* kept = F.zeta[c] + 64 * block + 8 * k;
+  for (int u = 0; u < 8; ++u) {
+    const T z = device::prox_of(staged, own.q[u], own.flat, 8 * n + u, coefficient[u], step);
+    zeta[u] = inside ? z : T(0);
+  }
+  if (keep && inside) {
+    T* kept = F.zeta[c] + 64 * (static_cast<size_t>(ty) * K.columns + tx) + 8 * n;
 #pragma unroll
-      for (int u = 0; u < 8; ++u) kept[u] = zeta[u];
-    }
-  } else {
-#pragma unroll
-    for (int u = 0; u < 8; ++u) zeta[u] = T(0);
+    for (int u = 0; u < 8; ++u) kept[u] = zeta[u];
   }
   T back_row[8];
   device::inverse8(zeta, back_row);
 #pragma unroll
-  for (int v = 0; v < 8; ++v) inverse_room[wave][k][8 * b + v] = back_row[v];
+  for (int v = 0; v < 8; ++v) inverse_room[wave][n][8 * b + v] = back_row[v];
   __syncthreads();
   T back_column[8], back[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) back_column[m] = inverse_room[wave][m][lane];
   device::inverse8(back_column, back);
-  if (!live) return;
-  const bool own_inside = ty < K.rows && tx < K.columns;
-  T* xbar = F.xbar + c * N;
-  T* tilde_x = F.tx + c * N;
+  // The lane's column of tiles is one of the blocks' (a tile beyond them keeps v): the
+  // lane's block in the coefficient phase is the one of its column here (b = lane / 8).
+  const bool own_inside = inside;
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     // x~: the inverse DCT of zeta (one sample to a cell), or v moved by the change of its
@@ -183,12 +186,12 @@ __global__ __launch_bounds__(256) void prox_kernel(const Component* __restrict__
     for (int a = 0; a < DV; ++a) {
 #pragma unroll
       for (int bb = 0; bb < DH; ++bb) {
-        const size_t at = static_cast<size_t>((8 * ty + m) * DV + a) * W + first_column + bb;
-        const T v = vs[m][a * DH + bb], here = xs[m][a * DH + bb];
+        const uint32_t where = at(top + m * DV + a, first_column + bb);
+        const T v = vs[m][a * DH + bb], old = here[m][a][bb];
         const T tilde = own_inside ? (DV * DH > 1 ? v + change : back[m]) : v;
-        xbar[at] = T(2) * tilde - here;
-        x[at] = rho * tilde + rest * here;
-        if (keep) tilde_x[at] = tilde;
+        xbar.store_if(live, 0, where, T(2) * tilde - old);
+        x.store_if(live, 0, where, rho * tilde + rest * old);
+        if (keep) tilde_x.store_if(live, 0, where, tilde);
       }
     }
   }
@@ -201,19 +204,20 @@ __global__ __launch_bounds__(256) void prox_any_cell_kernel(const Component* __r
                                                             int keep) {
   __shared__ T forward_room[4][8][65];
   __shared__ T inverse_room[4][8][65];
-  __shared__ device::StagedComponent staged;
+  __shared__ device::StagedArrays staged_arrays;
   const Component& K = *component;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int strips = (K.tiles_x + 7) >> 3;
-  const int strip = blockIdx.x % strips;
-  const int ty = (blockIdx.x / strips) * 4 + wave;
+  const int g = device::ordered(blockIdx.x, gridDim.x);
+  const int strip = g % strips;
+  const int ty = (g / strips) * 4 + wave;
   const int tx = strip * 8 + (lane >> 3);
   const int n = lane & 7;
   const bool live = ty < K.tiles_y && tx < K.tiles_x;
   const int b = lane >> 3, k = lane & 7;
   const int bx = strip * 8 + b;
   const bool inside = ty < K.rows && bx < K.columns;
-  device::stage(K, staged);
+  const device::Staged staged = device::stage(K, staged_arrays);
   const LevelRow own = level_row(K, ty, bx, k, inside);
   const int dv = K.down, dh = K.across;
   const size_t N = static_cast<size_t>(H) * W;
@@ -366,16 +370,19 @@ struct EdgeGather {
 };
 
 template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes>
-__global__ __launch_bounds__(256) void dual_kernel(Fields<T, S> F, int H, int W, int rows, Steps<T> steps, int keep) {
+__global__ __launch_bounds__(256) void dual_kernel(Fields<T, S> F, int H, int W, int rows, int groups_across,
+                                                   Steps<T> steps, int keep) {
+  // Work-group g takes 4 strips across and `rows` rows down; a run of them on each XCD.
+  const int g = device::ordered(blockIdx.x, gridDim.x);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int strip = blockIdx.x * 4 + wave;
+  const int strip = (g % groups_across) * 4 + wave;
   const int j0 = strip * kStripColumns;
   if (j0 >= W) return;  // the whole wavefront beyond the canvas
   const int j = j0 + lane;
   const bool writes = j < W;
   const bool right = j + 1 < W;
   const uint32_t column = static_cast<uint32_t>(min(j, W - 1));
-  const int first = blockIdx.y * rows;
+  const int first = (g / groups_across) * rows;
   const int end = min(first + rows, H);
   const size_t N = static_cast<size_t>(H) * W;
   const Planes<T> xbar(F.xbar, C, N);

@@ -819,13 +819,14 @@ class Solver {
   template <int C, bool TGV, bool COUPLED>
   void launch_dual(const Steps<T>& steps, int keep) {
     const int strips = (W_ + kStripColumns - 1) / kStripColumns;
-    const dim3 grid((strips + 3) / 4, (H_ + dual_rows_ - 1) / dual_rows_);
+    const int across = (strips + 3) / 4, down = (H_ + dual_rows_ - 1) / dual_rows_;
+    const dim3 grid(across * down);
     if (narrow_spans()) {
       dual_kernel<T, S, C, TGV, COUPLED, device::BufferPlanes>
-          <<<grid, 256, 0, stream_.stream>>>(fields_, H_, W_, dual_rows_, steps, keep);
+          <<<grid, 256, 0, stream_.stream>>>(fields_, H_, W_, dual_rows_, across, steps, keep);
     } else {
       dual_kernel<T, S, C, TGV, COUPLED, device::WidePlanes>
-          <<<grid, 256, 0, stream_.stream>>>(fields_, H_, W_, dual_rows_, steps, keep);
+          <<<grid, 256, 0, stream_.stream>>>(fields_, H_, W_, dual_rows_, across, steps, keep);
     }
   }
 
@@ -844,14 +845,20 @@ class Solver {
       const T step = static_cast<T>(plan_.tau / static_cast<double>(k.cells()));
       if (per_kernel) HIP_CHECK(hipEventRecord(primal_events_[first_event + c].begin, s));
       const int groups = prox_groups(c), live = keep ? 1 : 0;
+      const auto prox = [&](auto kernel) {
+        kernel<<<groups, 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho, rest, live);
+      };
+      // Through buffer resources where a component's planes span at most 2 GiB (x: one of T,
+      // p: two of S), else with 64-bit addresses.
+      const bool buffers = N_ * sizeof(T) <= (size_t{1} << 31) && 2 * N_ * sizeof(S) <= (size_t{1} << 31);
       if (k.down == 1 && k.across == 1) {
-        prox_kernel<T, S, 1, 1><<<groups, 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho, rest, live);
+        buffers ? prox(prox_kernel<T, S, 1, 1, device::BufferPlanes>) : prox(prox_kernel<T, S, 1, 1, device::WidePlanes>);
       } else if (k.down == 2 && k.across == 2) {
-        prox_kernel<T, S, 2, 2><<<groups, 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho, rest, live);
+        buffers ? prox(prox_kernel<T, S, 2, 2, device::BufferPlanes>) : prox(prox_kernel<T, S, 2, 2, device::WidePlanes>);
       } else if (k.down == 1 && k.across == 2) {
-        prox_kernel<T, S, 1, 2><<<groups, 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho, rest, live);
+        buffers ? prox(prox_kernel<T, S, 1, 2, device::BufferPlanes>) : prox(prox_kernel<T, S, 1, 2, device::WidePlanes>);
       } else if (k.down == 2 && k.across == 1) {
-        prox_kernel<T, S, 2, 1><<<groups, 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho, rest, live);
+        buffers ? prox(prox_kernel<T, S, 2, 1, device::BufferPlanes>) : prox(prox_kernel<T, S, 2, 1, device::WidePlanes>);
       } else {
         prox_any_cell_kernel<T, S><<<groups, 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho, rest,
                                                           live);
