@@ -32,6 +32,19 @@ unround_status unround_solve_start(const unround_input* input, const unround_set
                                    const unround_start* start, unround_observer observer, void* user,
                                    unround_result** result, char* message, size_t message_size);
 
+/* unround_solve of `count` inputs together on the GPU, with one settings (so the same
+ * iterations and records for all): results[k] is what unround_solve gives for inputs[k]
+ * (the same iterates, records and stopping iteration), to be freed with unround_result_free
+ * each. Every input's tiles run in the same launches, a run of iterations between two
+ * records is one HIP graph, and the records and stopping tests run on the device. No
+ * observer. Where latencies is not null, latencies[k] is the seconds from the batch's first
+ * iteration to the end of the run of iterations in which input k stopped. On an error no
+ * result is made. No counterpart in the reference: the batch of the reference's
+ * unround_solve calls (rust/capi/include/unround.h:138-140) a caller would otherwise make. */
+unround_status unround_solve_batch(const unround_input* inputs, size_t count, const unround_settings* settings,
+                                   unround_result** results, double* latencies, char* message,
+                                   size_t message_size);
+
 /* The durations of the solve's kernels, where the environment variable UNROUND_TIMING was
  * set during the solve (HIP events on the solve's stream): values[0] the seconds of the
  * primal kernels and [1] their launches, [2] and [3] those of the dual kernels, [4] the

@@ -124,9 +124,10 @@ void assemble(unround_result& result) {
   result.made.assign(C, false);
 }
 
-unround_status reconstruct(const unround_input* input, const unround_settings* settings, const unround_start* start,
-                           unround_observer observer, void* user, unround_result** result, std::string& message) {
-  if (input == nullptr || settings == nullptr || result == nullptr) {
+// The checks of one input and its frame and plan, into `made` and `plan`.
+unround_status frame_of(const unround_input* input, const unround_settings* settings, unround_result& made,
+                        unround::Plan& plan, std::string& message) {
+  if (input == nullptr || settings == nullptr) {
     message = "the input, the settings and the result are not null pointers";
     return UNROUND_ERROR_ARGUMENT;
   }
@@ -163,15 +164,40 @@ unround_status reconstruct(const unround_input* input, const unround_settings* s
     std::memcpy(in.table.data(), given.quant_table, sizeof(uint16_t) * 64);
     inputs.push_back(in);
   }
-  auto made = std::make_unique<unround_result>();
   int32_t status = UNROUND_OK;
-  if (!unround::make_frame(input->height, input->width, input->color_space, inputs, s, made->frame, message, status)) {
-    return status;
+  if (!unround::make_frame(input->height, input->width, input->color_space, inputs, s, made.frame, message, status)) {
+    return static_cast<unround_status>(status);
   }
-  unround::Plan plan;
   const bool solving = s.method == unround::Method::tv || s.method == unround::Method::tgv;
-  if (solving && !unround::make_plan(s, made->frame.channels(), plan, message)) return UNROUND_ERROR_OPTIONS;
-  if (!solving) plan.gamma.assign(made->frame.channels(), 1.0);
+  if (solving && !unround::make_plan(s, made.frame.channels(), plan, message)) return UNROUND_ERROR_OPTIONS;
+  if (!solving) plan.gamma.assign(made.frame.channels(), 1.0);
+  made.settings = s;
+  made.solver = solving;
+  return UNROUND_OK;
+}
+
+// The UNROUND_TIMING environment variable: 1 times the runs of iterations between records,
+// 2 also every launch.
+unround::Timing timing_asked() {
+  const char* timing = std::getenv("UNROUND_TIMING");
+  if (timing == nullptr) return unround::Timing::none;
+  if (std::string(timing) == "2") return unround::Timing::kernels;
+  if (std::string(timing) == "1") return unround::Timing::iterations;
+  return unround::Timing::none;
+}
+
+unround_status reconstruct(const unround_input* input, const unround_settings* settings, const unround_start* start,
+                           unround_observer observer, void* user, unround_result** result, std::string& message) {
+  if (result == nullptr) {
+    message = "the input, the settings and the result are not null pointers";
+    return UNROUND_ERROR_ARGUMENT;
+  }
+  auto made = std::make_unique<unround_result>();
+  unround::Plan plan;
+  const unround_status checked = frame_of(input, settings, *made, plan, message);
+  if (checked != UNROUND_OK) return checked;
+  const unround::Settings& s = settings->settings;
+  int32_t status = UNROUND_OK;
   unround::StartState state;
   if (start != nullptr) {
     if (start->coefficients != nullptr)
@@ -201,20 +227,55 @@ unround_status reconstruct(const unround_input* input, const unround_settings* s
       return observer(user, &raw) != 0;
     };
   }
-  // UNROUND_TIMING=1 times the runs of iterations between records; 2 also every launch.
-  const char* timing = std::getenv("UNROUND_TIMING");
-  const unround::Timing timed = timing == nullptr                ? unround::Timing::none
-                                : std::string(timing) == "2"     ? unround::Timing::kernels
-                                : std::string(timing) == "1"     ? unround::Timing::iterations
-                                                                 : unround::Timing::none;
-  if (!unround::solve_on_gpu(frame, s, plan, start != nullptr ? &state : nullptr, watch, timed, made->out, message,
-                             status)) {
-    return status;
+  if (!unround::solve_on_gpu(frame, s, plan, start != nullptr ? &state : nullptr, watch, timing_asked(), made->out,
+                             message, status)) {
+    return static_cast<unround_status>(status);
   }
-  made->settings = s;
-  made->solver = solving;
   assemble(*made);
   *result = made.release();
+  return UNROUND_OK;
+}
+
+unround_status reconstruct_batch(const unround_input* inputs, size_t count, const unround_settings* settings,
+                                 unround_result** results, double* latencies, std::string& message) {
+  if (results == nullptr || (count > 0 && inputs == nullptr) || settings == nullptr) {
+    message = "the inputs, the settings and the results are not null pointers";
+    return UNROUND_ERROR_ARGUMENT;
+  }
+  std::vector<std::unique_ptr<unround_result>> made(count);
+  std::vector<unround::Plan> plans(count);
+  for (size_t k = 0; k < count; ++k) {
+    made[k] = std::make_unique<unround_result>();
+    const unround_status checked = frame_of(&inputs[k], settings, *made[k], plans[k], message);
+    if (checked != UNROUND_OK) {
+      message = "input " + std::to_string(k) + ": " + message;
+      return checked;
+    }
+  }
+  const unround::Settings& s = settings->settings;
+  const bool solving = s.method == unround::Method::tv || s.method == unround::Method::tgv;
+  if (!solving && count > 0) {
+    message = "a batch iterates: use --method tv or tgv";
+    return UNROUND_ERROR_UNSUPPORTED;
+  }
+  std::vector<const unround::FrameModel*> frames;
+  std::vector<const unround::Plan*> plan_pointers;
+  for (size_t k = 0; k < count; ++k) {
+    frames.push_back(&made[k]->frame);
+    plan_pointers.push_back(&plans[k]);
+  }
+  std::vector<unround::SolveOutput> outs;
+  std::vector<double> seconds;
+  int32_t status = UNROUND_OK;
+  if (count > 0 && !unround::solve_batch_on_gpu(frames, plan_pointers, s, outs, seconds, message, status)) {
+    return static_cast<unround_status>(status);
+  }
+  for (size_t k = 0; k < count; ++k) {
+    made[k]->out = std::move(outs[k]);
+    assemble(*made[k]);
+    if (latencies != nullptr) latencies[k] = seconds[k];
+  }
+  for (size_t k = 0; k < count; ++k) results[k] = made[k].release();
   return UNROUND_OK;
 }
 
@@ -283,6 +344,14 @@ unround_status unround_solve_start(const unround_input* input, const unround_set
                                    unround_result** result, char* message, size_t message_size) {
   return guarded(message, message_size, [&](std::string& text) {
     return reconstruct(input, settings, start, observer, user, result, text);
+  });
+}
+
+unround_status unround_solve_batch(const unround_input* inputs, size_t count, const unround_settings* settings,
+                                   unround_result** results, double* latencies, char* message,
+                                   size_t message_size) {
+  return guarded(message, message_size, [&](std::string& text) {
+    return reconstruct_batch(inputs, count, settings, results, latencies, text);
   });
 }
 

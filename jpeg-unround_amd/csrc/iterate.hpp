@@ -68,16 +68,18 @@ __device__ __forceinline__ LevelRow level_row(const Component& K, int ty, int bx
 // in its registers from the descent to the write-back. Every load and store is issued by
 // every lane, a lane beyond the tiles addressing outside its resource (zero read, store
 // dropped), so that the loads of all rows are in flight together.
+//
+// prox_tile is work-group g's part of one component's step; prox_kernel runs a component
+// (work-groups in a run on each XCD), and the batch's kernel runs tiles of many images.
 template <typename T, typename S, int DV, int DH, template <typename> class Planes>
-__global__ __launch_bounds__(256) void prox_kernel(const Component* __restrict__ component, Fields<T, S> F, int c,
-                                                   int H, int W, T descent, T step, T rho, T rest, int keep) {
+__device__ __forceinline__ void prox_tile(const Component* __restrict__ component, Fields<T, S> F, int c, int H,
+                                          int W, T descent, T step, T rho, T rest, int keep, int g) {
   __shared__ T forward_room[4][8][65];
   __shared__ T inverse_room[4][8][65];
   __shared__ device::StagedArrays staged_arrays;
   const Component& K = *component;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int strips = (K.tiles_x + 7) >> 3;
-  const int g = device::ordered(blockIdx.x, gridDim.x);  // a run of work-groups on each XCD
   const int strip = g % strips;
   const int ty = (g / strips) * 4 + wave;  // the wave's row of tiles (blocks)
   const int b = lane >> 3, n = lane & 7;
@@ -195,6 +197,20 @@ __global__ __launch_bounds__(256) void prox_kernel(const Component* __restrict__
       }
     }
   }
+}
+
+// The wavefronts per SIMD a prox kernel is compiled for: as many as its registers allow
+// without spilling (the scheduler otherwise trades them for latency it does not need).
+template <typename T, int DV, int DH>
+constexpr int kProxWaves =
+    sizeof(T) == 4 ? (DV * DH == 1 ? 5 : 3) : (DV * DH == 1 ? 4 : (DV * DH == 2 ? 2 : 1));
+
+template <typename T, typename S, int DV, int DH, template <typename> class Planes>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kProxWaves<T, DV, DH>))) void prox_kernel(
+    const Component* __restrict__ component, Fields<T, S> F, int c, int H, int W, T descent, T step, T rho, T rest,
+    int keep) {
+  prox_tile<T, S, DV, DH, Planes>(component, F, c, H, W, descent, step, rho, rest, keep,
+                                  device::ordered(blockIdx.x, gridDim.x));
 }
 
 // The same for any cell up to 4 x 4, its samples read again for the write-back.
@@ -369,11 +385,12 @@ struct EdgeGather {
   }
 };
 
+//
+// dual_tile is work-group g's part: 4 strips across and `rows` rows down; dual_kernel runs
+// a frame (work-groups in a run on each XCD), and the batch's kernel runs tiles of many.
 template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes>
-__global__ __launch_bounds__(256) void dual_kernel(Fields<T, S> F, int H, int W, int rows, int groups_across,
-                                                   Steps<T> steps, int keep) {
-  // Work-group g takes 4 strips across and `rows` rows down; a run of them on each XCD.
-  const int g = device::ordered(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void dual_tile(Fields<T, S> F, int H, int W, int rows, int groups_across,
+                                          Steps<T> steps, int keep, int g) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int strip = (g % groups_across) * 4 + wave;
   const int j0 = strip * kStripColumns;
@@ -570,4 +587,11 @@ __global__ __launch_bounds__(256) void dual_kernel(Fields<T, S> F, int H, int W,
     edge_t = edge_t_ahead;
     edge_s = edge_s_ahead;
   }
+}
+
+template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes>
+__global__ __launch_bounds__(256) void dual_kernel(Fields<T, S> F, int H, int W, int rows, int groups_across,
+                                                   Steps<T> steps, int keep) {
+  dual_tile<T, S, C, TGV, COUPLED, Planes>(F, H, W, rows, groups_across, steps, keep,
+                                           device::ordered(blockIdx.x, gridDim.x));
 }

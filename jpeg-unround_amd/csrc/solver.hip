@@ -32,8 +32,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
+#include <optional>
 #include <set>
 #include <stdexcept>
 #include <string>
@@ -231,12 +233,16 @@ __device__ double group_reduce(double value, double* room) {
 // grad x~ - w~), TGV's |E w~|_F, the largest norm of div_2 r~, and the partial gap's sum
 // over the channels of gamma |p~ + div_2 r~|; each work-group's sums and largest into
 // partials[4 group + 0..3].
+//
+// Work-group `group` takes 64 columns and 4 rows, groups numbered across, then down, with
+// `across` groups to a row of them.
 template <typename T, typename S, bool TGV>
-__global__ __launch_bounds__(256) void norms_kernel(Fields<T, S> F, int C, int H, int W, double g0, double g1,
-                                                    double g2, int coupled, double* __restrict__ partials) {
+__device__ __forceinline__ void norms_tile(const Fields<T, S>& F, int C, int H, int W, double g0, double g1,
+                                           double g2, int coupled, double* __restrict__ partials, int across,
+                                           int group) {
   __shared__ double room[256];
-  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int j = (group % across) * 64 + (threadIdx.x & 63);
+  const int i = (group / across) * 4 + (threadIdx.x >> 6);
   const bool here = i < H && j < W;
   const size_t N = static_cast<size_t>(H) * W;
   const double gammas[3] = {g0, g1, g2};
@@ -293,7 +299,6 @@ __global__ __launch_bounds__(256) void norms_kernel(Fields<T, S> F, int C, int H
       if (coupled) most = sqrt(squares);
     }
   }
-  const int group = blockIdx.y * gridDim.x + blockIdx.x;
   const double s_first = group_reduce<false>(first, room);
   const double s_second = group_reduce<false>(second, room);
   const double s_most = group_reduce<true>(most, room);
@@ -306,13 +311,18 @@ __global__ __launch_bounds__(256) void norms_kernel(Fields<T, S> F, int C, int H
   }
 }
 
+template <typename T, typename S, bool TGV>
+__global__ __launch_bounds__(256) void norms_kernel(Fields<T, S> F, int C, int H, int W, double g0, double g1,
+                                                    double g2, int coupled, double* __restrict__ partials) {
+  norms_tile<T, S, TGV>(F, C, H, W, g0, g1, g2, coupled, partials, gridDim.x, blockIdx.y * gridDim.x + blockIdx.x);
+}
+
 // G's terms at the kept coefficients of one component, a work-group's sum each.
 template <typename T>
-__global__ __launch_bounds__(256) void value_kernel(const Component* __restrict__ component, const T* zeta,
-                                                    double* __restrict__ partials) {
+__device__ __forceinline__ void value_tile(const Component& K, const T* zeta, double* __restrict__ partials,
+                                           int group) {
   __shared__ double room[256];
-  const Component& K = *component;
-  const size_t index = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+  const size_t index = static_cast<size_t>(group) * 256 + threadIdx.x;
   const size_t count = static_cast<size_t>(K.rows) * K.columns * 64;
   double term = 0.0;
   if (index < count) {
@@ -321,22 +331,27 @@ __global__ __launch_bounds__(256) void value_kernel(const Component* __restrict_
     term = device::value_of(K, K.levels[index], K.flat[block] != 0, k, static_cast<double>(zeta[index]));
   }
   const double total = group_reduce<false>(term, room);
-  if (threadIdx.x == 0) partials[blockIdx.x] = total;
+  if (threadIdx.x == 0) partials[group] = total;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void value_kernel(const Component* __restrict__ component, const T* zeta,
+                                                    double* __restrict__ partials) {
+  value_tile<T>(*component, zeta, partials, blockIdx.x);
 }
 
 // G_c* at xi = gamma div p~ (from_r = 0) or xi = gamma div(-theta div_2 r~) (from_r = 1),
 // with the free samples' terms (frames.cpp:616-681): a work-group's sum into partials.
 template <typename T, typename S>
-__global__ __launch_bounds__(256) void conjugate_kernel(const Component* __restrict__ component, Fields<T, S> F, int c,
-                                                        int H, int W, double gamma, int from_r,
-                                                        const double* __restrict__ most, double alpha1, double radius,
-                                                        double* __restrict__ partials) {
+__device__ __forceinline__ void conjugate_tile(const Component& K, const Fields<T, S>& F, int c, int H, int W,
+                                               double gamma, int from_r, const double* __restrict__ most,
+                                               double alpha1, double radius, double* __restrict__ partials,
+                                               int group) {
   __shared__ double first[4][64];
   __shared__ double second[4][64];
   __shared__ double room[256];
-  const Component& K = *component;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane >> 3, n = lane & 7;
-  const int tile = blockIdx.x * 4 + wave;
+  const int tile = group * 4 + wave;
   const bool active = tile < K.tiles_y * K.tiles_x;
   const int ty = active ? tile / K.tiles_x : 0, tx = active ? tile % K.tiles_x : 0;
   const bool inside = active && ty < K.rows && tx < K.columns;
@@ -408,15 +423,22 @@ __global__ __launch_bounds__(256) void conjugate_kernel(const Component* __restr
     }
   }
   const double total = group_reduce<false>(term, room);
-  if (threadIdx.x == 0) partials[blockIdx.x] = total;
+  if (threadIdx.x == 0) partials[group] = total;
+}
+
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void conjugate_kernel(const Component* __restrict__ component, Fields<T, S> F, int c,
+                                                        int H, int W, double gamma, int from_r,
+                                                        const double* __restrict__ most, double alpha1, double radius,
+                                                        double* __restrict__ partials) {
+  conjugate_tile<T, S>(*component, F, c, H, W, gamma, from_r, most, alpha1, radius, partials, blockIdx.x);
 }
 
 // Sums (or the largest of) count partials, one work-group per column of a row-major
 // [count][columns] array: each lane takes every 256th in order, then a fixed tree.
-__global__ __launch_bounds__(256) void reduce_kernel(const double* __restrict__ partials, int count, int columns,
-                                                     unsigned largest_mask, double* __restrict__ out) {
+__device__ __forceinline__ void reduce_tile(const double* __restrict__ partials, int count, int columns,
+                                            unsigned largest_mask, double* __restrict__ out, int column) {
   __shared__ double room[256];
-  const int column = blockIdx.x;
   const bool largest = (largest_mask >> column) & 1u;
   double value = 0.0;
   for (int k = threadIdx.x; k < count; k += 256) {
@@ -425,6 +447,11 @@ __global__ __launch_bounds__(256) void reduce_kernel(const double* __restrict__ 
   }
   const double total = largest ? group_reduce<true>(value, room) : group_reduce<false>(value, room);
   if (threadIdx.x == 0) out[column] = total;
+}
+
+__global__ __launch_bounds__(256) void reduce_kernel(const double* __restrict__ partials, int count, int columns,
+                                                     unsigned largest_mask, double* __restrict__ out) {
+  reduce_tile(partials, count, columns, largest_mask, out, blockIdx.x);
 }
 
 // The final projection in binary64: the last canvas's coefficients (of the means of its
@@ -554,58 +581,17 @@ void upload_basis() {
 template <typename T, typename S>
 class Solver {
  public:
-  Solver(const FrameModel& frame, const Settings& settings, const Plan& plan, Timing timing)
-      : frame_(frame), settings_(settings), plan_(plan), timing_(timing) {}
+  // On its own stream, or on `stream` (a batch's) where one is given.
+  Solver(const FrameModel& frame, const Settings& settings, const Plan& plan, Timing timing,
+         hipStream_t stream = nullptr)
+      : frame_(frame), settings_(settings), plan_(plan), timing_(timing), stream_override_(stream) {
+    if (stream_override_ == nullptr) own_stream_.emplace();
+  }
 
   void run(const StartState* start, const Observer& observer, SolveOutput& out) {
     const auto began = std::chrono::steady_clock::now();
-    upload_basis();
-    C_ = static_cast<int>(frame_.channels());
-    H_ = static_cast<int>(frame_.rows);
-    W_ = static_cast<int>(frame_.columns);
-    N_ = static_cast<size_t>(H_) * W_;
-    tgv_ = settings_.method == Method::tgv;
-    const bool solving = settings_.method == Method::tv || settings_.method == Method::tgv;
-    coupled_ = settings_.coupled && C_ > 1;
-    if (C_ != 1 && C_ != 3) throw GpuError("a frame of 1 or 3 channels");
-    dual_rows_ = dual_rows_for(H_, W_);
-    if (const char* rows = std::getenv("UNROUND_DUAL_ROWS")) dual_rows_ = std::max(1, std::atoi(rows));
-    hipStream_t s = stream_.stream;
-    allocate();
-    upload_components();
-    // The start.
-    std::vector<DeviceBuffer> given(C_);
-    for (int c = 0; c < C_; ++c) {
-      const double* coefficients = start != nullptr && static_cast<size_t>(c) < start->coefficients.size()
-                                       ? start->coefficients[c]
-                                       : nullptr;
-      double* device_given = nullptr;
-      if (coefficients != nullptr) {
-        const size_t bytes = sizeof(double) * 64 * frame_.components[c].blocks();
-        given[c] = DeviceBuffer(bytes);
-        HIP_CHECK(hipMemcpyAsync(given[c].pointer, coefficients, bytes, hipMemcpyHostToDevice, s));
-        device_given = given[c].as<double>();
-      }
-      start_kernel<T, S><<<(tiles(c) + 3) / 4, 256, 0, s>>>(component(c), fields_, c, H_, W_, device_given);
-      HIP_CHECK(hipGetLastError());
-    }
-    const dim3 pixels((W_ + 63) / 64, (H_ + 3) / 4);
-    if (solving) {
-      upload_field(start != nullptr ? start->p : nullptr, fields_.p, 2);
-      if (tgv_) {
-        if (start != nullptr && start->w != nullptr) {
-          upload_field(start->w, fields_.w, 2);
-        } else {
-          gradient_kernel<T, S><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_);
-          HIP_CHECK(hipGetLastError());
-        }
-        upload_field(start != nullptr ? start->r : nullptr, fields_.r, 3);
-      }
-      const T radius = static_cast<T>(tgv_ ? settings_.alpha1 : settings_.alpha);
-      project_start_kernel<T, S><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_, tgv_, settings_.coupled ? 1 : 0, radius,
-                                                        static_cast<T>(settings_.alpha0));
-      HIP_CHECK(hipGetLastError());
-    }
+    const bool solving = prepare(start);
+    hipStream_t s = stream();
     out.tau = plan_.tau;
     out.sigma = plan_.sigma;
     uint64_t iteration = 0;
@@ -670,7 +656,64 @@ class Solver {
     finish(out);
   }
 
+  // Everything before the iterations: the device, the fields, the components and the start
+  // (x from the coefficients given or the centres, w = grad x unless given, p and r given
+  // or 0, projected onto their balls). Returns whether the method iterates at all.
+  bool prepare(const StartState* start) {
+    upload_basis();
+    C_ = static_cast<int>(frame_.channels());
+    H_ = static_cast<int>(frame_.rows);
+    W_ = static_cast<int>(frame_.columns);
+    N_ = static_cast<size_t>(H_) * W_;
+    tgv_ = settings_.method == Method::tgv;
+    const bool solving = settings_.method == Method::tv || settings_.method == Method::tgv;
+    coupled_ = settings_.coupled && C_ > 1;
+    if (C_ != 1 && C_ != 3) throw GpuError("a frame of 1 or 3 channels");
+    dual_rows_ = dual_rows_for(H_, W_);
+    if (const char* rows = std::getenv("UNROUND_DUAL_ROWS")) dual_rows_ = std::max(1, std::atoi(rows));
+    hipStream_t s = stream();
+    allocate();
+    upload_components();
+    // The start.
+    std::vector<DeviceBuffer> given(C_);
+    for (int c = 0; c < C_; ++c) {
+      const double* coefficients = start != nullptr && static_cast<size_t>(c) < start->coefficients.size()
+                                       ? start->coefficients[c]
+                                       : nullptr;
+      double* device_given = nullptr;
+      if (coefficients != nullptr) {
+        const size_t bytes = sizeof(double) * 64 * frame_.components[c].blocks();
+        given[c] = DeviceBuffer(bytes);
+        HIP_CHECK(hipMemcpyAsync(given[c].pointer, coefficients, bytes, hipMemcpyHostToDevice, s));
+        device_given = given[c].as<double>();
+      }
+      start_kernel<T, S><<<(tiles(c) + 3) / 4, 256, 0, s>>>(component(c), fields_, c, H_, W_, device_given);
+      HIP_CHECK(hipGetLastError());
+    }
+    const dim3 pixels((W_ + 63) / 64, (H_ + 3) / 4);
+    if (solving) {
+      upload_field(start != nullptr ? start->p : nullptr, fields_.p, 2);
+      if (tgv_) {
+        if (start != nullptr && start->w != nullptr) {
+          upload_field(start->w, fields_.w, 2);
+        } else {
+          gradient_kernel<T, S><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_);
+          HIP_CHECK(hipGetLastError());
+        }
+        upload_field(start != nullptr ? start->r : nullptr, fields_.r, 3);
+      }
+      const T radius = static_cast<T>(tgv_ ? settings_.alpha1 : settings_.alpha);
+      project_start_kernel<T, S><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_, tgv_, settings_.coupled ? 1 : 0, radius,
+                                                        static_cast<T>(settings_.alpha0));
+      HIP_CHECK(hipGetLastError());
+    }
+    return solving;
+  }
+
  private:
+  template <typename, typename>
+  friend class Batch;
+
   int tiles(int c) const {
     const ComponentModel& k = frame_.components[c];
     return static_cast<int>((frame_.rows / (8 * k.down)) * (frame_.columns / (8 * k.across)));
@@ -699,7 +742,7 @@ class Solver {
     const size_t t_bytes = sizeof(T) * (t_count + zeta_count);
     const size_t s_bytes = sizeof(S) * s_count;
     state_ = DeviceBuffer(t_bytes + s_bytes + 256);
-    HIP_CHECK(hipMemsetAsync(state_.pointer, 0, t_bytes + s_bytes + 256, stream_.stream));
+    HIP_CHECK(hipMemsetAsync(state_.pointer, 0, t_bytes + s_bytes + 256, stream()));
     T* at = state_.as<T>();
     const auto take = [&at](size_t n) {
       T* here = at;
@@ -756,9 +799,9 @@ class Solver {
       levels_.emplace_back(sizeof(int16_t) * std::max<size_t>(1, k.levels.size()));
       flats_.emplace_back(std::max<size_t>(1, k.flat.size()));
       HIP_CHECK(hipMemcpyAsync(levels_.back().pointer, k.levels.data(), sizeof(int16_t) * k.levels.size(),
-                               hipMemcpyHostToDevice, stream_.stream));
+                               hipMemcpyHostToDevice, stream()));
       HIP_CHECK(hipMemcpyAsync(flats_.back().pointer, k.flat.data(), k.flat.size(), hipMemcpyHostToDevice,
-                               stream_.stream));
+                               stream()));
       Component& d = host[c];
       d.levels = levels_.back().as<int16_t>();
       d.flat = flats_.back().as<uint8_t>();
@@ -782,8 +825,8 @@ class Solver {
     }
     components_ = DeviceBuffer(sizeof(Component) * C_);
     HIP_CHECK(hipMemcpyAsync(components_.pointer, host.data(), sizeof(Component) * C_, hipMemcpyHostToDevice,
-                             stream_.stream));
-    HIP_CHECK(hipStreamSynchronize(stream_.stream));
+                             stream()));
+    HIP_CHECK(hipStreamSynchronize(stream()));
   }
 
   // A field of the start, converted from binary64, or left at 0.
@@ -792,11 +835,11 @@ class Solver {
     if (given == nullptr) return;
     const size_t count = static_cast<size_t>(entries) * C_ * N_;
     DeviceBuffer staging(sizeof(double) * count);
-    HIP_CHECK(hipMemcpyAsync(staging.pointer, given, sizeof(double) * count, hipMemcpyHostToDevice, stream_.stream));
-    convert_kernel<F><<<static_cast<unsigned>((count + 255) / 256), 256, 0, stream_.stream>>>(staging.as<double>(),
+    HIP_CHECK(hipMemcpyAsync(staging.pointer, given, sizeof(double) * count, hipMemcpyHostToDevice, stream()));
+    convert_kernel<F><<<static_cast<unsigned>((count + 255) / 256), 256, 0, stream()>>>(staging.as<double>(),
                                                                                              field, count);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipStreamSynchronize(stream_.stream));
+    HIP_CHECK(hipStreamSynchronize(stream()));
   }
 
   // Rows a wavefront of the dual kernel walks down: long enough to spread its first rows'
@@ -823,15 +866,15 @@ class Solver {
     const dim3 grid(across * down);
     if (narrow_spans()) {
       dual_kernel<T, S, C, TGV, COUPLED, device::BufferPlanes>
-          <<<grid, 256, 0, stream_.stream>>>(fields_, H_, W_, dual_rows_, across, steps, keep);
+          <<<grid, 256, 0, stream()>>>(fields_, H_, W_, dual_rows_, across, steps, keep);
     } else {
       dual_kernel<T, S, C, TGV, COUPLED, device::WidePlanes>
-          <<<grid, 256, 0, stream_.stream>>>(fields_, H_, W_, dual_rows_, across, steps, keep);
+          <<<grid, 256, 0, stream()>>>(fields_, H_, W_, dual_rows_, across, steps, keep);
     }
   }
 
   void iterate(bool keep) {
-    hipStream_t s = stream_.stream;
+    hipStream_t s = stream();
     const T rho = static_cast<T>(plan_.relaxation), rest = static_cast<T>(1.0 - plan_.relaxation);
     const bool per_kernel = timing_ == Timing::kernels;
     if (per_kernel) {
@@ -893,7 +936,7 @@ class Solver {
 
   // The durations of the launches since the last collection, then their events freed.
   void collect_kernel_times(KernelTimes& times) {
-    HIP_CHECK(hipStreamSynchronize(stream_.stream));
+    HIP_CHECK(hipStreamSynchronize(stream()));
     const auto elapsed = [](const EventPair& pair) {
       float ms = 0.0f;
       HIP_CHECK(hipEventElapsedTime(&ms, pair.begin, pair.end));
@@ -918,7 +961,7 @@ class Solver {
 
   // The record of the tilde point (primal_dual.hpp:446-501).
   Record record(uint64_t iteration) {
-    hipStream_t s = stream_.stream;
+    hipStream_t s = stream();
     double* partials = partials_.as<double>();
     double* norm_partials = partials;
     double* value_partials = norm_partials + 4 * norm_partials_count_;
@@ -985,13 +1028,13 @@ class Solver {
   void tilde_canvas(std::vector<double>& canvas) {
     std::vector<T> values(static_cast<size_t>(C_) * N_);
     HIP_CHECK(hipMemcpyAsync(values.data(), fields_.tx, sizeof(T) * values.size(), hipMemcpyDeviceToHost,
-                             stream_.stream));
-    HIP_CHECK(hipStreamSynchronize(stream_.stream));
+                             stream()));
+    HIP_CHECK(hipStreamSynchronize(stream()));
     canvas.assign(values.begin(), values.end());
   }
 
   void finish(SolveOutput& out) {
-    hipStream_t s = stream_.stream;
+    hipStream_t s = stream();
     DeviceBuffer canvas(sizeof(double) * C_ * N_);
     std::vector<DeviceBuffer> coefficients;
     // Narrowed by 2^-30 of a unit: beyond what any binary64 block DCT of the result rounds
@@ -1023,7 +1066,9 @@ class Solver {
   int C_ = 0, H_ = 0, W_ = 0;
   size_t N_ = 0;
   bool tgv_ = false, coupled_ = false;
-  StreamHolder stream_;
+  hipStream_t stream_override_ = nullptr;
+  std::optional<StreamHolder> own_stream_;
+  hipStream_t stream() const { return stream_override_ != nullptr ? stream_override_ : own_stream_->stream; }
   DeviceBuffer state_, components_, partials_, record_out_;
   std::vector<DeviceBuffer> levels_, flats_;
   Fields<T, S> fields_{};
@@ -1032,6 +1077,8 @@ class Solver {
   // Rows a wavefront of the dual kernel walks down.
   int dual_rows_ = 32;
 };
+
+#include "batch.hpp"
 
 }  // namespace
 
@@ -1053,6 +1100,34 @@ bool solve_on_gpu(const FrameModel& frame, const Settings& settings, const Plan&
     } else {
       Solver<float, float> solver(frame, settings, plan, timing);
       solver.run(start, observer, out);
+    }
+    return true;
+  } catch (const GpuError& error) {
+    status = 6;
+    message = std::string("a GPU error: ") + error.what();
+    return false;
+  }
+}
+
+bool solve_batch_on_gpu(const std::vector<const FrameModel*>& frames, const std::vector<const Plan*>& plans,
+                        const Settings& settings, std::vector<SolveOutput>& outs, std::vector<double>& latencies,
+                        std::string& message, int32_t& status) {
+  try {
+    int devices = 0;
+    if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) {
+      status = 6;
+      message = "no HIP device: this library computes on an AMD GPU";
+      return false;
+    }
+    if (settings.precision == Precision::dual) {
+      Batch<double, double> batch(frames, plans, settings);
+      batch.run(outs, latencies);
+    } else if (settings.precision == Precision::mixed) {
+      Batch<float, bf16> batch(frames, plans, settings);
+      batch.run(outs, latencies);
+    } else {
+      Batch<float, float> batch(frames, plans, settings);
+      batch.run(outs, latencies);
     }
     return true;
   } catch (const GpuError& error) {

@@ -66,4 +66,12 @@ struct SolveOutput {
 bool solve_on_gpu(const FrameModel& frame, const Settings& settings, const Plan& plan, const StartState* start,
                   const Observer& observer, Timing timing, SolveOutput& out, std::string& message, int32_t& status);
 
+// Frames solved together on the GPU, with the same settings (and so the same iterations and
+// records) and each its own plan: the outputs are each frame's own solve's (its iterates,
+// records and stopping iteration), and latencies[m] the seconds from the first iteration
+// of the batch to the end of the run of iterations in which frame m stopped. No observer.
+bool solve_batch_on_gpu(const std::vector<const FrameModel*>& frames, const std::vector<const Plan*>& plans,
+                        const Settings& settings, std::vector<SolveOutput>& outs, std::vector<double>& latencies,
+                        std::string& message, int32_t& status);
+
 }  // namespace unround

@@ -17,7 +17,7 @@ import functools
 import warnings
 from collections.abc import Callable
 from dataclasses import dataclass
-from typing import Final
+from typing import Final, Sequence
 
 import numpy as np
 import numpy.typing as npt
@@ -165,6 +165,8 @@ def _declare_all(library: ctypes.CDLL) -> None:
     _declare(library, "unround_decode", i32, ctypes.c_char_p, size, handle, *made)
     _declare(library, "unround_solve", i32, ctypes.POINTER(_Input), handle, *made)
     _declare(library, "unround_solve_start", i32, ctypes.POINTER(_Input), handle, ctypes.POINTER(_Start), *made)
+    _declare(library, "unround_solve_batch", i32, ctypes.POINTER(_Input), size, handle, ctypes.POINTER(handle), doubles,
+             *message)
     _declare(library, "unround_result_free", None, handle)
     _declare(library, "unround_result_picture", doubles, handle, u64_out, u64_out, u64_out)
     _declare(library, "unround_result_planes", doubles, handle, u64_out)
@@ -604,6 +606,42 @@ def solve(image: jpegio.Image, settings: Settings | list[str] | None = None, *, 
     decoded = _reconstruct(call, observer)
     del kept
     return decoded
+
+
+def solve_batch(images: Sequence[jpegio.Image], settings: Settings | list[str] | None = None
+                ) -> tuple[list[Decoded], list[float]]:
+    """Reconstructs many images together on the GPU, this edition's: each result is what
+    solve() gives for its image (the same iterates, records and stopping iteration), with
+    the tiles of every image in the same launches and the stopping tests on the device. Also
+    returns each image's latency: the seconds from the batch's first iteration to the end of
+    the run of iterations in which it stopped. No observer."""
+    library = _library()
+    handle = _Settings(settings if isinstance(settings, list) else options(settings))
+    count = len(images)
+    inputs = (_Input * max(count, 1))()
+    kept: list = []
+    for index, image in enumerate(images):
+        given, levels, raw = _input_of(image)
+        inputs[index] = given
+        kept += [levels, raw]
+    results = (ctypes.c_void_p * max(count, 1))()
+    latencies = np.zeros(max(count, 1))
+    message = ctypes.create_string_buffer(_MESSAGE_SIZE)
+    status = int(library.unround_solve_batch(inputs, count, handle.handle, results,
+                                             latencies.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), message,
+                                             _MESSAGE_SIZE))
+    if status != Status.OK:
+        known = status in Status.__members__.values()
+        raise UnroundError(Status(status) if known else Status.INTERNAL, message.value.decode("utf-8", "replace"))
+    decoded: list[Decoded] = []
+    try:
+        for index in range(count):
+            decoded.append(_decoded(library, ctypes.c_void_p(results[index])))
+    finally:
+        for index in range(count):
+            library.unround_result_free(ctypes.c_void_p(results[index]))
+    del kept
+    return decoded, [float(value) for value in latencies[:count]]
 
 
 def to_rgb(planes: npt.ArrayLike) -> Array:

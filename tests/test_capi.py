@@ -108,6 +108,25 @@ def test_null_arguments_and_unsupported_paths_are_statuses():
     assert status == native.Status.ARGUMENT and len(short.value) == 7
 
 
+def test_batch_refusals_come_before_the_gpu():
+    library = native._library()
+    message = ctypes.create_string_buffer(256)
+    results = (ctypes.c_void_p * 2)()
+    status = library.unround_solve_batch(None, 2, None, results, None, message, 256)
+    assert status == native.Status.ARGUMENT and b"null" in message.value
+    # An empty batch is done at once; a bad input is named by its index.
+    assert native.solve_batch([], ["--method", "tgv"]) == ([], [])
+    grey = unround.jpegio.Component(np.zeros((2, 2, 8, 8), np.int16), np.ones(64, np.uint16))
+    good = unround.jpegio.Image(16, 16, unround.jpegio.ColorSpace.GRAYSCALE, (grey,))
+    bad = unround.jpegio.Image(17, 16, unround.jpegio.ColorSpace.GRAYSCALE, (grey,))
+    with pytest.raises(native.UnroundError) as caught:
+        native.solve_batch([good, bad], ["--method", "tgv"])
+    assert caught.value.status == native.Status.UNSUPPORTED and caught.value.message.startswith("input 1: ")
+    with pytest.raises(native.UnroundError) as caught:
+        native.solve_batch([good], ["--method", "mmse"])
+    assert caught.value.status == native.Status.UNSUPPORTED
+
+
 def test_refused_frames():
     grey = unround.jpegio.Component(np.zeros((2, 2, 8, 8), np.int16), np.ones(64, np.uint16))
     # Blocks that do not cover the picture, a colour space with the wrong count of
