@@ -130,6 +130,123 @@ def measured_traffic(kernel: str, workload_name: str, precision: str) -> float |
     return None
 
 
+def batch_workload(count: int = 64):
+    """The mixed batch: image k (seeds 1..count) of height and width uniform on the multiples of
+    8 in [96, 1920], quality uniform in [30, 90], grey or 4:2:0 colour, each drawn from its
+    own seed; with the uncompressed pictures for PSNR."""
+    import numpy as np
+
+    from unround import synthetic
+
+    images, originals, kinds = [], [], []
+    for seed in range(1, count + 1):
+        draw = np.random.default_rng(1000 + seed)
+        height, width = (int(v) * 8 for v in draw.integers(12, 241, size=2))
+        quality = int(draw.integers(30, 91))
+        if draw.random() < 0.5:
+            picture = synthetic.grey_picture(height, width, seed)
+            images.append(synthetic.encode_grey(picture, quality))
+            kinds.append("grey")
+        else:
+            picture = synthetic.colour_picture(height, width, seed)
+            images.append(synthetic.encode_colour420(picture, quality))
+            kinds.append("colour")
+        originals.append(picture)
+    return images, originals, kinds
+
+
+def psnr_of(decoded, original) -> float:
+    import numpy as np
+
+    from unround import native
+
+    planes = decoded.planes
+    ours = planes[0] if planes.shape[0] == 1 else native.to_rgb(planes)
+    error = np.mean((np.clip(np.round(ours), 0, 255) - original.astype(np.float64)) ** 2)
+    return float(10 * np.log10(255.0 ** 2 / max(error, 1e-30)))
+
+
+def batch_main(args) -> None:
+    """C3: the mixed batch, solved together (unround_solve_batch) and one image at a time
+    (unround_solve), TGV to a relative gap of 1e-4 checked every 10 iterations."""
+    import numpy as np
+    import torch
+
+    from unround import native
+
+    images, originals, kinds = batch_workload()
+    options = ["--method", "tgv", "--tolerance", "0", "--relative-tolerance", "1e-4", "--record-every", "10",
+               "--iterations", str(args.steps), "--precision", args.precision]
+    log(f"C3: {len(images)} images, {sum(k == 'colour' for k in kinds)} 4:2:0 colour; options {' '.join(options)}")
+    # Warm-up: the library, the kernels and the batch's choice of rows.
+    native.solve_batch(images[:4], options)
+    native.solve_batch(images, options[:-4] + ["--iterations", "20", "--precision", args.precision])
+    torch.cuda.synchronize()
+    began = time.perf_counter()
+    together, latencies = native.solve_batch(images, options)
+    torch.cuda.synchronize()
+    batch_seconds = time.perf_counter() - began
+    began = time.perf_counter()
+    alone = [native.solve(image, options) for image in images]
+    torch.cuda.synchronize()
+    plain_seconds = time.perf_counter() - began
+    same = all(np.array_equal(a.canvas, b.canvas) for a, b in zip(together, alone))
+    reference = None
+    if args.precision != "double":
+        reference, _ = native.solve_batch(images, options[:-2] + ["--precision", "double"])
+    psnrs = [psnr_of(d, o) for d, o in zip(together, originals)]
+    deltas = ([abs(p - psnr_of(r, o)) for p, r, o in zip(psnrs, reference, originals)]
+              if reference is not None else [0.0])
+    iterations = [d.result.iterations for d in together]
+    channel_samples = [int(np.prod(d.canvas.shape)) for d in together]
+    per_sample = 2 * field_bytes("tgv", args.precision)
+    moved = sum(n * k * per_sample for n, k in zip(channel_samples, iterations))
+    log(f"batched {batch_seconds:.3f} s, one at a time {plain_seconds:.3f} s; identical canvases {same}; "
+        f"iterations {min(iterations)}..{max(iterations)} (mean {np.mean(iterations):.0f}); "
+        f"PSNR {min(psnrs):.2f}..{max(psnrs):.2f} dB, largest |dPSNR| vs binary64 {max(deltas):.4f} dB")
+    value = len(images) / batch_seconds
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "images/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": round(batch_seconds / len(images) * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"single": "f32", "mixed": "f32+bf16", "double": "f64"}[args.precision],
+        "data": "synthetic",
+        "config": {
+            "workload": "C3: 64 images, sides U{96..1920 step 8}, q U[30,90], grey or 4:2:0, seeds 1..64, TGV to "
+                        "relative gap 1e-4 (every 10 iterations)",
+            "images": len(images),
+            "colour_images": sum(k == "colour" for k in kinds),
+            "samples": int(sum(channel_samples)),
+            "iteration_cap": args.steps,
+            "precision": args.precision,
+        },
+        "plain_images_per_s": round(len(images) / plain_seconds, 3),
+        "speedup_vs_plain": round(plain_seconds / batch_seconds, 3),
+        "latency_p50_s": round(float(np.percentile(latencies, 50)), 4),
+        "latency_p99_s": round(float(np.percentile(latencies, 99)), 4),
+        "identical_to_plain": bool(same),
+        "psnr_delta_vs_f64_max_db": round(max(deltas), 5),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "the batch's iterations (prox and dual launches over all images), whole solve",
+            "achieved": round(moved / batch_seconds / 1e9, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(moved / batch_seconds / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main() -> None:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -142,6 +259,10 @@ def main() -> None:
     if args.gpus != 1:
         raise SystemExit("this benchmark runs on one GPU: the path shards by picture across GPUs with no "
                          "communication, and no figure on several GPUs is measured here (--gpus 1)")
+
+    if args.config == "c3":
+        batch_main(args)
+        return
 
     import numpy as np
     import torch
@@ -221,7 +342,42 @@ def main() -> None:
         },
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(method),
     }
+    if args.config == "c4" and args.precision != "double":
+        line["accuracy"] = accuracy_against_binary64(image, decoded, base, args.steps)
     print(json.dumps(line), flush=True)
+
+
+def accuracy_against_binary64(image, decoded, base: list[str], steps: int) -> dict:
+    """The timed solve's result against the same solve in binary64 on the GPU: the largest
+    difference of the canvases, of PSNR against the uncompressed picture, and whether every
+    block of both requantizes to the file's levels."""
+    import numpy as np
+
+    from unround import native, synthetic
+
+    options = [value for value in base]
+    options[options.index("--precision") + 1] = "double"
+    reference = native.solve(image, options + ["--iterations", str(steps), "--record-every", "0"])
+    original = synthetic.grey_picture(int(image.height), int(image.width), 0)
+    h, w = int(image.height), int(image.width)
+
+    def psnr(planes):
+        error = np.mean((np.clip(np.round(planes[0, :h, :w]), 0, 255) - original) ** 2)
+        return float(10 * np.log10(255.0**2 / error))
+
+    def exact(result):
+        return bool(np.array_equal(synthetic.requantized(result.canvas[0], image, 0), image.components[0].coefficients))
+
+    return {
+        "reference": "the same solve in binary64 on the GPU",
+        "iterations": steps,
+        "max_abs_canvas": float(np.max(np.abs(decoded.canvas - reference.canvas))),
+        "psnr": round(psnr(decoded.planes), 5),
+        "psnr_binary64": round(psnr(reference.planes), 5),
+        "psnr_delta_db": round(psnr(decoded.planes) - psnr(reference.planes), 5),
+        "requantizes_exactly": exact(decoded),
+        "requantizes_exactly_binary64": exact(reference),
+    }
 
 
 if __name__ == "__main__":

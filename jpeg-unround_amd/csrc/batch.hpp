@@ -364,6 +364,10 @@ class Batch {
         best = rows;
         best_ms = ms;
       }
+      if (std::getenv("UNROUND_BATCH_LOG") != nullptr) {
+        std::fprintf(stderr, "batch of %zu: dual step with %d rows a wavefront, %.3f ms\n", solvers_.size(), rows,
+                     ms / 3.0f);
+      }
     }
     std::lock_guard<std::mutex> hold(choices.lock);
     choices.chosen[shape] = best;

@@ -116,6 +116,19 @@ def _levels(plane: np.ndarray, table: np.ndarray) -> np.ndarray:
     return np.rint(coefficients / table.reshape(8, 8).astype(np.float64)).astype(np.int16)
 
 
+def requantized(canvas: np.ndarray, image: jpegio.Image, index: int, cells: tuple[int, int] = (1, 1)) -> np.ndarray:
+    """The levels that a reconstruction's canvas plane (H, W) of component `index` quantizes
+    to: its cells' means, block DCT less the level shift, over the steps, rounded to the
+    nearest integer; (rows, columns, 8, 8) like the component's own levels."""
+    component = image.components[index]
+    rows, columns = component.coefficients.shape[:2]
+    down, across = cells
+    plane = np.asarray(canvas, dtype=np.float64)[: rows * 8 * down, : columns * 8 * across]
+    if down * across > 1:
+        plane = plane.reshape(rows * 8, down, columns * 8, across).mean(axis=(1, 3))
+    return _levels(plane, np.asarray(component.quant_table))
+
+
 def encode_grey(picture: np.ndarray, quality: int) -> jpegio.Image:
     """A greyscale picture quantized at a quality with the scaled luminance table."""
     height, width = picture.shape
