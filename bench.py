@@ -247,6 +247,72 @@ def batch_main(args) -> None:
     print(json.dumps(line), flush=True)
 
 
+def unrolled_main(args) -> None:
+    """C5: 16 grey 512x512 pictures at quality 30, 100 unrolled binary32 TGV iterations, the
+    state kept every 10; loss = the mean squared error to the uncompressed pictures; the
+    gradients of alpha0, alpha1 and a 512x512 weight map through the library's adjoint."""
+    import numpy as np
+    import torch
+
+    from unround import synthetic
+    from unround.unrolled import UnrolledTGV
+
+    count, size, iterations, every = 16, 512, 100, 10
+    originals = [synthetic.grey_picture(size, size, seed) for seed in range(1, count + 1)]
+    images = [synthetic.encode_grey(picture, 30) for picture in originals]
+    solve = UnrolledTGV(images, ["--method", "tgv"], iterations=iterations, checkpoint_every=every)
+    target = torch.tensor(np.stack(originals), dtype=torch.float32, device="cuda")
+    weight_map = torch.ones((size, size), dtype=torch.float32, device="cuda", requires_grad=True)
+    alpha0 = torch.tensor(2.0, dtype=torch.float64, requires_grad=True)
+    alpha1 = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+
+    def once():
+        canvas = solve(alpha0, alpha1, weight_map)
+        loss = ((canvas - target) ** 2).mean()
+        torch.cuda.synchronize()
+        middle = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        return float(loss), middle
+
+    once()  # warm-up
+    alpha0.grad = alpha1.grad = weight_map.grad = None
+    torch.cuda.synchronize()
+    began = time.perf_counter()
+    loss, middle = once()
+    ended = time.perf_counter()
+    forward, backward = middle - began, ended - middle
+    samples = count * size * size
+    log(f"C5: loss {loss:.4f}; forward {forward * 1e3:.1f} ms, backward {backward * 1e3:.1f} ms; "
+        f"dL/dalpha0 {float(alpha0.grad):.6g}, dL/dalpha1 {float(alpha1.grad):.6g}, "
+        f"|dL/dm| {float(weight_map.grad.norm()):.6g}")
+    line = {
+        "metric": METRIC,
+        "value": round(samples * iterations / (forward + backward) / 1e6, 2),
+        "unit": "Mpixel-iters/s (forward + backward)",
+        "n_gpus": 1,
+        "steps": iterations,
+        "warmup": 1,
+        "ms_per_step": round((forward + backward) / iterations * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "C5: 16 x 512x512 grey q30, 100 unrolled TGV iterations, checkpoint every 10",
+                   "pictures": count, "size": size, "iterations": iterations, "checkpoint_every": every},
+        "forward_ms": round(forward * 1e3, 2),
+        "backward_ms": round(backward * 1e3, 2),
+        "loss": round(loss, 6),
+        "grad_alpha0": float(alpha0.grad),
+        "grad_alpha1": float(alpha1.grad),
+        "grad_map_norm": float(weight_map.grad.norm()),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main() -> None:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -262,6 +328,9 @@ def main() -> None:
 
     if args.config == "c3":
         batch_main(args)
+        return
+    if args.config == "c5":
+        unrolled_main(args)
         return
 
     import numpy as np

@@ -45,6 +45,41 @@ unround_status unround_solve_batch(const unround_input* inputs, size_t count, co
                                    unround_result** results, double* latencies, char* message,
                                    size_t message_size);
 
+/* Unrolled TGV with its adjoint, for gradients with respect to alpha0, alpha1 and a map
+ * m(i, j) weighting the first-order term per pixel: p's ball at pixel (i, j) has radius
+ * alpha1 m(i, j) (the reference has one radius per channel, docs/math.md:544-566). Grey
+ * inputs of one size whose blocks cover them, no slack cost, --method tgv; binary32; the
+ * settings give everything but the alphas. No counterpart in the reference: for training
+ * the method's weights through a loss on its result. */
+typedef struct unround_unrolled unround_unrolled;
+
+/* A handle for `count` inputs, `iterations` iterations, the state kept every
+ * checkpoint_every iterations for the backward (memory: (iterations / checkpoint_every + 1
+ * + checkpoint_every) x 8 planes per input). */
+unround_status unround_unrolled_new(const unround_input* inputs, size_t count, const unround_settings* settings,
+                                    uint32_t iterations, uint32_t checkpoint_every, unround_unrolled** made,
+                                    char* message, size_t message_size);
+void unround_unrolled_free(unround_unrolled* handle);
+unround_status unround_unrolled_shape(const unround_unrolled* handle, uint64_t* count, uint64_t* height,
+                                      uint64_t* width);
+/* tau, sigma, the channel's gamma and rho. */
+unround_status unround_unrolled_constants(const unround_unrolled* handle, double* values);
+/* The start state, [count][x, w_x, w_y, p_x, p_y, r_xx, r_yy, r_xy][height][width], into host
+ * memory. */
+unround_status unround_unrolled_start(unround_unrolled* handle, double* state, char* message, size_t message_size);
+/* Input `frame`'s problem, [blocks][64][lower, upper, centre, weight], into host memory. */
+unround_status unround_unrolled_problem(const unround_unrolled* handle, uint64_t frame, double* values);
+/* The forward from the start: weight_map [height][width] and canvas [count][height][width]
+ * are device memory (floats); the canvas gets the last x. `stream` is the hipStream_t the
+ * call is ordered after and before (null: the default stream). */
+unround_status unround_unrolled_forward(unround_unrolled* handle, const float* weight_map, double alpha0,
+                                        double alpha1, float* canvas, void* stream, char* message,
+                                        size_t message_size);
+/* The backward of the last forward: grad_canvas [count][height][width] (device) to grad_map
+ * [height][width] (device) and grad_alphas = {dL/d alpha0, dL/d alpha1} (host). */
+unround_status unround_unrolled_backward(unround_unrolled* handle, const float* grad_canvas, float* grad_map,
+                                         double* grad_alphas, void* stream, char* message, size_t message_size);
+
 /* The durations of the solve's kernels, where the environment variable UNROUND_TIMING was
  * set during the solve (HIP events on the solve's stream): values[0] the seconds of the
  * primal kernels and [1] their launches, [2] and [3] those of the dual kernels, [4] the

@@ -40,6 +40,14 @@ struct unround_result {
   std::string warning;
 };
 
+// The unrolled solve's handle: its frames (with their settings) and plans, which the solve
+// reads, and the solve.
+struct unround_unrolled {
+  std::vector<std::unique_ptr<unround_result>> frames;
+  std::vector<unround::Plan> plans;
+  std::unique_ptr<unround::UnrolledSolve> solve;
+};
+
 namespace {
 
 const char* const kVersion = "unround 0.1.0 (MI355X, HIP gfx950, C interface)";
@@ -352,6 +360,113 @@ unround_status unround_solve_batch(const unround_input* inputs, size_t count, co
                                    size_t message_size) {
   return guarded(message, message_size, [&](std::string& text) {
     return reconstruct_batch(inputs, count, settings, results, latencies, text);
+  });
+}
+
+unround_status unround_unrolled_new(const unround_input* inputs, size_t count, const unround_settings* settings,
+                                    uint32_t iterations, uint32_t checkpoint_every, unround_unrolled** made,
+                                    char* message, size_t message_size) {
+  return guarded(message, message_size, [&](std::string& text) -> unround_status {
+    if (made == nullptr || settings == nullptr || inputs == nullptr || count == 0) {
+      text = "the inputs, the settings and the handle are not null pointers, and there is an input";
+      return UNROUND_ERROR_ARGUMENT;
+    }
+    *made = nullptr;
+    auto handle = std::make_unique<unround_unrolled>();
+    handle->plans.resize(count);
+    std::vector<const unround::FrameModel*> frames;
+    std::vector<const unround::Plan*> plans;
+    for (size_t k = 0; k < count; ++k) {
+      handle->frames.push_back(std::make_unique<unround_result>());
+      const unround_status checked = frame_of(&inputs[k], settings, *handle->frames[k], handle->plans[k], text);
+      if (checked != UNROUND_OK) {
+        text = "input " + std::to_string(k) + ": " + text;
+        return checked;
+      }
+      frames.push_back(&handle->frames[k]->frame);
+      plans.push_back(&handle->plans[k]);
+    }
+    int32_t status = UNROUND_OK;
+    handle->solve = unround::make_unrolled(frames, plans, settings->settings, static_cast<int>(iterations),
+                                           static_cast<int>(checkpoint_every), text, status);
+    if (handle->solve == nullptr) return static_cast<unround_status>(status);
+    *made = handle.release();
+    return UNROUND_OK;
+  });
+}
+
+void unround_unrolled_free(unround_unrolled* handle) { delete handle; }
+
+unround_status unround_unrolled_shape(const unround_unrolled* handle, uint64_t* count, uint64_t* height,
+                                      uint64_t* width) {
+  if (handle == nullptr) return UNROUND_ERROR_ARGUMENT;
+  if (count != nullptr) *count = static_cast<uint64_t>(handle->solve->frames());
+  if (height != nullptr) *height = static_cast<uint64_t>(handle->solve->height());
+  if (width != nullptr) *width = static_cast<uint64_t>(handle->solve->width());
+  return UNROUND_OK;
+}
+
+unround_status unround_unrolled_constants(const unround_unrolled* handle, double* values) {
+  if (handle == nullptr || values == nullptr) return UNROUND_ERROR_ARGUMENT;
+  handle->solve->constants(values);
+  return UNROUND_OK;
+}
+
+unround_status unround_unrolled_start(unround_unrolled* handle, double* state, char* message, size_t message_size) {
+  return guarded(message, message_size, [&](std::string& text) -> unround_status {
+    if (handle == nullptr || state == nullptr) {
+      text = "the handle and the state are not null pointers";
+      return UNROUND_ERROR_ARGUMENT;
+    }
+    handle->solve->start_state(state);
+    return UNROUND_OK;
+  });
+}
+
+unround_status unround_unrolled_problem(const unround_unrolled* handle, uint64_t frame, double* values) {
+  if (handle == nullptr || values == nullptr || frame >= handle->frames.size()) return UNROUND_ERROR_ARGUMENT;
+  try {
+    const unround::ComponentModel& k = handle->frames[frame]->frame.components[0];
+    for (size_t b = 0; b < k.blocks(); ++b) {
+      for (int q = 0; q < 64; ++q) {
+        double* entry = values + 4 * (64 * b + q);
+        double own_lower, own_upper;
+        k.coefficient(b, q, entry[0], entry[1], entry[2], own_lower, own_upper);
+        entry[3] = k.flat[b] != 0 ? k.flat_weights[q] : k.weights[q];
+      }
+    }
+    return UNROUND_OK;
+  } catch (...) {
+    return UNROUND_ERROR_INTERNAL;
+  }
+}
+
+unround_status unround_unrolled_forward(unround_unrolled* handle, const float* weight_map, double alpha0,
+                                        double alpha1, float* canvas, void* stream, char* message,
+                                        size_t message_size) {
+  return guarded(message, message_size, [&](std::string& text) -> unround_status {
+    if (handle == nullptr || weight_map == nullptr || canvas == nullptr) {
+      text = "the handle, the map and the canvas are not null pointers";
+      return UNROUND_ERROR_ARGUMENT;
+    }
+    if (!(alpha0 > 0.0) || !(alpha1 > 0.0)) {
+      text = "alpha0 and alpha1 above 0";
+      return UNROUND_ERROR_ARGUMENT;
+    }
+    handle->solve->forward(weight_map, alpha0, alpha1, canvas, stream);
+    return UNROUND_OK;
+  });
+}
+
+unround_status unround_unrolled_backward(unround_unrolled* handle, const float* grad_canvas, float* grad_map,
+                                         double* grad_alphas, void* stream, char* message, size_t message_size) {
+  return guarded(message, message_size, [&](std::string& text) -> unround_status {
+    if (handle == nullptr || grad_canvas == nullptr || grad_map == nullptr || grad_alphas == nullptr) {
+      text = "the handle, the gradients and the map's gradient are not null pointers";
+      return UNROUND_ERROR_ARGUMENT;
+    }
+    handle->solve->backward(grad_canvas, grad_map, grad_alphas, stream);
+    return UNROUND_OK;
   });
 }
 

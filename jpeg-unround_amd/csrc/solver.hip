@@ -714,6 +714,7 @@ class Solver {
  private:
   template <typename, typename>
   friend class Batch;
+  friend class Unrolled;
 
   int tiles(int c) const {
     const ComponentModel& k = frame_.components[c];
@@ -1080,8 +1081,58 @@ class Solver {
 };
 
 #include "batch.hpp"
+#include "unrolled.hpp"
+
+class UnrolledOnGpu final : public UnrolledSolve {
+ public:
+  UnrolledOnGpu(const std::vector<const FrameModel*>& frames, const std::vector<const Plan*>& plans,
+                const Settings& settings, int iterations, int every)
+      : solve_(frames, plans, settings, iterations, every) {}
+  int frames() const override { return solve_.frames(); }
+  int height() const override { return solve_.height(); }
+  int width() const override { return solve_.width(); }
+  void constants(double* out) const override {
+    const Plan& plan = solve_.plan();
+    out[0] = plan.tau;
+    out[1] = plan.sigma;
+    out[2] = plan.gamma[0];
+    out[3] = plan.relaxation;
+  }
+  void start_state(double* out) override { solve_.start_state(out); }
+  void forward(const float* map, double alpha0, double alpha1, float* canvas, void* stream) override {
+    solve_.forward(map, alpha0, alpha1, canvas, static_cast<hipStream_t>(stream));
+  }
+  void backward(const float* grad_canvas, float* grad_map, double* grad_alphas, void* stream) override {
+    solve_.backward(grad_canvas, grad_map, grad_alphas, static_cast<hipStream_t>(stream));
+  }
+
+ private:
+  Unrolled solve_;
+};
 
 }  // namespace
+
+std::unique_ptr<UnrolledSolve> make_unrolled(const std::vector<const FrameModel*>& frames,
+                                             const std::vector<const Plan*>& plans, const Settings& settings,
+                                             int iterations, int every, std::string& message, int32_t& status) {
+  try {
+    int devices = 0;
+    if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) {
+      status = 6;
+      message = "no HIP device: this library computes on an AMD GPU";
+      return nullptr;
+    }
+    return std::make_unique<UnrolledOnGpu>(frames, plans, settings, iterations, every);
+  } catch (const Refused& refusal) {
+    status = 3;
+    message = refusal.what();
+    return nullptr;
+  } catch (const GpuError& error) {
+    status = 6;
+    message = std::string("a GPU error: ") + error.what();
+    return nullptr;
+  }
+}
 
 bool solve_on_gpu(const FrameModel& frame, const Settings& settings, const Plan& plan, const StartState* start,
                   const Observer& observer, Timing timing, SolveOutput& out, std::string& message, int32_t& status) {

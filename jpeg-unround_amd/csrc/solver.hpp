@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <functional>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -70,6 +71,35 @@ bool solve_on_gpu(const FrameModel& frame, const Settings& settings, const Plan&
 // records) and each its own plan: the outputs are each frame's own solve's (its iterates,
 // records and stopping iteration), and latencies[m] the seconds from the first iteration
 // of the batch to the end of the run of iterations in which frame m stopped. No observer.
+// Unrolled TGV iterations on the GPU with their adjoint (grey frames of one size, blocks
+// covering them, binary32): the forward from the start with p's ball of radius
+// alpha1 m(i, j) at pixel (i, j) and r's of alpha0, keeping the state every `every`
+// iterations; the backward of a loss's gradient on the last x to the gradients of m, alpha0
+// and alpha1. Device pointers; `stream` a hipStream_t the calls are ordered after and
+// before (null: the default stream).
+class UnrolledSolve {
+ public:
+  virtual ~UnrolledSolve() = default;
+  virtual int frames() const = 0;
+  virtual int height() const = 0;
+  virtual int width() const = 0;
+  // tau, sigma, gamma, rho.
+  virtual void constants(double* out) const = 0;
+  // The start state [frame][x, w_x, w_y, p_x, p_y, r_xx, r_yy, r_xy][H][W], host memory.
+  virtual void start_state(double* out) = 0;
+  // canvas: [frame][H][W] floats; map: [H][W] floats.
+  virtual void forward(const float* map, double alpha0, double alpha1, float* canvas, void* stream) = 0;
+  // Of the last forward: grad_canvas [frame][H][W] to grad_map [H][W] (device) and
+  // grad_alphas[0] = dL/d alpha0, [1] = dL/d alpha1 (host).
+  virtual void backward(const float* grad_canvas, float* grad_map, double* grad_alphas, void* stream) = 0;
+};
+
+// Null with the reason and a status where the frames or the settings are refused or the
+// GPU fails. The frames and plans must outlive the result.
+std::unique_ptr<UnrolledSolve> make_unrolled(const std::vector<const FrameModel*>& frames,
+                                             const std::vector<const Plan*>& plans, const Settings& settings,
+                                             int iterations, int every, std::string& message, int32_t& status);
+
 bool solve_batch_on_gpu(const std::vector<const FrameModel*>& frames, const std::vector<const Plan*>& plans,
                         const Settings& settings, std::vector<SolveOutput>& outs, std::vector<double>& latencies,
                         std::string& message, int32_t& status);
