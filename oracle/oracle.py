@@ -134,6 +134,19 @@ def library() -> ctypes.CDLL:
         lib.oracle_mmse_centre.argtypes = (ctypes.c_int16, ctypes.c_double, ctypes.c_double)
         lib.oracle_sum_rows.restype = ctypes.c_double
         lib.oracle_sum_rows.argtypes = (dp, ctypes.c_uint64, ctypes.c_uint64)
+        u16p = ctypes.POINTER(ctypes.c_uint16)
+        lib.oracle_samples_intervals.restype = ctypes.c_int
+        lib.oracle_samples_intervals.argtypes = (u16p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_double, dp, dp, dp)
+        lib.oracle_samples_prox.argtypes = (u16p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
+                                            dp, ctypes.c_double, dp)
+        for name in ("oracle_samples_value", "oracle_samples_conjugate"):
+            getattr(lib, name).restype = ctypes.c_double
+            getattr(lib, name).argtypes = (u16p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
+                                           ctypes.c_double, ctypes.c_double, dp)
+        lib.oracle_samples_solve.restype = ctypes.c_void_p
+        lib.oracle_samples_solve.argtypes = (u16p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                             ctypes.c_double, ctypes.c_double, ctypes.POINTER(OracleOptions),
+                                             ctypes.c_char_p, ctypes.c_size_t)
         _library = lib
     return _library
 
@@ -389,3 +402,96 @@ def sum_rows(terms: np.ndarray) -> float:
 def colour_space_code(name: str) -> int:
     return {"grayscale": 1, "ycbcr": 2, "rgb": 3}[name]
 
+
+# Pictures of rounded samples (docs/math.md, 11).
+
+def _levels16(levels) -> np.ndarray:
+    return np.ascontiguousarray(levels, dtype=np.uint16)
+
+
+def samples_intervals(levels, bits: int, slack: float = 0.0) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """The lower and upper ends and the centres of levels of `bits` bits."""
+    values = _levels16(levels)
+    low, high, centre = (np.zeros(values.size) for _ in range(3))
+    made = library().oracle_samples_intervals(values.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), values.size,
+                                              bits, slack, _doubles(low), _doubles(high), _doubles(centre))
+    if not made:
+        raise ValueError("bits, a level or the slack refused")
+    return low.reshape(values.shape), high.reshape(values.shape), centre.reshape(values.shape)
+
+
+def samples_prox(levels, bits: int, values, tau: float, mu: float = 0.0, slack: float = 0.0) -> np.ndarray:
+    given = _levels16(levels)
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    out = np.zeros(given.size)
+    library().oracle_samples_prox(given.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), given.size, bits, mu, slack,
+                                  _doubles(v), tau, _doubles(out))
+    return out.reshape(given.shape)
+
+
+def _samples_sum(name: str, levels, bits: int, mu: float, slack: float, field) -> float:
+    given = _levels16(levels)
+    values = np.ascontiguousarray(field, dtype=np.float64)
+    channels, height, width = given.shape
+    return float(getattr(library(), name)(given.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), channels, height,
+                                          width, bits, mu, slack, _doubles(values)))
+
+
+def samples_value(levels, bits: int, x, mu: float = 0.0, slack: float = 0.0) -> float:
+    """G(x) of a (C, H, W) picture of levels."""
+    return _samples_sum("oracle_samples_value", levels, bits, mu, slack, x)
+
+
+def samples_conjugate(levels, bits: int, xi, mu: float = 0.0, slack: float = 0.0) -> float:
+    """G*(xi) of a (C, H, W) picture of levels."""
+    return _samples_sum("oracle_samples_conjugate", levels, bits, mu, slack, xi)
+
+
+@dataclass
+class SamplesResult:
+    picture: np.ndarray  # (C, H, W)
+    tau: float
+    sigma: float
+    iterations: int
+    stop: int
+    record_iterations: np.ndarray
+    primal: np.ndarray
+    dual: np.ndarray
+    theta: np.ndarray
+    partial: np.ndarray
+
+
+def solve_samples(levels, bits: int, options: list[str]) -> SamplesResult:
+    """TV or TGV within the intervals of a (C, H, W) picture of levels; --mu and --slack of
+    the options are the data term's."""
+    lib = library()
+    given = _levels16(levels)
+    channels, height, width = given.shape
+    parsed = parse_options(options, channels)
+    term = parsed.data[0]
+    mu = term.mu if term.has_mu else 0.0
+    message = ctypes.create_string_buffer(1024)
+    handle = lib.oracle_samples_solve(given.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), channels, height, width,
+                                      bits, mu, term.slack, ctypes.byref(parsed.options), message, 1024)
+    if not handle:
+        raise ValueError(message.value.decode())
+    try:
+        C, H, W = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        pointer = lib.oracle_result_canvas(handle, ctypes.byref(C), ctypes.byref(H), ctypes.byref(W))
+        shape = (C.value, H.value, W.value)
+        picture = np.ctypeslib.as_array(pointer, shape=(int(np.prod(shape)),)).copy().reshape(shape)
+        tau, sigma = ctypes.c_double(), ctypes.c_double()
+        iterations, stop = ctypes.c_uint64(), ctypes.c_int32()
+        lib.oracle_result_solver(handle, ctypes.byref(tau), ctypes.byref(sigma), ctypes.byref(iterations),
+                                 ctypes.byref(stop))
+        n = ctypes.c_uint64()
+        p = lib.oracle_result_record_iterations(handle, ctypes.byref(n))
+        record_iterations = np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else np.zeros(0, np.uint64)
+        values = []
+        for which in range(4):
+            p = lib.oracle_result_record_values(handle, which, ctypes.byref(n))
+            values.append(np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else np.zeros(0))
+        return SamplesResult(picture, tau.value, sigma.value, iterations.value, stop.value, record_iterations,
+                             *values)
+    finally:
+        lib.oracle_result_free(handle)

@@ -778,8 +778,182 @@ static void push_record(oracle_result* r, uint64_t iteration, double primal, dou
  * The primal-dual method (docs/math.md:568-682, 742-811;
  * cpp/src/pdhg.cpp:147-233, cpp/src/primal_dual.hpp:401-1013) */
 
+/* ---------------------------------------------------------------------------------
+ * Pictures of rounded samples (docs/math.md, 11; rust/unround/src/samples.rs:69-345). */
+
+typedef struct samples_model {
+  size_t C, H, W;
+  double *low, *high, *centre; /* C x H x W */
+  double mu;
+} samples_model;
+
+/* A double as mantissa / 2^power, exactly (samples.rs:286-300): 0, or a normal double. */
+static void dyadic_of(double value, __int128* mantissa, unsigned* power) {
+  uint64_t bits;
+  memcpy(&bits, &value, sizeof bits);
+  const unsigned exponent = (unsigned)((bits >> 52) & 0x7FFu);
+  if (exponent == 0) {
+    *mantissa = 0;
+    *power = 0;
+    return;
+  }
+  const __int128 m = (__int128)((bits & ((1ull << 52) - 1u)) | (1ull << 52));
+  *power = 1075u - exponent;
+  *mantissa = value < 0.0 ? -m : m;
+}
+
+/* Whether value < numerator / denominator exactly (samples.rs:302-307). */
+static int lies_below(double value, __int128 numerator, __int128 denominator) {
+  __int128 mantissa;
+  unsigned power;
+  dyadic_of(value, &mantissa, &power);
+  return mantissa * denominator < numerator * ((__int128)1 << power);
+}
+
+/* The least double at or above numerator / denominator, and the greatest double below it
+ * (samples.rs:309-330); the nearest is the correctly rounded quotient of two doubles. */
+static double at_or_above(__int128 numerator, __int128 denominator) {
+  const double near = (double)numerator / (double)denominator;
+  return lies_below(near, numerator, denominator) ? nextafter(near, INFINITY) : near;
+}
+static double under(__int128 numerator, __int128 denominator) {
+  const double near = (double)numerator / (double)denominator;
+  return lies_below(near, numerator, denominator) ? near : nextafter(near, -INFINITY);
+}
+
+/* The ends and centre of every level 0..top (samples.rs:252-284): without slack the
+ * doubles within max(0, (v - 1/2) s) <= x < (v + 1/2) s, or <= 255 at the last level;
+ * with a slack, max(0, ((v - 1/2) - slack) s) and min(255, ((v + 1/2) + slack) s) in
+ * floating point, s the double nearest 255 / top; the centre v s, the nearest double. */
+static void level_ends(uint32_t level, uint32_t top, double slack, double* low, double* high, double* centre) {
+  const __int128 wide = top, value = level;
+  double below, above;
+  if (slack == 0.0) {
+    below = at_or_above((2 * value - 1) * 255, 2 * wide);
+    above = under((2 * value + 1) * 255, 2 * wide);
+  } else {
+    const double step = 255.0 / (double)top;
+    const double own = (double)level;
+    below = ((own - 0.5) - slack) * step;
+    above = ((own + 0.5) + slack) * step;
+  }
+  *low = below > 0.0 ? below : 0.0;
+  *high = above < 255.0 ? above : 255.0;
+  *centre = (double)(value * 255) / (double)wide;
+}
+
+int oracle_samples_intervals(const uint16_t* levels, uint64_t count, uint32_t bits, double slack, double* low,
+                             double* high, double* centre) {
+  if (!(bits == 1 || bits == 2 || bits == 4 || bits == 8 || bits == 16) || !(slack >= 0.0 && isfinite(slack)))
+    return 0;
+  const uint32_t top = (1u << bits) - 1u;
+  for (uint64_t k = 0; k < count; ++k) {
+    if (levels[k] > top) return 0;
+    level_ends(levels[k], top, slack, &low[k], &high[k], &centre[k]);
+  }
+  return 1;
+}
+
+static int samples_make(samples_model* m, const uint16_t* levels, size_t C, size_t H, size_t W, uint32_t bits,
+                        double mu, double slack) {
+  const size_t count = C * H * W;
+  m->C = C;
+  m->H = H;
+  m->W = W;
+  m->mu = mu;
+  m->low = (double*)malloc(sizeof(double) * (count ? count : 1));
+  m->high = (double*)malloc(sizeof(double) * (count ? count : 1));
+  m->centre = (double*)malloc(sizeof(double) * (count ? count : 1));
+  if (count == 0 || !(mu >= 0.0 && isfinite(mu)) ||
+      !oracle_samples_intervals(levels, count, bits, slack, m->low, m->high, m->centre)) {
+    free(m->low);
+    free(m->high);
+    free(m->centre);
+    return 0;
+  }
+  return 1;
+}
+
+static void samples_free(samples_model* m) {
+  free(m->low);
+  free(m->high);
+  free(m->centre);
+}
+
+/* prox_{tau G}(v) of channel c's samples: clip((t c + v) r, a, b), t = tau mu, r = 1/(1+t)
+ * (samples.rs:171-182; t c + v fused, as the reference's multiply_add is with FMA). */
+static void samples_prox(const samples_model* m, size_t c, const double* v, double tau, double* out) {
+  const size_t N = m->H * m->W, at = c * N;
+  const double t = tau * m->mu;
+  const double r = 1.0 / (1.0 + t);
+  for (size_t e = 0; e < N; ++e) out[e] = clamp_to(fma(t, m->centre[at + e], v[e]) * r, m->low[at + e], m->high[at + e]);
+}
+
+/* G(x) = mu / 2 the sum of the squares from the centres, row by row (samples.rs:205-222). */
+static double samples_value(const samples_model* m, const double* x) {
+  if (m->mu == 0.0) return 0.0;
+  const size_t count = m->C * m->H * m->W;
+  double* terms = (double*)malloc(sizeof(double) * count);
+  for (size_t k = 0; k < count; ++k) {
+    const double away = x[k] - m->centre[k];
+    terms[k] = away * away;
+  }
+  const double total = oracle_sum_rows(terms, m->C * m->H, m->W);
+  free(terms);
+  return 0.5 * m->mu * total;
+}
+
+/* g*(s) of a sample (kernels.rs:1572-1590 without a charge). */
+static double samples_term(double s, double mu, double centre, double low, double high) {
+  if (mu > 0.0) {
+    const double best = clamp_to(centre + s / mu, low, high);
+    const double difference = best - centre;
+    return s * best - 0.5 * mu * (difference * difference);
+  }
+  return fmax(s * low, s * high);
+}
+
+/* G*_c at xi (one channel's plane), row by row (samples.rs:224-248). */
+static double samples_conjugate(const samples_model* m, size_t c, const double* xi) {
+  const size_t N = m->H * m->W, at = c * N;
+  double* terms = (double*)malloc(sizeof(double) * N);
+  for (size_t e = 0; e < N; ++e) terms[e] = samples_term(xi[e], m->mu, m->centre[at + e], m->low[at + e], m->high[at + e]);
+  const double total = oracle_sum_rows(terms, m->H, m->W);
+  free(terms);
+  return total;
+}
+
+void oracle_samples_prox(const uint16_t* levels, uint64_t count, uint32_t bits, double mu, double slack,
+                         const double* values, double tau, double* out) {
+  samples_model m;
+  if (!samples_make(&m, levels, 1, 1, count, bits, mu, slack)) return;
+  samples_prox(&m, 0, values, tau, out);
+  samples_free(&m);
+}
+
+double oracle_samples_value(const uint16_t* levels, uint64_t channels, uint64_t height, uint64_t width,
+                            uint32_t bits, double mu, double slack, const double* x) {
+  samples_model m;
+  if (!samples_make(&m, levels, channels, height, width, bits, mu, slack)) return NAN;
+  const double value = samples_value(&m, x);
+  samples_free(&m);
+  return value;
+}
+
+double oracle_samples_conjugate(const uint16_t* levels, uint64_t channels, uint64_t height, uint64_t width,
+                                uint32_t bits, double mu, double slack, const double* xi) {
+  samples_model m;
+  if (!samples_make(&m, levels, channels, height, width, bits, mu, slack)) return NAN;
+  double total = 0.0;
+  for (size_t c = 0; c < channels; ++c) total += samples_conjugate(&m, c, xi + c * height * width);
+  samples_free(&m);
+  return total;
+}
+
+
 typedef struct machine {
   const frame* f;
+  const samples_model* sm; /* in place of f for a picture of rounded samples */
   int tgv, coupled;
   size_t C, H, W, N;
   double tau, sigma, rho, rest;
@@ -854,7 +1028,11 @@ static void iterate(machine* M) {
     divergence(FIELD(M->p, c, 0, 2), FIELD(M->p, c, 1, 2), H, W, one);
     const double* x = FIELD(M->x, c, 0, 1);
     for (size_t e = 0; e < N; ++e) v[e] = x[e] + descent * one[e];
-    frame_prox(M->f, c, v, M->tau, FIELD(M->tx, c, 0, 1), M->zeta[c]);
+    if (M->sm != NULL) {
+      samples_prox(M->sm, c, v, M->tau, FIELD(M->tx, c, 0, 1));
+    } else {
+      frame_prox(M->f, c, v, M->tau, FIELD(M->tx, c, 0, 1), M->zeta[c]);
+    }
     if (M->tgv) {
       tensor_divergence(FIELD(M->r, c, 0, 3), FIELD(M->r, c, 1, 3), FIELD(M->r, c, 2, 3), H, W, one, two);
       for (size_t e = 0; e < N; ++e) {
@@ -965,7 +1143,7 @@ static void record(machine* M, double* primal, double* dual, double* theta_out, 
   const size_t C = M->C, H = M->H, W = M->W, N = M->N;
   const frame* f = M->f;
   double data = 0.0;
-  for (size_t c = 0; c < C; ++c) {
+  for (size_t c = 0; c < C && M->sm == NULL; ++c) {
     const problem* pr = &f->comps[c].pr;
     const size_t blocks = pr->rows * pr->columns;
     double* terms = (double*)malloc(sizeof(double) * 64u * (blocks ? blocks : 1));
@@ -974,6 +1152,7 @@ static void record(machine* M, double* primal, double* dual, double* theta_out, 
     data += blocks_sum(pr, terms);
     free(terms);
   }
+  if (M->sm != NULL) data = samples_value(M->sm, M->tx);
   double* first = (double*)calloc(N, sizeof(double));
   double* second = (double*)calloc(N, sizeof(double));
   double* squares = (double*)calloc(N, sizeof(double));
@@ -1014,7 +1193,7 @@ static void record(machine* M, double* primal, double* dual, double* theta_out, 
     for (size_t c = 0; c < C; ++c) {
       divergence(FIELD(M->tp, c, 0, 2), FIELD(M->tp, c, 1, 2), H, W, one);
       for (size_t e = 0; e < N; ++e) xi[e] = M->gamma[c] * one[e];
-      conjugates += frame_conjugate(f, c, xi, M->free_radius);
+      conjugates += (M->sm != NULL ? samples_conjugate(M->sm, c, xi) : frame_conjugate(f, c, xi, M->free_radius));
     }
     *dual = -conjugates;
     *theta_out = NAN;
@@ -1067,7 +1246,7 @@ static void record(machine* M, double* primal, double* dual, double* theta_out, 
       }
       divergence(g1, g2, H, W, one);
       for (size_t e = 0; e < N; ++e) xi[e] = M->gamma[c] * one[e];
-      conjugates += frame_conjugate(f, c, xi, M->free_radius);
+      conjugates += (M->sm != NULL ? samples_conjugate(M->sm, c, xi) : frame_conjugate(f, c, xi, M->free_radius));
     }
     *dual = -conjugates;
     *partial_out = NAN;
@@ -1076,7 +1255,7 @@ static void record(machine* M, double* primal, double* dual, double* theta_out, 
       for (size_t c = 0; c < C; ++c) {
         divergence(FIELD(M->tp, c, 0, 2), FIELD(M->tp, c, 1, 2), H, W, one);
         for (size_t e = 0; e < N; ++e) xi[e] = M->gamma[c] * one[e];
-        partial += frame_conjugate(f, c, xi, M->free_radius);
+        partial += (M->sm != NULL ? samples_conjugate(M->sm, c, xi) : frame_conjugate(f, c, xi, M->free_radius));
         const double* dx = divergences + (2 * c) * N;
         const double* dy = divergences + (2 * c + 1) * N;
         for (size_t e = 0; e < N; ++e) {
@@ -1101,6 +1280,38 @@ static void record(machine* M, double* primal, double* dual, double* theta_out, 
   free(xi);
   free(g1);
   free(g2);
+}
+
+/* The run (primal_dual.hpp:958-1013; samples.rs:613-626): records at 0 (where any are
+ * taken), every `every` iterations and at the last; a tolerance met stops it. The relative
+ * tolerance compares with |primal| where `absolute` (the samples' solvers), with primal
+ * otherwise. */
+static void run_machine(machine* M, oracle_result* result, double tolerance, double relative, double partial_tolerance,
+                        uint64_t every, uint64_t most, int absolute) {
+  const double samples = (double)M->C * (double)M->H * (double)M->W;
+  int stopped = 0;
+  uint64_t iteration = 0;
+#define TAKE(at)                                                                                      \
+  do {                                                                                                \
+    double primal, dual, theta, partial;                                                              \
+    record(M, &primal, &dual, &theta, &partial);                                                      \
+    push_record(result, (at), primal, dual, theta, partial);                                          \
+    const double gap = primal - dual;                                                                 \
+    int met = 0;                                                                                      \
+    if (tolerance > 0.0 && gap / samples <= tolerance) met = 1;                                       \
+    if (relative > 0.0 && gap <= relative * (absolute ? fabs(primal) : primal)) met = 1;              \
+    if (partial_tolerance > 0.0 && M->has_partial && partial / samples <= partial_tolerance) met = 1; \
+    if (met) stopped = 1;                                                                             \
+  } while (0)
+  if (every > 0 || most == 0) TAKE(0);
+  while (!stopped && iteration < most) {
+    iterate(M);
+    iteration += 1;
+    if ((every > 0 && iteration % every == 0) || iteration == most) TAKE(iteration);
+  }
+#undef TAKE
+  result->iterations = iteration;
+  result->stop = stopped ? 1 : 2;
 }
 
 oracle_result* oracle_solve(uint32_t height, uint32_t width, int32_t color_space, uint32_t count,
@@ -1298,36 +1509,12 @@ oracle_result* oracle_solve(uint32_t height, uint32_t width, int32_t color_space
   }
 
   /* The run (primal_dual.hpp:958-1013). */
-  int32_t stop = 0;
-  uint64_t iteration = 0;
   if (o->method != 0) {
-    const uint64_t every = o->record_every;
-    const double samples = (double)C * (double)H * (double)W;
-    int stopped = 0;
-#define TAKE(at)                                                                                   \
-  do {                                                                                             \
-    double primal, dual, theta, partial;                                                           \
-    record(&M, &primal, &dual, &theta, &partial);                                                  \
-    push_record(result, (at), primal, dual, theta, partial);                                       \
-    const double gap = primal - dual;                                                              \
-    int met = 0;                                                                                   \
-    if (tolerance > 0.0 && gap / samples <= tolerance) met = 1;                                    \
-    if (o->relative_tolerance > 0.0 && gap <= o->relative_tolerance * primal) met = 1;             \
-    if (o->partial_tolerance > 0.0 && M.has_partial && partial / samples <= o->partial_tolerance)  \
-      met = 1;                                                                                     \
-    if (met) stopped = 1;                                                                          \
-  } while (0)
-    if (every > 0 || most == 0) TAKE(0);
-    while (!stopped && iteration < most) {
-      iterate(&M);
-      iteration += 1;
-      if ((every > 0 && iteration % every == 0) || iteration == most) TAKE(iteration);
-    }
-#undef TAKE
-    stop = stopped ? 1 : 2;
+    run_machine(&M, result, tolerance, o->relative_tolerance, o->partial_tolerance, o->record_every, most, 0);
+  } else {
+    result->iterations = 0;
+    result->stop = 0;
   }
-  result->iterations = iteration;
-  result->stop = stop;
   result->canvas = copy_of(M.tx, C * N);
   for (size_t c = 0; c < C; ++c) {
     result->arrays[c][0] = M.zeta[c];
@@ -1342,5 +1529,92 @@ oracle_result* oracle_solve(uint32_t height, uint32_t width, int32_t color_space
   free(M.r);
   free(M.tr);
   for (size_t c = 0; c < C; ++c) problem_free(&f.comps[c].pr);
+  return result;
+}
+
+/* TV or TGV within the intervals of a picture of rounded samples (samples.rs:674-1080):
+ * the method of 5 with the data term of 11, from the centres (projected), w their gradient
+ * and the duals 0; the picture is the last tilde x. */
+oracle_result* oracle_samples_solve(const uint16_t* levels, uint32_t channels, uint32_t height, uint32_t width,
+                                    uint32_t bits, double mu, double slack, const oracle_options* options,
+                                    char* message, size_t message_size) {
+  const oracle_options* o = options;
+  if (o->method != 1 && o->method != 2) {
+    say(message, message_size, "the samples' methods are TV and TGV");
+    return NULL;
+  }
+  samples_model model;
+  if (channels == 0 || channels > 3 ||
+      !samples_make(&model, levels, channels, height, width, bits, mu, slack)) {
+    say(message, message_size, "a picture of 1 to 3 channels, levels of 1, 2, 4, 8 or 16 bits within range, "
+                               "and a weight and slack at least 0");
+    return NULL;
+  }
+  const int tgv = o->method == 2;
+  const size_t C = channels, H = height, W = width, N = H * W;
+  double gamma[3] = {1.0, 1.0, 1.0};
+  if (o->has_channel_weights)
+    for (size_t c = 0; c < C; ++c) gamma[c] = o->channel_weights[c];
+  double largest = 0.0;
+  for (size_t c = 0; c < C; ++c) largest = fmax(largest, gamma[c]);
+  const double bound = tgv ? (17.0 + sqrt(33.0)) / 2.0 : 8.0;
+  const double norm_squared = o->has_norm_squared ? o->norm_squared : bound * (largest * largest);
+  const double weight = tgv ? o->alpha1 : o->alpha;
+  const double scale = o->weight_scaling ? weight : 1.0;
+  const double default_ratio = tgv ? 0.3 : 10.0;
+  const double ratio = o->has_step_ratio ? o->step_ratio : default_ratio / (scale * scale);
+  const double tolerance = o->has_tolerance ? o->tolerance : 1e-4 * scale;
+  const uint64_t most = o->has_iterations ? o->iterations : 10000u;
+  oracle_result* result = (oracle_result*)calloc(1, sizeof(oracle_result));
+  result->channels = C;
+  result->rows = H;
+  result->columns = W;
+  result->tau = sqrt((ratio * o->step_product) / norm_squared);
+  result->sigma = sqrt(o->step_product / (ratio * norm_squared));
+
+  machine M;
+  memset(&M, 0, sizeof M);
+  M.sm = &model;
+  M.tgv = tgv;
+  M.coupled = o->coupled;
+  M.C = C;
+  M.H = H;
+  M.W = W;
+  M.N = N;
+  M.tau = result->tau;
+  M.sigma = result->sigma;
+  M.rho = o->has_relaxation ? o->relaxation : 1.9;
+  M.rest = 1.0 - M.rho;
+  M.alpha = o->alpha;
+  M.alpha1 = o->alpha1;
+  M.alpha0 = o->alpha0;
+  for (size_t c = 0; c < 3; ++c) M.gamma[c] = gamma[c];
+  M.has_partial = tgv && o->has_partial_radius;
+  M.partial_radius = o->partial_radius;
+  M.x = (double*)calloc(C * N, sizeof(double));
+  M.tx = (double*)calloc(C * N, sizeof(double));
+  M.p = (double*)calloc(2 * C * N, sizeof(double));
+  M.tp = (double*)calloc(2 * C * N, sizeof(double));
+  if (tgv) {
+    M.w = (double*)calloc(2 * C * N, sizeof(double));
+    M.tw = (double*)calloc(2 * C * N, sizeof(double));
+    M.r = (double*)calloc(3 * C * N, sizeof(double));
+    M.tr = (double*)calloc(3 * C * N, sizeof(double));
+  }
+  for (size_t k = 0; k < C * N; ++k) M.x[k] = clamp_to(model.centre[k], model.low[k], model.high[k]);
+  for (size_t c = 0; c < C && tgv; ++c) gradient(FIELD(M.x, c, 0, 1), H, W, FIELD(M.w, c, 0, 2), FIELD(M.w, c, 1, 2));
+  memcpy(M.tx, M.x, sizeof(double) * C * N);
+  if (tgv) memcpy(M.tw, M.w, sizeof(double) * 2 * C * N);
+  run_machine(&M, result, tolerance, o->relative_tolerance, o->partial_tolerance, o->record_every, most, 1);
+  result->canvas = copy_of(M.tx, C * N);
+  free(M.x);
+  free(M.tx);
+  free(M.p);
+  free(M.tp);
+  free(M.w);
+  free(M.tw);
+  free(M.r);
+  free(M.tr);
+  samples_free(&model);
   return result;
 }

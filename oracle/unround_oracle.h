@@ -114,6 +114,28 @@ void oracle_result_solver(const oracle_result* result, double* tau, double* sigm
 const uint64_t* oracle_result_record_iterations(const oracle_result* result, uint64_t* count);
 const double* oracle_result_record_values(const oracle_result* result, int32_t which, uint64_t* count);
 
+/* Pictures of rounded samples (docs/math.md, 11; rust/unround/src/samples.rs): the
+ * intervals and centres of levels of `bits` bits (1, 2, 4, 8 or 16) with a slack in
+ * levels, into low/high/centre (count each); 0 where the bits, a level or the slack are
+ * refused. */
+int oracle_samples_intervals(const uint16_t* levels, uint64_t count, uint32_t bits, double slack, double* low,
+                             double* high, double* centre);
+/* The proximal map of tau G at values (count), G's value at x and G* at xi, of a model of
+ * `channels x height x width` levels (count = their product). */
+void oracle_samples_prox(const uint16_t* levels, uint64_t count, uint32_t bits, double mu, double slack,
+                         const double* values, double tau, double* out);
+double oracle_samples_value(const uint16_t* levels, uint64_t channels, uint64_t height, uint64_t width,
+                            uint32_t bits, double mu, double slack, const double* x);
+double oracle_samples_conjugate(const uint16_t* levels, uint64_t channels, uint64_t height, uint64_t width,
+                                uint32_t bits, double mu, double slack, const double* xi);
+/* TV (options->method 1) or TGV (2) within the intervals: the picture is the canvas,
+ * channels x height x width; the defaults of rounded samples replace the options' where
+ * they are not given (has_...): tolerance 1e-4 and the step ratio 10 (TV) or 0.3 (TGV),
+ * each with the first-order weight as JPEG's are, and 10000 iterations. */
+oracle_result* oracle_samples_solve(const uint16_t* levels, uint32_t channels, uint32_t height, uint32_t width,
+                                    uint32_t bits, double mu, double slack, const oracle_options* options,
+                                    char* message, size_t message_size);
+
 /* Exposed parts, for tests of their own. */
 void oracle_basis(double out[64]);
 void oracle_dct_forward(const double in[64], double out[64]);
