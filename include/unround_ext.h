@@ -80,6 +80,23 @@ unround_status unround_unrolled_forward(unround_unrolled* handle, const float* w
 unround_status unround_unrolled_backward(unround_unrolled* handle, const float* grad_canvas, float* grad_map,
                                          double* grad_alphas, void* stream, char* message, size_t message_size);
 
+/* A picture of rounded samples given as arrays, reconstructed as unround_decode does a PNG
+ * file (rust/capi/include/unround.h:128-136; rust/unround/src/rounded.rs:71-170): levels of
+ * `bits` bits (1, 2, 4, 8 or 16), height x width x channels interleaved as a PNG file
+ * stores them (1 grey, 2 grey and alpha, 3 RGB, 4 RGB and alpha); the colours together by
+ * TV or TGV within the intervals of their levels (docs/math.md 11) with --mu and --slack
+ * alone of the data term and the defaults of rounded samples, and the alpha alone, without
+ * the colours' weights of channels. The result is a PNG file's: unround_result_format 2,
+ * unround_result_bits the bits, the picture interleaved as given and every sample within
+ * its interval, the canvas its planes, no components, the alpha's solver through
+ * unround_result_alpha_solver. observer, where it is not null, is called with user and
+ * each record after the first: the colours', then the alpha's. No counterpart in the
+ * reference's C ABI, whose unround_decode reads the levels from a PNG file: the same
+ * solve for levels a caller already holds. */
+unround_status unround_solve_samples(const uint16_t* levels, uint64_t height, uint64_t width, uint64_t channels,
+                                     uint32_t bits, const unround_settings* settings, unround_observer observer,
+                                     void* user, unround_result** result, char* message, size_t message_size);
+
 /* The durations of the solve's kernels, where the environment variable UNROUND_TIMING was
  * set during the solve (HIP events on the solve's stream): values[0] the seconds of the
  * primal kernels and [1] their launches, [2] and [3] those of the dual kernels, [4] the

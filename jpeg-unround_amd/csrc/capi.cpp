@@ -38,6 +38,14 @@ struct unround_result {
   mutable std::vector<std::array<std::vector<double>, 5>> problems;
   mutable std::vector<bool> made;
   std::string warning;
+  // A picture of rounded samples (a PNG file's): the format, the bits of its samples, and
+  // the alpha's solve where it has alpha; its frame has no components.
+  int32_t format = UNROUND_FORMAT_JPEG;
+  int32_t bits = 8;
+  bool has_alpha = false;
+  unround::SolveOutput alpha;
+  std::vector<uint64_t> alpha_record_iterations;
+  std::array<std::vector<double>, 5> alpha_record_values;
 };
 
 // The unrolled solve's handle: its frames (with their settings) and plans, which the solve
@@ -96,6 +104,22 @@ void rgb_of(const double* planes, size_t count, double* picture) {
   }
 }
 
+// A solve's records as the accessors give them: iterations, and seconds, primal, dual,
+// theta and partial.
+void records_of(const unround::SolveOutput& out, std::vector<uint64_t>& iterations,
+                std::array<std::vector<double>, 5>& values) {
+  iterations.clear();
+  for (auto& column : values) column.clear();
+  for (const unround::Record& r : out.records) {
+    iterations.push_back(r.iteration);
+    values[0].push_back(r.seconds);
+    values[1].push_back(r.primal);
+    values[2].push_back(r.dual);
+    values[3].push_back(r.theta);
+    values[4].push_back(r.partial);
+  }
+}
+
 // The canvas cut to the picture, and the picture of the colour space
 // (rust/unround/src/decode.rs:306-358).
 void assemble(unround_result& result) {
@@ -118,16 +142,7 @@ void assemble(unround_result& result) {
       for (size_t c = 0; c < C; ++c) result.picture[C * k + c] = result.planes[c * h * w + k];
     result.picture_channels = C;
   }
-  result.record_iterations.clear();
-  for (auto& values : result.record_values) values.clear();
-  for (const unround::Record& r : result.out.records) {
-    result.record_iterations.push_back(r.iteration);
-    result.record_values[0].push_back(r.seconds);
-    result.record_values[1].push_back(r.primal);
-    result.record_values[2].push_back(r.dual);
-    result.record_values[3].push_back(r.theta);
-    result.record_values[4].push_back(r.partial);
-  }
+  records_of(result.out, result.record_iterations, result.record_values);
   result.problems.assign(C, {});
   result.made.assign(C, false);
 }
@@ -194,6 +209,24 @@ unround::Timing timing_asked() {
   return unround::Timing::none;
 }
 
+// A record's callback to the caller's observer, for a canvas of C x H x W samples.
+unround::Observer watcher(unround_observer observer, void* user, uint64_t channels, uint64_t height, uint64_t width) {
+  if (observer == nullptr) return {};
+  const double samples = static_cast<double>(channels) * static_cast<double>(height) * static_cast<double>(width);
+  return [observer, user, channels, height, width, samples](const unround::Record& record, const double* canvas) {
+    unround_record raw{};
+    raw.iteration = record.iteration;
+    raw.gap = (record.primal - record.dual) / samples;
+    raw.primal = record.primal;
+    raw.dual = record.dual;
+    raw.channels = channels;
+    raw.height = height;
+    raw.width = width;
+    raw.canvas = canvas;
+    return observer(user, &raw) != 0;
+  };
+}
+
 unround_status reconstruct(const unround_input* input, const unround_settings* settings, const unround_start* start,
                            unround_observer observer, void* user, unround_result** result, std::string& message) {
   if (result == nullptr) {
@@ -219,27 +252,96 @@ unround_status reconstruct(const unround_input* input, const unround_settings* s
     }
   }
   const unround::FrameModel& frame = made->frame;
-  const double samples = static_cast<double>(frame.channels()) * frame.rows * frame.columns;
-  unround::Observer watch;
-  if (observer != nullptr) {
-    watch = [observer, user, &frame, samples](const unround::Record& record, const double* canvas) {
-      unround_record raw{};
-      raw.iteration = record.iteration;
-      raw.gap = (record.primal - record.dual) / samples;
-      raw.primal = record.primal;
-      raw.dual = record.dual;
-      raw.channels = frame.channels();
-      raw.height = frame.rows;
-      raw.width = frame.columns;
-      raw.canvas = canvas;
-      return observer(user, &raw) != 0;
-    };
-  }
+  const unround::Observer watch = watcher(observer, user, frame.channels(), frame.rows, frame.columns);
   if (!unround::solve_on_gpu(frame, s, plan, start != nullptr ? &state : nullptr, watch, timing_asked(), made->out,
                              message, status)) {
     return static_cast<unround_status>(status);
   }
   assemble(*made);
+  *result = made.release();
+  return UNROUND_OK;
+}
+
+// A picture of rounded samples, interleaved, reconstructed (rust/unround/src/rounded.rs:
+// 71-170): the colours by the settings' method and weights of channels, the alpha alone.
+unround_status reconstruct_samples(const uint16_t* levels, uint64_t height, uint64_t width, uint64_t channels,
+                                   uint32_t bits, const unround_settings* settings, unround_observer observer,
+                                   void* user, unround_result** result, std::string& message) {
+  if (result == nullptr || settings == nullptr || levels == nullptr) {
+    message = "the levels, the settings and the result are not null pointers";
+    return UNROUND_ERROR_ARGUMENT;
+  }
+  *result = nullptr;
+  if (channels < 1 || channels > 4) {
+    message = "a picture of 1 to 4 channels (grey, grey and alpha, RGB, RGB and alpha), not " +
+              std::to_string(channels);
+    return UNROUND_ERROR_ARGUMENT;
+  }
+  if (height == 0 || width == 0 || height > (uint64_t{1} << 31) || width > (uint64_t{1} << 31) ||
+      height * width > (uint64_t{1} << 40)) {
+    message = "a picture of at least one and at most 2^40 pixels";
+    return UNROUND_ERROR_ARGUMENT;
+  }
+  unround::Settings s = settings->settings;
+  if (s.method != unround::Method::tv && s.method != unround::Method::tgv) {
+    message = "a picture of rounded samples is reconstructed by TV or TGV";
+    return UNROUND_ERROR_OPTIONS;
+  }
+  const uint64_t colours = (channels == 2 || channels == 4) ? channels - 1 : channels;
+  const size_t plane = static_cast<size_t>(height * width);
+  const auto planes_of = [&](uint64_t first, uint64_t count) {
+    std::vector<uint16_t> out(count * plane);
+    for (uint64_t c = 0; c < count; ++c)
+      for (size_t k = 0; k < plane; ++k) out[c * plane + k] = levels[k * channels + first + c];
+    return out;
+  };
+  const auto h = static_cast<uint32_t>(height), w = static_cast<uint32_t>(width);
+  int32_t status = UNROUND_OK;
+  unround::SamplesModel colour_model;
+  const std::vector<uint16_t> colour_levels = planes_of(0, colours);
+  if (!unround::make_samples(colour_levels.data(), static_cast<uint32_t>(colours), h, w, bits, s, colour_model,
+                             message, status)) {
+    return static_cast<unround_status>(status);
+  }
+  unround::Plan plan;
+  if (!unround::make_samples_plan(s, colours, plan, message)) return UNROUND_ERROR_OPTIONS;
+  auto made = std::make_unique<unround_result>();
+  if (!unround::solve_samples_on_gpu(colour_model, s, plan, watcher(observer, user, colours, height, width),
+                                     timing_asked(), made->out, message, status)) {
+    return static_cast<unround_status>(status);
+  }
+  if (colours < channels) {
+    // The alpha alone: one channel, without the colours' weights of channels.
+    unround::Settings alone = s;
+    alone.channel_weights.reset();
+    unround::SamplesModel alpha_model;
+    const std::vector<uint16_t> alpha_levels = planes_of(channels - 1, 1);
+    if (!unround::make_samples(alpha_levels.data(), 1, h, w, bits, alone, alpha_model, message, status)) {
+      return static_cast<unround_status>(status);
+    }
+    unround::Plan alpha_plan;
+    if (!unround::make_samples_plan(alone, 1, alpha_plan, message)) return UNROUND_ERROR_OPTIONS;
+    if (!unround::solve_samples_on_gpu(alpha_model, alone, alpha_plan, watcher(observer, user, 1, height, width),
+                                       timing_asked(), made->alpha, message, status)) {
+      return static_cast<unround_status>(status);
+    }
+    made->has_alpha = true;
+    records_of(made->alpha, made->alpha_record_iterations, made->alpha_record_values);
+  }
+  made->format = UNROUND_FORMAT_PNG;
+  made->bits = static_cast<int32_t>(bits);
+  made->settings = s;
+  made->solver = true;
+  made->frame.height = made->frame.rows = h;
+  made->frame.width = made->frame.columns = w;
+  made->frame.color_space = channels <= 2 ? UNROUND_COLOR_GRAYSCALE : UNROUND_COLOR_RGB;
+  made->planes = made->out.canvas;
+  if (made->has_alpha) made->planes.insert(made->planes.end(), made->alpha.canvas.begin(), made->alpha.canvas.end());
+  made->picture.resize(channels * plane);
+  for (uint64_t c = 0; c < channels; ++c)
+    for (size_t k = 0; k < plane; ++k) made->picture[k * channels + c] = made->planes[c * plane + k];
+  made->picture_channels = channels;
+  records_of(made->out, made->record_iterations, made->record_values);
   *result = made.release();
   return UNROUND_OK;
 }
@@ -352,6 +454,14 @@ unround_status unround_solve_start(const unround_input* input, const unround_set
                                    unround_result** result, char* message, size_t message_size) {
   return guarded(message, message_size, [&](std::string& text) {
     return reconstruct(input, settings, start, observer, user, result, text);
+  });
+}
+
+unround_status unround_solve_samples(const uint16_t* levels, uint64_t height, uint64_t width, uint64_t channels,
+                                     uint32_t bits, const unround_settings* settings, unround_observer observer,
+                                     void* user, unround_result** result, char* message, size_t message_size) {
+  return guarded(message, message_size, [&](std::string& text) {
+    return reconstruct_samples(levels, height, width, channels, bits, settings, observer, user, result, text);
   });
 }
 
@@ -502,6 +612,13 @@ int32_t unround_result_color_space(const unround_result* result) {
 const double* unround_result_canvas(const unround_result* result, uint64_t* channels, uint64_t* height,
                                     uint64_t* width) {
   if (result == nullptr) return nullptr;
+  if (result->format == UNROUND_FORMAT_PNG) {
+    // The planes: nothing lies beyond the picture.
+    if (channels != nullptr) *channels = result->picture_channels;
+    if (height != nullptr) *height = result->frame.height;
+    if (width != nullptr) *width = result->frame.width;
+    return result->planes.data();
+  }
   if (channels != nullptr) *channels = result->frame.channels();
   if (height != nullptr) *height = result->frame.rows;
   if (width != nullptr) *width = result->frame.columns;
@@ -577,16 +694,29 @@ const double* unround_result_record_values(const unround_result* result, int32_t
   return result->record_values[which].data();
 }
 
-int32_t unround_result_alpha_solver(const unround_result*, uint64_t*, int32_t*) { return 0; }
-
-const uint64_t* unround_result_alpha_record_iterations(const unround_result*, uint64_t* count) {
-  if (count != nullptr) *count = 0;
-  return nullptr;
+int32_t unround_result_alpha_solver(const unround_result* result, uint64_t* iterations, int32_t* stop) {
+  if (result == nullptr || !result->has_alpha) return 0;
+  if (iterations != nullptr) *iterations = result->alpha.iterations;
+  if (stop != nullptr) *stop = result->alpha.stop;
+  return 1;
 }
 
-const double* unround_result_alpha_record_values(const unround_result*, int32_t, uint64_t* count) {
-  if (count != nullptr) *count = 0;
-  return nullptr;
+const uint64_t* unround_result_alpha_record_iterations(const unround_result* result, uint64_t* count) {
+  if (result == nullptr || !result->has_alpha) {
+    if (count != nullptr) *count = 0;
+    return nullptr;
+  }
+  if (count != nullptr) *count = result->alpha_record_iterations.size();
+  return result->alpha_record_iterations.data();
+}
+
+const double* unround_result_alpha_record_values(const unround_result* result, int32_t which, uint64_t* count) {
+  if (result == nullptr || !result->has_alpha || which < 0 || which > 4) {
+    if (count != nullptr) *count = 0;
+    return nullptr;
+  }
+  if (count != nullptr) *count = result->alpha_record_values[which].size();
+  return result->alpha_record_values[which].data();
 }
 
 const uint8_t* unround_result_icc_profile(const unround_result*, uint64_t* size) {
@@ -596,9 +726,9 @@ const uint8_t* unround_result_icc_profile(const unround_result*, uint64_t* size)
 
 int32_t unround_result_exif_orientation(const unround_result*) { return 0; }
 
-int32_t unround_result_format(const unround_result* result) { return result == nullptr ? 0 : UNROUND_FORMAT_JPEG; }
+int32_t unround_result_format(const unround_result* result) { return result == nullptr ? 0 : result->format; }
 
-int32_t unround_result_bits(const unround_result* result) { return result == nullptr ? 0 : 8; }
+int32_t unround_result_bits(const unround_result* result) { return result == nullptr ? 0 : result->bits; }
 
 const char* unround_result_warning(const unround_result* result) {
   return result == nullptr ? nullptr : result->warning.c_str();

@@ -296,7 +296,31 @@ bool make_frame(uint32_t height, uint32_t width, int32_t color_space, const std:
   return true;
 }
 
+namespace {
+
+// The defaults of a plan: the ratio of the steps, the tolerance and the most iterations,
+// each of TV and of TGV.
+struct PlanDefaults {
+  double tv_ratio, tgv_ratio;
+  double tv_tolerance, tgv_tolerance;
+  uint64_t tv_iterations, tgv_iterations;
+};
+
+bool plan_with(const Settings& s, size_t channels, const PlanDefaults& d, Plan& plan, std::string& message);
+
+}  // namespace
+
 bool make_plan(const Settings& s, size_t channels, Plan& plan, std::string& message) {
+  return plan_with(s, channels, PlanDefaults{10.0, 3.0, 2e-4, 2e-3, 5000u, 10000u}, plan, message);
+}
+
+bool make_samples_plan(const Settings& s, size_t channels, Plan& plan, std::string& message) {
+  return plan_with(s, channels, PlanDefaults{10.0, 0.3, 1e-4, 1e-4, 10000u, 10000u}, plan, message);
+}
+
+namespace {
+
+bool plan_with(const Settings& s, size_t channels, const PlanDefaults& d, Plan& plan, std::string& message) {
   std::vector<std::string> errors;
   if (s.channel_weights && s.channel_weights->size() != channels) {
     errors.push_back("the channel weights are one for each of the " + std::to_string(channels) + " channels, not " +
@@ -317,11 +341,11 @@ bool make_plan(const Settings& s, size_t channels, Plan& plan, std::string& mess
   const double bound = tgv ? (17.0 + std::sqrt(33.0)) / 2.0 : 8.0;
   const double norm_squared = s.norm_squared ? *s.norm_squared : bound * (largest * largest);
   const double weight = tgv ? s.alpha1 : s.alpha;
-  const double default_ratio = tgv ? 3.0 : 10.0;
-  const double default_tolerance = tgv ? 2e-3 : 2e-4;
+  const double default_ratio = tgv ? d.tgv_ratio : d.tv_ratio;
+  const double default_tolerance = tgv ? d.tgv_tolerance : d.tv_tolerance;
   const double ratio = s.step_ratio ? *s.step_ratio : (s.weight_scaling ? default_ratio / (weight * weight) : default_ratio);
   plan.tolerance = s.tolerance ? *s.tolerance : (s.weight_scaling ? default_tolerance * weight : default_tolerance);
-  plan.iterations = s.iterations ? *s.iterations : (tgv ? 10000u : 5000u);
+  plan.iterations = s.iterations ? *s.iterations : (tgv ? d.tgv_iterations : d.tv_iterations);
   plan.tau = std::sqrt((ratio * s.step_product) / norm_squared);
   plan.sigma = std::sqrt(s.step_product / (ratio * norm_squared));
   plan.relative_tolerance = s.relative_tolerance;
@@ -331,6 +355,102 @@ bool make_plan(const Settings& s, size_t channels, Plan& plan, std::string& mess
   plan.relaxation = s.relaxation ? *s.relaxation : 1.9;
   plan.record_every = s.record_every;
   plan.free_radius = s.free_radius;
+  return true;
+}
+
+// Whether x < n / m exactly, for a double x within a factor 2 of n / m and whole n, m > 0
+// below 2^53: x m = p + e with p = x m rounded and e = fma(x, m, -p) exactly; p - n is
+// exact (Sterbenz) and, where not 0, a multiple of p's unit in the last place, so larger
+// than |e| and its sign is the sign of x m - n; where 0, the sign is e's.
+bool lies_below(double x, double n, double m) {
+  const double p = x * m;
+  const double e = std::fma(x, m, -p);
+  const double d = p - n;
+  return d < 0.0 || (d == 0.0 && e < 0.0);
+}
+
+// The least double at or above n / m, and the greatest double below it.
+double at_or_above(double n, double m) {
+  const double near = n / m;
+  return lies_below(near, n, m) ? std::nextafter(near, std::numeric_limits<double>::infinity()) : near;
+}
+double below(double n, double m) {
+  const double near = n / m;
+  return lies_below(near, n, m) ? near : std::nextafter(near, -std::numeric_limits<double>::infinity());
+}
+
+}  // namespace
+
+bool make_samples(const uint16_t* levels, uint32_t channels, uint32_t height, uint32_t width, uint32_t bits,
+                  const Settings& settings, SamplesModel& model, std::string& message, int32_t& status) {
+  status = 4;
+  if (settings.data.size() != 1) {
+    message = "a picture of rounded samples takes one value of each option of the data term, not one for each "
+              "component";
+    return false;
+  }
+  const DataTerm& term = settings.data[0];
+  DataTerm defaults;
+  defaults.mu = term.mu;
+  defaults.slack = term.slack;
+  if (term.mu_scale != defaults.mu_scale || term.mu_power != defaults.mu_power ||
+      term.mu_chroma != defaults.mu_chroma || term.dc_weight != defaults.dc_weight || term.mmse != defaults.mmse ||
+      term.power != defaults.power || term.slack_cost != defaults.slack_cost ||
+      term.flat_weight != defaults.flat_weight || term.flat_levels != defaults.flat_levels ||
+      term.flat_dc != defaults.flat_dc) {
+    message = "of the data term, a picture of rounded samples takes --mu and --slack alone: the others are for JPEG "
+              "files";
+    return false;
+  }
+  const size_t count = static_cast<size_t>(channels) * height * width;
+  if (levels == nullptr || count == 0) {
+    message = "a picture of " + std::to_string(channels) + " x " + std::to_string(height) + " x " +
+              std::to_string(width) + " levels, not empty";
+    return false;
+  }
+  if (bits != 1 && bits != 2 && bits != 4 && bits != 8 && bits != 16) {
+    message = "samples of 1, 2, 4, 8 or 16 bits, not " + std::to_string(bits);
+    return false;
+  }
+  const double mu = term.mu.value_or(0.0);
+  if (!(at_least_zero(mu) && at_least_zero(term.slack))) {
+    message = "the weight of the data term and the slack are at least 0 and finite";
+    return false;
+  }
+  const uint32_t top = (1u << bits) - 1u;
+  for (size_t k = 0; k < count; ++k) {
+    if (levels[k] > top) {
+      message = "a level of " + std::to_string(bits) + " bits is at most " + std::to_string(top) + ", not " +
+                std::to_string(levels[k]);
+      return false;
+    }
+  }
+  model.channels = channels;
+  model.height = height;
+  model.width = width;
+  model.bits = bits;
+  model.mu = mu;
+  model.slack = term.slack;
+  model.levels.assign(levels, levels + count);
+  model.low.resize(top + 1);
+  model.high.resize(top + 1);
+  model.centre.resize(top + 1);
+  const double wide = static_cast<double>(top), scale = 255.0 / wide;
+  for (uint32_t v = 0; v <= top; ++v) {
+    const double own = static_cast<double>(v);
+    double lower, upper;
+    if (term.slack == 0.0) {
+      lower = v == 0 ? 0.0 : at_or_above((2.0 * own - 1.0) * 255.0, 2.0 * wide);
+      upper = v == top ? 255.0 : below((2.0 * own + 1.0) * 255.0, 2.0 * wide);
+    } else {
+      lower = ((own - 0.5) - term.slack) * scale;
+      upper = ((own + 0.5) + term.slack) * scale;
+    }
+    model.low[v] = std::max(lower, 0.0);
+    model.high[v] = std::min(upper, 255.0);
+    model.centre[v] = (own * 255.0) / wide;
+  }
+  status = 0;
   return true;
 }
 

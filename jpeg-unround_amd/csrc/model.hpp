@@ -84,6 +84,36 @@ struct Plan {
 
 bool make_plan(const Settings& settings, size_t channels, Plan& plan, std::string& message);
 
+// A picture of rounded samples (docs/math.md 11; rust/unround/src/samples.rs:56-140):
+// channels x height x width levels of `bits` bits (1, 2, 4, 8 or 16), channel by channel,
+// each standing for the values that round to it on the scale of 0 to 255, with the data
+// term's weight mu (0: the intervals alone) and a slack in levels. The ends and centres of
+// every level 0..top, in binary64: without a slack the doubles x with
+// max(0, (v - 1/2) s) <= x < (v + 1/2) s, or x <= 255 at the top level, s = 255 / top,
+// each end exact; with a slack, ((v -+ 1/2) -+ slack) s in floating point, within 0 and
+// 255; the centre the double nearest v s.
+struct SamplesModel {
+  uint32_t channels = 0, height = 0, width = 0, bits = 8;
+  double mu = 0.0, slack = 0.0;
+  std::vector<uint16_t> levels;  // channels x height x width
+  std::vector<double> low, high, centre;  // per level, 0..top
+
+  uint32_t top() const { return (1u << bits) - 1u; }
+  size_t count() const { return static_cast<size_t>(channels) * height * width; }
+};
+
+// The model of the levels under the settings, whose data term holds --mu and --slack alone
+// (rust/unround/src/rounded.rs:44-70; samples.rs:74-110): false with the reason where
+// refused (status 4, options).
+bool make_samples(const uint16_t* levels, uint32_t channels, uint32_t height, uint32_t width, uint32_t bits,
+                  const Settings& settings, SamplesModel& model, std::string& message, int32_t& status);
+
+// The plan of TV or TGV on a picture of rounded samples: the defaults are a tolerance of
+// 1e-4, 10000 iterations and a step ratio of 10 (TV) or 0.3 (TGV), the tolerance times the
+// first-order weight and the ratio over its square unless weight scaling is off
+// (rust/unround/src/samples.rs:25-52).
+bool make_samples_plan(const Settings& settings, size_t channels, Plan& plan, std::string& message);
+
 // The Laplace model (docs/math.md 2.1-2.2; cpp/src/laplace.cpp:86-164).
 double laplace_scale(uint64_t zeros, uint64_t others, uint64_t sum, double step);
 double laplace_shrinkage(double rho);

@@ -341,6 +341,36 @@ __global__ __launch_bounds__(256) void value_kernel(const Component* __restrict_
   value_tile<T>(*component, zeta, partials, blockIdx.x);
 }
 
+// div p~ (from_r = 0) or div(-theta div_2 r~) (from_r = 1) of channel c at (i, j), in
+// binary64: the divergence that the dual values take their xi from.
+template <typename T, typename S>
+__device__ __forceinline__ double dual_divergence_at(const Fields<T, S>& F, int c, int i, int j, int H, int W,
+                                                     int from_r, double theta) {
+  const size_t N = static_cast<size_t>(H) * W;
+  if (!from_r) return device::divergence_at<double>(F.tp + (2 * c) * N, F.tp + (2 * c + 1) * N, i, j, H, W);
+  const S* rxx = F.tr + (3 * c) * N;
+  const S* ryy = F.tr + (3 * c + 1) * N;
+  const S* rxy = F.tr + (3 * c + 2) * N;
+  // g = -theta div_2 r~, then div g = delta_y g_y + delta_x g_x.
+  double dx, dy, gy_up = 0.0, gx_left = 0.0;
+  device::tensor_divergence_at<double>(rxx, ryy, rxy, i, j, H, W, dx, dy);
+  const double gx_here = -theta * dx, gy_here = -theta * dy;
+  if (i > 0) {
+    device::tensor_divergence_at<double>(rxx, ryy, rxy, i - 1, j, H, W, dx, dy);
+    gy_up = -theta * dy;
+  }
+  if (j > 0) {
+    device::tensor_divergence_at<double>(rxx, ryy, rxy, i, j - 1, H, W, dx, dy);
+    gx_left = -theta * dx;
+  }
+  const bool top = i == 0, bottom = i == H - 1;
+  const double down_part = (!top && !bottom) ? gy_here - gy_up : (!bottom ? gy_here : (!top ? -gy_up : 0.0));
+  const double across_part = W == 1 ? 0.0 : (j == 0 ? gx_here : (j == W - 1 ? -gx_left : gx_here - gx_left));
+  double div = down_part;
+  div += across_part;
+  return div;
+}
+
 // G_c* at xi = gamma div p~ (from_r = 0) or xi = gamma div(-theta div_2 r~) (from_r = 1),
 // with the free samples' terms (frames.cpp:616-681): a work-group's sum into partials.
 template <typename T, typename S>
@@ -357,14 +387,8 @@ __device__ __forceinline__ void conjugate_tile(const Component& K, const Fields<
   const int ty = active ? tile / K.tiles_x : 0, tx = active ? tile % K.tiles_x : 0;
   const bool inside = active && ty < K.rows && tx < K.columns;
   const int dv = K.down, dh = K.across;
-  const size_t N = static_cast<size_t>(H) * W;
   double theta = 1.0;
   if (from_r) theta = *most > alpha1 ? alpha1 / *most : 1.0;
-  const S* px = F.tp + (2 * c) * N;
-  const S* py = F.tp + (2 * c + 1) * N;
-  const S* rxx = F.tr + (3 * c) * N;
-  const S* ryy = F.tr + (3 * c + 1) * N;
-  const S* rxy = F.tr + (3 * c + 2) * N;
   double xis[kMaxCell * kMaxCell];
   double sum = 0.0;
 #pragma unroll
@@ -374,31 +398,7 @@ __device__ __forceinline__ void conjugate_tile(const Component& K, const Fields<
     for (int a = 0; a < kMaxCell; ++a) {
       if (active && a < dv && b < dh) {
         const int i = (8 * ty + m) * dv + a, j = (8 * tx + n) * dh + b;
-        double div = 0.0;
-        if (!from_r) {
-          div = device::divergence_at<double>(px, py, i, j, H, W);
-        } else {
-          // g = -theta div_2 r~, then div g = delta_y g_y + delta_x g_x.
-          double dx, dy, gy_up = 0.0, gx_left = 0.0;
-          device::tensor_divergence_at<double>(rxx, ryy, rxy, i, j, H, W, dx, dy);
-          const double gx_here = -theta * dx, gy_here = -theta * dy;
-          if (i > 0) {
-            device::tensor_divergence_at<double>(rxx, ryy, rxy, i - 1, j, H, W, dx, dy);
-            gy_up = -theta * dy;
-          }
-          if (j > 0) {
-            device::tensor_divergence_at<double>(rxx, ryy, rxy, i, j - 1, H, W, dx, dy);
-            gx_left = -theta * dx;
-          }
-          const bool top = i == 0, bottom = i == H - 1;
-          const double down_part =
-              (!top && !bottom) ? gy_here - gy_up : (!bottom ? gy_here : (!top ? -gy_up : 0.0));
-          const double across_part =
-              W == 1 ? 0.0 : (j == 0 ? gx_here : (j == W - 1 ? -gx_left : gx_here - gx_left));
-          div = down_part;
-          div += across_part;
-        }
-        const double xi = gamma * div;
+        const double xi = gamma * dual_divergence_at(F, c, i, j, H, W, from_r, theta);
         xis[kMaxCell * a + b] = xi;
         column = a == 0 ? xi : column + xi;
       }
@@ -588,6 +588,7 @@ class Solver {
       : frame_(frame), settings_(settings), plan_(plan), timing_(timing), stream_override_(stream) {
     if (stream_override_ == nullptr) own_stream_.emplace();
   }
+  virtual ~Solver() = default;
 
   void run(const StartState* start, const Observer& observer, SolveOutput& out) {
     const auto began = std::chrono::steady_clock::now();
@@ -612,17 +613,16 @@ class Solver {
         if (plan_.partial_tolerance > 0.0 && plan_.has_partial && record.partial / samples <= plan_.partial_tolerance)
           met = true;
         out.records.push_back(record);
-        if (met) {
-          stop = 1;
-          stopped = true;
-          return;
-        }
+        // The observer sees every record after the first, the one that meets a tolerance
+        // too, which then stops the solver as converged (rust/unround/src/pdhg.rs:645-665).
+        bool observed = false;
         if (observer && at > 0) {
           tilde_canvas(canvas_room);
-          if (observer(record, canvas_room.data())) {
-            stop = 3;
-            stopped = true;
-          }
+          observed = observer(record, canvas_room.data());
+        }
+        if (met || observed) {
+          stop = met ? 1 : 3;
+          stopped = true;
         }
       };
       if (every > 0 || most == 0) take(0);
@@ -674,23 +674,7 @@ class Solver {
     if (const char* rows = std::getenv("UNROUND_DUAL_ROWS")) dual_rows_ = std::max(1, std::atoi(rows));
     hipStream_t s = stream();
     allocate();
-    upload_components();
-    // The start.
-    std::vector<DeviceBuffer> given(C_);
-    for (int c = 0; c < C_; ++c) {
-      const double* coefficients = start != nullptr && static_cast<size_t>(c) < start->coefficients.size()
-                                       ? start->coefficients[c]
-                                       : nullptr;
-      double* device_given = nullptr;
-      if (coefficients != nullptr) {
-        const size_t bytes = sizeof(double) * 64 * frame_.components[c].blocks();
-        given[c] = DeviceBuffer(bytes);
-        HIP_CHECK(hipMemcpyAsync(given[c].pointer, coefficients, bytes, hipMemcpyHostToDevice, s));
-        device_given = given[c].as<double>();
-      }
-      start_kernel<T, S><<<(tiles(c) + 3) / 4, 256, 0, s>>>(component(c), fields_, c, H_, W_, device_given);
-      HIP_CHECK(hipGetLastError());
-    }
+    start_primal(start);
     const dim3 pixels((W_ + 63) / 64, (H_ + 3) / 4);
     if (solving) {
       upload_field(start != nullptr ? start->p : nullptr, fields_.p, 2);
@@ -711,10 +695,46 @@ class Solver {
     return solving;
   }
 
- private:
+ protected:
   template <typename, typename>
   friend class Batch;
   friend class Unrolled;
+
+  // What depends on the data term: a JPEG frame's here, a picture of rounded samples' in
+  // SamplesSolver.
+
+  // The coefficients kept (zeta), and the work-groups of the kernels of G and G*.
+  virtual void data_sizes(size_t& zeta, size_t& value_groups, size_t& conjugate_groups) const {
+    zeta = value_groups = conjugate_groups = 0;
+    for (int c = 0; c < C_; ++c) {
+      zeta += 64 * frame_.components[c].blocks();
+      value_groups += (64 * frame_.components[c].blocks() + 255) / 256;
+      conjugate_groups += static_cast<size_t>((tiles(c) + 3) / 4);
+    }
+  }
+
+  // The data term onto the device, and x and x~ of the start: the canvas of the
+  // coefficients given (or the centres) clipped to their intervals.
+  virtual void start_primal(const StartState* start) {
+    hipStream_t s = stream();
+    upload_components();
+    std::vector<DeviceBuffer> given(C_);
+    for (int c = 0; c < C_; ++c) {
+      const double* coefficients = start != nullptr && static_cast<size_t>(c) < start->coefficients.size()
+                                       ? start->coefficients[c]
+                                       : nullptr;
+      double* device_given = nullptr;
+      if (coefficients != nullptr) {
+        const size_t bytes = sizeof(double) * 64 * frame_.components[c].blocks();
+        given[c] = DeviceBuffer(bytes);
+        HIP_CHECK(hipMemcpyAsync(given[c].pointer, coefficients, bytes, hipMemcpyHostToDevice, s));
+        device_given = given[c].as<double>();
+      }
+      start_kernel<T, S><<<(tiles(c) + 3) / 4, 256, 0, s>>>(component(c), fields_, c, H_, W_, device_given);
+      HIP_CHECK(hipGetLastError());
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
 
   int tiles(int c) const {
     const ComponentModel& k = frame_.components[c];
@@ -739,8 +759,8 @@ class Solver {
       t_count += 3 * 2 * C * N_;  // w, w_next, tw
       s_count += 3 * 3 * C * N_;  // r, r_next, tr
     }
-    size_t zeta_count = 0;
-    for (const ComponentModel& k : frame_.components) zeta_count += 64 * k.blocks();
+    size_t zeta_count = 0, value_groups = 0, conjugate_groups = 0;
+    data_sizes(zeta_count, value_groups, conjugate_groups);
     const size_t t_bytes = sizeof(T) * (t_count + zeta_count);
     const size_t s_bytes = sizeof(S) * s_count;
     state_ = DeviceBuffer(t_bytes + s_bytes + 256);
@@ -777,11 +797,6 @@ class Solver {
     }
     // Partials of the records: as many as the work-groups of their kernels.
     const size_t pixel_groups = static_cast<size_t>((W_ + 63) / 64) * ((H_ + 3) / 4);
-    size_t value_groups = 0, conjugate_groups = 0;
-    for (int c = 0; c < C_; ++c) {
-      value_groups += (64 * frame_.components[c].blocks() + 255) / 256;
-      conjugate_groups += static_cast<size_t>((tiles(c) + 3) / 4);
-    }
     norm_partials_count_ = pixel_groups;
     value_partials_count_ = value_groups;
     conjugate_partials_count_ = conjugate_groups;
@@ -875,14 +890,13 @@ class Solver {
     }
   }
 
-  void iterate(bool keep) {
+  // The proximal step of every component, each timed on its own where timing asks.
+  virtual void primal_step(bool keep) {
     hipStream_t s = stream();
     const T rho = static_cast<T>(plan_.relaxation), rest = static_cast<T>(1.0 - plan_.relaxation);
     const bool per_kernel = timing_ == Timing::kernels;
-    if (per_kernel) {
+    if (per_kernel)
       for (int c = 0; c < C_; ++c) primal_events_.emplace_back();
-      dual_events_.emplace_back();
-    }
     const size_t first_event = per_kernel ? primal_events_.size() - static_cast<size_t>(C_) : 0;
     for (int c = 0; c < C_; ++c) {
       const ComponentModel& k = frame_.components[c];
@@ -911,6 +925,14 @@ class Solver {
       HIP_CHECK(hipGetLastError());
       if (per_kernel) HIP_CHECK(hipEventRecord(primal_events_[first_event + c].end, s));
     }
+  }
+
+  void iterate(bool keep) {
+    hipStream_t s = stream();
+    const T rho = static_cast<T>(plan_.relaxation), rest = static_cast<T>(1.0 - plan_.relaxation);
+    const bool per_kernel = timing_ == Timing::kernels;
+    primal_step(keep);
+    if (per_kernel) dual_events_.emplace_back();
     Steps<T> steps{};
     for (int c = 0; c < C_; ++c) {
       steps.descent[c] = static_cast<T>(plan_.tau * plan_.gamma[c]);
@@ -982,6 +1004,33 @@ class Solver {
     // results[0..3]: first, second, most, residual.
     reduce_kernel<<<4, 256, 0, s>>>(norm_partials, static_cast<int>(norm_partials_count_), 4, 1u << 2, results);
     HIP_CHECK(hipGetLastError());
+    data_records(value_partials, conjugate_partials, partial_partials, results);
+    double host[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    HIP_CHECK(hipMemcpyAsync(host, results, sizeof(double) * 7, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    Record record;
+    record.iteration = iteration;
+    const double data = data_value(host[4]);
+    if (!tgv_) {
+      record.primal = settings_.alpha * host[0] + data;
+      record.theta = std::nan("");
+      record.partial = std::nan("");
+    } else {
+      record.primal = (settings_.alpha1 * host[0] + settings_.alpha0 * host[1]) + data;
+      const double most = host[2];
+      record.theta = most > settings_.alpha1 ? settings_.alpha1 / most : 1.0;
+      record.partial = plan_.has_partial ? record.primal + host[6] + plan_.partial_radius * host[3] : std::nan("");
+    }
+    record.dual = -host[5];
+    return record;
+  }
+
+  // The data term's parts of a record: G's sum into results[4], G* at the dual's xi (TV:
+  // gamma div p~; TGV: the feasible dual's, from the largest norm of div_2 r~ in results[2])
+  // into results[5] and, for TGV's partial gap, G* at gamma div p~ into results[6].
+  virtual void data_records(double* value_partials, double* conjugate_partials, double* partial_partials,
+                            double* results) {
+    hipStream_t s = stream();
     size_t offset = 0;
     for (int c = 0; c < C_; ++c) {
       const size_t groups = (64 * frame_.components[c].blocks() + 255) / 256;
@@ -1007,25 +1056,10 @@ class Solver {
     };
     conjugates(tgv_ ? 1 : 0, conjugate_partials, results + 5);
     if (tgv_ && plan_.has_partial) conjugates(0, partial_partials, results + 6);
-    double host[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    HIP_CHECK(hipMemcpyAsync(host, results, sizeof(double) * 7, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    Record record;
-    record.iteration = iteration;
-    const double data = host[4];
-    if (!tgv_) {
-      record.primal = settings_.alpha * host[0] + data;
-      record.theta = std::nan("");
-      record.partial = std::nan("");
-    } else {
-      record.primal = (settings_.alpha1 * host[0] + settings_.alpha0 * host[1]) + data;
-      const double most = host[2];
-      record.theta = most > settings_.alpha1 ? settings_.alpha1 / most : 1.0;
-      record.partial = plan_.has_partial ? record.primal + host[6] + plan_.partial_radius * host[3] : std::nan("");
-    }
-    record.dual = -host[5];
-    return record;
   }
+
+  // G of the sum its kernels reduced.
+  virtual double data_value(double sum) const { return sum; }
 
   void tilde_canvas(std::vector<double>& canvas) {
     std::vector<T> values(static_cast<size_t>(C_) * N_);
@@ -1035,7 +1069,8 @@ class Solver {
     canvas.assign(values.begin(), values.end());
   }
 
-  void finish(SolveOutput& out) {
+  // The final projection onto the data term's intervals, in binary64, into out.
+  virtual void finish(SolveOutput& out) {
     hipStream_t s = stream();
     DeviceBuffer canvas(sizeof(double) * C_ * N_);
     std::vector<DeviceBuffer> coefficients;
@@ -1081,6 +1116,7 @@ class Solver {
 };
 
 #include "batch.hpp"
+#include "samples.hpp"
 #include "unrolled.hpp"
 
 class UnrolledOnGpu final : public UnrolledSolve {
@@ -1152,6 +1188,34 @@ bool solve_on_gpu(const FrameModel& frame, const Settings& settings, const Plan&
     } else {
       Solver<float, float> solver(frame, settings, plan, timing);
       solver.run(start, observer, out);
+    }
+    return true;
+  } catch (const GpuError& error) {
+    status = 6;
+    message = std::string("a GPU error: ") + error.what();
+    return false;
+  }
+}
+
+bool solve_samples_on_gpu(const SamplesModel& model, const Settings& settings, const Plan& plan,
+                          const Observer& observer, Timing timing, SolveOutput& out, std::string& message,
+                          int32_t& status) {
+  try {
+    int devices = 0;
+    if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) {
+      status = 6;
+      message = "no HIP device: this library computes on an AMD GPU";
+      return false;
+    }
+    if (settings.precision == Precision::dual) {
+      SamplesSolver<double, double> solver(model, settings, plan, timing);
+      solver.run(nullptr, observer, out);
+    } else if (settings.precision == Precision::mixed) {
+      SamplesSolver<float, bf16> solver(model, settings, plan, timing);
+      solver.run(nullptr, observer, out);
+    } else {
+      SamplesSolver<float, float> solver(model, settings, plan, timing);
+      solver.run(nullptr, observer, out);
     }
     return true;
   } catch (const GpuError& error) {

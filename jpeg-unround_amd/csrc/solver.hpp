@@ -67,6 +67,13 @@ struct SolveOutput {
 bool solve_on_gpu(const FrameModel& frame, const Settings& settings, const Plan& plan, const StartState* start,
                   const Observer& observer, Timing timing, SolveOutput& out, std::string& message, int32_t& status);
 
+// A picture of rounded samples (TV or TGV within its intervals) on the current HIP device:
+// out.canvas is the picture, channels x height x width, every sample within its interval;
+// no coefficients.
+bool solve_samples_on_gpu(const SamplesModel& model, const Settings& settings, const Plan& plan,
+                          const Observer& observer, Timing timing, SolveOutput& out, std::string& message,
+                          int32_t& status);
+
 // Frames solved together on the GPU, with the same settings (and so the same iterations and
 // records) and each its own plan: the outputs are each frame's own solve's (its iterates,
 // records and stopping iteration), and latencies[m] the seconds from the first iteration

@@ -42,6 +42,8 @@ __all__ = [
     "held_options",
     "options",
     "solve",
+    "solve_batch",
+    "solve_samples",
     "to_rgb",
     "to_ycbcr",
     "version",
@@ -53,6 +55,7 @@ Observer = Callable[["Record"], "bool | None"]
 ABI_VERSION: Final = 5
 
 _MESSAGE_SIZE: Final = 4096
+_FORMAT_PNG: Final = 2
 _BLOCK_SIZE: Final = 64
 
 
@@ -167,6 +170,11 @@ def _declare_all(library: ctypes.CDLL) -> None:
     _declare(library, "unround_solve_start", i32, ctypes.POINTER(_Input), handle, ctypes.POINTER(_Start), *made)
     _declare(library, "unround_solve_batch", i32, ctypes.POINTER(_Input), size, handle, ctypes.POINTER(handle), doubles,
              *message)
+    _declare(library, "unround_solve_samples", i32, ctypes.POINTER(ctypes.c_uint16), u64, u64, u64, ctypes.c_uint32,
+             handle, *made)
+    _declare(library, "unround_result_alpha_solver", i32, handle, u64_out, i32_out)
+    _declare(library, "unround_result_alpha_record_iterations", u64_out, handle, u64_out)
+    _declare(library, "unround_result_alpha_record_values", doubles, handle, i32, u64_out)
     _declare(library, "unround_result_free", None, handle)
     _declare(library, "unround_result_picture", doubles, handle, u64_out, u64_out, u64_out)
     _declare(library, "unround_result_planes", doubles, handle, u64_out)
@@ -443,16 +451,20 @@ def _problem(library: ctypes.CDLL, result: ctypes.c_void_p, index: int) -> tuple
     return coefficients, Problem((int(down.value), int(across.value)), lower, upper, centres, steps, weights)
 
 
-def _solution(library: ctypes.CDLL, result: ctypes.c_void_p) -> Solution | None:
+def _solution(library: ctypes.CDLL, result: ctypes.c_void_p, *, alpha: bool = False) -> Solution | None:
+    """The solver of the result, or of its alpha (python/src/unround/native.py:548-575)."""
+    solver, recorded_at, values_of = (
+        (library.unround_result_alpha_solver, library.unround_result_alpha_record_iterations,
+         library.unround_result_alpha_record_values) if alpha else
+        (library.unround_result_solver, library.unround_result_record_iterations, library.unround_result_record_values))
     iterations, stop = ctypes.c_uint64(), ctypes.c_int32()
-    if not library.unround_result_solver(result, ctypes.byref(iterations), ctypes.byref(stop)):
+    if not solver(result, ctypes.byref(iterations), ctypes.byref(stop)):
         return None
     count = ctypes.c_uint64()
-    recorded = library.unround_result_record_iterations(result, ctypes.byref(count))
+    recorded = recorded_at(result, ctypes.byref(count))
     size = int(count.value)
     steps = np.array(np.ctypeslib.as_array(recorded, shape=(size,)), dtype=np.int64) if size else np.zeros(0, np.int64)
-    values = [_checked(library.unround_result_record_values(result, which, ctypes.byref(count)), count, (size,))
-              for which in range(5)]
+    values = [_checked(values_of(result, which, ctypes.byref(count)), count, (size,)) for which in range(5)]
     seconds, primal, dual, scaling, partial_gap = values
     return Solution(int(iterations.value), Stop(int(stop.value)), History(steps, seconds, primal, dual, scaling,
                                                                           partial_gap))
@@ -491,7 +503,9 @@ def _decoded(library: ctypes.CDLL, result: ctypes.c_void_p) -> Decoded:
         result=_solution(library, result),
         icc_profile=None,
         exif_orientation=int(library.unround_result_exif_orientation(result)),
+        input_format="png" if int(library.unround_result_format(result)) == _FORMAT_PNG else "jpeg",
         bits=int(library.unround_result_bits(result)),
+        alpha_result=_solution(library, result, alpha=True),
         kernel_times=_times(library, result),
     )
 
@@ -606,6 +620,33 @@ def solve(image: jpegio.Image, settings: Settings | list[str] | None = None, *, 
     decoded = _reconstruct(call, observer)
     del kept
     return decoded
+
+
+def solve_samples(levels: npt.ArrayLike, bits: int = 8, settings: Settings | list[str] | None = None, *,
+                  observer: Observer | None = None) -> Decoded:
+    """Reconstructs a picture of rounded samples on the GPU, as decode() does a PNG file
+    (rust/unround/src/rounded.rs:71-170): levels (height, width) or (height, width, channels)
+    of `bits` bits, channels 1 grey, 2 grey and alpha, 3 RGB, 4 RGB and alpha; TV or TGV
+    within the intervals of the levels with --mu and --slack alone of the data term, the
+    colours together and the alpha alone. The Decoded is a PNG file's: input_format "png",
+    the bits, no components, and alpha_result where there is alpha."""
+    library = _library()
+    handle = _Settings(settings if isinstance(settings, list) else options(settings))
+    given = np.asarray(levels)
+    if given.ndim not in (2, 3):
+        raise ValueError(f"levels are (height, width) or (height, width, channels), not {given.shape}")
+    if given.size and (np.min(given) < 0 or np.max(given) > 65535):
+        raise ValueError("levels lie in 0..65535")
+    values = np.ascontiguousarray(given, dtype=np.uint16)
+    height, width = int(values.shape[0]), int(values.shape[1])
+    channels = 1 if values.ndim == 2 else int(values.shape[2])
+
+    def call(callback: object, result: object, message: ctypes.Array) -> int:
+        return int(library.unround_solve_samples(values.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), height, width,
+                                                 channels, bits, handle.handle, callback, None, result, message,
+                                                 _MESSAGE_SIZE))
+
+    return _reconstruct(call, observer)
 
 
 def solve_batch(images: Sequence[jpegio.Image], settings: Settings | list[str] | None = None

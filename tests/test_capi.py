@@ -127,6 +127,26 @@ def test_batch_refusals_come_before_the_gpu():
     assert caught.value.status == native.Status.UNSUPPORTED
 
 
+def test_samples_refusals_come_before_the_gpu():
+    # What a PNG file's reconstruction refuses (rust/unround/src/rounded.rs:44-90;
+    # rust/unround/src/samples.rs:74-110), refused before any GPU work.
+    levels = np.zeros((4, 5, 3), np.uint16)
+    cases = [
+        (np.zeros((4, 5, 5), np.uint16), 8, ["--method", "tgv"], native.Status.ARGUMENT, "1 to 4 channels"),
+        (levels, 3, ["--method", "tgv"], native.Status.OPTIONS, "bits"),
+        (levels + 256, 8, ["--method", "tgv"], native.Status.OPTIONS, "at most 255"),
+        (levels, 8, ["--method", "mmse"], native.Status.OPTIONS, "TV or TGV"),
+        (levels, 8, ["--method", "tv", "--flat-weight", "1"], native.Status.OPTIONS, "--mu and --slack alone"),
+        (levels, 8, ["--method", "tv", "--mu", "1,2,3"], native.Status.OPTIONS, "one value of each option"),
+        (levels, 8, ["--method", "tv", "--channel-weights", "1,1"], native.Status.OPTIONS, "channel weights"),
+    ]
+    for given, bits, options, status, words in cases:
+        with pytest.raises(native.UnroundError) as caught:
+            native.solve_samples(given, bits, options)
+        assert caught.value.status == status, (options, caught.value.message)
+        assert words in caught.value.message, (options, caught.value.message)
+
+
 def test_refused_frames():
     grey = unround.jpegio.Component(np.zeros((2, 2, 8, 8), np.int16), np.ones(64, np.uint16))
     # Blocks that do not cover the picture, a colour space with the wrong count of

@@ -44,8 +44,8 @@ def test_gpu_double_reproduces_case(name):
         assert np.all(coefficients > problem.lower) and np.all(coefficients < problem.upper)
     expected = case.canvas()
     tolerance = case.data["tolerance_canvas"]
-    # The observer sees every record after the first but a stopping one (as the reference's
-    # run loop calls it, cpp/src/primal_dual.hpp:984-998).
+    # The observer sees every record after the first, the stopping one too (as the
+    # reference's run loop behind its C interface calls it, rust/unround/src/pdhg.rs:645-665).
     if seen and decoded.result is not None and seen[-1].iteration == decoded.result.iterations:
         raw = seen[-1].canvas
         distance = float(np.sqrt(np.sum((raw - expected) ** 2)))
