@@ -43,7 +43,8 @@ def build() -> Path:
     with ThreadPoolExecutor(max_workers=min(8, len(sources))) as pool:
         objects = list(pool.map(compile_one, sources))
     temporary = TARGET.with_suffix(".so.tmp")
-    subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", str(temporary), *map(str, objects)], check=True)
+    subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", str(temporary), *map(str, objects), "-lz"],
+                   check=True)
     temporary.replace(TARGET)
     return TARGET
 

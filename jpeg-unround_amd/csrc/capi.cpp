@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "files.hpp"
 #include "model.hpp"
 #include "settings.hpp"
 #include "solver.hpp"
@@ -46,6 +47,7 @@ struct unround_result {
   unround::SolveOutput alpha;
   std::vector<uint64_t> alpha_record_iterations;
   std::array<std::vector<double>, 5> alpha_record_values;
+  std::vector<uint8_t> icc_profile;
 };
 
 // The unrolled solve's handle: its frames (with their settings) and plans, which the solve
@@ -346,6 +348,43 @@ unround_status reconstruct_samples(const uint16_t* levels, uint64_t height, uint
   return UNROUND_OK;
 }
 
+// A file reconstructed (rust/capi/src/lib.rs:532-565): a PNG file within the intervals of
+// its rounded samples; a JPEG file needs a JPEG reader, which this library has not.
+unround_status decode_file(const uint8_t* data, size_t size, const unround_settings* settings,
+                           unround_observer observer, void* user, unround_result** result, std::string& message) {
+  if (data == nullptr || settings == nullptr || result == nullptr) {
+    message = "the data, the settings and the result are not null pointers";
+    return UNROUND_ERROR_ARGUMENT;
+  }
+  *result = nullptr;
+  if (!unround::is_png(data, size)) {
+    message = "reading JPEG files is not part of this library: give the components as arrays to unround_solve";
+    return UNROUND_ERROR_UNSUPPORTED;
+  }
+  // What a PNG file takes is checked before it is read (rust/unround/src/rounded.rs:77-95).
+  const unround::Settings& s = settings->settings;
+  double mu = 0.0, slack = 0.0;
+  if (!unround::samples_data_term(s, mu, slack, message)) return UNROUND_ERROR_OPTIONS;
+  if (s.method != unround::Method::tv && s.method != unround::Method::tgv) {
+    message = "a PNG file is reconstructed by TV or TGV";
+    return UNROUND_ERROR_OPTIONS;
+  }
+  unround::PngImage image;
+  int32_t status = UNROUND_OK;
+  if (!unround::read_png(data, size, s.max_pixels, image, message, status)) {
+    // The file layer's statuses (rust/unround/src/rounded.rs:90-94): unsupported stays so,
+    // a limit is the options', anything else a file that cannot be read.
+    return status == 3 ? UNROUND_ERROR_UNSUPPORTED : (status == 4 ? UNROUND_ERROR_OPTIONS : UNROUND_ERROR_READ);
+  }
+  const unround_status done = reconstruct_samples(image.samples.data(), image.height, image.width, image.channels,
+                                                  image.bits, settings, observer, user, result, message);
+  if (done == UNROUND_OK) {
+    (*result)->icc_profile = std::move(image.icc_profile);
+    (*result)->warning = image.warning;
+  }
+  return done;
+}
+
 unround_status reconstruct_batch(const unround_input* inputs, size_t count, const unround_settings* settings,
                                  unround_result** results, double* latencies, std::string& message) {
   if (results == nullptr || (count > 0 && inputs == nullptr) || settings == nullptr) {
@@ -433,13 +472,12 @@ size_t unround_settings_options(const unround_settings* settings, char* buffer, 
   }
 }
 
-unround_status unround_decode(const uint8_t*, size_t, const unround_settings*, unround_observer, void*,
-                              unround_result** result, char* message, size_t message_size) {
-  if (result != nullptr) *result = nullptr;
-  write_message(message, message_size,
-                "reading JPEG and PNG files is not part of this library: give the components as arrays to "
-                "unround_solve");
-  return UNROUND_ERROR_UNSUPPORTED;
+unround_status unround_decode(const uint8_t* data, size_t size, const unround_settings* settings,
+                              unround_observer observer, void* user, unround_result** result, char* message,
+                              size_t message_size) {
+  return guarded(message, message_size, [&](std::string& text) {
+    return decode_file(data, size, settings, observer, user, result, text);
+  });
 }
 
 unround_status unround_solve(const unround_input* input, const unround_settings* settings, unround_observer observer,
@@ -719,9 +757,13 @@ const double* unround_result_alpha_record_values(const unround_result* result, i
   return result->alpha_record_values[which].data();
 }
 
-const uint8_t* unround_result_icc_profile(const unround_result*, uint64_t* size) {
-  if (size != nullptr) *size = 0;
-  return nullptr;
+const uint8_t* unround_result_icc_profile(const unround_result* result, uint64_t* size) {
+  if (result == nullptr || result->icc_profile.empty()) {
+    if (size != nullptr) *size = 0;
+    return nullptr;
+  }
+  if (size != nullptr) *size = result->icc_profile.size();
+  return result->icc_profile.data();
 }
 
 int32_t unround_result_exif_orientation(const unround_result*) { return 0; }

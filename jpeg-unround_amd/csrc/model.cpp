@@ -381,18 +381,14 @@ double below(double n, double m) {
 
 }  // namespace
 
-bool make_samples(const uint16_t* levels, uint32_t channels, uint32_t height, uint32_t width, uint32_t bits,
-                  const Settings& settings, SamplesModel& model, std::string& message, int32_t& status) {
-  status = 4;
+bool samples_data_term(const Settings& settings, double& mu, double& slack, std::string& message) {
   if (settings.data.size() != 1) {
     message = "a picture of rounded samples takes one value of each option of the data term, not one for each "
               "component";
     return false;
   }
   const DataTerm& term = settings.data[0];
-  DataTerm defaults;
-  defaults.mu = term.mu;
-  defaults.slack = term.slack;
+  const DataTerm defaults;
   if (term.mu_scale != defaults.mu_scale || term.mu_power != defaults.mu_power ||
       term.mu_chroma != defaults.mu_chroma || term.dc_weight != defaults.dc_weight || term.mmse != defaults.mmse ||
       term.power != defaults.power || term.slack_cost != defaults.slack_cost ||
@@ -402,6 +398,16 @@ bool make_samples(const uint16_t* levels, uint32_t channels, uint32_t height, ui
               "files";
     return false;
   }
+  mu = term.mu.value_or(0.0);
+  slack = term.slack;
+  return true;
+}
+
+bool make_samples(const uint16_t* levels, uint32_t channels, uint32_t height, uint32_t width, uint32_t bits,
+                  const Settings& settings, SamplesModel& model, std::string& message, int32_t& status) {
+  status = 4;
+  double mu = 0.0, slack = 0.0;
+  if (!samples_data_term(settings, mu, slack, message)) return false;
   const size_t count = static_cast<size_t>(channels) * height * width;
   if (levels == nullptr || count == 0) {
     message = "a picture of " + std::to_string(channels) + " x " + std::to_string(height) + " x " +
@@ -412,8 +418,7 @@ bool make_samples(const uint16_t* levels, uint32_t channels, uint32_t height, ui
     message = "samples of 1, 2, 4, 8 or 16 bits, not " + std::to_string(bits);
     return false;
   }
-  const double mu = term.mu.value_or(0.0);
-  if (!(at_least_zero(mu) && at_least_zero(term.slack))) {
+  if (!(at_least_zero(mu) && at_least_zero(slack))) {
     message = "the weight of the data term and the slack are at least 0 and finite";
     return false;
   }
@@ -430,7 +435,7 @@ bool make_samples(const uint16_t* levels, uint32_t channels, uint32_t height, ui
   model.width = width;
   model.bits = bits;
   model.mu = mu;
-  model.slack = term.slack;
+  model.slack = slack;
   model.levels.assign(levels, levels + count);
   model.low.resize(top + 1);
   model.high.resize(top + 1);
@@ -439,12 +444,12 @@ bool make_samples(const uint16_t* levels, uint32_t channels, uint32_t height, ui
   for (uint32_t v = 0; v <= top; ++v) {
     const double own = static_cast<double>(v);
     double lower, upper;
-    if (term.slack == 0.0) {
+    if (slack == 0.0) {
       lower = v == 0 ? 0.0 : at_or_above((2.0 * own - 1.0) * 255.0, 2.0 * wide);
       upper = v == top ? 255.0 : below((2.0 * own + 1.0) * 255.0, 2.0 * wide);
     } else {
-      lower = ((own - 0.5) - term.slack) * scale;
-      upper = ((own + 0.5) + term.slack) * scale;
+      lower = ((own - 0.5) - slack) * scale;
+      upper = ((own + 0.5) + slack) * scale;
     }
     model.low[v] = std::max(lower, 0.0);
     model.high[v] = std::min(upper, 255.0);

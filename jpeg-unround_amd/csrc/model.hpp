@@ -102,6 +102,11 @@ struct SamplesModel {
   size_t count() const { return static_cast<size_t>(channels) * height * width; }
 };
 
+// The data term of a picture of rounded samples from the settings: --mu (0 where it follows
+// JPEG's rule) and --slack, one value each; the other options of the data term are JPEG's
+// and refused where they are not their defaults (rust/unround/src/rounded.rs:44-70).
+bool samples_data_term(const Settings& settings, double& mu, double& slack, std::string& message);
+
 // The model of the levels under the settings, whose data term holds --mu and --slack alone
 // (rust/unround/src/rounded.rs:44-70; samples.rs:74-110): false with the reason where
 // refused (status 4, options).

@@ -189,6 +189,7 @@ def _declare_all(library: ctypes.CDLL) -> None:
     _declare(library, "unround_result_record_values", doubles, handle, i32, u64_out)
     _declare(library, "unround_result_kernel_times", i32, handle, doubles, size)
     _declare(library, "unround_result_exif_orientation", i32, handle)
+    _declare(library, "unround_result_icc_profile", ctypes.POINTER(ctypes.c_uint8), handle, u64_out)
     _declare(library, "unround_result_format", i32, handle)
     _declare(library, "unround_result_bits", i32, handle)
     _declare(library, "unround_result_warning", ctypes.c_char_p, handle)
@@ -490,6 +491,8 @@ def _decoded(library: ctypes.CDLL, result: ctypes.c_void_p) -> Decoded:
     planes = _checked(planes_pointer, planes_count, (count, rows, columns))
     canvas = _copy(canvas_pointer, (count, int(canvas_height.value), int(canvas_width.value)))
     components = [_problem(library, result, index) for index in range(int(library.unround_result_components(result)))]
+    profile_size = ctypes.c_uint64()
+    profile = library.unround_result_icc_profile(result, ctypes.byref(profile_size))
     warning = (library.unround_result_warning(result) or b"").decode("utf-8", "replace")
     if warning:
         warnings.warn(warning, UnroundWarning, stacklevel=4)
@@ -501,7 +504,7 @@ def _decoded(library: ctypes.CDLL, result: ctypes.c_void_p) -> Decoded:
         coefficients=tuple(coefficients for coefficients, _ in components),
         problems=tuple(problem for _, problem in components),
         result=_solution(library, result),
-        icc_profile=None,
+        icc_profile=ctypes.string_at(profile, int(profile_size.value)) if profile else None,
         exif_orientation=int(library.unround_result_exif_orientation(result)),
         input_format="png" if int(library.unround_result_format(result)) == _FORMAT_PNG else "jpeg",
         bits=int(library.unround_result_bits(result)),
@@ -549,14 +552,17 @@ def _reconstruct(call: Callable, observer: Observer | None) -> Decoded:
         library.unround_result_free(result)
 
 
-def decode(source: bytes | jpegio.Image, settings: Settings | None = None, *,
+def decode(source: bytes | jpegio.Image, settings: Settings | list[str] | None = None, *,
            observer: Observer | None = None) -> Decoded:
-    """A file's bytes are not read here (the library has no JPEG reader): UnroundError with
-    Status.UNSUPPORTED; an Image is reconstructed as solve() does."""
+    """Reconstructs a file's bytes (python/src/unround/native.py:677-716): a PNG file within
+    the intervals of its rounded samples, by TV or TGV with mu and slack alone of the data
+    term, its colours together and its alpha alone; a JPEG file needs a JPEG reader, which
+    the library has not (UnroundError with Status.UNSUPPORTED). An Image is reconstructed
+    as solve() does."""
     if isinstance(source, jpegio.Image):
         return solve(source, settings, observer=observer)
     library = _library()
-    handle = _Settings(options(settings))
+    handle = _Settings(settings if isinstance(settings, list) else options(settings))
     buffer = bytes(source)
 
     def call(callback: object, result: object, message: ctypes.Array) -> int:

@@ -184,3 +184,46 @@ def test_an_observer_stops_the_solve():
     decoded = gpu(levels, 8, ["--method", "tv"], observer=lambda record: record.iteration >= 30)
     assert decoded.result.stop == native.Stop.OBSERVER and decoded.result.iterations == 30
     assert_within(planes_of(decoded), levels, 8)
+
+
+def test_a_png_file_decodes_as_its_levels_solve():
+    # unround_decode of a PNG file (rust/capi/src/lib.rs:550-554): the file layer's levels,
+    # then the solve of solve_samples, bit for bit; the ICC profile kept; a tRNS chunk warned of.
+    import io
+    import struct
+    import warnings
+
+    from PIL import Image
+
+    colours = noisy(3, 20, 28, 8, seed=21)
+    alpha = smooth(1, 20, 28)[1]
+    rgba = interleaved(np.concatenate([colours, alpha])).astype(np.uint8)
+    profile = bytearray(160)
+    profile[0:4] = struct.pack(">I", 160)
+    profile[36:40] = b"acsp"
+    buffer = io.BytesIO()
+    Image.fromarray(rgba, mode="RGBA").save(buffer, "PNG", icc_profile=bytes(profile))
+    options = ["--method", "tgv", "--iterations", "120", "--tolerance", "0"]
+    print("decode of a 20x28 RGBA PNG file, 120 iterations", flush=True)
+    decoded = native.decode(buffer.getvalue(), options)
+    direct = native.solve_samples(rgba, 8, options)
+    assert decoded.input_format == "png" and decoded.bits == 8
+    assert np.array_equal(decoded.picture, direct.picture) and np.array_equal(decoded.planes, direct.planes)
+    assert decoded.icc_profile == bytes(profile)
+    assert decoded.alpha_result is not None and decoded.alpha_result.iterations == 120
+    assert int(decoded.color_space) == 3
+    # An RGB file with a tRNS chunk: reconstructed, and the colour not taken is said.
+    rgb = np.ascontiguousarray(rgba[..., :3])
+    buffer = io.BytesIO()
+    Image.fromarray(rgb, mode="RGB").save(buffer, "PNG", transparency=(1, 2, 3))
+    data = buffer.getvalue()
+    assert b"tRNS" in data
+    with warnings.catch_warnings(record=True) as caught:
+        warnings.simplefilter("always")
+        decoded = native.decode(data, options)
+    assert any("tRNS" in str(w.message) for w in caught)
+    assert decoded.alpha_result is None and decoded.picture.shape == (20, 28, 3)
+    # What a PNG file does not take is refused before it is read.
+    with pytest.raises(native.UnroundError) as refused:
+        native.decode(data, ["--method", "mmse"])
+    assert refused.value.status == native.Status.OPTIONS
