@@ -1,6 +1,7 @@
 """The benchmark: TGV primal-dual iterations on one MI355X, as Mpixel-iterations per second.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c4] [--precision single|mixed|double]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c4|png|c3|c5]
+                [--precision single|mixed|double]
 
 A step is one iteration of the method over the whole canvas of the workload (all its
 channels). The default workload is the 2048 x 2048 RGB picture of seed 0, JFIF YCbCr
@@ -25,6 +26,11 @@ workload. iteration_achieved is the same for a whole iteration (2 x the state's 
 
 cpu_baseline: the CPU oracle (binary64, one thread) on a bounded sample of the same kind
 of workload, timed on this host.
+
+--config png: the same measurement for a picture of rounded samples (unround_solve_samples,
+the solve behind a PNG file): 2048 x 2048 RGB levels of 8 bits, TGV in binary32; its
+primal kernel is the per-sample proximal step (samples_prox_kernel), the data term 2
+bytes a sample.
 """
 
 from __future__ import annotations
@@ -61,7 +67,21 @@ def workload(name: str):
     if name == "c4":
         picture = synthetic.grey_picture(4096, 4096, 0)
         return synthetic.encode_grey(picture, 40), "tgv", "C4: 4096x4096 grey q40, TGV"
+    if name == "png":
+        return samples_levels(2048, 2048), "tgv", "PNG: 2048x2048 RGB 8-bit rounded samples, TGV"
     raise SystemExit(f"unknown config {name}")
+
+
+def samples_levels(height: int, width: int):
+    """A picture of rounded samples: the synthetic colour picture of seed 0 rounded to 8 bits,
+    (height, width, 3) uint16 levels."""
+    import numpy as np
+
+    from unround import synthetic
+
+    picture = synthetic.colour_picture(height, width, 0)
+    whole = np.floor(picture)
+    return np.clip(whole + (picture - whole >= 0.5), 0, 255).astype(np.uint16)
 
 
 def field_bytes(method: str, precision: str) -> float:
@@ -71,13 +91,29 @@ def field_bytes(method: str, precision: str) -> float:
     return iterate + 2 * dual if method == "tv" else 3 * iterate + 5 * dual
 
 
-def cpu_baseline(method: str, budget_seconds: float = 12.0) -> dict:
+def cpu_baseline(method: str, budget_seconds: float = 12.0, samples: bool = False) -> dict:
     """The oracle on a 512 x 512 sample of the workload's kind, about budget_seconds of work."""
     import numpy as np
 
     from oracle import oracle
     from unround import synthetic
 
+    if samples:
+        levels = np.ascontiguousarray(np.moveaxis(samples_levels(512, 512), -1, 0))
+
+        def run_samples(iterations: int) -> tuple[float, int]:
+            options = ["--method", method, "--iterations", str(iterations), "--tolerance", "0", "--record-every", "0"]
+            began = time.perf_counter()
+            result = oracle.solve_samples(levels, 8, options)
+            return time.perf_counter() - began, int(np.prod(result.picture.shape))
+
+        short, _ = run_samples(2)
+        longer, _ = run_samples(6)
+        iterations = max(4, int(budget_seconds / max((longer - short) / 4, 1e-6)))
+        seconds, count = run_samples(iterations)
+        return {"value": round(count * iterations / seconds / 1e6, 3), "unit": "Mpixel-iters/s", "cores": 1,
+                "kind": "port", "sample": f"512x512 RGB 8-bit rounded samples TGV, {iterations} iterations in "
+                                          f"{seconds:.1f} s (records and setup included)"}
     if method == "tv":
         picture = synthetic.grey_picture(512, 512, 0)
         image = synthetic.encode_grey(picture, 50)
@@ -340,14 +376,21 @@ def main() -> None:
 
     image, method, description = workload(args.config)
     log(f"workload {description}; library {native.library_path()}")
+    rounded_samples = args.config == "png"
+
+    def solve(options: list[str]):
+        if rounded_samples:
+            return native.solve_samples(image, 8, options)
+        return native.solve(image, options)
+
     base = ["--method", method, "--tolerance", "0", "--precision", args.precision]
     if args.warmup > 0:
-        native.solve(image, base + ["--iterations", str(args.warmup), "--record-every", "0"])
+        solve(base + ["--iterations", str(args.warmup), "--record-every", "0"])
     # The timed solve: one pair of events around its iterations, nothing between launches.
     os.environ["UNROUND_TIMING"] = "1"
     torch.cuda.synchronize()
     began = time.perf_counter()
-    decoded = native.solve(image, base + ["--iterations", str(args.steps), "--record-every", "0"])
+    decoded = solve(base + ["--iterations", str(args.steps), "--record-every", "0"])
     torch.cuda.synchronize()
     wall = time.perf_counter() - began
     times = decoded.kernel_times
@@ -355,7 +398,7 @@ def main() -> None:
     # The kernels' own durations: events around every launch, in a solve of their own.
     kernel_steps = min(args.steps, 100)
     os.environ["UNROUND_TIMING"] = "2"
-    launches = native.solve(image, base + ["--iterations", str(kernel_steps), "--record-every", "0"]).kernel_times
+    launches = solve(base + ["--iterations", str(kernel_steps), "--record-every", "0"]).kernel_times
     os.environ.pop("UNROUND_TIMING", None)
     assert launches is not None and launches.dual_launches == kernel_steps, "the library took no kernel timings"
 
@@ -363,7 +406,8 @@ def main() -> None:
     samples = channels * rows * columns
     seconds = times.iterations_seconds
     value = samples * args.steps / seconds / 1e6
-    coefficients = sum(int(np.prod(c.shape)) for c in decoded.coefficients)
+    # The data term's levels, 2 bytes each: every coefficient's, or every sample's.
+    coefficients = samples if rounded_samples else sum(int(np.prod(c.shape)) for c in decoded.coefficients)
     per_sample = 2 * field_bytes(method, args.precision) + 2 * coefficients / samples
     iteration_achieved = per_sample * samples / (seconds / args.steps) / 1e9
     dual_launch = launches.dual_seconds / launches.dual_launches
@@ -390,8 +434,8 @@ def main() -> None:
         "data": "synthetic",
         "config": {
             "workload": description,
-            "height": int(image.height),
-            "width": int(image.width),
+            "height": int(rows),
+            "width": int(columns),
             "channels": int(channels),
             "samples_per_iteration": int(samples),
             "precision": args.precision,
@@ -409,8 +453,18 @@ def main() -> None:
             "iteration_achieved": round(iteration_achieved, 1),
             "iteration_frac": round(iteration_achieved / HBM_PEAK_GBS, 4),
         },
-        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(method),
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(method, samples=rounded_samples),
     }
+    if rounded_samples:
+        # The per-sample proximal step reads x, p_x, p_y and the level and writes x- and x.
+        iterate = 8 if args.precision == "double" else 4
+        dual = 2 if args.precision == "mixed" else iterate
+        primal_bytes = (iterate + 2 * dual + 2 + 2 * iterate) * samples
+        primal_achieved = primal_bytes / primal_launch / 1e9
+        line["roofline"]["primal"] = {"kernel": "samples_prox_kernel", "bytes_per_launch": int(primal_bytes),
+                                      "launch_us": round(primal_launch * 1e6, 2),
+                                      "achieved": round(primal_achieved, 1),
+                                      "frac": round(primal_achieved / HBM_PEAK_GBS, 4)}
     if args.config == "c4" and args.precision != "double":
         line["accuracy"] = accuracy_against_binary64(image, decoded, base, args.steps)
     print(json.dumps(line), flush=True)

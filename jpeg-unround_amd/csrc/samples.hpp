@@ -34,31 +34,43 @@ __global__ __launch_bounds__(256) void samples_start_kernel(Fields<T, S> F, Samp
 // The proximal step of every channel (samples.rs:960-966, 171-182): v = x + tau gamma_c
 // div p (backward differences), x~ = clip((t c + v) r, a, b), then x- = 2 x~ - x and x
 // relaxed in place, and x~ kept where a record follows. A work-group takes 64 columns and
-// 4 rows, a lane a pixel; the neighbours of div p on the left and above are loads of the
-// lanes beside it and of the row above, which the caches serve.
-template <typename T, typename S>
-__global__ __launch_bounds__(256) void samples_prox_kernel(Fields<T, S> F, SampleTerm<T> term, int C, int H, int W,
+// 4 rows, a lane a pixel and its C channels; the neighbours of div p on the left and above
+// are loads of the lanes beside it and of the row above, which the caches serve. Every
+// load of every channel is issued before the first store, so that they are in flight
+// together.
+template <typename T, typename S, int C>
+__global__ __launch_bounds__(256) void samples_prox_kernel(Fields<T, S> F, SampleTerm<T> term, int H, int W,
                                                            Steps<T> steps, T t, T r, int keep) {
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
   const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (i >= H || j >= W) return;
   const size_t N = static_cast<size_t>(H) * W, at = static_cast<size_t>(i) * W + j;
+  const S* __restrict__ p = F.p;
+  T* __restrict__ x = F.x;
+  const uint16_t* __restrict__ levels = term.levels;
+  T across[C], left[C], down[C], up[C], old[C];
+  uint16_t level[C];
+#pragma unroll
   for (int c = 0; c < C; ++c) {
-    const S* px = F.p + (2 * c) * N;
-    const S* py = F.p + (2 * c + 1) * N;
-    const T across = device::widen(px[at]);
-    const T left = j > 0 ? static_cast<T>(device::widen(px[at - 1])) : T(0);
-    const T down = device::widen(py[at]);
-    const T up = i > 0 ? static_cast<T>(device::widen(py[at - W])) : T(0);
-    T div = backward_down_of(down, up, i, H);
-    div += backward_across_of(across, left, j, W);
-    const size_t k = c * N + at;
-    const T old = F.x[k];
-    const T v = old + steps.descent[c] * div;
-    const T* entry = term.ends + 4 * static_cast<size_t>(term.levels[k]);
+    const S* px = p + (2 * c) * N;
+    const S* py = p + (2 * c + 1) * N;
+    across[c] = device::widen(px[at]);
+    left[c] = j > 0 ? static_cast<T>(device::widen(px[at - 1])) : T(0);
+    down[c] = device::widen(py[at]);
+    up[c] = i > 0 ? static_cast<T>(device::widen(py[at - W])) : T(0);
+    old[c] = x[c * N + at];
+    level[c] = levels[c * N + at];
+  }
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    T div = backward_down_of(down[c], up[c], i, H);
+    div += backward_across_of(across[c], left[c], j, W);
+    const T v = old[c] + steps.descent[c] * div;
+    const T* entry = term.ends + 4 * static_cast<size_t>(level[c]);
     const T tilde = device::clamp_to(fma(t, entry[2], v) * r, entry[0], entry[1]);
-    F.xbar[k] = T(2) * tilde - old;
-    F.x[k] = steps.rho * tilde + steps.rest * old;
+    const size_t k = c * N + at;
+    F.xbar[k] = T(2) * tilde - old[c];
+    x[k] = steps.rho * tilde + steps.rest * old[c];
     if (keep) F.tx[k] = tilde;
   }
 }
@@ -200,9 +212,11 @@ class SamplesSolver : private SamplesShape, public Solver<T, S> {
     steps.rho = static_cast<T>(this->plan_.relaxation);
     steps.rest = static_cast<T>(1.0 - this->plan_.relaxation);
     const double t = this->plan_.tau * model_.mu;
-    samples_prox_kernel<T, S><<<pixels(), 256, 0, s>>>(this->fields_, term_, this->C_, this->H_, this->W_, steps,
-                                                       static_cast<T>(t), static_cast<T>(1.0 / (1.0 + t)),
-                                                       keep ? 1 : 0);
+    const auto prox = [&](auto kernel) {
+      kernel<<<pixels(), 256, 0, s>>>(this->fields_, term_, this->H_, this->W_, steps, static_cast<T>(t),
+                                      static_cast<T>(1.0 / (1.0 + t)), keep ? 1 : 0);
+    };
+    this->C_ == 1 ? prox(samples_prox_kernel<T, S, 1>) : prox(samples_prox_kernel<T, S, 3>);
     HIP_CHECK(hipGetLastError());
     if (per_kernel) HIP_CHECK(hipEventRecord(this->primal_events_.back().end, s));
   }
