@@ -14,9 +14,12 @@
  * Different settings and results can be used on different threads at once, and one can
  * be read by several.
  *
- * Reading files (libjpeg) is not part of this library: unround_decode and
- * unround_project return UNROUND_ERROR_UNSUPPORTED, and so do the writers of TIFF, PNG
- * and PNM, unround_orient and unround_main.
+ * unround_decode reads PNG files (the file layer, include/unround_jpegio.h, on zlib) and
+ * reconstructs them within the intervals of their rounded samples on the GPU; a JPEG
+ * file needs a JPEG reader, which this library has not yet: unround_decode of one, and
+ * unround_project, return UNROUND_ERROR_UNSUPPORTED, and so does unround_main (the
+ * command line). The writers of TIFF, PNG and PNM, with the rounding and blue-noise
+ * dither of docs/math.md 10, and unround_orient are this library's own.
  */
 #ifndef UNROUND_H
 #define UNROUND_H
@@ -111,7 +114,10 @@ typedef struct unround_result unround_result;
 typedef int32_t unround_format;
 enum { UNROUND_FORMAT_JPEG = 1, UNROUND_FORMAT_PNG = 2 };
 
-/* rust/capi/include/unround.h:128-134: reading files is not part of this library. */
+/* rust/capi/include/unround.h:128-134: a PNG file (known by its signature) within the
+ * intervals of its rounded samples, by TV or TGV with --mu and --slack alone of the data
+ * term, its colours together and its alpha alone; a JPEG file is UNROUND_ERROR_UNSUPPORTED
+ * (no JPEG reader). */
 unround_status unround_decode(const uint8_t* data, size_t size, const unround_settings* settings,
                               unround_observer observer, void* user, unround_result** result, char* message,
                               size_t message_size);
@@ -155,7 +161,11 @@ const char* unround_result_warning(const unround_result* result);
 const double* unround_result_excess(const unround_result* result, uint64_t component, uint64_t* count);
 
 /* ---------------------------------------------------------------------------------
- * Output and conversion (rust/capi/include/unround.h:246-302) */
+ * Output and conversion (rust/capi/include/unround.h:235-290) */
+
+/* rust/capi/include/unround.h:246-254 */
+typedef int32_t unround_dither;
+enum { UNROUND_DITHER_NONE = 0, UNROUND_DITHER_BLUE_NOISE = 1, UNROUND_DITHER_BLUE_NOISE_TOGETHER = 2 };
 
 unround_status unround_tiff(const double* samples, uint64_t height, uint64_t width, uint64_t channels, int32_t ycbcr,
                             const uint8_t* icc, uint64_t icc_size, uint8_t** bytes, uint64_t* size, char* message,
@@ -173,6 +183,14 @@ unround_status unround_orient(const double* samples, uint64_t height, uint64_t w
 unround_status unround_to_rgb(const double* planes, uint64_t height, uint64_t width, double* picture);
 unround_status unround_to_ycbcr(const double* picture, uint64_t height, uint64_t width, double* planes);
 int32_t unround_main(size_t count, const char* const* arguments);
+
+/* ---------------------------------------------------------------------------------
+ * The file layer through this library (rust/capi/include/unround.h:292-304): the
+ * functions of include/unround_jpegio.h under the names the reference passes them on by. */
+int32_t unround_jpeg_abi_version(void);
+const char* unround_jpeg_libpng_version(void);
+int32_t unround_jpeg_check_icc(const uint8_t* profile, uint64_t size, int32_t channels, char* reason,
+                               size_t reason_size);
 
 #ifdef __cplusplus
 }

@@ -1,10 +1,10 @@
 /* The file layer of JPEG-Unround, MI355X edition: a drop-in for the reference's C layer
  * between the files and the reconstruction (c/jpegio/include/unround/jpegio.h, ABI version
  * 3), which the reference builds on libjpeg-turbo and libpng. Here it is written on zlib
- * alone: the PNG reader and writer are this library's own. The enum values, structure
- * layouts and function signatures are the reference's, so that its callers (the ctypes
- * module python/src/unround/jpegio.py, the Rust crate jpegio-sys) bind this library
- * unchanged; each declaration cites the one it replaces.
+ * alone: the PNG reader and writer and the check of ICC profiles are this library's own.
+ * The enum values, structure layouts and function signatures are the reference's, so that
+ * its callers (the ctypes module python/src/unround/jpegio.py, the Rust crate jpegio-sys)
+ * bind this library unchanged; each declaration cites the one it replaces.
  *
  * Every function returns a status and copies the message of a failure, or the first
  * warning of a success, into the caller's buffer (NUL-terminated, cut to its size; a null
@@ -51,6 +51,27 @@ typedef struct unround_jpegio_png_image {
   uint64_t icc_profile_size;
 } unround_jpegio_png_image;
 
+/* A picture to write as a PNG file (c/jpegio/include/unround/jpegio.h:134-157): rows from
+ * the top, the samples of a pixel together (grey or RGB, then alpha, not premultiplied).
+ * 48 bytes. */
+typedef struct unround_jpegio_png {
+  uint32_t width;
+  uint32_t height;
+  int32_t channels;    /* 1 to 4 */
+  int32_t bits;        /* 8 or 16 */
+  int32_t compression; /* zlib's level 0 to 9; -1 zlib's default */
+  int32_t reserved;    /* 0 */
+  const void* samples; /* uint8_t, or uint16_t in this machine's byte order */
+  const uint8_t* icc_profile;
+  uint64_t icc_profile_size;
+} unround_jpegio_png;
+
+/* Bytes that this layer allocated (c/jpegio/include/unround/jpegio.h:171-175). */
+typedef struct unround_jpegio_bytes {
+  uint8_t* data;
+  uint64_t size;
+} unround_jpegio_bytes;
+
 /* c/jpegio/include/unround/jpegio.h:180-181 */
 int32_t unround_jpegio_abi_version(void);
 
@@ -71,6 +92,25 @@ unround_jpegio_status unround_jpegio_read_png(const uint8_t* data, size_t size, 
 
 /* c/jpegio/include/unround/jpegio.h:252-254 */
 void unround_jpegio_png_image_free(unround_jpegio_png_image* image);
+
+/* Whether an ICC profile goes with a picture of the channels (1, 2: grey; 3, 4: RGB): at
+ * least 132 bytes, the length its header gives, a multiple of 4 from version 4 on, the
+ * signature 'acsp', its tags within it, a rendering intent below 0xFFFF, a class neither
+ * abstract nor a device link, the picture's data colour space and XYZ or Lab as its
+ * connection space (c/jpegio/include/unround/jpegio.h:212-223). 1 where it does; 0 with
+ * the reason where it does not. */
+int32_t unround_jpegio_check_icc(const uint8_t* profile, uint64_t size, int32_t channels, char* reason,
+                                 size_t reason_size);
+
+/* Writes a picture as a PNG file into memory (c/jpegio/include/unround/jpegio.h:225-233):
+ * 8 or 16 bits, grey or RGB with or without alpha, not interlaced, each row by the filter
+ * of the least sum, with the ICC profile where unround_jpegio_check_icc takes it (else the
+ * message says why). file owns the bytes until unround_jpegio_bytes_free(). */
+unround_jpegio_status unround_jpegio_write_png(const unround_jpegio_png* png, unround_jpegio_bytes* file,
+                                               char* message, size_t message_size);
+
+/* c/jpegio/include/unround/jpegio.h:235-237 */
+void unround_jpegio_bytes_free(unround_jpegio_bytes* bytes);
 
 #ifdef __cplusplus
 }

@@ -791,38 +791,148 @@ int32_t unround_result_kernel_times(const unround_result* result, double* values
 }
 
 namespace {
-unround_status unsupported(char* message, size_t size, const char* what) {
-  write_message(message, size, std::string(what) + " is not part of this library");
-  return UNROUND_ERROR_UNSUPPORTED;
+
+// Bytes handed over to the caller, who frees them with unround_bytes_free.
+void hand_over(const std::vector<uint8_t>& made, uint8_t** bytes, uint64_t* size) {
+  uint8_t* out = static_cast<uint8_t*>(std::malloc(std::max<size_t>(1, made.size())));
+  if (out == nullptr) throw std::bad_alloc();
+  if (!made.empty()) std::memcpy(out, made.data(), made.size());
+  *bytes = out;
+  *size = made.size();
 }
+
+// The picture's samples and sizes checked as the reference's C interface does
+// (rust/capi/src/lib.rs:1259-1269).
+unround_status picture_given(const double* samples, uint64_t height, uint64_t width, uint64_t channels,
+                             std::string& message) {
+  const unsigned __int128 count = static_cast<unsigned __int128>(height) * width * channels;
+  if (count > (static_cast<unsigned __int128>(1) << 62)) {
+    message = "a picture too large";
+    return UNROUND_ERROR_ARGUMENT;
+  }
+  if (samples == nullptr) {
+    message = "the samples are a null pointer";
+    return UNROUND_ERROR_ARGUMENT;
+  }
+  return UNROUND_OK;
+}
+
+unround::Dither dither_of(int32_t dither) { return static_cast<unround::Dither>(dither); }
+
+bool dither_known(int32_t dither, std::string& message) {
+  if (dither >= 0 && dither <= 2) return true;
+  message = "the dither is 0, 1 or 2, not " + std::to_string(dither);
+  return false;
+}
+
 }  // namespace
 
-unround_status unround_tiff(const double*, uint64_t, uint64_t, uint64_t, int32_t, const uint8_t*, uint64_t,
-                            uint8_t** bytes, uint64_t* size, char* message, size_t message_size) {
-  if (bytes != nullptr) *bytes = nullptr;
-  if (size != nullptr) *size = 0;
-  return unsupported(message, message_size, "writing TIFF");
+unround_status unround_tiff(const double* samples, uint64_t height, uint64_t width, uint64_t channels, int32_t ycbcr,
+                            const uint8_t* icc, uint64_t icc_size, uint8_t** bytes, uint64_t* size, char* message,
+                            size_t message_size) {
+  std::string warning;
+  const unround_status status = guarded(message, message_size, [&](std::string& text) -> unround_status {
+    if (bytes == nullptr || size == nullptr) {
+      text = "the bytes and the size are not null pointers";
+      return UNROUND_ERROR_ARGUMENT;
+    }
+    const unround_status given = picture_given(samples, height, width, channels, text);
+    if (given != UNROUND_OK) return given;
+    std::vector<uint8_t> file;
+    if (!unround::tiff_bytes(samples, height, width, channels, ycbcr != 0, icc, icc != nullptr ? icc_size : 0, file,
+                             warning, text)) {
+      return UNROUND_ERROR_OPTIONS;
+    }
+    hand_over(file, bytes, size);
+    return UNROUND_OK;
+  });
+  if (status == UNROUND_OK) write_message(message, message_size, warning);
+  return status;
 }
 
-unround_status unround_png(const double*, uint64_t, uint64_t, uint64_t, int32_t, int32_t, int32_t, const uint8_t*,
-                           uint64_t, uint8_t** bytes, uint64_t* size, char* message, size_t message_size) {
-  if (bytes != nullptr) *bytes = nullptr;
-  if (size != nullptr) *size = 0;
-  return unsupported(message, message_size, "writing PNG");
-}
-
-unround_status unround_pnm(const double*, uint64_t, uint64_t, uint64_t, int32_t, int32_t, uint8_t** bytes,
+unround_status unround_png(const double* samples, uint64_t height, uint64_t width, uint64_t channels, int32_t sixteen,
+                           int32_t dither, int32_t compression, const uint8_t* icc, uint64_t icc_size, uint8_t** bytes,
                            uint64_t* size, char* message, size_t message_size) {
-  if (bytes != nullptr) *bytes = nullptr;
-  if (size != nullptr) *size = 0;
-  return unsupported(message, message_size, "writing PNM");
+  std::string warning;
+  const unround_status status = guarded(message, message_size, [&](std::string& text) -> unround_status {
+    if (bytes == nullptr || size == nullptr) {
+      text = "the bytes and the size are not null pointers";
+      return UNROUND_ERROR_ARGUMENT;
+    }
+    if (compression < -1 || compression > 9) {
+      text = "zlib's level is 0 to 9, or -1, not " + std::to_string(compression);
+      return UNROUND_ERROR_OPTIONS;
+    }
+    if (!dither_known(dither, text)) return UNROUND_ERROR_OPTIONS;
+    const unround_status given = picture_given(samples, height, width, channels, text);
+    if (given != UNROUND_OK) return given;
+    if (height > 0x7FFFFFFFu || width > 0x7FFFFFFFu) {
+      text = "a side of the picture is larger than PNG allows";
+      return UNROUND_ERROR_OPTIONS;
+    }
+    std::vector<uint16_t> levels;
+    if (!unround::quantize(samples, height, width, channels, sixteen != 0, dither_of(dither), levels, text)) {
+      return UNROUND_ERROR_OPTIONS;
+    }
+    std::vector<uint8_t> eight;
+    const void* data = levels.data();
+    if (sixteen == 0) {
+      eight.assign(levels.begin(), levels.end());
+      data = eight.data();
+    }
+    std::vector<uint8_t> file;
+    int32_t layer = 0;
+    if (!unround::png_bytes(data, static_cast<uint32_t>(width), static_cast<uint32_t>(height),
+                            static_cast<int32_t>(channels), sixteen != 0 ? 16 : 8, compression, icc,
+                            icc != nullptr ? icc_size : 0, file, warning, text, layer)) {
+      return layer == 1 ? UNROUND_ERROR_OPTIONS : UNROUND_ERROR_IO;
+    }
+    hand_over(file, bytes, size);
+    return UNROUND_OK;
+  });
+  if (status == UNROUND_OK) write_message(message, message_size, warning);
+  return status;
+}
+
+unround_status unround_pnm(const double* samples, uint64_t height, uint64_t width, uint64_t channels, int32_t sixteen,
+                           int32_t dither, uint8_t** bytes, uint64_t* size, char* message, size_t message_size) {
+  return guarded(message, message_size, [&](std::string& text) -> unround_status {
+    if (bytes == nullptr || size == nullptr) {
+      text = "the bytes and the size are not null pointers";
+      return UNROUND_ERROR_ARGUMENT;
+    }
+    if (!dither_known(dither, text)) return UNROUND_ERROR_OPTIONS;
+    const unround_status given = picture_given(samples, height, width, channels, text);
+    if (given != UNROUND_OK) return given;
+    std::vector<uint8_t> file;
+    if (!unround::pnm_bytes(samples, height, width, channels, sixteen != 0, dither_of(dither), file, text)) {
+      return UNROUND_ERROR_OPTIONS;
+    }
+    hand_over(file, bytes, size);
+    return UNROUND_OK;
+  });
 }
 
 void unround_bytes_free(uint8_t* bytes, uint64_t) { std::free(bytes); }
 
-unround_status unround_orient(const double*, uint64_t, uint64_t, uint64_t, int32_t, double*, uint64_t*, uint64_t*,
+unround_status unround_orient(const double* samples, uint64_t height, uint64_t width, uint64_t channels,
+                              int32_t orientation, double* turned, uint64_t* turned_height, uint64_t* turned_width,
                               char* message, size_t message_size) {
-  return unsupported(message, message_size, "turning a picture upright");
+  return guarded(message, message_size, [&](std::string& text) -> unround_status {
+    const unround_status given = picture_given(samples, height, width, channels, text);
+    if (given != UNROUND_OK) return given;
+    if (turned == nullptr) {
+      text = "the turned picture is a null pointer";
+      return UNROUND_ERROR_ARGUMENT;
+    }
+    size_t rows = 0, columns = 0;
+    if (!unround::orient(samples, height, width, channels, orientation, turned, rows, columns, text)) {
+      return UNROUND_ERROR_OPTIONS;
+    }
+    if (turned_height != nullptr) *turned_height = rows;
+    if (turned_width != nullptr) *turned_width = columns;
+    return UNROUND_OK;
+  });
 }
 
 unround_status unround_to_rgb(const double* planes, uint64_t height, uint64_t width, double* picture) {

@@ -5,9 +5,11 @@
 
 #include <zlib.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "files.hpp"
 
@@ -84,6 +86,72 @@ void unround_jpegio_png_image_free(unround_jpegio_png_image* image) {
   std::free(image->samples);
   std::free(image->icc_profile);
   *image = unround_jpegio_png_image{};
+}
+
+int32_t unround_jpegio_check_icc(const uint8_t* profile, uint64_t size, int32_t channels, char* reason,
+                                 size_t reason_size) {
+  try {
+    std::string why;
+    const bool taken = unround::check_icc(profile, size, channels, why);
+    copy_message(reason, reason_size, why);
+    return taken ? 1 : 0;
+  } catch (...) {
+    copy_message(reason, reason_size, "out of memory");
+    return 0;
+  }
+}
+
+unround_jpegio_status unround_jpegio_write_png(const unround_jpegio_png* png, unround_jpegio_bytes* file,
+                                               char* message, size_t message_size) {
+  if (file == nullptr || (message == nullptr && message_size > 0)) return UNROUND_JPEGIO_ERROR_ARGUMENT;
+  *file = unround_jpegio_bytes{};
+  if (png == nullptr) {
+    copy_message(message, message_size, "no picture is given");
+    return UNROUND_JPEGIO_ERROR_ARGUMENT;
+  }
+  if (png->reserved != 0) {
+    copy_message(message, message_size, "the reserved field is 0");
+    return UNROUND_JPEGIO_ERROR_ARGUMENT;
+  }
+  try {
+    std::vector<uint8_t> bytes;
+    std::string warning, text;
+    int32_t status = UNROUND_JPEGIO_OK;
+    if (!unround::png_bytes(png->samples, png->width, png->height, png->channels, png->bits, png->compression,
+                            png->icc_profile, png->icc_profile_size, bytes, warning, text, status)) {
+      copy_message(message, message_size, text);
+      return status;
+    }
+    file->data = released(bytes.data(), bytes.size());
+    if (file->data == nullptr) {
+      copy_message(message, message_size, "out of memory");
+      return UNROUND_JPEGIO_ERROR_MEMORY;
+    }
+    file->size = bytes.size();
+    copy_message(message, message_size, warning);
+    return UNROUND_JPEGIO_OK;
+  } catch (...) {
+    *file = unround_jpegio_bytes{};
+    copy_message(message, message_size, "out of memory");
+    return UNROUND_JPEGIO_ERROR_MEMORY;
+  }
+}
+
+void unround_jpegio_bytes_free(unround_jpegio_bytes* bytes) {
+  if (bytes == nullptr) return;
+  std::free(bytes->data);
+  *bytes = unround_jpegio_bytes{};
+}
+
+// The file layer under the names the reference's C interface passes it on by
+// (rust/capi/include/unround.h:292-304).
+int32_t unround_jpeg_abi_version(void) { return unround_jpegio_abi_version(); }
+
+const char* unround_jpeg_libpng_version(void) { return unround_jpegio_libpng_version(); }
+
+int32_t unround_jpeg_check_icc(const uint8_t* profile, uint64_t size, int32_t channels, char* reason,
+                               size_t reason_size) {
+  return unround_jpegio_check_icc(profile, size, channels, reason, reason_size);
 }
 
 }  // extern "C"

@@ -41,10 +41,14 @@ __all__ = [
     "decode",
     "held_options",
     "options",
+    "oriented",
+    "png_bytes",
+    "pnm_bytes",
     "solve",
     "solve_batch",
     "solve_samples",
     "to_rgb",
+    "tiff_bytes",
     "to_ycbcr",
     "version",
 ]
@@ -193,6 +197,13 @@ def _declare_all(library: ctypes.CDLL) -> None:
     _declare(library, "unround_result_format", i32, handle)
     _declare(library, "unround_result_bits", i32, handle)
     _declare(library, "unround_result_warning", ctypes.c_char_p, handle)
+    bytes_out = (ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8)), u64_out, *message)
+    icc = (ctypes.c_char_p, u64)
+    _declare(library, "unround_tiff", i32, doubles, u64, u64, u64, i32, *icc, *bytes_out)
+    _declare(library, "unround_png", i32, doubles, u64, u64, u64, i32, i32, i32, *icc, *bytes_out)
+    _declare(library, "unround_pnm", i32, doubles, u64, u64, u64, i32, i32, *bytes_out)
+    _declare(library, "unround_bytes_free", None, ctypes.POINTER(ctypes.c_uint8), u64)
+    _declare(library, "unround_orient", i32, doubles, u64, u64, u64, i32, doubles, u64_out, u64_out, *message)
     _declare(library, "unround_to_rgb", i32, doubles, u64, u64, doubles)
     _declare(library, "unround_to_ycbcr", i32, doubles, u64, u64, doubles)
 
@@ -689,6 +700,102 @@ def solve_batch(images: Sequence[jpegio.Image], settings: Settings | list[str] |
             library.unround_result_free(ctypes.c_void_p(results[index]))
     del kept
     return decoded, [float(value) for value in latencies[:count]]
+
+
+def _samples(picture: npt.ArrayLike) -> tuple[Array, int, int, int]:
+    samples = np.ascontiguousarray(picture, dtype=np.float64)
+    if samples.ndim == 2:
+        return samples, int(samples.shape[0]), int(samples.shape[1]), 1
+    if samples.ndim == 3:
+        return samples, int(samples.shape[0]), int(samples.shape[1]), int(samples.shape[2])
+    raise ValueError(f"a picture is (height, width) or (height, width, channels), not {samples.shape}")
+
+
+def _written(name: str, picture: npt.ArrayLike, *arguments: object) -> bytes:
+    """The file a writer of the library makes of the picture, with the arguments after its size;
+    its warning, where it has one, as an UnroundWarning (python/src/unround/native.py:801-822)."""
+    library = _library()
+    samples, height, width, channels = _samples(picture)
+    pointer = ctypes.POINTER(ctypes.c_uint8)()
+    size = ctypes.c_uint64()
+    message = ctypes.create_string_buffer(_MESSAGE_SIZE)
+    status = getattr(library, name)(samples.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), height, width, channels,
+                                    *arguments, ctypes.byref(pointer), ctypes.byref(size), message, _MESSAGE_SIZE)
+    text = message.value.decode("utf-8", "replace")
+    if status != Status.OK:
+        known = status in Status.__members__.values()
+        raise UnroundError(Status(status) if known else Status.INTERNAL, text)
+    try:
+        data = ctypes.string_at(pointer, int(size.value))
+    finally:
+        library.unround_bytes_free(pointer, size)
+    if text:
+        warnings.warn(text, UnroundWarning, stacklevel=3)
+    return data
+
+
+def _profile(icc_profile: bytes | None) -> tuple[bytes | None, int]:
+    return (None, 0) if icc_profile is None else (bytes(icc_profile), len(icc_profile))
+
+
+def tiff_bytes(picture: npt.ArrayLike, *, ycbcr: bool = False, icc_profile: bytes | None = None) -> bytes:
+    """A TIFF of the picture's samples in binary64, bit for bit (python/src/unround/native.py:829-844):
+    (height, width) or (height, width, channels), 1 to 4, alpha last. With ycbcr the three samples of
+    a pixel are JFIF's Y, Cb and Cr, and the file says so. The ICC profile is embedded where it goes
+    with the picture; otherwise it is left out, with an UnroundWarning."""
+    return _written("unround_tiff", picture, 1 if ycbcr else 0, *_profile(icc_profile))
+
+
+_DITHERS: Final = {("none", None): 0, ("blue-noise", None): 1, ("blue-noise", "apart"): 1,
+                   ("blue-noise", "together"): 2}
+
+
+def _dither(dither: str, channels: str | None) -> int:
+    found = _DITHERS.get((dither, channels))
+    if found is None:
+        raise ValueError(f"the dither is blue-noise (channels apart or together) or none, not {dither}"
+                         + ("" if channels is None else f" with the channels {channels}"))
+    return found
+
+
+def png_bytes(picture: npt.ArrayLike, *, sixteen: bool = False, dither: str = "blue-noise",
+              dither_channels: str | None = None, compression: int | None = None,
+              icc_profile: bytes | None = None) -> bytes:
+    """A PNG file of the picture's samples, of 8 or 16 bits (python/src/unround/native.py:858-879):
+    dither "blue-noise" (the ordered dither of the blue-noise matrix; dither_channels "apart", the
+    default, or "together") or "none" (each sample rounded, halves away from 0) (docs/math.md, 10);
+    compression zlib's level 0 to 9, None for zlib's default; the ICC profile as tiff_bytes takes it."""
+    level = -1 if compression is None else compression
+    if not -1 <= level <= 9:
+        raise ValueError(f"zlib's level is 0 to 9, or None, not {compression}")
+    return _written("unround_png", picture, 1 if sixteen else 0, _dither(dither, dither_channels), level,
+                    *_profile(icc_profile))
+
+
+def pnm_bytes(picture: npt.ArrayLike, *, sixteen: bool = False, dither: str = "blue-noise",
+              dither_channels: str | None = None) -> bytes:
+    """A PNM file (P5 grey, P6 RGB) of the picture's samples, made as png_bytes makes them."""
+    return _written("unround_pnm", picture, 1 if sixteen else 0, _dither(dither, dither_channels))
+
+
+def oriented(picture: npt.ArrayLike, orientation: int) -> Array:
+    """The picture, (height, width) or (height, width, channels), turned upright as EXIF's
+    orientation (1 to 8, 0 for none) says; from 5 on the rows become columns. Every sample moves
+    bit for bit (python/src/unround/native.py:889-919)."""
+    library = _library()
+    samples, height, width, channels = _samples(picture)
+    if not -(2**31) <= orientation < 2**31:
+        raise ValueError(f"an EXIF orientation is 1 to 8, or 0 for none, not {orientation}")
+    turned = np.empty(samples.size, dtype=np.float64)
+    rows, columns = ctypes.c_uint64(), ctypes.c_uint64()
+    message = ctypes.create_string_buffer(_MESSAGE_SIZE)
+    status = library.unround_orient(samples.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), height, width, channels,
+                                    orientation, turned.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                    ctypes.byref(rows), ctypes.byref(columns), message, _MESSAGE_SIZE)
+    if status != Status.OK:
+        raise UnroundError(Status(status), message.value.decode("utf-8", "replace"))
+    shape = (int(rows.value), int(columns.value)) + (() if samples.ndim == 2 else (channels,))
+    return turned.reshape(shape)
 
 
 def to_rgb(planes: npt.ArrayLike) -> Array:
