@@ -27,6 +27,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "unround_jpegio.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -188,9 +190,17 @@ int32_t unround_main(size_t count, const char* const* arguments);
  * The file layer through this library (rust/capi/include/unround.h:292-304): the
  * functions of include/unround_jpegio.h under the names the reference passes them on by. */
 int32_t unround_jpeg_abi_version(void);
+const char* unround_jpeg_libjpeg_version(void);
 const char* unround_jpeg_libpng_version(void);
 int32_t unround_jpeg_check_icc(const uint8_t* profile, uint64_t size, int32_t channels, char* reason,
                                size_t reason_size);
+unround_jpegio_status unround_jpeg_read(const uint8_t* data, size_t size, const unround_jpegio_options* options,
+                                        unround_jpegio_image* image, char* message, size_t message_size);
+void unround_jpeg_image_free(unround_jpegio_image* image);
+unround_jpegio_status unround_jpeg_decode_planes(const uint8_t* data, size_t size,
+                                                 const unround_jpegio_options* options, unround_jpegio_planes* planes,
+                                                 char* message, size_t message_size);
+void unround_jpeg_planes_free(unround_jpegio_planes* planes);
 
 #ifdef __cplusplus
 }

@@ -48,6 +48,7 @@ struct unround_result {
   std::vector<uint64_t> alpha_record_iterations;
   std::array<std::vector<double>, 5> alpha_record_values;
   std::vector<uint8_t> icc_profile;
+  int32_t exif_orientation = 0;
 };
 
 // The unrolled solve's handle: its frames (with their settings) and plans, which the solve
@@ -348,8 +349,42 @@ unround_status reconstruct_samples(const uint16_t* levels, uint64_t height, uint
   return UNROUND_OK;
 }
 
+// A JPEG file's components as the solve's input, over the image's arrays.
+unround_input input_of(const unround::JpegImage& image, std::vector<unround_component>& components) {
+  components.assign(image.components.size(), unround_component{});
+  for (size_t c = 0; c < image.components.size(); ++c) {
+    const unround::JpegComponent& from = image.components[c];
+    unround_component& to = components[c];
+    to.levels = from.coefficients.data();
+    to.rows = from.blocks_down;
+    to.columns = from.blocks_across;
+    to.h_samp_factor = from.h;
+    to.v_samp_factor = from.v;
+    std::memcpy(to.quant_table, from.table.data(), sizeof(to.quant_table));
+  }
+  unround_input input{};
+  input.height = image.height;
+  input.width = image.width;
+  input.color_space = image.color_space;
+  input.count = static_cast<uint32_t>(components.size());
+  input.components = components.data();
+  return input;
+}
+
+// A JPEG file read with the settings' limits; a failure of the reading is a file that
+// cannot be read (rust/unround/src/error.rs:46-50).
+unround_status read_file(const uint8_t* data, size_t size, const unround::Settings& s, unround::JpegImage& image,
+                         std::string& message) {
+  int32_t status = 0;
+  if (!unround::read_jpeg(data, size, s.max_pixels, static_cast<int32_t>(std::min<uint64_t>(s.max_scans, 0x7FFFFFFF)),
+                          s.warnings_are_errors, image, message, status)) {
+    return UNROUND_ERROR_READ;
+  }
+  return UNROUND_OK;
+}
+
 // A file reconstructed (rust/capi/src/lib.rs:532-565): a PNG file within the intervals of
-// its rounded samples; a JPEG file needs a JPEG reader, which this library has not.
+// its rounded samples, any other as a JPEG file, its components by the settings.
 unround_status decode_file(const uint8_t* data, size_t size, const unround_settings* settings,
                            unround_observer observer, void* user, unround_result** result, std::string& message) {
   if (data == nullptr || settings == nullptr || result == nullptr) {
@@ -358,8 +393,18 @@ unround_status decode_file(const uint8_t* data, size_t size, const unround_setti
   }
   *result = nullptr;
   if (!unround::is_png(data, size)) {
-    message = "reading JPEG files is not part of this library: give the components as arrays to unround_solve";
-    return UNROUND_ERROR_UNSUPPORTED;
+    unround::JpegImage image;
+    const unround_status read = read_file(data, size, settings->settings, image, message);
+    if (read != UNROUND_OK) return read;
+    std::vector<unround_component> components;
+    const unround_input input = input_of(image, components);
+    const unround_status done = reconstruct(&input, settings, nullptr, observer, user, result, message);
+    if (done == UNROUND_OK) {
+      (*result)->icc_profile = std::move(image.icc_profile);
+      (*result)->exif_orientation = image.exif_orientation;
+      (*result)->warning = image.warnings > 0 ? image.warning : "";
+    }
+    return done;
   }
   // What a PNG file takes is checked before it is read (rust/unround/src/rounded.rs:77-95).
   const unround::Settings& s = settings->settings;
@@ -766,7 +811,9 @@ const uint8_t* unround_result_icc_profile(const unround_result* result, uint64_t
   return result->icc_profile.data();
 }
 
-int32_t unround_result_exif_orientation(const unround_result*) { return 0; }
+int32_t unround_result_exif_orientation(const unround_result* result) {
+  return result == nullptr ? 0 : result->exif_orientation;
+}
 
 int32_t unround_result_format(const unround_result* result) { return result == nullptr ? 0 : result->format; }
 

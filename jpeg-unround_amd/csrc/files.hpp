@@ -35,6 +35,45 @@ bool is_png(const uint8_t* data, size_t size);
 bool read_png(const uint8_t* data, size_t size, uint64_t max_pixels, PngImage& image, std::string& message,
               int32_t& status);
 
+// A JPEG file's component (c/jpegio/include/unround/jpegio.h:69-91): its identifier,
+// sampling factors and table slot, its samples and the blocks that carry them, the
+// quantization table (natural order) its slot held when its first scan began, and its
+// coefficients, blocks_down x blocks_across blocks of 64 in natural order (no dummy blocks).
+struct JpegComponent {
+  int id = 0, h = 1, v = 1, slot = 0;
+  uint32_t width = 0, height = 0, blocks_across = 0, blocks_down = 0;
+  std::array<uint16_t, 64> table{};
+  std::vector<int16_t> coefficients;
+};
+
+// A JPEG file as the reconstruction reads it (c/jpegio/include/unround/jpegio.h:93-108).
+struct JpegImage {
+  uint32_t width = 0, height = 0;
+  int color_space = 1;  // 1 grey, 2 YCbCr, 3 RGB
+  int max_h = 1, max_v = 1;
+  bool progressive = false, arithmetic = false;
+  int exif_orientation = 0;
+  int warnings = 0;
+  std::string warning;
+  std::vector<JpegComponent> components;
+  std::vector<uint8_t> icc_profile;
+};
+
+// Whether the data begins with SOI.
+bool is_jpeg(const uint8_t* data, size_t size);
+
+// Reads a JPEG file's coefficients and metadata; false with the reason and a status of the
+// file layer (2 data it cannot read, 3 a JPEG outside this reader's scope, 4 more pixels than
+// max_pixels (2^28 where 0) or scans than max_scans (500 where 0), 5 out of memory). A
+// corrupt-data warning fails the read where warnings_are_errors.
+bool read_jpeg(const uint8_t* data, size_t size, uint64_t max_pixels, int32_t max_scans, bool warnings_are_errors,
+               JpegImage& image, std::string& message, int32_t& status);
+
+// The component planes of a JPEG file: each component's blocks by the accurate integer
+// inverse DCT of libjpeg (jidctint.c), level-shifted and limited to 0-255, no smoothing, no
+// upsampling: blocks_down * 8 rows of blocks_across * 8 samples.
+void jpeg_planes(const JpegImage& image, std::vector<std::vector<uint8_t>>& planes);
+
 // How samples become levels of 8 or 16 bits (docs/math.md 10): rounded, halves away from
 // 0; or the ordered dither of the blue-noise matrix, the channels of a pixel by the matrix
 // shifted apart, or together.

@@ -134,8 +134,8 @@ def encode_grey(picture: np.ndarray, quality: int) -> jpegio.Image:
     height, width = picture.shape
     table = ijg_table(LUMA, quality)
     plane = _padded(picture, -(-height // 8) * 8, -(-width // 8) * 8)
-    component = jpegio.Component(_levels(plane, table), table, 1, 1)
-    return jpegio.Image(height, width, jpegio.ColorSpace.GRAYSCALE, (component,))
+    component = jpegio.make_component(_levels(plane, table), table, 1, 1)
+    return jpegio.make_image(height, width, jpegio.ColorSpace.GRAYSCALE, (component,))
 
 
 def encode_colour420(picture: np.ndarray, quality: int) -> jpegio.Image:
@@ -155,8 +155,8 @@ def encode_colour420(picture: np.ndarray, quality: int) -> jpegio.Image:
         chroma.append(np.floor(means + 0.5))
     luma_table, chroma_table = ijg_table(LUMA, quality), ijg_table(CHROMA, quality)
     components = (
-        jpegio.Component(_levels(luma, luma_table), luma_table, 2, 2),
-        jpegio.Component(_levels(chroma[0], chroma_table), chroma_table, 1, 1),
-        jpegio.Component(_levels(chroma[1], chroma_table), chroma_table, 1, 1),
+        jpegio.make_component(_levels(luma, luma_table), luma_table, 2, 2),
+        jpegio.make_component(_levels(chroma[0], chroma_table), chroma_table, 1, 1),
+        jpegio.make_component(_levels(chroma[1], chroma_table), chroma_table, 1, 1),
     )
-    return jpegio.Image(height, width, jpegio.ColorSpace.YCBCR, components)
+    return jpegio.make_image(height, width, jpegio.ColorSpace.YCBCR, components)

@@ -96,9 +96,10 @@ def test_null_arguments_and_unsupported_paths_are_statuses():
     assert b"null" in message.value
     with pytest.raises(native.UnroundError) as caught:
         native.decode(b"\xff\xd8\xff", unround.Settings())
-    assert caught.value.status == native.Status.UNSUPPORTED
-    image = unround.jpegio.Image(8, 8, unround.jpegio.ColorSpace.GRAYSCALE,
-                                 (unround.jpegio.Component(np.zeros((1, 1, 8, 8), np.int16), np.ones(64, np.uint16)),))
+    assert caught.value.status == native.Status.READ
+    image = unround.jpegio.make_image(
+        8, 8, unround.jpegio.ColorSpace.GRAYSCALE,
+        (unround.jpegio.make_component(np.zeros((1, 1, 8, 8), np.int16), np.ones(64, np.uint16)),))
     with pytest.raises(native.UnroundError) as caught:
         native.solve(image, unround.Settings(method="subgradient"))
     assert caught.value.status == native.Status.UNSUPPORTED
@@ -116,9 +117,9 @@ def test_batch_refusals_come_before_the_gpu():
     assert status == native.Status.ARGUMENT and b"null" in message.value
     # An empty batch is done at once; a bad input is named by its index.
     assert native.solve_batch([], ["--method", "tgv"]) == ([], [])
-    grey = unround.jpegio.Component(np.zeros((2, 2, 8, 8), np.int16), np.ones(64, np.uint16))
-    good = unround.jpegio.Image(16, 16, unround.jpegio.ColorSpace.GRAYSCALE, (grey,))
-    bad = unround.jpegio.Image(17, 16, unround.jpegio.ColorSpace.GRAYSCALE, (grey,))
+    grey = unround.jpegio.make_component(np.zeros((2, 2, 8, 8), np.int16), np.ones(64, np.uint16))
+    good = unround.jpegio.make_image(16, 16, unround.jpegio.ColorSpace.GRAYSCALE, (grey,))
+    bad = unround.jpegio.make_image(17, 16, unround.jpegio.ColorSpace.GRAYSCALE, (grey,))
     with pytest.raises(native.UnroundError) as caught:
         native.solve_batch([good, bad], ["--method", "tgv"])
     assert caught.value.status == native.Status.UNSUPPORTED and caught.value.message.startswith("input 1: ")
@@ -148,16 +149,16 @@ def test_samples_refusals_come_before_the_gpu():
 
 
 def test_refused_frames():
-    grey = unround.jpegio.Component(np.zeros((2, 2, 8, 8), np.int16), np.ones(64, np.uint16))
+    grey = unround.jpegio.make_component(np.zeros((2, 2, 8, 8), np.int16), np.ones(64, np.uint16))
     # Blocks that do not cover the picture, a colour space with the wrong count of
     # components, a step of 0, channel weights not one per channel.
     cases = [
-        (unround.jpegio.Image(17, 16, unround.jpegio.ColorSpace.GRAYSCALE, (grey,)), unround.Settings(), 3),
-        (unround.jpegio.Image(16, 16, unround.jpegio.ColorSpace.YCBCR, (grey,)), unround.Settings(), 3),
-        (unround.jpegio.Image(16, 16, unround.jpegio.ColorSpace.GRAYSCALE,
-                              (unround.jpegio.Component(np.zeros((2, 2, 8, 8), np.int16), np.zeros(64, np.uint16)),)),
+        (unround.jpegio.make_image(17, 16, unround.jpegio.ColorSpace.GRAYSCALE, (grey,)), unround.Settings(), 3),
+        (unround.jpegio.make_image(16, 16, unround.jpegio.ColorSpace.YCBCR, (grey,)), unround.Settings(), 3),
+        (unround.jpegio.make_image(16, 16, unround.jpegio.ColorSpace.GRAYSCALE,
+                              (unround.jpegio.make_component(np.zeros((2, 2, 8, 8), np.int16), np.zeros(64, np.uint16)),)),
          unround.Settings(), 4),
-        (unround.jpegio.Image(16, 16, unround.jpegio.ColorSpace.GRAYSCALE, (grey,)),
+        (unround.jpegio.make_image(16, 16, unround.jpegio.ColorSpace.GRAYSCALE, (grey,)),
          unround.Settings(tgv=unround.TGV(channel_weights=(1.0, 2.0))), 4),
     ]
     for image, settings, status in cases:

@@ -20,10 +20,10 @@ PDHG_CASES = [name for name in cases.names() if name != "grey-subgradient"]
 
 def image_of(case: cases.Case) -> unround.jpegio.Image:
     components = tuple(
-        unround.jpegio.Component(case.levels[c], case.tables[c], case.components[c][0], case.components[c][1])
+        unround.jpegio.make_component(case.levels[c], case.tables[c], case.components[c][0], case.components[c][1])
         for c in range(len(case.components))
     )
-    return unround.jpegio.Image(case.height, case.width, unround.jpegio.ColorSpace(case.space), components)
+    return unround.jpegio.make_image(case.height, case.width, unround.jpegio.ColorSpace(case.space), components)
 
 
 @pytest.mark.parametrize("name", PDHG_CASES)
