@@ -49,6 +49,8 @@ struct unround_result {
   std::array<std::vector<double>, 5> alpha_record_values;
   std::vector<uint8_t> icc_profile;
   int32_t exif_orientation = 0;
+  // Of a projection, each component's coefficients' excess; empty for a reconstruction.
+  std::vector<std::vector<double>> excess;
 };
 
 // The unrolled solve's handle: its frames (with their settings) and plans, which the solve
@@ -430,6 +432,93 @@ unround_status decode_file(const uint8_t* data, size_t size, const unround_setti
   return done;
 }
 
+// A picture made elsewhere measured against a JPEG file's intervals and projected onto them
+// (docs/math.md 12; rust/capi/src/lib.rs:708-752; rust/unround/src/picture.rs:60-137): its
+// canvas as an encoder makes the file's (JFIF's Y, Cb and Cr of its RGB for a file in YCbCr,
+// its own channels otherwise; each plane extended over whole MCUs by repeating its last
+// column, then its last row), each coefficient's excess, and the projection as a
+// reconstruction without a solver.
+unround_status project_picture(const uint8_t* data, size_t size, const unround_settings* settings,
+                               const double* picture, uint64_t height, uint64_t width, uint64_t channels,
+                               unround_result** result, std::string& message) {
+  if (data == nullptr || settings == nullptr || result == nullptr) {
+    message = "the data, the settings and the result are not null pointers";
+    return UNROUND_ERROR_ARGUMENT;
+  }
+  *result = nullptr;
+  if (unround::is_png(data, size)) {
+    message = "a picture is projected onto the intervals of a JPEG file, not of a PNG file";
+    return UNROUND_ERROR_OPTIONS;
+  }
+  const unsigned __int128 count = static_cast<unsigned __int128>(height) * width * channels;
+  if (count > (static_cast<unsigned __int128>(1) << 62)) {
+    message = "a picture too large";
+    return UNROUND_ERROR_ARGUMENT;
+  }
+  if (picture == nullptr) {
+    message = "the samples are a null pointer";
+    return UNROUND_ERROR_ARGUMENT;
+  }
+  unround::JpegImage image;
+  const unround_status read = read_file(data, size, settings->settings, image, message);
+  if (read != UNROUND_OK) return read;
+  if (height != image.height || width != image.width) {
+    message = "a picture of the file is " + std::to_string(image.height) + " x " + std::to_string(image.width) +
+              ", not " + std::to_string(height) + " x " + std::to_string(width);
+    return UNROUND_ERROR_OPTIONS;
+  }
+  std::vector<unround_component> components;
+  const unround_input input = input_of(image, components);
+  auto made = std::make_unique<unround_result>();
+  unround::Plan plan;
+  const unround_status checked = frame_of(&input, settings, *made, plan, message);
+  if (checked != UNROUND_OK) return checked;
+  const unround::FrameModel& frame = made->frame;
+  const size_t expected = image.color_space == UNROUND_COLOR_GRAYSCALE ? 1 : 3;
+  const size_t h = image.height, w = image.width, plane = h * w;
+  if (channels != expected) {
+    message = "a picture of the file is " + std::to_string(h) + " x " + std::to_string(w) + " x " +
+              std::to_string(expected) + " samples, not in " + std::to_string(channels) + " channels";
+    return UNROUND_ERROR_OPTIONS;
+  }
+  for (size_t k = 0; k < plane * channels; ++k) {
+    if (!std::isfinite(picture[k])) {
+      message = "a picture's samples are finite";
+      return UNROUND_ERROR_OPTIONS;
+    }
+  }
+  std::vector<double> planes(plane * channels);
+  if (image.color_space == UNROUND_COLOR_YCBCR) {
+    unround_to_ycbcr(picture, h, w, planes.data());
+  } else {
+    for (size_t k = 0; k < plane; ++k)
+      for (size_t c = 0; c < channels; ++c) planes[c * plane + k] = picture[k * channels + c];
+  }
+  const size_t rows = frame.rows, columns = frame.columns;
+  std::vector<double> canvas(channels * rows * columns);
+  for (size_t c = 0; c < channels; ++c) {
+    for (size_t i = 0; i < rows; ++i) {
+      const double* row = planes.data() + c * plane + std::min(i, h - 1) * w;
+      double* out = canvas.data() + (c * rows + i) * columns;
+      std::memcpy(out, row, sizeof(double) * w);
+      std::fill(out + w, out + columns, row[w - 1]);
+    }
+  }
+  int32_t status = UNROUND_OK;
+  std::vector<std::vector<double>> excess;
+  if (!unround::project_on_gpu(frame, settings->settings, canvas, made->out, excess, message, status)) {
+    return static_cast<unround_status>(status);
+  }
+  made->solver = false;
+  made->excess = std::move(excess);
+  assemble(*made);
+  made->icc_profile = std::move(image.icc_profile);
+  made->exif_orientation = image.exif_orientation;
+  made->warning = image.warnings > 0 ? image.warning : "";
+  *result = made.release();
+  return UNROUND_OK;
+}
+
 unround_status reconstruct_batch(const unround_input* inputs, size_t count, const unround_settings* settings,
                                  unround_result** results, double* latencies, std::string& message) {
   if (results == nullptr || (count > 0 && inputs == nullptr) || settings == nullptr) {
@@ -663,12 +752,12 @@ unround_status unround_unrolled_backward(unround_unrolled* handle, const float* 
   });
 }
 
-unround_status unround_project(const uint8_t*, size_t, const unround_settings*, const double*, uint64_t, uint64_t,
-                               uint64_t, unround_result** result, char* message, size_t message_size) {
-  if (result != nullptr) *result = nullptr;
-  write_message(message, message_size, "projecting a picture onto a file's intervals needs a JPEG reader, "
-                                       "which is not part of this library");
-  return UNROUND_ERROR_UNSUPPORTED;
+unround_status unround_project(const uint8_t* data, size_t size, const unround_settings* settings,
+                               const double* picture, uint64_t height, uint64_t width, uint64_t channels,
+                               unround_result** result, char* message, size_t message_size) {
+  return guarded(message, message_size, [&](std::string& text) {
+    return project_picture(data, size, settings, picture, height, width, channels, result, text);
+  });
 }
 
 void unround_result_free(unround_result* result) { delete result; }
@@ -823,9 +912,13 @@ const char* unround_result_warning(const unround_result* result) {
   return result == nullptr ? nullptr : result->warning.c_str();
 }
 
-const double* unround_result_excess(const unround_result*, uint64_t, uint64_t* count) {
-  if (count != nullptr) *count = 0;
-  return nullptr;
+const double* unround_result_excess(const unround_result* result, uint64_t component, uint64_t* count) {
+  if (result == nullptr || component >= result->excess.size()) {
+    if (count != nullptr) *count = 0;
+    return nullptr;
+  }
+  if (count != nullptr) *count = result->excess[component].size();
+  return result->excess[component].data();
 }
 
 int32_t unround_result_kernel_times(const unround_result* result, double* values, size_t count) {

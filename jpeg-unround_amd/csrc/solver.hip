@@ -461,7 +461,7 @@ __global__ __launch_bounds__(256) void reduce_kernel(const double* __restrict__ 
 template <typename T>
 __global__ __launch_bounds__(256) void final_kernel(const Component* __restrict__ component, const T* tx_field, int W,
                                                     double margin, double* __restrict__ canvas,
-                                                    double* __restrict__ coefficients) {
+                                                    double* __restrict__ coefficients, double* __restrict__ excess) {
   __shared__ double first[4][64];
   __shared__ double second[4][64];
   const Component& K = *component;
@@ -500,6 +500,8 @@ __global__ __launch_bounds__(256) void final_kernel(const Component* __restrict_
     device::ends_of(K, K.levels[64 * block + lane], lane, lower, upper, centre, own_lower, own_upper);
     zeta = device::clamp_to(e, lower + margin, upper - margin);
     coefficients[64 * block + lane] = zeta;
+    // How far the canvas's coefficient lies outside its interval, in steps (0 within it).
+    if (excess != nullptr) excess[64 * block + lane] = (fmax(lower - e, 0.0) + fmax(e - upper, 0.0)) / K.steps[lane];
   }
   first[wave][lane] = zeta;
   __syncthreads();
@@ -1071,19 +1073,58 @@ class Solver {
 
   // The final projection onto the data term's intervals, in binary64, into out.
   virtual void finish(SolveOutput& out) {
-    hipStream_t s = stream();
-    DeviceBuffer canvas(sizeof(double) * C_ * N_);
-    std::vector<DeviceBuffer> coefficients;
     // Narrowed by 2^-30 of a unit: beyond what any binary64 block DCT of the result rounds
     // by, so that the coefficients requantize to their levels whatever the ties' rule.
-    const double margin = std::ldexp(1.0, -30);
+    project_tilde(out, std::ldexp(1.0, -30), nullptr);
+  }
+
+ public:
+  // A canvas (C x H x W binary64) projected onto the intervals (docs/math.md 12;
+  // rust/unround/src/frames.rs:408-432): each block's coefficients, of its cells' means,
+  // clipped to their intervals and the cells moved by the change; and how far each
+  // coefficient lay outside its interval, in steps (frames.rs:492-504). Of a Solver of
+  // binary64 iterates.
+  void project(const std::vector<double>& canvas, SolveOutput& out, std::vector<std::vector<double>>& excess) {
+    static_assert(sizeof(T) == sizeof(double), "the projection is taken in binary64");
+    C_ = static_cast<int>(frame_.channels());
+    H_ = static_cast<int>(frame_.rows);
+    W_ = static_cast<int>(frame_.columns);
+    N_ = static_cast<size_t>(H_) * W_;
+    tgv_ = false;
+    if (canvas.size() != static_cast<size_t>(C_) * N_) throw GpuError("a canvas of the frame's size");
+    allocate();
+    upload_components();
+    HIP_CHECK(hipMemcpyAsync(fields_.tx, canvas.data(), sizeof(double) * canvas.size(), hipMemcpyHostToDevice,
+                             stream()));
+    project_tilde(out, 0.0, &excess);
+    out.iterations = 0;
+    out.stop = 0;
+  }
+
+ protected:
+  // The final projection of x~ into out (and, where asked, the excess of each coefficient).
+  void project_tilde(SolveOutput& out, double margin, std::vector<std::vector<double>>* excess) {
+    hipStream_t s = stream();
+    DeviceBuffer canvas(sizeof(double) * C_ * N_);
+    std::vector<DeviceBuffer> coefficients, excesses;
     out.coefficients.assign(C_, {});
     for (int c = 0; c < C_; ++c) {
-      coefficients.emplace_back(sizeof(double) * std::max<size_t>(1, 64 * frame_.components[c].blocks()));
+      const size_t count = std::max<size_t>(1, 64 * frame_.components[c].blocks());
+      coefficients.emplace_back(sizeof(double) * count);
+      excesses.emplace_back(excess != nullptr ? sizeof(double) * count : 0);
       final_kernel<T><<<(tiles(c) + 3) / 4, 256, 0, s>>>(component(c), fields_.tx + c * N_, W_, margin,
                                                          canvas.as<double>() + c * N_,
-                                                         coefficients.back().as<double>());
+                                                         coefficients.back().as<double>(),
+                                                         excesses.back().as<double>());
       HIP_CHECK(hipGetLastError());
+    }
+    if (excess != nullptr) {
+      excess->assign(C_, {});
+      for (int c = 0; c < C_; ++c) {
+        (*excess)[c].resize(64 * frame_.components[c].blocks());
+        HIP_CHECK(hipMemcpyAsync((*excess)[c].data(), excesses[c].pointer, sizeof(double) * (*excess)[c].size(),
+                                 hipMemcpyDeviceToHost, s));
+      }
     }
     out.canvas.resize(static_cast<size_t>(C_) * N_);
     HIP_CHECK(hipMemcpyAsync(out.canvas.data(), canvas.pointer, sizeof(double) * out.canvas.size(),
@@ -1189,6 +1230,28 @@ bool solve_on_gpu(const FrameModel& frame, const Settings& settings, const Plan&
       Solver<float, float> solver(frame, settings, plan, timing);
       solver.run(start, observer, out);
     }
+    return true;
+  } catch (const GpuError& error) {
+    status = 6;
+    message = std::string("a GPU error: ") + error.what();
+    return false;
+  }
+}
+
+bool project_on_gpu(const FrameModel& frame, const Settings& settings, const std::vector<double>& canvas,
+                    SolveOutput& out, std::vector<std::vector<double>>& excess, std::string& message,
+                    int32_t& status) {
+  try {
+    int devices = 0;
+    if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) {
+      status = 6;
+      message = "no HIP device: this library computes on an AMD GPU";
+      return false;
+    }
+    upload_basis();
+    const Plan plan;
+    Solver<double, double> solver(frame, settings, plan, Timing::none);
+    solver.project(canvas, out, excess);
     return true;
   } catch (const GpuError& error) {
     status = 6;

@@ -67,6 +67,14 @@ struct SolveOutput {
 bool solve_on_gpu(const FrameModel& frame, const Settings& settings, const Plan& plan, const StartState* start,
                   const Observer& observer, Timing timing, SolveOutput& out, std::string& message, int32_t& status);
 
+// A canvas (C x H x W binary64, the frame's) projected onto the frame's intervals on the
+// current HIP device (docs/math.md 12): out.canvas and out.coefficients as a final
+// projection makes them, without a margin, and excess[c] how far each coefficient of
+// component c lay outside its interval, in steps.
+bool project_on_gpu(const FrameModel& frame, const Settings& settings, const std::vector<double>& canvas,
+                    SolveOutput& out, std::vector<std::vector<double>>& excess, std::string& message,
+                    int32_t& status);
+
 // A picture of rounded samples (TV or TGV within its intervals) on the current HIP device:
 // out.canvas is the picture, channels x height x width, every sample within its interval;
 // no coefficients.

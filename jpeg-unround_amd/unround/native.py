@@ -14,9 +14,11 @@ from __future__ import annotations
 import ctypes
 import enum
 import functools
+import os
 import warnings
 from collections.abc import Callable
 from dataclasses import dataclass
+from pathlib import Path
 from typing import Final, Sequence
 
 import numpy as np
@@ -44,6 +46,7 @@ __all__ = [
     "oriented",
     "png_bytes",
     "pnm_bytes",
+    "project",
     "solve",
     "solve_batch",
     "solve_samples",
@@ -194,6 +197,9 @@ def _declare_all(library: ctypes.CDLL) -> None:
     _declare(library, "unround_result_kernel_times", i32, handle, doubles, size)
     _declare(library, "unround_result_exif_orientation", i32, handle)
     _declare(library, "unround_result_icc_profile", ctypes.POINTER(ctypes.c_uint8), handle, u64_out)
+    _declare(library, "unround_result_excess", doubles, handle, u64, u64_out)
+    _declare(library, "unround_project", i32, ctypes.c_char_p, size, handle, doubles, u64, u64, u64,
+             ctypes.POINTER(handle), *message)
     _declare(library, "unround_result_format", i32, handle)
     _declare(library, "unround_result_bits", i32, handle)
     _declare(library, "unround_result_warning", ctypes.c_char_p, handle)
@@ -302,8 +308,14 @@ def _subgradient_options(method: SubgradientOptions) -> list[str]:
     return result
 
 
-def options(settings: Settings | None = None) -> list[str]:
-    """The command line's options that are the settings (python/src/unround/native.py:305-332)."""
+def options(settings: Settings | None = None, *, max_pixels: int = 0, max_scans: int = 0,
+            warnings_are_errors: bool = False) -> list[str]:
+    """The command line's options that are the settings (python/src/unround/native.py:321-353),
+    with the limits of reading a file."""
+    reading = ([f"--max-pixels={max_pixels}"] if max_pixels else []) + (
+        [f"--max-scans={max_scans}"] if max_scans else []) + (["--warnings-are-errors"] if warnings_are_errors else [])
+    if isinstance(settings, list):
+        return settings + reading
     settings = Settings() if settings is None else settings
     weights: TV | TGV = settings.tgv if settings.method == "tgv" else settings.tv
     result = ["--method", settings.method, "--alpha", _number(settings.tv.alpha)]
@@ -314,7 +326,7 @@ def options(settings: Settings | None = None) -> list[str]:
     result += _data_options(settings.data)
     result += _pdhg_options(settings.pdhg)
     result += _subgradient_options(settings.subgradient)
-    return result
+    return result + reading
 
 
 class _Settings:
@@ -502,6 +514,13 @@ def _decoded(library: ctypes.CDLL, result: ctypes.c_void_p) -> Decoded:
     planes = _checked(planes_pointer, planes_count, (count, rows, columns))
     canvas = _copy(canvas_pointer, (count, int(canvas_height.value), int(canvas_width.value)))
     components = [_problem(library, result, index) for index in range(int(library.unround_result_components(result)))]
+    excess = []
+    for index, (coefficients, _) in enumerate(components):
+        count = ctypes.c_uint64()
+        pointer = library.unround_result_excess(result, index, ctypes.byref(count))
+        if not pointer:
+            break
+        excess.append(_checked(pointer, count, coefficients.shape))
     profile_size = ctypes.c_uint64()
     profile = library.unround_result_icc_profile(result, ctypes.byref(profile_size))
     warning = (library.unround_result_warning(result) or b"").decode("utf-8", "replace")
@@ -520,6 +539,7 @@ def _decoded(library: ctypes.CDLL, result: ctypes.c_void_p) -> Decoded:
         input_format="png" if int(library.unround_result_format(result)) == _FORMAT_PNG else "jpeg",
         bits=int(library.unround_result_bits(result)),
         alpha_result=_solution(library, result, alpha=True),
+        excess=tuple(excess) if components and len(excess) == len(components) else None,
         kernel_times=_times(library, result),
     )
 
@@ -563,24 +583,51 @@ def _reconstruct(call: Callable, observer: Observer | None) -> Decoded:
         library.unround_result_free(result)
 
 
-def decode(source: bytes | jpegio.Image, settings: Settings | list[str] | None = None, *,
-           observer: Observer | None = None) -> Decoded:
-    """Reconstructs a file's bytes (python/src/unround/native.py:677-716): a PNG file within
-    the intervals of its rounded samples, by TV or TGV with mu and slack alone of the data
-    term, its colours together and its alpha alone; a JPEG file needs a JPEG reader, which
-    the library has not (UnroundError with Status.UNSUPPORTED). An Image is reconstructed
-    as solve() does."""
+def decode(source: bytes | str | os.PathLike | jpegio.Image, settings: Settings | list[str] | None = None, *,
+           observer: Observer | None = None, max_pixels: int = 0, max_scans: int = 0,
+           warnings_are_errors: bool = False) -> Decoded:
+    """Reconstructs a JPEG file or a PNG file, its bytes or its path (python/src/unround/
+    native.py:677-716): a PNG file within the intervals of its rounded samples, by TV or TGV
+    with mu and slack alone of the data term, its colours together and its alpha alone; any
+    other as a JPEG file, read by the library's reader with these limits of reading. An Image
+    is reconstructed as solve() does. A warning of the reading is said as an UnroundWarning."""
     if isinstance(source, jpegio.Image):
+        if max_pixels or max_scans or warnings_are_errors:
+            raise ValueError("an image read already takes no limits of reading")
         return solve(source, settings, observer=observer)
     library = _library()
-    handle = _Settings(settings if isinstance(settings, list) else options(settings))
-    buffer = bytes(source)
+    handle = _Settings(options(settings, max_pixels=max_pixels, max_scans=max_scans,
+                               warnings_are_errors=warnings_are_errors))
+    buffer = Path(source).read_bytes() if isinstance(source, (str, os.PathLike)) else bytes(source)
 
     def call(callback: object, result: object, message: ctypes.Array) -> int:
         return int(library.unround_decode(buffer, len(buffer), handle.handle, callback, None, result, message,
                                           _MESSAGE_SIZE))
 
     return _reconstruct(call, observer)
+
+
+def project(picture: npt.ArrayLike, source: bytes | str | os.PathLike, settings: Settings | list[str] | None = None,
+            *, max_pixels: int = 0, max_scans: int = 0, warnings_are_errors: bool = False) -> Decoded:
+    """Measures a picture made elsewhere against a JPEG file's intervals and projects it onto
+    them on the GPU (docs/math.md, 12; python/src/unround/native.py:718-755): picture (height,
+    width) of a grey file or (height, width, 3) RGB of a file in colour, as the file stores
+    it; the result is the projection as a reconstruction without a solver (result None), and
+    its excess how far each coefficient lay outside its interval, in steps."""
+    library = _library()
+    samples, height, width, channels = _samples(picture)
+    handle = _Settings(options(settings, max_pixels=max_pixels, max_scans=max_scans,
+                               warnings_are_errors=warnings_are_errors))
+    buffer = Path(source).read_bytes() if isinstance(source, (str, os.PathLike)) else bytes(source)
+    pointer = samples.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+    def call(_callback: object, result: object, message: ctypes.Array) -> int:
+        return int(library.unround_project(buffer, len(buffer), handle.handle, pointer, height, width, channels,
+                                           result, message, _MESSAGE_SIZE))
+
+    projected = _reconstruct(call, None)
+    del samples
+    return projected
 
 
 def _input_of(image: jpegio.Image) -> tuple[_Input, list[np.ndarray], object]:

@@ -47,3 +47,54 @@ def test_decode_of_a_jpeg_file_is_the_solve_of_its_components(channels, options)
                                    c.h_samp_factor, c.v_samp_factor) for c in image.components]
     expected = oracle.solve(image.height, image.width, int(image.color_space), components, settings)
     assert np.max(np.abs(decoded.canvas - expected.canvas)) <= 1e-8
+
+
+def block_dct(plane: np.ndarray) -> np.ndarray:
+    """The orthonormal 8 x 8 DCT of a plane's blocks, (rows, columns, 8, 8)."""
+    k = np.arange(8)
+    basis = np.sqrt(2 / 8) * np.cos(np.pi * (2 * k[None, :] + 1) * k[:, None] / 16)
+    basis[0] /= np.sqrt(2)
+    rows, columns = plane.shape[0] // 8, plane.shape[1] // 8
+    blocks = plane.reshape(rows, 8, columns, 8).transpose(0, 2, 1, 3)
+    return basis @ blocks @ basis.T
+
+
+def test_a_picture_made_elsewhere_is_measured_and_projected():
+    # docs/math.md 12: the canvas of Pillow's own decoding, extended by its edges; each
+    # coefficient's excess in steps against the intervals; the projection's coefficients
+    # clipped to them; and the projection of the projection moves nothing.
+    made = picture(37, 45, 1, seed=5)
+    data = jpeg(made, quality=60)
+    shown = np.asarray(Image.open(io.BytesIO(data))).astype(np.float64)
+    settings = ["--method", "mmse"]
+    print("projection of Pillow's decoding of a 37x45 grey JPEG file", flush=True)
+    projected = native.project(shown, data, settings)
+    assert projected.result is None and projected.excess is not None
+    problem = projected.problems[0]
+    rows, columns = problem.lower.shape[:2]
+    canvas = np.pad(shown, ((0, 8 * rows - 37), (0, 8 * columns - 45)), mode="edge") - 0.0
+    coefficients = block_dct(canvas)
+    steps = problem.steps.reshape(1, 1, 8, 8)
+    expected = (np.maximum(problem.lower - coefficients, 0) + np.maximum(coefficients - problem.upper, 0)) / steps
+    assert np.max(np.abs(projected.excess[0] - expected)) <= 1e-9
+    assert np.count_nonzero(expected > 1e-9) > 0
+    clipped = np.clip(coefficients, problem.lower, problem.upper)
+    assert np.max(np.abs(projected.coefficients[0] - clipped)) <= 1e-9
+    assert np.all(projected.coefficients[0] >= problem.lower) and np.all(projected.coefficients[0] <= problem.upper)
+    # The projection of the projection's picture: within the intervals at the blocks that the
+    # picture covers whole (docs/math.md 12: the samples beyond it moved too, and cut).
+    again = native.project(projected.picture, data, settings)
+    assert np.max(again.excess[0][: 37 // 8, : 45 // 8]) <= 1e-9
+
+
+def test_a_colour_picture_is_projected_through_jfif_s_ycbcr():
+    made = picture(32, 48, 3, seed=6)
+    data = jpeg(made, quality=70, subsampling=2)
+    shown = np.asarray(Image.open(io.BytesIO(data))).astype(np.float64)
+    print("projection of a 32x48 4:2:0 JPEG file's decoding", flush=True)
+    projected = native.project(shown, data, ["--method", "mmse"])
+    assert projected.picture.shape == (32, 48, 3) and len(projected.excess) == 3
+    for c, problem in enumerate(projected.problems):
+        assert np.all(projected.coefficients[c] >= problem.lower) and np.all(projected.coefficients[c] <= problem.upper)
+    again = native.project(projected.picture, data, ["--method", "mmse"])
+    assert max(float(np.max(e)) for e in again.excess) <= 1e-6

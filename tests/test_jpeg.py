@@ -341,3 +341,19 @@ def test_decode_reads_a_jpeg_file_and_refuses_what_it_cannot_read():
     with pytest.raises(native.UnroundError) as caught:
         native.decode(b"not a picture", ["--method", "tgv"])
     assert caught.value.status == native.Status.READ
+
+
+def test_projection_refusals_come_before_the_gpu():
+    # rust/capi/src/lib.rs:708-752: a PNG file, a picture of another size or other channels.
+    picture = smooth_picture(16, 24, 1, seed=2)
+    data = pillow_file(picture, quality=60)
+    png = io.BytesIO()
+    Image.fromarray(picture).save(png, "PNG")
+    cases = [(png.getvalue(), np.zeros((16, 24)), native.Status.OPTIONS, "PNG"),
+             (data, np.zeros((16, 23)), native.Status.OPTIONS, "16 x 24"),
+             (data, np.zeros((16, 24, 3)), native.Status.OPTIONS, "channels"),
+             (data[:20], np.zeros((16, 24)), native.Status.READ, "")]
+    for source, given, status, words in cases:
+        with pytest.raises(native.UnroundError) as caught:
+            native.project(given, source, ["--method", "mmse"])
+        assert caught.value.status == status and words in caught.value.message, caught.value.message
