@@ -397,6 +397,52 @@ struct BufferPlanes {
   __device__ __forceinline__ void store(int plane, uint32_t index, X value) const {
     store_at(plane, index * static_cast<uint32_t>(sizeof(X)), value);
   }
+  // Samples index and index + 1 (index even, so that the pair is aligned), zeros where
+  // `valid` is false: one load of twice the width.
+  __device__ __forceinline__ void load_pair_if(bool valid, int plane, uint32_t index, X& first, X& second) const {
+    const int offset = static_cast<int>(valid ? index * static_cast<uint32_t>(sizeof(X)) : kOutside);
+    const int scalar = static_cast<int>(static_cast<uint32_t>(plane) * plane_bytes);
+    if constexpr (sizeof(X) == 2) {
+      const unsigned int both = __builtin_bit_cast(unsigned int, __builtin_amdgcn_raw_buffer_load_b32(resource, offset,
+                                                                                                      scalar, 0));
+      first = __builtin_bit_cast(X, static_cast<unsigned short>(both & 0xFFFFu));
+      second = __builtin_bit_cast(X, static_cast<unsigned short>(both >> 16));
+    } else if constexpr (sizeof(X) == 4) {
+      using Pair = unsigned int __attribute__((ext_vector_type(2)));
+      const Pair both = __builtin_bit_cast(Pair, __builtin_amdgcn_raw_buffer_load_b64(resource, offset, scalar, 0));
+      // The words go through plain variables: a bit cast of a vector element (`both.y`)
+      // is lowered by clang as a load from the vector's first element.
+      const unsigned int low = both[0], high = both[1];
+      first = __builtin_bit_cast(X, low);
+      second = __builtin_bit_cast(X, high);
+    } else {
+      using Quad = unsigned int __attribute__((ext_vector_type(4)));
+      using Pair = unsigned int __attribute__((ext_vector_type(2)));
+      const Quad both = __builtin_bit_cast(Quad, __builtin_amdgcn_raw_buffer_load_b128(resource, offset, scalar, 0));
+      first = __builtin_bit_cast(X, Pair{both.x, both.y});
+      second = __builtin_bit_cast(X, Pair{both.z, both.w});
+    }
+  }
+  // The stores of index and index + 1 (index even) where `valid`, dropped elsewhere.
+  __device__ __forceinline__ void store_pair_if(bool valid, int plane, uint32_t index, X first, X second) const {
+    const int offset = static_cast<int>(valid ? index * static_cast<uint32_t>(sizeof(X)) : kOutside);
+    const int scalar = static_cast<int>(static_cast<uint32_t>(plane) * plane_bytes);
+    if constexpr (sizeof(X) == 2) {
+      const unsigned int both = static_cast<unsigned int>(__builtin_bit_cast(unsigned short, first)) |
+                                (static_cast<unsigned int>(__builtin_bit_cast(unsigned short, second)) << 16);
+      __builtin_amdgcn_raw_buffer_store_b32(both, resource, offset, scalar, 0);
+    } else if constexpr (sizeof(X) == 4) {
+      using Pair = unsigned int __attribute__((ext_vector_type(2)));
+      const Pair both{__builtin_bit_cast(unsigned int, first), __builtin_bit_cast(unsigned int, second)};
+      __builtin_amdgcn_raw_buffer_store_b64(both, resource, offset, scalar, 0);
+    } else {
+      using Quad = unsigned int __attribute__((ext_vector_type(4)));
+      using Pair = unsigned int __attribute__((ext_vector_type(2)));
+      const Pair a = __builtin_bit_cast(Pair, first), b = __builtin_bit_cast(Pair, second);
+      const Quad both{a.x, a.y, b.x, b.y};
+      __builtin_amdgcn_raw_buffer_store_b128(both, resource, offset, scalar, 0);
+    }
+  }
   __device__ __forceinline__ X load_at(int plane, uint32_t bytes) const {
     const int offset = static_cast<int>(bytes);
     const int scalar = static_cast<int>(static_cast<uint32_t>(plane) * plane_bytes);
@@ -434,6 +480,14 @@ struct WidePlanes {
   }
   __device__ __forceinline__ void store_if(bool valid, int plane, uint32_t index, X value) const {
     if (valid) base[plane * N + index] = value;
+  }
+  __device__ __forceinline__ void load_pair_if(bool valid, int plane, uint32_t index, X& first, X& second) const {
+    first = load_if(valid, plane, index);
+    second = load_if(valid, plane, index + 1);
+  }
+  __device__ __forceinline__ void store_pair_if(bool valid, int plane, uint32_t index, X first, X second) const {
+    store_if(valid, plane, index, first);
+    store_if(valid, plane, index + 1, second);
   }
 };
 

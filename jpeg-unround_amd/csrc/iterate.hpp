@@ -95,20 +95,29 @@ __device__ __forceinline__ void prox_tile(const Component* __restrict__ componen
   // The loads: p_y of the canvas row above, and per canvas row x, p_x and p_y at the lane's
   // columns and p_x of the column on its left (lane 0 only; the others take their left
   // lane's).
+  // Cells two samples wide take each pair of samples of a row in one load of twice the width
+  // (the pair is aligned: the first column and the canvas width are even).
   T up[DH], here[8][DV][DH], across[8][DV][DH], down[8][DV][DH], edge[8][DV];
+  const auto load_row = [&](const auto& planes, bool valid, int plane, uint32_t where, auto (&out)[DH]) {
+    if constexpr (DH == 2) {
+      decltype(planes.load(0, 0)) first, second;
+      planes.load_pair_if(valid, plane, where, first, second);
+      out[0] = device::widen(first);
+      out[1] = device::widen(second);
+    } else {
 #pragma unroll
-  for (int bb = 0; bb < DH; ++bb) up[bb] = device::widen(p.load_if(live && top > 0, 1, at(top - 1, first_column + bb)));
+      for (int bb = 0; bb < DH; ++bb) out[bb] = device::widen(planes.load_if(valid, plane, where + bb));
+    }
+  };
+  load_row(p, live && top > 0, 1, at(top - 1, first_column), up);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
 #pragma unroll
     for (int a = 0; a < DV; ++a) {
       const int i = top + m * DV + a;
-#pragma unroll
-      for (int bb = 0; bb < DH; ++bb) {
-        here[m][a][bb] = x.load_if(live, 0, at(i, first_column + bb));
-        across[m][a][bb] = device::widen(p.load_if(live, 0, at(i, first_column + bb)));
-        down[m][a][bb] = device::widen(p.load_if(live, 1, at(i, first_column + bb)));
-      }
+      load_row(x, live, 0, at(i, first_column), here[m][a]);
+      load_row(p, live, 0, at(i, first_column), across[m][a]);
+      load_row(p, live, 1, at(i, first_column), down[m][a]);
       edge[m][a] = device::widen(p.load_if(live && lane == 0 && first_column > 0, 0, at(i, first_column - 1)));
     }
   }
@@ -186,14 +195,27 @@ __device__ __forceinline__ void prox_tile(const Component* __restrict__ componen
     const T change = back[m] - column[m];
 #pragma unroll
     for (int a = 0; a < DV; ++a) {
+      T bars[DH], relaxed[DH], tildes[DH];
 #pragma unroll
       for (int bb = 0; bb < DH; ++bb) {
-        const uint32_t where = at(top + m * DV + a, first_column + bb);
         const T v = vs[m][a * DH + bb], old = here[m][a][bb];
         const T tilde = own_inside ? (DV * DH > 1 ? v + change : back[m]) : v;
-        xbar.store_if(live, 0, where, T(2) * tilde - old);
-        x.store_if(live, 0, where, rho * tilde + rest * old);
-        if (keep) tilde_x.store_if(live, 0, where, tilde);
+        bars[bb] = T(2) * tilde - old;
+        relaxed[bb] = rho * tilde + rest * old;
+        tildes[bb] = tilde;
+      }
+      const uint32_t where = at(top + m * DV + a, first_column);
+      if constexpr (DH == 2) {
+        xbar.store_pair_if(live, 0, where, bars[0], bars[1]);
+        x.store_pair_if(live, 0, where, relaxed[0], relaxed[1]);
+        if (keep) tilde_x.store_pair_if(live, 0, where, tildes[0], tildes[1]);
+      } else {
+#pragma unroll
+        for (int bb = 0; bb < DH; ++bb) {
+          xbar.store_if(live, 0, where + bb, bars[bb]);
+          x.store_if(live, 0, where + bb, relaxed[bb]);
+          if (keep) tilde_x.store_if(live, 0, where + bb, tildes[bb]);
+        }
       }
     }
   }
@@ -205,11 +227,22 @@ template <typename T, int DV, int DH>
 constexpr int kProxWaves =
     sizeof(T) == 4 ? (DV * DH == 1 ? 5 : 3) : (DV * DH == 1 ? 4 : (DV * DH == 2 ? 2 : 1));
 
+// The steps of the components one prox launch takes, from its first: component c + k has
+// descent[k] (tau gamma_c) and step[k] (tau over its cell's samples).
+template <typename T>
+struct ProxSteps {
+  T descent[3];
+  T step[3];
+};
+
+// Components c .. c + gridDim.y - 1 of one cell shape and size, blockIdx.y the component: the
+// chroma of 4:2:0 in one launch, so that its work-groups fill the GPU together.
 template <typename T, typename S, int DV, int DH, template <typename> class Planes>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kProxWaves<T, DV, DH>))) void prox_kernel(
-    const Component* __restrict__ component, Fields<T, S> F, int c, int H, int W, T descent, T step, T rho, T rest,
+    const Component* __restrict__ component, Fields<T, S> F, int c, int H, int W, ProxSteps<T> steps, T rho, T rest,
     int keep) {
-  prox_tile<T, S, DV, DH, Planes>(component, F, c, H, W, descent, step, rho, rest, keep,
+  const int k = static_cast<int>(blockIdx.y);
+  prox_tile<T, S, DV, DH, Planes>(component + k, F, c + k, H, W, steps.descent[k], steps.step[k], rho, rest, keep,
                                   device::ordered(blockIdx.x, gridDim.x));
 }
 

@@ -892,22 +892,38 @@ class Solver {
     }
   }
 
-  // The proximal step of every component, each timed on its own where timing asks.
+  // Whether component c and the one after it share a launch: the same cell shape and the
+  // same tiles (the chroma of a frame).
+  bool shares_launch(int c) const {
+    if (c + 1 >= C_) return false;
+    const ComponentModel& a = frame_.components[c];
+    const ComponentModel& b = frame_.components[c + 1];
+    return a.down == b.down && a.across == b.across && tiles(c) == tiles(c + 1) && prox_groups(c) == prox_groups(c + 1);
+  }
+
+  // The proximal step of every component: components of one cell shape and size in one
+  // launch, each launch timed on its own where timing asks.
   virtual void primal_step(bool keep) {
     hipStream_t s = stream();
     const T rho = static_cast<T>(plan_.relaxation), rest = static_cast<T>(1.0 - plan_.relaxation);
     const bool per_kernel = timing_ == Timing::kernels;
-    if (per_kernel)
-      for (int c = 0; c < C_; ++c) primal_events_.emplace_back();
-    const size_t first_event = per_kernel ? primal_events_.size() - static_cast<size_t>(C_) : 0;
-    for (int c = 0; c < C_; ++c) {
+    for (int c = 0; c < C_;) {
+      int count = 1;
+      while (shares_launch(c + count - 1)) ++count;
       const ComponentModel& k = frame_.components[c];
-      const T descent = static_cast<T>(plan_.tau * plan_.gamma[c]);
-      const T step = static_cast<T>(plan_.tau / static_cast<double>(k.cells()));
-      if (per_kernel) HIP_CHECK(hipEventRecord(primal_events_[first_event + c].begin, s));
+      ProxSteps<T> steps{};
+      for (int m = 0; m < count; ++m) {
+        steps.descent[m] = static_cast<T>(plan_.tau * plan_.gamma[c + m]);
+        steps.step[m] = static_cast<T>(plan_.tau / static_cast<double>(frame_.components[c + m].cells()));
+      }
+      if (per_kernel) {
+        primal_events_.emplace_back();
+        HIP_CHECK(hipEventRecord(primal_events_.back().begin, s));
+      }
       const int groups = prox_groups(c), live = keep ? 1 : 0;
+      const dim3 grid(static_cast<unsigned>(groups), static_cast<unsigned>(count));
       const auto prox = [&](auto kernel) {
-        kernel<<<groups, 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho, rest, live);
+        kernel<<<grid, 256, 0, s>>>(component(c), fields_, c, H_, W_, steps, rho, rest, live);
       };
       // Through buffer resources where a component's planes span at most 2 GiB (x: one of T,
       // p: two of S), else with 64-bit addresses.
@@ -921,11 +937,14 @@ class Solver {
       } else if (k.down == 2 && k.across == 1) {
         buffers ? prox(prox_kernel<T, S, 2, 1, device::BufferPlanes>) : prox(prox_kernel<T, S, 2, 1, device::WidePlanes>);
       } else {
-        prox_any_cell_kernel<T, S><<<groups, 256, 0, s>>>(component(c), fields_, c, H_, W_, descent, step, rho, rest,
-                                                          live);
+        for (int m = 0; m < count; ++m) {
+          prox_any_cell_kernel<T, S><<<groups, 256, 0, s>>>(component(c + m), fields_, c + m, H_, W_,
+                                                            steps.descent[m], steps.step[m], rho, rest, live);
+        }
       }
       HIP_CHECK(hipGetLastError());
-      if (per_kernel) HIP_CHECK(hipEventRecord(primal_events_[first_event + c].end, s));
+      if (per_kernel) HIP_CHECK(hipEventRecord(primal_events_.back().end, s));
+      c += count;
     }
   }
 

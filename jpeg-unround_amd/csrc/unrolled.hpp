@@ -527,8 +527,11 @@ class Unrolled {
     const UnrolledSteps k = steps_of(f);
     const float step = static_cast<float>(plan.tau);  // one sample a cell
     const float rest = 1.0f - k.rho;
+    ProxSteps<float> steps{};
+    steps.descent[0] = k.descent;
+    steps.step[0] = step;
     prox_kernel<float, float, 1, 1, device::BufferPlanes><<<v.prox_groups(0), 256, 0, s>>>(
-        v.component(0), v.fields_, 0, H_, W_, k.descent, step, k.rho, rest, 0);
+        v.component(0), v.fields_, 0, H_, W_, steps, k.rho, rest, 0);
     HIP_CHECK(hipGetLastError());
     Fields<float, float>& F = v.fields_;
     const UnrolledState now{F.x, F.w, F.p, F.r};
