@@ -68,8 +68,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kProxWaves<
   const BatchTile t = tiles[device::ordered(blockIdx.x, gridDim.x)];
   if (active[t.image] == 0) return;
   const ImageSlot<T, S>& m = slots[t.image];
-  prox_tile<T, S, DV, DH, Planes>(m.components + t.part, m.F, t.part, m.H, m.W, m.descent[t.part], m.step[t.part],
-                                  rho, rest, keep, t.group);
+  // zeta's pointer read from the slot at its index: an index into a copy of the fields
+  // would place the copy in private memory, written by every lane.
+  prox_tile<T, S, DV, DH, Planes>(m.components + t.part, m.F, t.part, m.F.zeta[t.part], m.H, m.W, m.descent[t.part],
+                                  m.step[t.part], rho, rest, keep, t.group);
 }
 
 template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes>

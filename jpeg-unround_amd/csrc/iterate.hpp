@@ -72,8 +72,9 @@ __device__ __forceinline__ LevelRow level_row(const Component& K, int ty, int bx
 // prox_tile is work-group g's part of one component's step; prox_kernel runs a component
 // (work-groups in a run on each XCD), and the batch's kernel runs tiles of many images.
 template <typename T, typename S, int DV, int DH, template <typename> class Planes>
-__device__ __forceinline__ void prox_tile(const Component* __restrict__ component, Fields<T, S> F, int c, int H,
-                                          int W, T descent, T step, T rho, T rest, int keep, int g) {
+__device__ __forceinline__ void prox_tile(const Component* __restrict__ component, Fields<T, S> F, int c,
+                                          T* __restrict__ zeta_kept, int H, int W, T descent, T step, T rho, T rest,
+                                          int keep, int g) {
   __shared__ T forward_room[4][8][65];
   __shared__ T inverse_room[4][8][65];
   __shared__ device::StagedArrays staged_arrays;
@@ -172,7 +173,7 @@ __device__ __forceinline__ void prox_tile(const Component* __restrict__ componen
     zeta[u] = inside ? z : T(0);
   }
   if (keep && inside) {
-    T* kept = F.zeta[c] + 64 * (static_cast<size_t>(ty) * K.columns + tx) + 8 * n;
+    T* kept = zeta_kept + 64 * (static_cast<size_t>(ty) * K.columns + tx) + 8 * n;
 #pragma unroll
     for (int u = 0; u < 8; ++u) kept[u] = zeta[u];
   }
@@ -242,8 +243,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kProxWaves<
     const Component* __restrict__ component, Fields<T, S> F, int c, int H, int W, ProxSteps<T> steps, T rho, T rest,
     int keep) {
   const int k = static_cast<int>(blockIdx.y);
-  prox_tile<T, S, DV, DH, Planes>(component + k, F, c + k, H, W, steps.descent[k], steps.step[k], rho, rest, keep,
-                                  device::ordered(blockIdx.x, gridDim.x));
+  prox_tile<T, S, DV, DH, Planes>(component + k, F, c + k, F.zeta[c + k], H, W, steps.descent[k], steps.step[k], rho,
+                                  rest, keep, device::ordered(blockIdx.x, gridDim.x));
 }
 
 // The same for any cell up to 4 x 4, its samples read again for the write-back.
@@ -409,12 +410,15 @@ struct EdgeGather {
     }
   }
 
-  // This lane's sample for row i (rows clamped into the canvas), or 0.
-  __device__ __forceinline__ void load(int i, int H, int W, T& t, T& s) const {
+  // This lane's sample for row i (rows clamped into the canvas), or 0; 0 also where
+  // `wanted` is false.
+  __device__ __forceinline__ void load(int i, int H, int W, T& t, T& s, bool wanted = true) const {
     t = T(0);
     s = T(0);
-    if (t_source != nullptr) t = t_source[static_cast<size_t>(min(i + t_down, H - 1)) * W + t_column];
-    if (s_source != nullptr) s = device::widen(s_source[static_cast<size_t>(min(i + s_down, H - 1)) * W + s_column]);
+    if (wanted && t_source != nullptr) t = t_source[static_cast<size_t>(min(i + t_down, H - 1)) * W + t_column];
+    if (wanted && s_source != nullptr) {
+      s = device::widen(s_source[static_cast<size_t>(min(i + s_down, H - 1)) * W + s_column]);
+    }
   }
 };
 
@@ -497,19 +501,21 @@ __device__ __forceinline__ void dual_tile(Fields<T, S> F, int H, int W, int rows
   for (int i = first; i < end; ++i) {
     const bool below = i + 1 < H;
     // The loads of the next iteration: x- and r two rows down, p and w one row down, and the
-    // edge samples one row down.
+    // edge samples one row down; none in the run's last row, whose next rows belong to the
+    // run below (an unwanted buffer load reads nothing from memory).
+    const bool ahead = i + 1 < end;
     T bar_ahead[C], r_ahead[C][3], p_ahead[C][2], w_ahead[C][2], edge_t_ahead, edge_s_ahead;
-    edges.load(i + 1, H, W, edge_t_ahead, edge_s_ahead);
+    edges.load(i + 1, H, W, edge_t_ahead, edge_s_ahead, ahead);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      bar_ahead[c] = xbar.load(c, index(i + 2));
+      bar_ahead[c] = xbar.load_if(ahead, c, index(i + 2));
 #pragma unroll
-      for (int e = 0; e < 2; ++e) p_ahead[c][e] = device::widen(p.load(2 * c + e, index(i + 1)));
+      for (int e = 0; e < 2; ++e) p_ahead[c][e] = device::widen(p.load_if(ahead, 2 * c + e, index(i + 1)));
       if constexpr (TGV) {
 #pragma unroll
-        for (int e = 0; e < 3; ++e) r_ahead[c][e] = device::widen(r.load(3 * c + e, index(i + 2)));
+        for (int e = 0; e < 3; ++e) r_ahead[c][e] = device::widen(r.load_if(ahead, 3 * c + e, index(i + 2)));
 #pragma unroll
-        for (int e = 0; e < 2; ++e) w_ahead[c][e] = w.load(2 * c + e, index(i + 1));
+        for (int e = 0; e < 2; ++e) w_ahead[c][e] = w.load_if(ahead, 2 * c + e, index(i + 1));
       }
     }
 
