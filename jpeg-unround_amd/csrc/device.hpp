@@ -343,6 +343,67 @@ __device__ __forceinline__ void tensor_divergence_at(const S* rxx, const S* ryy,
   dy += forward_down<R>(ryy, i, j, H, W);
 }
 
+// A cell's shape (samples down, across) as a type, for choosing a kernel's instance.
+template <int DOWN, int ACROSS>
+struct Shape {
+  static constexpr int first = DOWN, second = ACROSS;
+};
+
+// The same differences with every sample loaded first, at positions clamped into the
+// canvas, and the case chosen after: no load sits under a branch, so that all of a lane's
+// loads are in flight together. The values are those of the helpers above.
+template <typename R, typename S>
+__device__ __forceinline__ R clamped_at(const S* f, int i, int j, int H, int W) {
+  return at<R>(f, min(max(i, 0), H - 1), min(max(j, 0), W - 1), W);
+}
+
+template <typename R, typename S>
+__device__ __forceinline__ R backward_across_loaded(const S* f, int i, int j, int H, int W) {
+  const R here = clamped_at<R>(f, i, j, H, W), left = clamped_at<R>(f, i, j - 1, H, W);
+  R value = here - left;
+  value = j == W - 1 ? -left : value;
+  value = j == 0 ? here : value;
+  return W == 1 ? R(0) : value;
+}
+
+template <typename R, typename S>
+__device__ __forceinline__ R backward_down_loaded(const S* f, int i, int j, int H, int W) {
+  const R here = clamped_at<R>(f, i, j, H, W), up = clamped_at<R>(f, i - 1, j, H, W);
+  const bool first = i == 0, last = i == H - 1;
+  R value = here - up;
+  value = first && !last ? here : value;
+  value = !first && last ? -up : value;
+  return first && last ? R(0) : value;
+}
+
+template <typename R, typename S>
+__device__ __forceinline__ R forward_across_loaded(const S* f, int i, int j, int H, int W) {
+  const R here = clamped_at<R>(f, i, j, H, W), right = clamped_at<R>(f, i, j + 1, H, W);
+  return j + 1 < W ? right - here : R(0);
+}
+
+template <typename R, typename S>
+__device__ __forceinline__ R forward_down_loaded(const S* f, int i, int j, int H, int W) {
+  const R here = clamped_at<R>(f, i, j, H, W), below = clamped_at<R>(f, i + 1, j, H, W);
+  return i + 1 < H ? below - here : R(0);
+}
+
+template <typename R, typename S>
+__device__ __forceinline__ R divergence_loaded(const S* px, const S* py, int i, int j, int H, int W) {
+  R value = backward_down_loaded<R>(py, i, j, H, W);
+  value += backward_across_loaded<R>(px, i, j, H, W);
+  return value;
+}
+
+template <typename R, typename S>
+__device__ __forceinline__ void tensor_divergence_loaded(const S* rxx, const S* ryy, const S* rxy, int i, int j, int H,
+                                                         int W, R& dx, R& dy) {
+  dx = forward_across_loaded<R>(rxx, i, j, H, W);
+  dx += forward_down_loaded<R>(rxy, i, j, H, W);
+  dy = forward_across_loaded<R>(rxy, i, j, H, W);
+  dy += forward_down_loaded<R>(ryy, i, j, H, W);
+}
+
 // radius / |p| beyond the ball, 1 within (cpp/include/unround/operators.hpp:112-115).
 template <typename T>
 __device__ __forceinline__ T ball_scale(T square, T radius) {

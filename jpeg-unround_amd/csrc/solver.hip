@@ -213,27 +213,100 @@ __global__ void project_start_kernel(Fields<T, S> F, int C, int H, int W, int tg
 // ---------------------------------------------------------------------------------------
 // Records, in binary64 (primal_dual.hpp:446-501, 791-895; frames.cpp:616-681).
 
-// A work-group's sum (or largest value) of one value per lane, in a fixed tree.
+// A work-group's sum (or largest value) of one value per lane, in a fixed order, for every
+// lane: within each wavefront a butterfly (the pairs are the same in every lane, and a + b
+// is b + a, so all lanes end with the same bits), then the wavefronts' results in order.
 template <bool LARGEST>
 __device__ double group_reduce(double value, double* room) {
-  room[threadIdx.x] = value;
-  __syncthreads();
-  for (int half = 128; half > 0; half >>= 1) {
-    if (static_cast<int>(threadIdx.x) < half) {
-      const double other = room[threadIdx.x + half];
-      room[threadIdx.x] = LARGEST ? fmax(room[threadIdx.x], other) : room[threadIdx.x] + other;
-    }
-    __syncthreads();
+#pragma unroll
+  for (int mask = 32; mask > 0; mask >>= 1) {
+    const double other = __shfl_xor(value, mask, 64);
+    value = LARGEST ? fmax(value, other) : value + other;
   }
-  const double total = room[0];
+  const int waves = static_cast<int>(blockDim.x) >> 6;
+  if ((threadIdx.x & 63) == 0) room[threadIdx.x >> 6] = value;
+  __syncthreads();
+  double total = room[0];
+  for (int w = 1; w < waves; ++w) total = LARGEST ? fmax(total, room[w]) : total + room[w];
   __syncthreads();
   return total;
 }
 
-// The norms of the tilde point's pixels: the first order (TV's gradient, or TGV's
-// grad x~ - w~), TGV's |E w~|_F, the largest norm of div_2 r~, and the partial gap's sum
-// over the channels of gamma |p~ + div_2 r~|; each work-group's sums and largest into
-// partials[4 group + 0..3].
+// The norms at one pixel of the tilde point (every channel): the first order (TV's
+// gradient, or TGV's grad x~ - w~), TGV's |E w~|_F, the largest norm of div_2 r~ and the
+// partial gap's sum over the channels of gamma |p~ + div_2 r~|. Every load is made first.
+template <typename T, typename S, bool TGV, int C>
+__device__ __forceinline__ void pixel_norms(const Fields<T, S>& F, int H, int W, int i, int j, const double* gammas,
+                                            int coupled, double& first, double& second, double& most,
+                                            double& residual) {
+  const size_t N = static_cast<size_t>(H) * W;
+  const size_t at = static_cast<size_t>(min(i, H - 1)) * W + min(j, W - 1);
+  double a[C], b[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const T* x = F.tx + c * N;
+    a[c] = device::forward_across_loaded<double>(x, i, j, H, W);
+    b[c] = device::forward_down_loaded<double>(x, i, j, H, W);
+    if (TGV) {
+      a[c] = a[c] + -1.0 * static_cast<double>(F.tw[(2 * c) * N + at]);
+      b[c] = b[c] + -1.0 * static_cast<double>(F.tw[(2 * c + 1) * N + at]);
+    }
+  }
+  double squares = 0.0;
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    squares += (coupled ? gammas[c] * gammas[c] : 1.0) * (a[c] * a[c] + b[c] * b[c]);
+    if (!coupled) {
+      first += gammas[c] * sqrt(squares);
+      squares = 0.0;
+    }
+  }
+  if (coupled) first = sqrt(squares);
+  if constexpr (TGV) {
+    double xx[C], yy[C], xy[C], dx[C], dy[C], pa[C], pb[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const T* wx = F.tw + (2 * c) * N;
+      const T* wy = F.tw + (2 * c + 1) * N;
+      xx[c] = device::backward_across_loaded<double>(wx, i, j, H, W);
+      yy[c] = device::backward_down_loaded<double>(wy, i, j, H, W);
+      xy[c] = device::backward_down_loaded<double>(wx, i, j, H, W);
+      xy[c] += device::backward_across_loaded<double>(wy, i, j, H, W);
+      xy[c] = 0.5 * xy[c];
+      device::tensor_divergence_loaded<double>(F.tr + (3 * c) * N, F.tr + (3 * c + 1) * N, F.tr + (3 * c + 2) * N, i,
+                                               j, H, W, dx[c], dy[c]);
+      pa[c] = static_cast<double>(device::widen(F.tp[(2 * c) * N + at]));
+      pb[c] = static_cast<double>(device::widen(F.tp[(2 * c + 1) * N + at]));
+    }
+    squares = 0.0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      squares += (coupled ? gammas[c] * gammas[c] : 1.0) * (xx[c] * xx[c] + yy[c] * yy[c] + 2.0 * (xy[c] * xy[c]));
+      if (!coupled) {
+        second += gammas[c] * sqrt(squares);
+        squares = 0.0;
+      }
+    }
+    if (coupled) second = sqrt(squares);
+    squares = 0.0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      squares += 1.0 * (dx[c] * dx[c] + dy[c] * dy[c]);
+      if (!coupled) {
+        most = fmax(most, sqrt(squares));
+        squares = 0.0;
+      }
+      const double ra = pa[c] + 1.0 * dx[c];
+      const double rb = pb[c] + 1.0 * dy[c];
+      residual += gammas[c] * sqrt(1.0 * (ra * ra + rb * rb));
+    }
+    if (coupled) most = sqrt(squares);
+  }
+}
+
+// The norms of the tilde point's pixels, each work-group's sums and largest into
+// partials[4 group + 0..3] (first order, TGV's second order, the largest |div_2 r~|, the
+// partial-gap residual).
 //
 // Work-group `group` takes 64 columns and 4 rows, groups numbered across, then down, with
 // `across` groups to a row of them.
@@ -245,61 +318,14 @@ __device__ __forceinline__ void norms_tile(const Fields<T, S>& F, int C, int H, 
   const int j = (group % across) * 64 + (threadIdx.x & 63);
   const int i = (group / across) * 4 + (threadIdx.x >> 6);
   const bool here = i < H && j < W;
-  const size_t N = static_cast<size_t>(H) * W;
   const double gammas[3] = {g0, g1, g2};
   double first = 0.0, second = 0.0, most = 0.0, residual = 0.0;
-  if (here) {
-    const size_t at = static_cast<size_t>(i) * W + j;
-    double squares = 0.0;
-    for (int c = 0; c < C; ++c) {
-      const T* x = F.tx + c * N;
-      double a = device::forward_across<double>(x, i, j, W);
-      double b = device::forward_down<double>(x, i, j, H, W);
-      if (TGV) {
-        a = a + -1.0 * static_cast<double>(F.tw[(2 * c) * N + at]);
-        b = b + -1.0 * static_cast<double>(F.tw[(2 * c + 1) * N + at]);
-      }
-      squares += (coupled ? gammas[c] * gammas[c] : 1.0) * (a * a + b * b);
-      if (!coupled) {
-        first += gammas[c] * sqrt(squares);
-        squares = 0.0;
-      }
-    }
-    if (coupled) first = sqrt(squares);
-    if (TGV) {
-      squares = 0.0;
-      for (int c = 0; c < C; ++c) {
-        const T* wx = F.tw + (2 * c) * N;
-        const T* wy = F.tw + (2 * c + 1) * N;
-        const double xx = device::backward_across<double>(wx, i, j, W);
-        const double yy = device::backward_down<double>(wy, i, j, H, W);
-        double xy = device::backward_down<double>(wx, i, j, H, W);
-        xy += device::backward_across<double>(wy, i, j, W);
-        xy = 0.5 * xy;
-        squares += (coupled ? gammas[c] * gammas[c] : 1.0) * (xx * xx + yy * yy + 2.0 * (xy * xy));
-        if (!coupled) {
-          second += gammas[c] * sqrt(squares);
-          squares = 0.0;
-        }
-      }
-      if (coupled) second = sqrt(squares);
-      squares = 0.0;
-      for (int c = 0; c < C; ++c) {
-        double dx, dy;
-        device::tensor_divergence_at<double>(F.tr + (3 * c) * N, F.tr + (3 * c + 1) * N, F.tr + (3 * c + 2) * N, i, j,
-                                             H, W, dx, dy);
-        squares += 1.0 * (dx * dx + dy * dy);
-        if (!coupled) {
-          most = fmax(most, sqrt(squares));
-          squares = 0.0;
-        }
-        const double ra = static_cast<double>(device::widen(F.tp[(2 * c) * N + at])) + 1.0 * dx;
-        const double rb = static_cast<double>(device::widen(F.tp[(2 * c + 1) * N + at])) + 1.0 * dy;
-        residual += gammas[c] * sqrt(1.0 * (ra * ra + rb * rb));
-      }
-      if (coupled) most = sqrt(squares);
-    }
+  if (C == 1) {
+    pixel_norms<T, S, TGV, 1>(F, H, W, i, j, gammas, coupled, first, second, most, residual);
+  } else {
+    pixel_norms<T, S, TGV, 3>(F, H, W, i, j, gammas, coupled, first, second, most, residual);
   }
+  if (!here) first = second = most = residual = 0.0;
   const double s_first = group_reduce<false>(first, room);
   const double s_second = group_reduce<false>(second, room);
   const double s_most = group_reduce<true>(most, room);
@@ -342,33 +368,83 @@ __global__ __launch_bounds__(256) void value_kernel(const Component* __restrict_
 }
 
 // div p~ (from_r = 0) or div(-theta div_2 r~) (from_r = 1) of channel c at (i, j), in
-// binary64: the divergence that the dual values take their xi from.
+// binary64: the divergence that the dual values take their xi from. Every load is made
+// first (at clamped positions), the cases chosen after.
 template <typename T, typename S>
 __device__ __forceinline__ double dual_divergence_at(const Fields<T, S>& F, int c, int i, int j, int H, int W,
                                                      int from_r, double theta) {
   const size_t N = static_cast<size_t>(H) * W;
-  if (!from_r) return device::divergence_at<double>(F.tp + (2 * c) * N, F.tp + (2 * c + 1) * N, i, j, H, W);
+  if (!from_r) return device::divergence_loaded<double>(F.tp + (2 * c) * N, F.tp + (2 * c + 1) * N, i, j, H, W);
   const S* rxx = F.tr + (3 * c) * N;
   const S* ryy = F.tr + (3 * c + 1) * N;
   const S* rxy = F.tr + (3 * c + 2) * N;
   // g = -theta div_2 r~, then div g = delta_y g_y + delta_x g_x.
-  double dx, dy, gy_up = 0.0, gx_left = 0.0;
-  device::tensor_divergence_at<double>(rxx, ryy, rxy, i, j, H, W, dx, dy);
+  double dx, dy, up_x, up_y, left_x, left_y;
+  device::tensor_divergence_loaded<double>(rxx, ryy, rxy, i, j, H, W, dx, dy);
+  device::tensor_divergence_loaded<double>(rxx, ryy, rxy, i - 1, j, H, W, up_x, up_y);
+  device::tensor_divergence_loaded<double>(rxx, ryy, rxy, i, j - 1, H, W, left_x, left_y);
   const double gx_here = -theta * dx, gy_here = -theta * dy;
-  if (i > 0) {
-    device::tensor_divergence_at<double>(rxx, ryy, rxy, i - 1, j, H, W, dx, dy);
-    gy_up = -theta * dy;
-  }
-  if (j > 0) {
-    device::tensor_divergence_at<double>(rxx, ryy, rxy, i, j - 1, H, W, dx, dy);
-    gx_left = -theta * dx;
-  }
+  const double gy_up = i > 0 ? -theta * up_y : 0.0;
+  const double gx_left = j > 0 ? -theta * left_x : 0.0;
   const bool top = i == 0, bottom = i == H - 1;
   const double down_part = (!top && !bottom) ? gy_here - gy_up : (!bottom ? gy_here : (!top ? -gy_up : 0.0));
   const double across_part = W == 1 ? 0.0 : (j == 0 ? gx_here : (j == W - 1 ? -gx_left : gx_here - gx_left));
   double div = down_part;
   div += across_part;
   return div;
+}
+
+// A lane's G* term of one block (and the free samples' terms of its cells): xi at the
+// lane's cells, their sums, the block DCT through LDS, the conjugate at the lane's
+// coefficient. DV and DH are the cell's shape, or 0 for the shape at run time (any up to
+// kMaxCell); with them fixed, every sample's loads are made before any is used.
+template <typename T, typename S, int DV, int DH>
+__device__ __forceinline__ double conjugate_terms(const Component& K, const Fields<T, S>& F, int c, int H, int W,
+                                                  double gamma, int from_r, double theta, double radius, bool active,
+                                                  bool inside, int ty, int tx, int lane, double* first,
+                                                  double* second) {
+  constexpr int kDown = DV > 0 ? DV : kMaxCell, kAcross = DH > 0 ? DH : kMaxCell;
+  const int m = lane >> 3, n = lane & 7;
+  const int dv = DV > 0 ? DV : K.down, dh = DH > 0 ? DH : K.across;
+  double xis[kDown * kAcross];
+  double sum = 0.0;
+#pragma unroll
+  for (int b = 0; b < kAcross; ++b) {
+    double column = 0.0;
+#pragma unroll
+    for (int a = 0; a < kDown; ++a) {
+      if ((DV > 0 || active) && a < dv && b < dh) {
+        const int i = (8 * ty + m) * dv + a, j = (8 * tx + n) * dh + b;
+        const double xi = gamma * dual_divergence_at(F, c, i, j, H, W, from_r, theta);
+        xis[kAcross * a + b] = xi;
+        column = a == 0 ? xi : column + xi;
+      }
+    }
+    if (b < dh) sum = b == 0 ? column : sum + column;
+  }
+  first[lane] = sum;
+  __syncthreads();
+  second[lane] = device::forward_down(first, m, n);
+  __syncthreads();
+  const double s = device::forward_along(second, m, n);
+  const size_t block = static_cast<size_t>(ty) * K.columns + tx;
+  double term = inside ? device::conjugate_of(K, K.levels[64 * block + lane], K.flat[block] != 0, lane, s) : 0.0;
+  if (K.free_samples && active) {
+    // <zeta, m> + R |zeta|_1 of the samples: within the blocks R |xi - mean of its cell|,
+    // beyond them 128 xi + R |xi| (docs/math.md:911-949).
+    const double mean = sum / static_cast<double>(dv * dh);
+#pragma unroll
+    for (int a = 0; a < kDown; ++a) {
+#pragma unroll
+      for (int bb = 0; bb < kAcross; ++bb) {
+        if (a < dv && bb < dh) {
+          const double xi = xis[kAcross * a + bb];
+          term += inside ? radius * fabs(xi - mean) : 128.0 * xi + radius * fabs(xi);
+        }
+      }
+    }
+  }
+  return term;
 }
 
 // G_c* at xi = gamma div p~ (from_r = 0) or xi = gamma div(-theta div_2 r~) (from_r = 1),
@@ -381,47 +457,30 @@ __device__ __forceinline__ void conjugate_tile(const Component& K, const Fields<
   __shared__ double first[4][64];
   __shared__ double second[4][64];
   __shared__ double room[256];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane >> 3, n = lane & 7;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int tile = group * 4 + wave;
   const bool active = tile < K.tiles_y * K.tiles_x;
   const int ty = active ? tile / K.tiles_x : 0, tx = active ? tile % K.tiles_x : 0;
   const bool inside = active && ty < K.rows && tx < K.columns;
-  const int dv = K.down, dh = K.across;
   double theta = 1.0;
   if (from_r) theta = *most > alpha1 ? alpha1 / *most : 1.0;
-  double xis[kMaxCell * kMaxCell];
-  double sum = 0.0;
-#pragma unroll
-  for (int b = 0; b < kMaxCell; ++b) {
-    double column = 0.0;
-#pragma unroll
-    for (int a = 0; a < kMaxCell; ++a) {
-      if (active && a < dv && b < dh) {
-        const int i = (8 * ty + m) * dv + a, j = (8 * tx + n) * dh + b;
-        const double xi = gamma * dual_divergence_at(F, c, i, j, H, W, from_r, theta);
-        xis[kMaxCell * a + b] = xi;
-        column = a == 0 ? xi : column + xi;
-      }
-    }
-    if (b < dh) sum = b == 0 ? column : sum + column;
-  }
-  first[wave][lane] = sum;
-  __syncthreads();
-  second[wave][lane] = device::forward_down(first[wave], m, n);
-  __syncthreads();
-  const double s = device::forward_along(second[wave], m, n);
-  const size_t block = static_cast<size_t>(ty) * K.columns + tx;
-  double term = inside ? device::conjugate_of(K, K.levels[64 * block + lane], K.flat[block] != 0, lane, s) : 0.0;
-  if (K.free_samples && active) {
-    // <zeta, m> + R |zeta|_1 of the samples: within the blocks R |xi - mean of its cell|,
-    // beyond them 128 xi + R |xi| (docs/math.md:911-949).
-    const double mean = sum / static_cast<double>(dv * dh);
-    for (int a = 0; a < dv; ++a) {
-      for (int bb = 0; bb < dh; ++bb) {
-        const double xi = xis[kMaxCell * a + bb];
-        term += inside ? radius * fabs(xi - mean) : 128.0 * xi + radius * fabs(xi);
-      }
-    }
+  // The cell's shape is the component's: the same in every wavefront of the work-group.
+  const auto terms = [&](auto shape) {
+    constexpr int kDV = decltype(shape)::first, kDH = decltype(shape)::second;
+    return conjugate_terms<T, S, kDV, kDH>(K, F, c, H, W, gamma, from_r, theta, radius, active, inside, ty, tx, lane,
+                                           first[wave], second[wave]);
+  };
+  double term;
+  if (K.down == 1 && K.across == 1) {
+    term = terms(device::Shape<1, 1>{});
+  } else if (K.down == 2 && K.across == 2) {
+    term = terms(device::Shape<2, 2>{});
+  } else if (K.down == 1 && K.across == 2) {
+    term = terms(device::Shape<1, 2>{});
+  } else if (K.down == 2 && K.across == 1) {
+    term = terms(device::Shape<2, 1>{});
+  } else {
+    term = terms(device::Shape<0, 0>{});
   }
   const double total = group_reduce<false>(term, room);
   if (threadIdx.x == 0) partials[group] = total;
