@@ -375,10 +375,12 @@ __device__ __forceinline__ T lane_value(T v, int source) {
   }
 }
 
-// The gather of a strip's edge samples for one row. Lanes 3c + e of the T gather hold w_x,
-// w_y on the left (e = 0, 1) and x- on the right (e = 2) of channel c; lanes 9c + e of the
-// S gather hold p_x, p_y, r_xx, r_yy, r_xy on the left (e = 0..4), r_yy and r_xy on the
-// left one row down (e = 5, 6), and r_xx, r_xy on the right (e = 7, 8).
+// The gather of a work-group's edge samples for one row: the left ones for its first
+// wavefront (`left`), the right ones for its last (`right`); the strips inside the
+// work-group hand theirs over through LDS (EdgeRoom). Lanes 3c + e of the T gather hold
+// w_x, w_y on the left (e = 0, 1) and x- on the right (e = 2) of channel c; lanes 9c + e
+// of the S gather hold p_x, p_y, r_xx, r_yy, r_xy on the left (e = 0..4), r_yy and r_xy on
+// the left one row down (e = 5, 6), and r_xx, r_xy on the right (e = 7, 8).
 template <typename T, typename S, int C, bool TGV>
 struct EdgeGather {
   const T* t_source = nullptr;
@@ -386,16 +388,16 @@ struct EdgeGather {
   int t_down = 0, s_down = 0;
   size_t t_column = 0, s_column = 0;
 
-  __device__ EdgeGather(const Fields<T, S>& F, int lane, int H, int W, int j0) {
+  __device__ EdgeGather(const Fields<T, S>& F, int lane, int H, int W, int j0, bool left_side, bool right_side) {
     const size_t N = static_cast<size_t>(H) * W;
     const size_t left = static_cast<size_t>(max(j0 - 1, 0));
     const size_t right = static_cast<size_t>(min(j0 + kStripColumns, W - 1));
     if (lane < 3 * C) {
       const int c = lane / 3, e = lane % 3;
-      if (e == 2) {
+      if (e == 2 && right_side) {
         t_source = F.xbar + c * N;
         t_column = right;
-      } else if (TGV) {
+      } else if (TGV && e < 2 && left_side) {
         t_source = F.w + (2 * c + e) * N;
         t_column = left;
       }
@@ -404,9 +406,11 @@ struct EdgeGather {
       const int c = lane / 9, e = lane % 9;
       // The entry of p (e < 2) or r for each e, two bits each: 0, 1, 0, 1, 2, 1, 2, 0, 2.
       const int entry = (0x22644 >> (2 * e)) & 3;
-      s_source = (e < 2 ? F.p + (2 * c) * N : F.r + (3 * c) * N) + entry * N;
-      s_down = (e == 5 || e == 6) ? 1 : 0;
-      s_column = e < 7 ? left : right;
+      if (e < 7 ? left_side : right_side) {
+        s_source = (e < 2 ? F.p + (2 * c) * N : F.r + (3 * c) * N) + entry * N;
+        s_down = (e == 5 || e == 6) ? 1 : 0;
+        s_column = e < 7 ? left : right;
+      }
     }
   }
 
@@ -422,28 +426,49 @@ struct EdgeGather {
   }
 };
 
+// What each wavefront of a work-group hands to its neighbours every row: lane 63 the
+// samples that the wavefront on its right takes as its left column's (w_x, w_y, p_x, p_y,
+// r_xx, r_yy, r_xy of the row, r_yy, r_xy of the row below), lane 0 those that the one on
+// its left takes as its right column's (x-, r_xx, r_xy). Two rows' worth, taken in turn,
+// so that one barrier a row parts a row's writes from the reads of the row before.
+template <typename T, int C>
+struct EdgeRoom {
+  T left[2][4][C][9];
+  T right[2][4][C][3];
+};
+
 //
 // dual_tile is work-group g's part: 4 strips across and `rows` rows down; dual_kernel runs
 // a frame (work-groups in a run on each XCD), and the batch's kernel runs tiles of many.
 template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes>
 __device__ __forceinline__ void dual_tile(Fields<T, S> F, int H, int W, int rows, int groups_across,
                                           Steps<T> steps, int keep, int g) {
+  __shared__ EdgeRoom<T, C> room;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int strip = (g % groups_across) * 4 + wave;
   const int j0 = strip * kStripColumns;
-  if (j0 >= W) return;  // the whole wavefront beyond the canvas
+  const int first = (g / groups_across) * rows;
+  const int end = min(first + rows, H);
+  if (j0 >= W) {
+    // The whole wavefront beyond the canvas: it keeps the work-group's barriers only.
+    if (TGV && first > 0) __syncthreads();
+    for (int i = first; i < end; ++i) __syncthreads();
+    return;
+  }
+  // The edge samples of the strips beside: through LDS from the wavefronts beside, or for
+  // the work-group's outer strips, from memory. A neighbour beyond the canvas is never
+  // read (its column is the canvas' edge, where the differences are zero).
+  const bool left_gathered = wave == 0, right_gathered = wave == 3;
   const int j = j0 + lane;
   const bool writes = j < W;
   const bool right = j + 1 < W;
   const uint32_t column = static_cast<uint32_t>(min(j, W - 1));
-  const int first = (g / groups_across) * rows;
-  const int end = min(first + rows, H);
   const size_t N = static_cast<size_t>(H) * W;
   const Planes<T> xbar(F.xbar, C, N);
   const Planes<S> p(F.p, 2 * C, N), p_next(F.p_next, 2 * C, N), tp(F.tp, 2 * C, N);
   const Planes<T> w(F.w, 2 * C, N), w_next(F.w_next, 2 * C, N), tw(F.tw, 2 * C, N);
   const Planes<S> r(F.r, 3 * C, N), r_next(F.r_next, 3 * C, N), tr(F.tr, 3 * C, N);
-  const EdgeGather<T, S, C, TGV> edges(F, lane, H, W, j0);
+  const EdgeGather<T, S, C, TGV> edges(F, lane, H, W, j0, left_gathered, right_gathered);
   // The index of the lane's sample in row i, i clamped into the canvas (a clamped load
   // only feeds differences that the canvas' edges take as zero).
   const auto index = [&](int i) { return static_cast<uint32_t>(min(i, H - 1)) * static_cast<uint32_t>(W) + column; };
@@ -480,19 +505,37 @@ __device__ __forceinline__ void dual_tile(Fields<T, S> F, int H, int W, int rows
 #pragma unroll
       for (int e = 0; e < 2; ++e) w_here[c][e] = w.load(2 * c + e, index(first));
       w_up[c][0] = w_up[c][1] = T(0);
-      if (first > 0) {
-        T r_up[3], p_up[2], old_up[2], tilde_up[2];
+    }
+  }
+  if constexpr (TGV) {
+    if (first > 0) {
+      // w- of the row above, whose right column's r_xx, r_xy come from the wavefront on
+      // the right (its lane 0) through the room's second half.
+      T r_up[C][3], p_up[C][2], old_up[C][2];
 #pragma unroll
-        for (int e = 0; e < 3; ++e) r_up[e] = device::widen(r.load(3 * c + e, index(first - 1)));
+      for (int c = 0; c < C; ++c) {
+#pragma unroll
+        for (int e = 0; e < 3; ++e) r_up[c][e] = device::widen(r.load(3 * c + e, index(first - 1)));
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-          p_up[e] = device::widen(p.load(2 * c + e, index(first - 1)));
-          old_up[e] = w.load(2 * c + e, index(first - 1));
+          p_up[c][e] = device::widen(p.load(2 * c + e, index(first - 1)));
+          old_up[c][e] = w.load(2 * c + e, index(first - 1));
         }
-        const T edge_xx = lane_value(up_s, 9 * c + 7), edge_xy = lane_value(up_s, 9 * c + 8);
-        const T shifted_xx = from_right(r_up[0]), shifted_xy = from_right(r_up[2]);
+        if (lane == 0) {
+          room.right[1][wave][c][1] = r_up[c][0];
+          room.right[1][wave][c][2] = r_up[c][2];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        T tilde_up[2];
+        const T edge_xx = right_gathered ? lane_value(up_s, 9 * c + 7) : room.right[1][wave + 1][c][1];
+        const T edge_xy = right_gathered ? lane_value(up_s, 9 * c + 8) : room.right[1][wave + 1][c][2];
+        const T shifted_xx = from_right(r_up[c][0]), shifted_xy = from_right(r_up[c][2]);
         const T right_xx = lane == 63 ? edge_xx : shifted_xx, right_xy = lane == 63 ? edge_xy : shifted_xy;
-        w_step(old_up, p_up, r_up, r_here[c], right_xx, right_xy, right, true, steps.descent[c], tilde_up, w_up[c]);
+        w_step(old_up[c], p_up[c], r_up[c], r_here[c], right_xx, right_xy, right, true, steps.descent[c], tilde_up,
+               w_up[c]);
       }
     }
   }
@@ -519,28 +562,56 @@ __device__ __forceinline__ void dual_tile(Fields<T, S> F, int H, int W, int rows
       }
     }
 
+    // This row's edge samples for the wavefronts beside, then theirs for this one.
+    const int slot = (i - first) & 1;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      if (lane == 0) {
+        room.right[slot][wave][c][0] = bar[c];
+        if constexpr (TGV) {
+          room.right[slot][wave][c][1] = r_here[c][0];
+          room.right[slot][wave][c][2] = r_here[c][2];
+        }
+      }
+      if (TGV && lane == 63) {
+        const T handed[9] = {w_here[c][0], w_here[c][1], p_here[c][0], p_here[c][1], r_here[c][0],
+                             r_here[c][1], r_here[c][2], r_below[c][1], r_below[c][2]};
+#pragma unroll
+        for (int e = 0; e < 9; ++e) room.left[slot][wave][c][e] = handed[e];
+      }
+    }
+    __syncthreads();
+    // The left column's sample e of channel c (the S gather's order, w_x and w_y at 7, 8)
+    // and the right column's (x-, r_xx, r_xy).
+    const auto left_of = [&](int c, int e) {
+      if (left_gathered) return e >= 7 ? lane_value(edge_t, 3 * c + (e - 7)) : lane_value(edge_s, 9 * c + e);
+      return room.left[slot][wave - 1][c][e < 7 ? e + 2 : e - 7];
+    };
+    const auto right_of = [&](int c, int e) {
+      if (right_gathered) return e == 0 ? lane_value(edge_t, 3 * c + 2) : lane_value(edge_s, 9 * c + 6 + e);
+      return room.right[slot][wave + 1][c][e];
+    };
     T step_p[C][2], step_r[C][3], tilde_w[C][2], bar_w[C][2];
     T square = T(0), tensor_square = T(0);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const T a = steps.ascent[c];
-      const T edge_bar = lane_value(edge_t, 3 * c + 2), shifted_bar = from_right(bar[c]);
+      const T edge_bar = right_of(c, 0), shifted_bar = from_right(bar[c]);
       const T bar_right = lane == 63 ? edge_bar : shifted_bar;
       const T g1 = right ? bar_right - bar[c] : T(0);
       const T g2 = below ? bar_next[c] - bar[c] : T(0);
       if constexpr (TGV) {
-        const T edge_xx = lane_value(edge_s, 9 * c + 7), edge_xy = lane_value(edge_s, 9 * c + 8);
+        const T edge_xx = right_of(c, 1), edge_xy = right_of(c, 2);
         const T shifted_xx = from_right(r_here[c][0]), shifted_xy = from_right(r_here[c][2]);
         const T right_xx = lane == 63 ? edge_xx : shifted_xx, right_xy = lane == 63 ? edge_xy : shifted_xy;
         w_step(w_here[c], p_here[c], r_here[c], r_below[c], right_xx, right_xy, right, below, steps.descent[c],
                tilde_w[c], bar_w[c]);
         // w- of the column on the left: the lane on the left's, or for lane 0, made from the
         // edge samples and its own r (every lane makes it; lane 0 keeps it).
-        const T old_left[2] = {lane_value(edge_t, 3 * c), lane_value(edge_t, 3 * c + 1)};
-        const T p_left[2] = {lane_value(edge_s, 9 * c), lane_value(edge_s, 9 * c + 1)};
-        const T r_left[3] = {lane_value(edge_s, 9 * c + 2), lane_value(edge_s, 9 * c + 3),
-                             lane_value(edge_s, 9 * c + 4)};
-        const T r_left_below[3] = {T(0), lane_value(edge_s, 9 * c + 5), lane_value(edge_s, 9 * c + 6)};
+        const T old_left[2] = {left_of(c, 7), left_of(c, 8)};
+        const T p_left[2] = {left_of(c, 0), left_of(c, 1)};
+        const T r_left[3] = {left_of(c, 2), left_of(c, 3), left_of(c, 4)};
+        const T r_left_below[3] = {T(0), left_of(c, 5), left_of(c, 6)};
         T tilde_left[2], bar_left[2];
         w_step(old_left, p_left, r_left, r_left_below, r_here[c][0], r_here[c][2], true, below, steps.descent[c],
                tilde_left, bar_left);
