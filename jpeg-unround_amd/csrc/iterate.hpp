@@ -383,12 +383,14 @@ __device__ __forceinline__ T lane_value(T v, int source) {
 // the left one row down (e = 5, 6), and r_xx, r_xy on the right (e = 7, 8).
 template <typename T, typename S, int C, bool TGV>
 struct EdgeGather {
-  const T* t_source = nullptr;
-  const S* s_source = nullptr;
+  const T* t_source;
+  const S* s_source;
   int t_down = 0, s_down = 0;
   size_t t_column = 0, s_column = 0;
+  bool gathers;  // the same in every lane of the wavefront
 
-  __device__ EdgeGather(const Fields<T, S>& F, int lane, int H, int W, int j0, bool left_side, bool right_side) {
+  __device__ EdgeGather(const Fields<T, S>& F, int lane, int H, int W, int j0, bool left_side, bool right_side)
+      : t_source(F.xbar), s_source(F.p), gathers(left_side || right_side) {
     const size_t N = static_cast<size_t>(H) * W;
     const size_t left = static_cast<size_t>(max(j0 - 1, 0));
     const size_t right = static_cast<size_t>(min(j0 + kStripColumns, W - 1));
@@ -414,15 +416,15 @@ struct EdgeGather {
     }
   }
 
-  // This lane's sample for row i (rows clamped into the canvas), or 0; 0 also where
-  // `wanted` is false.
-  __device__ __forceinline__ void load(int i, int H, int W, T& t, T& s, bool wanted = true) const {
-    t = T(0);
-    s = T(0);
-    if (wanted && t_source != nullptr) t = t_source[static_cast<size_t>(min(i + t_down, H - 1)) * W + t_column];
-    if (wanted && s_source != nullptr) {
-      s = device::widen(s_source[static_cast<size_t>(min(i + s_down, H - 1)) * W + s_column]);
-    }
+  // This lane's samples for row i (rows clamped into the canvas), loaded with no branch
+  // between the lanes: a lane without a sample of its own reads the first of x- (or p),
+  // which no one takes. S is kept as stored and widened where it is taken, a row later,
+  // so that nothing waits for these loads now. A wavefront that gathers nothing loads
+  // nothing.
+  __device__ __forceinline__ void load(int i, int H, int W, T& t, S& s) const {
+    if (!gathers) return;
+    t = t_source[static_cast<size_t>(min(i + t_down, H - 1)) * W + t_column];
+    if constexpr (TGV) s = s_source[static_cast<size_t>(min(i + s_down, H - 1)) * W + s_column];
   }
 };
 
@@ -487,7 +489,8 @@ __device__ __forceinline__ void dual_tile(Fields<T, S> F, int H, int W, int rows
   // The rows of the start: x- and r of the first two rows, p and w of the first, the edge
   // samples of the first row, and w- of the row above.
   T bar[C], bar_next[C], r_here[C][3], r_below[C][3], p_here[C][2], w_here[C][2], w_up[C][2];
-  T edge_t, edge_s, up_t, up_s;
+  T edge_t = T(0), up_t = T(0);
+  S edge_s{}, up_s{};
   edges.load(first, H, W, edge_t, edge_s);
   if (TGV && first > 0) edges.load(first - 1, H, W, up_t, up_s);
 #pragma unroll
@@ -530,8 +533,9 @@ __device__ __forceinline__ void dual_tile(Fields<T, S> F, int H, int W, int rows
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         T tilde_up[2];
-        const T edge_xx = right_gathered ? lane_value(up_s, 9 * c + 7) : room.right[1][wave + 1][c][1];
-        const T edge_xy = right_gathered ? lane_value(up_s, 9 * c + 8) : room.right[1][wave + 1][c][2];
+        const T up_wide = device::widen(up_s);
+        const T edge_xx = right_gathered ? lane_value(up_wide, 9 * c + 7) : room.right[1][wave + 1][c][1];
+        const T edge_xy = right_gathered ? lane_value(up_wide, 9 * c + 8) : room.right[1][wave + 1][c][2];
         const T shifted_xx = from_right(r_up[c][0]), shifted_xy = from_right(r_up[c][2]);
         const T right_xx = lane == 63 ? edge_xx : shifted_xx, right_xy = lane == 63 ? edge_xy : shifted_xy;
         w_step(old_up[c], p_up[c], r_up[c], r_here[c], right_xx, right_xy, right, true, steps.descent[c], tilde_up,
@@ -547,8 +551,9 @@ __device__ __forceinline__ void dual_tile(Fields<T, S> F, int H, int W, int rows
     // edge samples one row down; none in the run's last row, whose next rows belong to the
     // run below (an unwanted buffer load reads nothing from memory).
     const bool ahead = i + 1 < end;
-    T bar_ahead[C], r_ahead[C][3], p_ahead[C][2], w_ahead[C][2], edge_t_ahead, edge_s_ahead;
-    edges.load(i + 1, H, W, edge_t_ahead, edge_s_ahead, ahead);
+    T bar_ahead[C], r_ahead[C][3], p_ahead[C][2], w_ahead[C][2], edge_t_ahead = T(0);
+    S edge_s_ahead{};
+    if (ahead) edges.load(i + 1, H, W, edge_t_ahead, edge_s_ahead);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       bar_ahead[c] = xbar.load_if(ahead, c, index(i + 2));
@@ -583,12 +588,13 @@ __device__ __forceinline__ void dual_tile(Fields<T, S> F, int H, int W, int rows
     __syncthreads();
     // The left column's sample e of channel c (the S gather's order, w_x and w_y at 7, 8)
     // and the right column's (x-, r_xx, r_xy).
+    const T edge_wide = device::widen(edge_s);
     const auto left_of = [&](int c, int e) {
-      if (left_gathered) return e >= 7 ? lane_value(edge_t, 3 * c + (e - 7)) : lane_value(edge_s, 9 * c + e);
+      if (left_gathered) return e >= 7 ? lane_value(edge_t, 3 * c + (e - 7)) : lane_value(edge_wide, 9 * c + e);
       return room.left[slot][wave - 1][c][e < 7 ? e + 2 : e - 7];
     };
     const auto right_of = [&](int c, int e) {
-      if (right_gathered) return e == 0 ? lane_value(edge_t, 3 * c + 2) : lane_value(edge_s, 9 * c + 6 + e);
+      if (right_gathered) return e == 0 ? lane_value(edge_t, 3 * c + 2) : lane_value(edge_wide, 9 * c + 6 + e);
       return room.right[slot][wave + 1][c][e];
     };
     T step_p[C][2], step_r[C][3], tilde_w[C][2], bar_w[C][2];
