@@ -313,7 +313,7 @@ class Batch {
         r.conjugate_offset[c] = conjugate_at;
         if (c < v.C_) {
           value_at += static_cast<int>((64 * frames_[m]->components[c].blocks() + 255) / 256);
-          conjugate_at += (v.tiles(c) + 3) / 4;
+          conjugate_at += v.prox_groups(c);
         }
       }
       r.results = v.record_out_.template as<double>();
@@ -440,7 +440,7 @@ class Batch {
         auto& list = prox[{static_cast<int>(k.down), static_cast<int>(k.across), buffers_prox}];
         for (int g = 0; g < v.prox_groups(c); ++g) list.push_back({m, c, g});
         for (int g = 0; g < static_cast<int>((64 * k.blocks() + 255) / 256); ++g) values.push_back({m, c, g});
-        for (int g = 0; g < (v.tiles(c) + 3) / 4; ++g) {
+        for (int g = 0; g < v.prox_groups(c); ++g) {
           conjugates.push_back({m, c, g});
           if (tgv_ && plan.has_partial) partial_conjugates.push_back({m, c, g});
         }

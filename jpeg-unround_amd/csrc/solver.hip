@@ -394,81 +394,100 @@ __device__ __forceinline__ double dual_divergence_at(const Fields<T, S>& F, int 
   return div;
 }
 
-// A lane's G* term of one block (and the free samples' terms of its cells): xi at the
-// lane's cells, their sums, the block DCT through LDS, the conjugate at the lane's
-// coefficient. DV and DH are the cell's shape, or 0 for the shape at run time (any up to
-// kMaxCell); with them fixed, every sample's loads are made before any is used.
+// A lane's G* terms of one strip (and the free samples' terms of its cells), laid out as
+// the prox's: a wavefront takes one row of blocks x 8 blocks across, a lane per cell
+// column, so that the loads of each canvas row are coalesced stretches. The lane's xi down
+// its column, the cell sums, the DCT down the column in registers, an LDS transpose, the
+// DCT along the rows, the conjugate at each of the lane's 8 coefficients (row n of block
+// b). DV and DH are the cell's shape, or 0 for the shape at run time (any up to kMaxCell).
 template <typename T, typename S, int DV, int DH>
 __device__ __forceinline__ double conjugate_terms(const Component& K, const Fields<T, S>& F, int c, int H, int W,
-                                                  double gamma, int from_r, double theta, double radius, bool active,
-                                                  bool inside, int ty, int tx, int lane, double* first,
-                                                  double* second) {
+                                                  double gamma, int from_r, double theta, double radius, int g,
+                                                  double (*room)[65]) {
   constexpr int kDown = DV > 0 ? DV : kMaxCell, kAcross = DH > 0 ? DH : kMaxCell;
-  const int m = lane >> 3, n = lane & 7;
   const int dv = DV > 0 ? DV : K.down, dh = DH > 0 ? DH : K.across;
-  double xis[kDown * kAcross];
-  double sum = 0.0;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int strips = (K.tiles_x + 7) >> 3;
+  const int ty = (g / strips) * 4 + wave;
+  const int b = lane >> 3, n = lane & 7;
+  const int tx = (g % strips) * 8 + b;
+  const bool live = (ty < K.tiles_y) & (tx < K.tiles_x);
+  const bool inside = (ty < K.rows) & (tx < K.columns);
+  const int first_column = (8 * tx + n) * dh;
+  const int top = 8 * ty * dv;
+  // The free samples' terms (frames.cpp:616-681; docs/math.md:911-949), <zeta, m> +
+  // R |zeta|_1: within the blocks R |xi - mean of its cell|, beyond them 128 xi + R |xi|.
+  double column[8], free_terms = 0.0;
 #pragma unroll
-  for (int b = 0; b < kAcross; ++b) {
-    double column = 0.0;
+  for (int m = 0; m < 8; ++m) {
+    double xis[kDown * kAcross];
+    double sum = 0.0;
 #pragma unroll
-    for (int a = 0; a < kDown; ++a) {
-      if ((DV > 0 || active) && a < dv && b < dh) {
-        const int i = (8 * ty + m) * dv + a, j = (8 * tx + n) * dh + b;
-        const double xi = gamma * dual_divergence_at(F, c, i, j, H, W, from_r, theta);
-        xis[kAcross * a + b] = xi;
-        column = a == 0 ? xi : column + xi;
-      }
-    }
-    if (b < dh) sum = b == 0 ? column : sum + column;
-  }
-  first[lane] = sum;
-  __syncthreads();
-  second[lane] = device::forward_down(first, m, n);
-  __syncthreads();
-  const double s = device::forward_along(second, m, n);
-  const size_t block = static_cast<size_t>(ty) * K.columns + tx;
-  double term = inside ? device::conjugate_of(K, K.levels[64 * block + lane], K.flat[block] != 0, lane, s) : 0.0;
-  if (K.free_samples && active) {
-    // <zeta, m> + R |zeta|_1 of the samples: within the blocks R |xi - mean of its cell|,
-    // beyond them 128 xi + R |xi| (docs/math.md:911-949).
-    const double mean = sum / static_cast<double>(dv * dh);
+    for (int bb = 0; bb < kAcross; ++bb) {
+      double part = 0.0;
 #pragma unroll
-    for (int a = 0; a < kDown; ++a) {
-#pragma unroll
-      for (int bb = 0; bb < kAcross; ++bb) {
+      for (int a = 0; a < kDown; ++a) {
         if (a < dv && bb < dh) {
-          const double xi = xis[kAcross * a + bb];
-          term += inside ? radius * fabs(xi - mean) : 128.0 * xi + radius * fabs(xi);
+          const int i = top + m * dv + a, j = first_column + bb;
+          const double xi = gamma * dual_divergence_at(F, c, i, j, H, W, from_r, theta);
+          xis[kAcross * a + bb] = xi;
+          part = a == 0 ? xi : part + xi;
+        }
+      }
+      if (bb < dh) sum = bb == 0 ? part : sum + part;
+    }
+    column[m] = sum;
+    if (K.free_samples) {
+      const double mean = sum / static_cast<double>(dv * dh);
+#pragma unroll
+      for (int a = 0; a < kDown; ++a) {
+#pragma unroll
+        for (int bb = 0; bb < kAcross; ++bb) {
+          if (a < dv && bb < dh) {
+            const double xi = xis[kAcross * a + bb];
+            free_terms += inside ? radius * fabs(xi - mean) : 128.0 * xi + radius * fabs(xi);
+          }
         }
       }
     }
   }
-  return term;
+  double y[8];
+  device::forward8(column, y);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) room[k][lane] = y[k];
+  __syncthreads();
+  double row[8], coefficient[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) row[u] = room[n][8 * b + u];
+  device::forward8(row, coefficient);
+  const LevelRow own = level_row(K, ty, tx, n, inside);
+  double term = 0.0;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const double t = device::conjugate_of(K, own.q[u], own.flat, 8 * n + u, coefficient[u]);
+    term += inside ? t : 0.0;
+  }
+  term += free_terms;
+  return live ? term : 0.0;
 }
 
 // G_c* at xi = gamma div p~ (from_r = 0) or xi = gamma div(-theta div_2 r~) (from_r = 1),
-// with the free samples' terms (frames.cpp:616-681): a work-group's sum into partials.
+// with the free samples' terms (frames.cpp:616-681): work-group `group`'s sum into
+// partials, its tiles those of the prox's work-group `group`.
 template <typename T, typename S>
 __device__ __forceinline__ void conjugate_tile(const Component& K, const Fields<T, S>& F, int c, int H, int W,
                                                double gamma, int from_r, const double* __restrict__ most,
                                                double alpha1, double radius, double* __restrict__ partials,
                                                int group) {
-  __shared__ double first[4][64];
-  __shared__ double second[4][64];
-  __shared__ double room[256];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int tile = group * 4 + wave;
-  const bool active = tile < K.tiles_y * K.tiles_x;
-  const int ty = active ? tile / K.tiles_x : 0, tx = active ? tile % K.tiles_x : 0;
-  const bool inside = active && ty < K.rows && tx < K.columns;
+  __shared__ double room[4][8][65];
+  __shared__ double sums[256];
+  const int wave = threadIdx.x >> 6;
   double theta = 1.0;
   if (from_r) theta = *most > alpha1 ? alpha1 / *most : 1.0;
   // The cell's shape is the component's: the same in every wavefront of the work-group.
   const auto terms = [&](auto shape) {
     constexpr int kDV = decltype(shape)::first, kDH = decltype(shape)::second;
-    return conjugate_terms<T, S, kDV, kDH>(K, F, c, H, W, gamma, from_r, theta, radius, active, inside, ty, tx, lane,
-                                           first[wave], second[wave]);
+    return conjugate_terms<T, S, kDV, kDH>(K, F, c, H, W, gamma, from_r, theta, radius, group, room[wave]);
   };
   double term;
   if (K.down == 1 && K.across == 1) {
@@ -482,7 +501,7 @@ __device__ __forceinline__ void conjugate_tile(const Component& K, const Fields<
   } else {
     term = terms(device::Shape<0, 0>{});
   }
-  const double total = group_reduce<false>(term, room);
+  const double total = group_reduce<false>(term, sums);
   if (threadIdx.x == 0) partials[group] = total;
 }
 
@@ -770,7 +789,7 @@ class Solver {
     for (int c = 0; c < C_; ++c) {
       zeta += 64 * frame_.components[c].blocks();
       value_groups += (64 * frame_.components[c].blocks() + 255) / 256;
-      conjugate_groups += static_cast<size_t>((tiles(c) + 3) / 4);
+      conjugate_groups += static_cast<size_t>(prox_groups(c));
     }
   }
 
@@ -1128,7 +1147,7 @@ class Solver {
     const auto conjugates = [&](int from_r, double* into, double* result) {
       size_t at = 0;
       for (int c = 0; c < C_; ++c) {
-        const int groups = (tiles(c) + 3) / 4;
+        const int groups = prox_groups(c);
         conjugate_kernel<T, S><<<groups, 256, 0, s>>>(component(c), fields_, c, H_, W_, plan_.gamma[c], from_r,
                                                       results + 2, settings_.alpha1, plan_.free_radius, into + at);
         HIP_CHECK(hipGetLastError());
