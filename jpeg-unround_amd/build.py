@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("UNROUND_CXX", "/opt/rocm/lib/llvm/bin/clang++")
 ARCH = os.environ.get("UNROUND_ARCH", "gfx950")
 
-COMMON = ["-O3", "-fPIC", "-std=c++20", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", f"-I{SOURCES}"]
+COMMON = ["-O3", "-fPIC", "-pthread", "-std=c++20", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", f"-I{SOURCES}"]
 # Device code: gfx950 only; packed FP32 instructions off (they are made by the compiler
 # from plain float code, and are kept out on purpose).
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
@@ -43,7 +43,7 @@ def build() -> Path:
     with ThreadPoolExecutor(max_workers=min(8, len(sources))) as pool:
         objects = list(pool.map(compile_one, sources))
     temporary = TARGET.with_suffix(".so.tmp")
-    subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", str(temporary), *map(str, objects), "-lz"],
+    subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", str(temporary), *map(str, objects), "-lz", "-pthread"],
                    check=True)
     temporary.replace(TARGET)
     return TARGET

@@ -5,6 +5,7 @@
 #include "unround.h"
 #include "unround_ext.h"
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdlib>
@@ -13,6 +14,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "files.hpp"
@@ -519,6 +521,25 @@ unround_status project_picture(const uint8_t* data, size_t size, const unround_s
   return UNROUND_OK;
 }
 
+// body(k) for every k below count, on up to 8 threads, each a contiguous run of k (this
+// thread takes the runs of any thread that cannot be had). body must not throw.
+template <typename Body>
+void for_each_index(size_t count, const Body& body) {
+  const size_t workers = std::min<size_t>({8, count, std::max(1u, std::thread::hardware_concurrency())});
+  const auto run = [&](size_t t) {
+    for (size_t k = count * t / workers; k < count * (t + 1) / workers; ++k) body(k);
+  };
+  std::vector<std::thread> threads;
+  size_t started = 1;
+  try {
+    for (; started < workers; ++started) threads.emplace_back(run, started);
+  } catch (...) {
+  }
+  if (workers > 0) run(0);
+  for (size_t t = started; t < workers; ++t) run(t);
+  for (std::thread& thread : threads) thread.join();
+}
+
 unround_status reconstruct_batch(const unround_input* inputs, size_t count, const unround_settings* settings,
                                  unround_result** results, double* latencies, std::string& message) {
   if (results == nullptr || (count > 0 && inputs == nullptr) || settings == nullptr) {
@@ -527,12 +548,25 @@ unround_status reconstruct_batch(const unround_input* inputs, size_t count, cons
   }
   std::vector<std::unique_ptr<unround_result>> made(count);
   std::vector<unround::Plan> plans(count);
+  for (size_t k = 0; k < count; ++k) made[k] = std::make_unique<unround_result>();
+  // The frames (host work, each input alone) on a few threads; the first refusal reported.
+  std::vector<unround_status> checked(count, UNROUND_OK);
+  std::vector<std::string> messages(count);
+  for_each_index(count, [&](size_t k) {
+    try {
+      checked[k] = frame_of(&inputs[k], settings, *made[k], plans[k], messages[k]);
+    } catch (const std::bad_alloc&) {
+      checked[k] = UNROUND_ERROR_INTERNAL;
+      messages[k] = "out of host memory";
+    } catch (...) {
+      checked[k] = UNROUND_ERROR_INTERNAL;
+      messages[k] = "the frame could not be made";
+    }
+  });
   for (size_t k = 0; k < count; ++k) {
-    made[k] = std::make_unique<unround_result>();
-    const unround_status checked = frame_of(&inputs[k], settings, *made[k], plans[k], message);
-    if (checked != UNROUND_OK) {
-      message = "input " + std::to_string(k) + ": " + message;
-      return checked;
+    if (checked[k] != UNROUND_OK) {
+      message = "input " + std::to_string(k) + ": " + messages[k];
+      return checked[k];
     }
   }
   const unround::Settings& s = settings->settings;
@@ -555,8 +589,21 @@ unround_status reconstruct_batch(const unround_input* inputs, size_t count, cons
   }
   for (size_t k = 0; k < count; ++k) {
     made[k]->out = std::move(outs[k]);
-    assemble(*made[k]);
     if (latencies != nullptr) latencies[k] = seconds[k];
+  }
+  std::vector<char> assembled(count, 1);
+  for_each_index(count, [&](size_t k) {
+    try {
+      assemble(*made[k]);
+    } catch (...) {
+      assembled[k] = 0;
+    }
+  });
+  for (size_t k = 0; k < count; ++k) {
+    if (!assembled[k]) {
+      message = "input " + std::to_string(k) + ": out of host memory for its result";
+      return UNROUND_ERROR_INTERNAL;
+    }
   }
   for (size_t k = 0; k < count; ++k) results[k] = made[k].release();
   return UNROUND_OK;
@@ -817,13 +864,30 @@ const double* unround_result_problem(const unround_result* result, uint64_t comp
       auto& arrays = result->problems[component];
       const size_t entries = 64 * k.blocks();
       for (int a = 0; a < 3; ++a) arrays[a].resize(entries);
-      for (size_t b = 0; b < k.blocks(); ++b) {
-        for (int q = 0; q < 64; ++q) {
-          double own_lower, own_upper;
-          k.coefficient(b, q, arrays[0][64 * b + q], arrays[1][64 * b + q], arrays[2][64 * b + q], own_lower,
-                        own_upper);
+      // Each block's intervals depend on that block alone: runs of blocks on a few threads.
+      const auto fill = [&](size_t from, size_t to) {
+        for (size_t b = from; b < to; ++b) {
+          for (int q = 0; q < 64; ++q) {
+            double own_lower, own_upper;
+            k.coefficient(b, q, arrays[0][64 * b + q], arrays[1][64 * b + q], arrays[2][64 * b + q], own_lower,
+                          own_upper);
+          }
         }
+      };
+      const size_t blocks = k.blocks();
+      const size_t workers = blocks < 4096 ? 1 : std::min<size_t>(8, std::max(1u, std::thread::hardware_concurrency()));
+      std::vector<std::thread> threads;
+      size_t started = 1;
+      try {
+        for (; started < workers; ++started) {
+          threads.emplace_back(fill, blocks * started / workers, blocks * (started + 1) / workers);
+        }
+      } catch (...) {
+        // No more threads to be had: this one takes the runs not started.
       }
+      fill(0, blocks / workers);
+      for (size_t t = started; t < workers; ++t) fill(blocks * t / workers, blocks * (t + 1) / workers);
+      for (std::thread& thread : threads) thread.join();
       arrays[3].assign(k.steps.begin(), k.steps.end());
       arrays[4].assign(k.weights.begin(), k.weights.end());
       result->made[component] = true;
