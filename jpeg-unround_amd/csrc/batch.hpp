@@ -19,13 +19,22 @@ struct BatchTile {
   int group;  // the work-group within the frame's own launch
 };
 
+// A work-group of the batch's dual step: strip group `group` of picture `image` from row
+// `first`, `rows` rows; image < 0 for a place that holds no work.
+struct DualTile {
+  int image;
+  int first;
+  int rows;
+  int group;
+};
+
 // What a step of one frame reads, for one parity of the ping-pong buffers.
 template <typename T, typename S>
 struct ImageSlot {
   Fields<T, S> F;
   const Component* components;
   int H, W;
-  int rows, across;  // the dual step's rows per wavefront and work-groups across
+  int across;  // the dual step's strip groups across
   Steps<T> steps;
   T descent[3], step[3];  // the prox step's descent and coefficient step per component
 };
@@ -74,14 +83,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kProxWaves<
                                   m.step[t.part], rho, rest, keep, t.group);
 }
 
+// The table is laid out in the order the hardware hands out work-groups (each XCD's share
+// its own run of places, long runs of rows first), so the tiles are taken as they stand.
 template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes>
-__global__ __launch_bounds__(256) void dual_batch_kernel(const BatchTile* __restrict__ tiles,
+__global__ __launch_bounds__(256) void dual_batch_kernel(const DualTile* __restrict__ tiles,
                                                          const ImageSlot<T, S>* __restrict__ slots,
                                                          const int* __restrict__ active, int keep) {
-  const BatchTile t = tiles[device::ordered(blockIdx.x, gridDim.x)];
-  if (active[t.image] == 0) return;
+  const DualTile t = tiles[blockIdx.x];
+  if (t.image < 0 || active[t.image] == 0) return;
   const ImageSlot<T, S>& m = slots[t.image];
-  dual_tile<T, S, C, TGV, COUPLED, Planes>(m.F, m.H, m.W, m.rows, m.across, m.steps, keep, t.group);
+  dual_run<T, S, C, TGV, COUPLED, Planes>(m.F, m.H, m.W, t.first, min(t.first + t.rows, m.H), t.group, m.steps,
+                                          keep);
 }
 
 template <typename T, typename S, bool TGV>
@@ -396,7 +408,8 @@ class Batch {
   void build_tables(int rows) {
     const int M = static_cast<int>(solvers_.size());
     std::vector<ImageSlot<T, S>> even(M), odd(M);
-    std::map<Key, std::vector<BatchTile>> prox, dual;
+    std::map<Key, std::vector<BatchTile>> prox;
+    std::map<Key, std::vector<std::array<int, 3>>> dual;  // per kind: (picture, rows, strip groups)
     std::vector<BatchTile> norms, values, conjugates, partial_conjugates;
     std::vector<ReduceJob> first_jobs, last_jobs;
     const T rho = static_cast<T>(plans_[0]->relaxation);
@@ -411,7 +424,6 @@ class Batch {
       e.components = v.component(0);
       e.H = v.H_;
       e.W = v.W_;
-      e.rows = rows;
       const int strips = (v.W_ + kStripColumns - 1) / kStripColumns;
       e.across = (strips + 3) / 4;
       e.steps = Steps<T>{};
@@ -445,9 +457,7 @@ class Batch {
           if (tgv_ && plan.has_partial) partial_conjugates.push_back({m, c, g});
         }
       }
-      const int down = (v.H_ + rows - 1) / rows;
-      auto& list = dual[{v.C_, v.coupled_ ? 1 : 0, v.narrow_spans() ? 1 : 0}];
-      for (int g = 0; g < e.across * down; ++g) list.push_back({m, 0, g});
+      dual[{v.C_, v.coupled_ ? 1 : 0, v.narrow_spans() ? 1 : 0}].push_back({m, v.H_, e.across});
       for (int g = 0; g < static_cast<int>(v.norm_partials_count_); ++g) norms.push_back({m, 0, g});
       // The reductions, as a frame's record makes them.
       double* partials = v.partials_.template as<double>();
@@ -480,7 +490,15 @@ class Batch {
       return made;
     };
     prox_launches_ = launches(prox);
-    dual_launches_ = launches(dual);
+    dual_launches_.clear();
+    for (auto& [key, pictures] : dual) {
+      const std::vector<DualTile> list = dual_table(pictures, rows, tail_rows(key));
+      Launch launch;
+      launch.key = key;
+      launch.count = static_cast<int>(list.size());
+      launch.tiles = upload(list);
+      dual_launches_.push_back(std::move(launch));
+    }
     const auto single = [this](std::vector<BatchTile>& list) {
       Launch launch;
       launch.count = static_cast<int>(list.size());
@@ -495,6 +513,82 @@ class Batch {
     last_jobs_count_ = static_cast<int>(last_jobs.size());
     first_jobs_ = upload(first_jobs);
     last_jobs_ = upload(last_jobs);
+  }
+
+  // The dual step's table for the pictures of one kind (picture, rows, strip groups): runs of
+  // `rows` rows, the pictures in turn, dealt to the 8 XCDs in shares of about equal work;
+  // the last `tail` rows of strip groups of each share cut into runs of 8, so that the
+  // share's work-groups end close together. Laid out in the order the hardware hands out
+  // work-groups (group g to XCD g mod 8, each XCD's in turn); places past a share's end
+  // hold no work.
+  static std::vector<DualTile> dual_table(const std::vector<std::array<int, 3>>& pictures, int rows, long tail) {
+    struct Run {
+      int image, first, rows, across;
+      long work() const { return static_cast<long>(rows) * across; }
+    };
+    std::vector<Run> runs;
+    long total = 0;
+    for (const auto& [image, H, across] : pictures) {
+      for (int first = 0; first < H; first += rows) {
+        runs.push_back({image, first, std::min(rows, H - first), across});
+        total += runs.back().work();
+      }
+    }
+    constexpr int kXcds = 8;
+    std::vector<std::vector<DualTile>> shares(kXcds);
+    size_t next = 0;
+    long dealt = 0;
+    for (int x = 0; x < kXcds; ++x) {
+      const long goal = total * (x + 1) / kXcds;
+      std::vector<Run> mine;
+      while (next < runs.size() && (dealt < goal || x == kXcds - 1)) {
+        dealt += runs[next].work();
+        mine.push_back(runs[next++]);
+      }
+      size_t cut = mine.size();
+      for (long kept = 0; cut > 0 && kept < tail && rows > 8;) kept += mine[--cut].work();
+      std::vector<DualTile>& share = shares[x];
+      const auto emit = [&share](int image, int first, int count, int across) {
+        for (int g = 0; g < across; ++g) share.push_back({image, first, count, g});
+      };
+      for (size_t k = 0; k < cut; ++k) emit(mine[k].image, mine[k].first, mine[k].rows, mine[k].across);
+      for (size_t k = cut; k < mine.size(); ++k) {
+        const Run& r = mine[k];
+        for (int first = r.first; first < r.first + r.rows; first += 8) {
+          emit(r.image, first, std::min(8, r.first + r.rows - first), r.across);
+        }
+      }
+    }
+    size_t longest = 0;
+    for (const auto& share : shares) longest = std::max(longest, share.size());
+    std::vector<DualTile> table(kXcds * longest, DualTile{-1, 0, 0, 0});
+    for (int x = 0; x < kXcds; ++x) {
+      for (size_t k = 0; k < shares[x].size(); ++k) table[k * kXcds + x] = shares[x][k];
+    }
+    return table;
+  }
+
+  // The rows of strip groups to cut into short runs at the end of an XCD's share: two
+  // rounds of the work-groups its CUs hold at once, 8 rows each.
+  long tail_rows(const Key& key) const {
+    int device = 0, units = 0, resident = 0;
+    HIP_CHECK(hipGetDevice(&device));
+    HIP_CHECK(hipDeviceGetAttribute(&units, hipDeviceAttributeMultiprocessorCount, device));
+    const int C = key[0], coupled = key[1], narrow = key[2];
+    const auto held = [&resident](auto kernel) {
+      HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kernel, 256, 0));
+    };
+    if (C == 1) {
+      narrow ? held(dual_batch_kernel<T, S, 1, true, false, device::BufferPlanes>)
+             : held(dual_batch_kernel<T, S, 1, true, false, device::WidePlanes>);
+    } else if (coupled) {
+      narrow ? held(dual_batch_kernel<T, S, 3, true, true, device::BufferPlanes>)
+             : held(dual_batch_kernel<T, S, 3, true, true, device::WidePlanes>);
+    } else {
+      narrow ? held(dual_batch_kernel<T, S, 3, true, false, device::BufferPlanes>)
+             : held(dual_batch_kernel<T, S, 3, true, false, device::WidePlanes>);
+    }
+    return 2L * std::max(resident, 1) * std::max(units / 8, 1) * 8;
   }
 
   void launch_proxes(int parity, int keep) {
@@ -525,7 +619,7 @@ class Batch {
 
   template <int C, bool TGV, bool COUPLED>
   void launch_dual(const Launch& launch, const ImageSlot<T, S>* slots, int keep) {
-    const auto* tiles = launch.tiles.template as<BatchTile>();
+    const auto* tiles = launch.tiles.template as<DualTile>();
     if (launch.key[2]) {
       dual_batch_kernel<T, S, C, TGV, COUPLED, device::BufferPlanes>
           <<<launch.count, 256, 0, stream_.stream>>>(tiles, slots, state_.active, keep);

@@ -442,15 +442,15 @@ struct EdgeRoom {
 //
 // dual_tile is work-group g's part: 4 strips across and `rows` rows down; dual_kernel runs
 // a frame (work-groups in a run on each XCD), and the batch's kernel runs tiles of many.
+//
+// dual_run is the work of 4 strips across (strip group `group`) from row `first` to `end`.
 template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes>
-__device__ __forceinline__ void dual_tile(Fields<T, S> F, int H, int W, int rows, int groups_across,
-                                          Steps<T> steps, int keep, int g) {
+__device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first, int end, int group, Steps<T> steps,
+                                         int keep) {
   __shared__ EdgeRoom<T, C> room;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int strip = (g % groups_across) * 4 + wave;
+  const int strip = group * 4 + wave;
   const int j0 = strip * kStripColumns;
-  const int first = (g / groups_across) * rows;
-  const int end = min(first + rows, H);
   if (j0 >= W) {
     // The whole wavefront beyond the canvas: it keeps the work-group's barriers only.
     if (TGV && first > 0) __syncthreads();
@@ -703,6 +703,13 @@ __device__ __forceinline__ void dual_tile(Fields<T, S> F, int H, int W, int rows
     edge_t = edge_t_ahead;
     edge_s = edge_s_ahead;
   }
+}
+
+template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes>
+__device__ __forceinline__ void dual_tile(Fields<T, S> F, int H, int W, int rows, int groups_across,
+                                          Steps<T> steps, int keep, int g) {
+  const int first = (g / groups_across) * rows;
+  dual_run<T, S, C, TGV, COUPLED, Planes>(F, H, W, first, min(first + rows, H), g % groups_across, steps, keep);
 }
 
 template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes>
