@@ -77,3 +77,12 @@ def test_records_every_seven_alternate_parity():
 def test_no_records_until_the_end():
     same_as_alone(mixed_images(SPECS[2:4]), ["--method", "tgv", "--iterations", "35", "--record-every", "0",
                                              "--tolerance", "0"])
+
+
+def test_long_runs_with_short_tails(monkeypatch):
+    # Runs of 32 rows: each XCD's share of the dual step's table holds long runs first and
+    # its last stretch cut into runs of 8 (the shares here are a few times that stretch),
+    # with pictures whose widths leave strip groups part empty.
+    monkeypatch.setenv("UNROUND_DUAL_ROWS", "32")
+    images = mixed_images([(2048, 2040, True, 75), (1024, 1544, False, 60), (200, 136, True, 40)])
+    same_as_alone(images, ["--method", "tgv", "--iterations", "20", "--tolerance", "0"])
