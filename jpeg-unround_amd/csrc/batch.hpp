@@ -79,8 +79,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kProxWaves<
   const ImageSlot<T, S>& m = slots[t.image];
   // zeta's pointer read from the slot at its index: an index into a copy of the fields
   // would place the copy in private memory, written by every lane.
-  prox_tile<T, S, DV, DH, Planes>(m.components + t.part, m.F, t.part, m.F.zeta[t.part], m.H, m.W, m.descent[t.part],
-                                  m.step[t.part], rho, rest, keep, t.group);
+  prox_tile<T, S, DV, DH, Planes>(m.components + t.part, m.F, t.part, m.F.zeta[t.part], m.F.x, m.H, m.W,
+                                  m.descent[t.part], m.step[t.part], rho, rest, keep, t.group);
 }
 
 // The table is laid out in the order the hardware hands out work-groups (each XCD's share

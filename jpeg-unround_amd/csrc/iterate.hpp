@@ -73,8 +73,8 @@ __device__ __forceinline__ LevelRow level_row(const Component& K, int ty, int bx
 // (work-groups in a run on each XCD), and the batch's kernel runs tiles of many images.
 template <typename T, typename S, int DV, int DH, template <typename> class Planes>
 __device__ __forceinline__ void prox_tile(const Component* __restrict__ component, Fields<T, S> F, int c,
-                                          T* __restrict__ zeta_kept, int H, int W, T descent, T step, T rho, T rest,
-                                          int keep, int g) {
+                                          T* __restrict__ zeta_kept, T* x_out, int H, int W, T descent, T step, T rho,
+                                          T rest, int keep, int g) {
   __shared__ T forward_room[4][8][65];
   __shared__ T inverse_room[4][8][65];
   __shared__ device::StagedArrays staged_arrays;
@@ -88,6 +88,7 @@ __device__ __forceinline__ void prox_tile(const Component* __restrict__ componen
   const bool live = (ty < K.tiles_y) & (tx < K.tiles_x);
   const size_t N = static_cast<size_t>(H) * W;
   const Planes<T> x(F.x + c * N, 1, N), xbar(F.xbar + c * N, 1, N), tilde_x(F.tx + c * N, 1, N);
+  const Planes<T> x_next(x_out + c * N, 1, N);  // the relaxed x: in place, or elsewhere
   const Planes<S> p(F.p + (2 * c) * N, 2, N);  // p_x, p_y
   const int first_column = (8 * tx + n) * DH;  // the lane's first canvas column
   const int top = 8 * ty * DV;                 // the wave's first canvas row
@@ -208,13 +209,13 @@ __device__ __forceinline__ void prox_tile(const Component* __restrict__ componen
       const uint32_t where = at(top + m * DV + a, first_column);
       if constexpr (DH == 2) {
         xbar.store_pair_if(live, 0, where, bars[0], bars[1]);
-        x.store_pair_if(live, 0, where, relaxed[0], relaxed[1]);
+        x_next.store_pair_if(live, 0, where, relaxed[0], relaxed[1]);
         if (keep) tilde_x.store_pair_if(live, 0, where, tildes[0], tildes[1]);
       } else {
 #pragma unroll
         for (int bb = 0; bb < DH; ++bb) {
           xbar.store_if(live, 0, where + bb, bars[bb]);
-          x.store_if(live, 0, where + bb, relaxed[bb]);
+          x_next.store_if(live, 0, where + bb, relaxed[bb]);
           if (keep) tilde_x.store_if(live, 0, where + bb, tildes[bb]);
         }
       }
@@ -243,8 +244,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kProxWaves<
     const Component* __restrict__ component, Fields<T, S> F, int c, int H, int W, ProxSteps<T> steps, T rho, T rest,
     int keep) {
   const int k = static_cast<int>(blockIdx.y);
-  prox_tile<T, S, DV, DH, Planes>(component + k, F, c + k, F.zeta[c + k], H, W, steps.descent[k], steps.step[k], rho,
-                                  rest, keep, device::ordered(blockIdx.x, gridDim.x));
+  prox_tile<T, S, DV, DH, Planes>(component + k, F, c + k, F.zeta[c + k], F.x, H, W, steps.descent[k], steps.step[k],
+                                  rho, rest, keep, device::ordered(blockIdx.x, gridDim.x));
 }
 
 // The same for any cell up to 4 x 4, its samples read again for the write-back.

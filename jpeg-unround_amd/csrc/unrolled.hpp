@@ -56,6 +56,32 @@ struct UnrolledSteps {
   float rho;
 };
 
+// Per frame: the iteration's scalars and the prox's coefficient step (tau: one sample a cell).
+struct UnrolledFrame {
+  UnrolledSteps k;
+  float step;
+};
+
+// Where every frame's planes lie for one launch (the launch's frame is blockIdx.z, or
+// blockIdx.y for the block kernels): frame f's at each base plus f times its stride, a
+// state being 8 planes of N samples [x, w_x, w_y, p_x, p_y, r_xx, r_yy, r_xy].
+struct UnrolledFrames {
+  float* state;        // [B][8][N]: the iteration's input state
+  float* next;         // [B][8][N]: the forward's output state
+  float* bars;         // [B][N]: x-
+  float* after;        // [B][8][N]: the gradients after the iteration
+  float* before;       // [B][8][N]: and before it
+  float* work;         // [B][9][N]: the planes between the adjoint's kernels
+  float* sensitivity;  // [2][B][N]: the radius's, then alpha0's
+  const UnrolledFrame* frames;
+  const Component* components;
+  int count;
+};
+
+__device__ __forceinline__ float* frame_state(float* base, int f, size_t N) {
+  return base + static_cast<size_t>(f) * 8 * N;
+}
+
 // w- at (i, j) from w, p and r (r at (i, j + 1) and (i + 1, j) too); w~ into `tilde`.
 __device__ __forceinline__ void w_bar_at(const float* w, const float* p, const float* r, size_t N, int i, int j, int H,
                                          int W, float descent, float tilde[2], float bar[2]) {
@@ -90,14 +116,21 @@ __device__ __forceinline__ void ascents_at(const UnrolledState& s, const float* 
   r_hat[2] = s.r[2 * N + at] + k.ascent * xy;
 }
 
-// The forward dual step of one frame with a radius per pixel: w, p and r into `next`.
-__global__ __launch_bounds__(256) void unrolled_dual_kernel(UnrolledState s, const float* __restrict__ xbar,
-                                                            UnrolledState next, const float* __restrict__ radius,
-                                                            int H, int W, UnrolledSteps k) {
+// The forward dual step of every frame with a radius per pixel: w, p and r into the next
+// state.
+__global__ __launch_bounds__(256) void unrolled_dual_kernel(UnrolledFrames u, const float* __restrict__ radius,
+                                                            int H, int W) {
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
   const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int f = static_cast<int>(blockIdx.z);
   if (i >= H || j >= W) return;
   const size_t N = static_cast<size_t>(H) * W, at = static_cast<size_t>(i) * W + j;
+  float* const now = frame_state(u.state, f, N);
+  float* const later = frame_state(u.next, f, N);
+  const UnrolledState s{now, now + N, now + 3 * N, now + 5 * N};
+  const UnrolledState next{later, later + N, later + 3 * N, later + 5 * N};
+  const float* xbar = u.bars + f * N;
+  const UnrolledSteps k = u.frames[f].k;
   float p_hat[2], r_hat[3], w_tilde[2];
   ascents_at(s, xbar, N, i, j, H, W, k, p_hat, r_hat, w_tilde);
   const float scale = device::ball_scale(p_hat[0] * p_hat[0] + p_hat[1] * p_hat[1], radius[at]);
@@ -129,16 +162,44 @@ struct UnrolledWork {
   float* alpha0;   // += alpha0's
 };
 
+// Frame f's views for the adjoint's kernels.
+struct UnrolledBack {
+  UnrolledState s;
+  const float* xbar;
+  UnrolledGrads after, before;
+  UnrolledWork work;
+  UnrolledSteps k;
+};
+__device__ __forceinline__ UnrolledBack back_of(const UnrolledFrames& u, int f, size_t N) {
+  float* const kept = frame_state(u.state, f, N);
+  float* const a = frame_state(u.after, f, N);
+  float* const b = frame_state(u.before, f, N);
+  float* const w = u.work + static_cast<size_t>(f) * 9 * N;
+  UnrolledBack v;
+  v.s = UnrolledState{kept, kept + N, kept + 3 * N, kept + 5 * N};
+  v.xbar = u.bars + f * N;
+  v.after = UnrolledGrads{a, a + N, a + 3 * N, a + 5 * N};
+  v.before = UnrolledGrads{b, b + N, b + 3 * N, b + 5 * N};
+  v.work = UnrolledWork{w,         w + 2 * N, w + 5 * N, w + 6 * N, w + 8 * N, u.sensitivity + f * N,
+                        u.sensitivity + (static_cast<size_t>(u.count) + f) * N};
+  v.k = u.frames[f].k;
+  return v;
+}
+
 // 1. The balls' Jacobians: grad p^ and grad r^ from the gradients after the iteration
 // (`after`), and the radius and alpha0 sensitivities added up.
-__global__ __launch_bounds__(256) void unrolled_balls_back_kernel(UnrolledState s, const float* __restrict__ xbar,
-                                                                  UnrolledGrads after, UnrolledWork work,
-                                                                  const float* __restrict__ radius, int H, int W,
-                                                                  UnrolledSteps k) {
+__global__ __launch_bounds__(256) void unrolled_balls_back_kernel(UnrolledFrames u, const float* __restrict__ radius,
+                                                                  int H, int W) {
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
   const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (i >= H || j >= W) return;
   const size_t N = static_cast<size_t>(H) * W, at = static_cast<size_t>(i) * W + j;
+  const UnrolledBack v = back_of(u, static_cast<int>(blockIdx.z), N);
+  const UnrolledState& s = v.s;
+  const float* xbar = v.xbar;
+  const UnrolledGrads& after = v.after;
+  const UnrolledWork& work = v.work;
+  const UnrolledSteps& k = v.k;
   float p_hat[2], r_hat[3], w_tilde[2];
   ascents_at(s, xbar, N, i, j, H, W, k, p_hat, r_hat, w_tilde);
   // p~ = P(p^): where outside the ball, p~ = R p^ / n.
@@ -169,13 +230,16 @@ __global__ __launch_bounds__(256) void unrolled_balls_back_kernel(UnrolledState 
 
 // 2. grad x- (sigma g grad^T grad p^), grad w- (-sigma g grad p^ + sigma g E^T grad r^),
 // then grad w~, grad w, and grad p less div's part.
-__global__ __launch_bounds__(256) void unrolled_bars_back_kernel(UnrolledGrads after, UnrolledWork work,
-                                                                 UnrolledGrads before, int H, int W,
-                                                                 UnrolledSteps k) {
+__global__ __launch_bounds__(256) void unrolled_bars_back_kernel(UnrolledFrames u, int H, int W) {
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
   const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (i >= H || j >= W) return;
   const size_t N = static_cast<size_t>(H) * W, at = static_cast<size_t>(i) * W + j;
+  const UnrolledBack v = back_of(u, static_cast<int>(blockIdx.z), N);
+  const UnrolledGrads& after = v.after;
+  const UnrolledGrads& before = v.before;
+  const UnrolledWork& work = v.work;
+  const UnrolledSteps& k = v.k;
   const auto g = [&](const float* plane, int di, int dj) {
     const int a = i + di, b = j + dj;
     return (a >= 0 && a < H && b >= 0 && b < W) ? plane[static_cast<size_t>(a) * W + b] : 0.0f;
@@ -204,12 +268,16 @@ __global__ __launch_bounds__(256) void unrolled_bars_back_kernel(UnrolledGrads a
 }
 
 // 3. grad r: (1 - rho) grad r+ + grad r^ + div_2^T (tau g grad w~).
-__global__ __launch_bounds__(256) void unrolled_r_back_kernel(UnrolledGrads after, UnrolledWork work,
-                                                              UnrolledGrads before, int H, int W, UnrolledSteps k) {
+__global__ __launch_bounds__(256) void unrolled_r_back_kernel(UnrolledFrames u, int H, int W) {
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
   const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (i >= H || j >= W) return;
   const size_t N = static_cast<size_t>(H) * W, at = static_cast<size_t>(i) * W + j;
+  const UnrolledBack v = back_of(u, static_cast<int>(blockIdx.z), N);
+  const UnrolledGrads& after = v.after;
+  const UnrolledGrads& before = v.before;
+  const UnrolledWork& work = v.work;
+  const UnrolledSteps& k = v.k;
   const auto g = [&](const float* plane, int di, int dj) {
     const int a = i + di, b = j + dj;
     return (a >= 0 && a < H && b >= 0 && b < W) ? k.descent * plane[static_cast<size_t>(a) * W + b] : 0.0f;
@@ -228,13 +296,18 @@ __global__ __launch_bounds__(256) void unrolled_r_back_kernel(UnrolledGrads afte
 // 4. The data term's block: one wavefront per block, a lane per sample (and per
 // coefficient). grad x~ = 2 grad x- + rho grad x+; grad zeta = DCT grad x~; through the
 // clip's slope; grad v = IDCT; grad x = grad v - grad x- + (1 - rho) grad x+.
-__global__ __launch_bounds__(256) void unrolled_prox_back_kernel(const Component* __restrict__ component,
-                                                                 UnrolledState s, UnrolledGrads after,
-                                                                 UnrolledWork work, UnrolledGrads before, int H, int W,
-                                                                 float step, UnrolledSteps k) {
+__global__ __launch_bounds__(256) void unrolled_prox_back_kernel(UnrolledFrames u, int H, int W) {
   __shared__ float first[4][64];
   __shared__ float second[4][64];
-  const Component& K = *component;
+  const int f = static_cast<int>(blockIdx.y);
+  const UnrolledBack views = back_of(u, f, static_cast<size_t>(H) * W);
+  const UnrolledState& s = views.s;
+  const UnrolledGrads& after = views.after;
+  const UnrolledGrads& before = views.before;
+  const UnrolledWork& work = views.work;
+  const UnrolledSteps& k = views.k;
+  const float step = u.frames[f].step;
+  const Component& K = u.components[f];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane >> 3, n = lane & 7;
   const int tile = blockIdx.x * 4 + wave;
   const bool active = tile < K.rows * K.columns;
@@ -282,16 +355,37 @@ __global__ __launch_bounds__(256) void unrolled_prox_back_kernel(const Component
 }
 
 // 5. grad p += div^T (tau g grad v): div p = B_x p_x + B_y p_y.
-__global__ __launch_bounds__(256) void unrolled_div_back_kernel(UnrolledWork work, UnrolledGrads before, int H, int W) {
+__global__ __launch_bounds__(256) void unrolled_div_back_kernel(UnrolledFrames u, int H, int W) {
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
   const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (i >= H || j >= W) return;
   const size_t N = static_cast<size_t>(H) * W, at = static_cast<size_t>(i) * W + j;
+  const UnrolledBack v = back_of(u, static_cast<int>(blockIdx.z), N);
+  const UnrolledGrads& before = v.before;
+  const UnrolledWork& work = v.work;
   const float here = work.v[at];
   const float right = j + 1 < W ? work.v[at + 1] : 0.0f;
   const float below = i + 1 < H ? work.v[at + W] : 0.0f;
   before.p[at] += backward_transpose(here, right, j, W);
   before.p[N + at] += backward_transpose(here, below, i, H);
+}
+
+// The forward prox of every frame (frame blockIdx.y): x of the state relaxed into the next
+// state, x- into `bars`; no tilde copies.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kProxWaves<float, 1, 1>))) void
+unrolled_prox_kernel(UnrolledFrames u, int H, int W) {
+  const int f = static_cast<int>(blockIdx.y);
+  const size_t N = static_cast<size_t>(H) * W;
+  const UnrolledFrame scalars = u.frames[f];
+  float* const now = frame_state(u.state, f, N);
+  Fields<float, float> F{};
+  F.x = now;
+  F.xbar = u.bars + f * N;
+  F.tx = F.xbar;  // not written: no tilde copies here
+  F.p = now + 3 * N;
+  prox_tile<float, float, 1, 1, device::BufferPlanes>(u.components + f, F, 0, nullptr, frame_state(u.next, f, N), H,
+                                                      W, scalars.k.descent, scalars.step, scalars.k.rho,
+                                                      1.0f - scalars.k.rho, 0, device::ordered(blockIdx.x, gridDim.x));
 }
 
 __global__ void scale_kernel(const float* __restrict__ map, float factor, float* __restrict__ out, size_t count) {
@@ -359,8 +453,13 @@ class Unrolled {
       solvers_.push_back(std::make_unique<Solver<float, float>>(*frames_[f], settings_, *plans_[f], Timing::none, s));
       solvers_.back()->prepare(nullptr);
     }
-    // The start, kept; checkpoints; a run's states and x-; gradients; work planes.
+    // Every frame's state side by side (two of them, the iteration's input and output);
+    // x-; the start, kept; checkpoints; a run's states and x-; gradients; work planes. The
+    // frames' components side by side, their level tables staying their solvers'.
     const size_t state = 8 * N_;
+    states_[0] = DeviceBuffer(sizeof(float) * B * state);
+    states_[1] = DeviceBuffer(sizeof(float) * B * state);
+    bars_ = DeviceBuffer(sizeof(float) * B * N_);
     start_ = DeviceBuffer(sizeof(float) * B * state);
     checkpoints_ = DeviceBuffer(sizeof(float) * B * state * (iterations_ / every_ + 1));
     run_states_ = DeviceBuffer(sizeof(float) * B * state * every_);
@@ -368,7 +467,13 @@ class Unrolled {
     grads_ = DeviceBuffer(sizeof(float) * B * state * 2);
     work_ = DeviceBuffer(sizeof(float) * B * N_ * 11);
     radius_ = DeviceBuffer(sizeof(float) * N_);
-    for (int f = 0; f < B; ++f) save(f, start_.as<float>() + f * state);
+    components_ = DeviceBuffer(sizeof(Component) * B);
+    scalars_ = DeviceBuffer(sizeof(UnrolledFrame) * B);
+    for (int f = 0; f < B; ++f) {
+      HIP_CHECK(hipMemcpyAsync(components_.as<Component>() + f, solvers_[f]->component(0), sizeof(Component),
+                               hipMemcpyDeviceToDevice, s));
+      keep_start(f, start_.as<float>() + f * state);
+    }
     HIP_CHECK(hipStreamSynchronize(s));
   }
 
@@ -394,17 +499,18 @@ class Unrolled {
     alpha1_ = static_cast<float>(alpha1);
     scale_kernel<<<static_cast<unsigned>((N_ + 255) / 256), 256, 0, s>>>(map, alpha1_, radius_.as<float>(), N_);
     HIP_CHECK(hipGetLastError());
-    const size_t state = 8 * N_;
-    for (int f = 0; f < frames(); ++f) load(f, start_.as<float>() + f * state);
+    upload_scalars();
+    const int B = frames();
+    const size_t all = static_cast<size_t>(B) * 8 * N_;
+    live_ = 0;
+    copy(states_[live_].as<float>(), start_.as<float>(), all);
     for (int it = 0; it < iterations_; ++it) {
-      if (it % every_ == 0) {
-        for (int f = 0; f < frames(); ++f) save(f, checkpoint(it / every_, f));
-      }
-      for (int f = 0; f < frames(); ++f) step(f);
+      if (it % every_ == 0) copy(checkpoint(it / every_), states_[live_].as<float>(), all);
+      step(bars_.as<float>());
     }
-    for (int f = 0; f < frames(); ++f) {
-      HIP_CHECK(hipMemcpyAsync(canvas + f * N_, live_[f].x, sizeof(float) * N_, hipMemcpyDeviceToDevice, s));
-    }
+    // x of every frame: the first of its 8 planes.
+    HIP_CHECK(hipMemcpy2DAsync(canvas, sizeof(float) * N_, states_[live_].pointer, sizeof(float) * 8 * N_,
+                               sizeof(float) * N_, static_cast<size_t>(B), hipMemcpyDeviceToDevice, s));
     forwarded_ = true;
     join(s, caller);
   }
@@ -416,14 +522,13 @@ class Unrolled {
     hipStream_t s = stream_.stream;
     join(caller, s);
     const int B = frames();
-    const size_t state = 8 * N_;
+    const size_t state = 8 * N_, all = static_cast<size_t>(B) * state;
     float* g = grads_.as<float>();
     // The gradients after the last iteration: the canvas's for x, 0 for the rest.
     fill_kernel<<<static_cast<unsigned>((B * state + 255) / 256), 256, 0, s>>>(g, 0.0f, B * state);
     HIP_CHECK(hipGetLastError());
-    for (int f = 0; f < B; ++f) {
-      HIP_CHECK(hipMemcpyAsync(g + f * state, grad_canvas + f * N_, sizeof(float) * N_, hipMemcpyDeviceToDevice, s));
-    }
+    HIP_CHECK(hipMemcpy2DAsync(g, sizeof(float) * state, grad_canvas, sizeof(float) * N_, sizeof(float) * N_,
+                               static_cast<size_t>(B), hipMemcpyDeviceToDevice, s));
     float* sensitivities = work_.as<float>() + B * N_ * 9;  // radius [B][N], then alpha0 [B][N]
     fill_kernel<<<static_cast<unsigned>((2 * B * N_ + 255) / 256), 256, 0, s>>>(sensitivities, 0.0f, 2 * B * N_);
     HIP_CHECK(hipGetLastError());
@@ -432,26 +537,20 @@ class Unrolled {
     for (int run = runs - 1; run >= 0; --run) {
       const int first = run * every_, last = std::min(iterations_, first + every_);
       // Recompute the run, keeping each iteration's state and x-.
-      for (int f = 0; f < B; ++f) load(f, checkpoint(run, f));
+      copy(states_[live_].as<float>(), checkpoint(run), all);
       for (int it = first; it < last; ++it) {
-        for (int f = 0; f < B; ++f) {
-          save(f, run_states_.as<float>() + (static_cast<size_t>(it - first) * B + f) * state);
-          step(f);
-          HIP_CHECK(hipMemcpyAsync(run_bars_.as<float>() + (static_cast<size_t>(it - first) * B + f) * N_,
-                                   solvers_[f]->fields_.xbar, sizeof(float) * N_, hipMemcpyDeviceToDevice, s));
-        }
+        copy(run_states_.as<float>() + static_cast<size_t>(it - first) * all, states_[live_].as<float>(), all);
+        step(run_bars_.as<float>() + static_cast<size_t>(it - first) * B * N_);
       }
       for (int it = last - 1; it >= first; --it) {
-        for (int f = 0; f < B; ++f) {
-          float* kept = run_states_.as<float>() + (static_cast<size_t>(it - first) * B + f) * state;
-          const float* xbar = run_bars_.as<float>() + (static_cast<size_t>(it - first) * B + f) * N_;
-          UnrolledState at{kept, kept + N_, kept + 3 * N_, kept + 5 * N_};
-          float* after_planes = g + (static_cast<size_t>(parity) * B + f) * state;
-          float* before_planes = g + (static_cast<size_t>(1 - parity) * B + f) * state;
-          UnrolledGrads after{after_planes, after_planes + N_, after_planes + 3 * N_, after_planes + 5 * N_};
-          UnrolledGrads before{before_planes, before_planes + N_, before_planes + 3 * N_, before_planes + 5 * N_};
-          step_back(f, at, xbar, after, before, sensitivities + f * N_, sensitivities + (B + f) * N_);
-        }
+        UnrolledFrames u = frames_view();
+        u.state = run_states_.as<float>() + static_cast<size_t>(it - first) * all;
+        u.bars = run_bars_.as<float>() + static_cast<size_t>(it - first) * B * N_;
+        u.after = g + static_cast<size_t>(parity) * all;
+        u.before = g + static_cast<size_t>(1 - parity) * all;
+        u.work = work_.as<float>();
+        u.sensitivity = sensitivities;
+        step_back(u);
         parity = 1 - parity;
       }
     }
@@ -484,88 +583,80 @@ class Unrolled {
     HIP_CHECK(hipEventDestroy(event));
   }
 
-  float* checkpoint(int index, int f) {
-    return checkpoints_.as<float>() + (static_cast<size_t>(index) * frames() + f) * 8 * N_;
+  // The states of every frame at checkpoint `index`.
+  float* checkpoint(int index) { return checkpoints_.as<float>() + static_cast<size_t>(index) * frames() * 8 * N_; }
+
+  void copy(float* to, const float* from, size_t count) {
+    HIP_CHECK(hipMemcpyAsync(to, from, sizeof(float) * count, hipMemcpyDeviceToDevice, stream_.stream));
   }
 
-  UnrolledSteps steps_of(int f) const {
-    const Plan& plan = *plans_[f];
-    UnrolledSteps k;
-    k.descent = static_cast<float>(plan.tau * plan.gamma[0]);
-    k.ascent = static_cast<float>(plan.sigma * plan.gamma[0]);
-    k.alpha0 = alpha0_;
-    k.rho = static_cast<float>(plan.relaxation);
-    return k;
+  // The frames' scalars (alpha0 is the forward's) onto the device.
+  void upload_scalars() {
+    scalars_host_.assign(frames(), UnrolledFrame{});
+    for (int f = 0; f < frames(); ++f) {
+      const Plan& plan = *plans_[f];
+      UnrolledFrame& k = scalars_host_[f];
+      k.k.descent = static_cast<float>(plan.tau * plan.gamma[0]);
+      k.k.ascent = static_cast<float>(plan.sigma * plan.gamma[0]);
+      k.k.alpha0 = alpha0_;
+      k.k.rho = static_cast<float>(plan.relaxation);
+      k.step = static_cast<float>(plan.tau);  // one sample a cell
+    }
+    HIP_CHECK(hipMemcpyAsync(scalars_.pointer, scalars_host_.data(), sizeof(UnrolledFrame) * scalars_host_.size(),
+                             hipMemcpyHostToDevice, stream_.stream));
   }
 
-  // The live state of frame f to and from 8 planes.
-  void save(int f, float* to) {
+  UnrolledFrames frames_view() {
+    UnrolledFrames u{};
+    u.frames = scalars_.as<UnrolledFrame>();
+    u.components = components_.as<Component>();
+    u.count = frames();
+    return u;
+  }
+
+  // The start state of frame f, from its solver's fields, to 8 planes.
+  void keep_start(int f, float* to) {
     const Fields<float, float>& F = solvers_[f]->fields_;
-    hipStream_t s = stream_.stream;
-    HIP_CHECK(hipMemcpyAsync(to, F.x, sizeof(float) * N_, hipMemcpyDeviceToDevice, s));
-    HIP_CHECK(hipMemcpyAsync(to + N_, F.w, sizeof(float) * 2 * N_, hipMemcpyDeviceToDevice, s));
-    HIP_CHECK(hipMemcpyAsync(to + 3 * N_, F.p, sizeof(float) * 2 * N_, hipMemcpyDeviceToDevice, s));
-    HIP_CHECK(hipMemcpyAsync(to + 5 * N_, F.r, sizeof(float) * 3 * N_, hipMemcpyDeviceToDevice, s));
-  }
-  void load(int f, const float* from) {
-    const Fields<float, float>& F = solvers_[f]->fields_;
-    hipStream_t s = stream_.stream;
-    HIP_CHECK(hipMemcpyAsync(F.x, from, sizeof(float) * N_, hipMemcpyDeviceToDevice, s));
-    HIP_CHECK(hipMemcpyAsync(F.w, from + N_, sizeof(float) * 2 * N_, hipMemcpyDeviceToDevice, s));
-    HIP_CHECK(hipMemcpyAsync(F.p, from + 3 * N_, sizeof(float) * 2 * N_, hipMemcpyDeviceToDevice, s));
-    HIP_CHECK(hipMemcpyAsync(F.r, from + 5 * N_, sizeof(float) * 3 * N_, hipMemcpyDeviceToDevice, s));
-    live_.resize(frames());
-    live_[f] = UnrolledState{F.x, F.w, F.p, F.r};
+    copy(to, F.x, N_);
+    copy(to + N_, F.w, 2 * N_);
+    copy(to + 3 * N_, F.p, 2 * N_);
+    copy(to + 5 * N_, F.r, 3 * N_);
   }
 
-  // One forward iteration of frame f: the prox kernel, then the dual step with the radius
-  // per pixel into the other buffers, which then become the live ones.
-  void step(int f) {
-    Solver<float, float>& v = *solvers_[f];
+  // One forward iteration of every frame: the prox (x relaxed into the other state, x-
+  // into `bars`), then the dual step with the radius per pixel into the other state,
+  // which then becomes the live one.
+  void step(float* bars) {
     hipStream_t s = stream_.stream;
-    const Plan& plan = *plans_[f];
-    const UnrolledSteps k = steps_of(f);
-    const float step = static_cast<float>(plan.tau);  // one sample a cell
-    const float rest = 1.0f - k.rho;
-    ProxSteps<float> steps{};
-    steps.descent[0] = k.descent;
-    steps.step[0] = step;
-    prox_kernel<float, float, 1, 1, device::BufferPlanes><<<v.prox_groups(0), 256, 0, s>>>(
-        v.component(0), v.fields_, 0, H_, W_, steps, k.rho, rest, 0);
+    UnrolledFrames u = frames_view();
+    u.state = states_[live_].as<float>();
+    u.next = states_[1 - live_].as<float>();
+    u.bars = bars;
+    const unsigned B = static_cast<unsigned>(frames());
+    unrolled_prox_kernel<<<dim3(static_cast<unsigned>(solvers_[0]->prox_groups(0)), B), 256, 0, s>>>(u, H_, W_);
     HIP_CHECK(hipGetLastError());
-    Fields<float, float>& F = v.fields_;
-    const UnrolledState now{F.x, F.w, F.p, F.r};
-    const UnrolledState next{F.x, F.w_next, F.p_next, F.r_next};
-    const dim3 pixels((W_ + 63) / 64, (H_ + 3) / 4);
-    unrolled_dual_kernel<<<pixels, 256, 0, s>>>(now, F.xbar, next, radius_.as<float>(), H_, W_, k);
+    const dim3 pixels((W_ + 63) / 64, (H_ + 3) / 4, B);
+    unrolled_dual_kernel<<<pixels, 256, 0, s>>>(u, radius_.as<float>(), H_, W_);
     HIP_CHECK(hipGetLastError());
-    std::swap(F.w, F.w_next);
-    std::swap(F.p, F.p_next);
-    std::swap(F.r, F.r_next);
-    live_.resize(frames());
-    live_[f] = UnrolledState{F.x, F.w, F.p, F.r};
+    live_ = 1 - live_;
   }
 
-  // The adjoint of one iteration of frame f, from its input state `at` and its x-.
-  void step_back(int f, const UnrolledState& at, const float* xbar, const UnrolledGrads& after,
-                 const UnrolledGrads& before, float* radius_sens, float* alpha0_sens) {
-    Solver<float, float>& v = *solvers_[f];
+  // The adjoint of one iteration of every frame, from its input state and x- (u.state,
+  // u.bars), the gradients after it (u.after) into those before it (u.before).
+  void step_back(const UnrolledFrames& u) {
     hipStream_t s = stream_.stream;
-    const UnrolledSteps k = steps_of(f);
-    float* w = work_.as<float>() + static_cast<size_t>(f) * 9 * N_;
-    UnrolledWork work{w, w + 2 * N_, w + 5 * N_, w + 6 * N_, w + 8 * N_, radius_sens, alpha0_sens};
-    const dim3 pixels((W_ + 63) / 64, (H_ + 3) / 4);
-    unrolled_balls_back_kernel<<<pixels, 256, 0, s>>>(at, xbar, after, work, radius_.as<float>(), H_, W_, k);
+    const unsigned B = static_cast<unsigned>(frames());
+    const dim3 pixels((W_ + 63) / 64, (H_ + 3) / 4, B);
+    unrolled_balls_back_kernel<<<pixels, 256, 0, s>>>(u, radius_.as<float>(), H_, W_);
     HIP_CHECK(hipGetLastError());
-    unrolled_bars_back_kernel<<<pixels, 256, 0, s>>>(after, work, before, H_, W_, k);
+    unrolled_bars_back_kernel<<<pixels, 256, 0, s>>>(u, H_, W_);
     HIP_CHECK(hipGetLastError());
-    unrolled_r_back_kernel<<<pixels, 256, 0, s>>>(after, work, before, H_, W_, k);
+    unrolled_r_back_kernel<<<pixels, 256, 0, s>>>(u, H_, W_);
     HIP_CHECK(hipGetLastError());
-    const int blocks = static_cast<int>(frames_[f]->components[0].blocks());
-    unrolled_prox_back_kernel<<<(blocks + 3) / 4, 256, 0, s>>>(v.component(0), at, after, work, before, H_, W_,
-                                                               static_cast<float>(plans_[f]->tau), k);
+    const unsigned blocks = static_cast<unsigned>(frames_[0]->components[0].blocks());
+    unrolled_prox_back_kernel<<<dim3((blocks + 3) / 4, B), 256, 0, s>>>(u, H_, W_);
     HIP_CHECK(hipGetLastError());
-    unrolled_div_back_kernel<<<pixels, 256, 0, s>>>(work, before, H_, W_);
+    unrolled_div_back_kernel<<<pixels, 256, 0, s>>>(u, H_, W_);
     HIP_CHECK(hipGetLastError());
   }
 
@@ -577,8 +668,10 @@ class Unrolled {
   size_t N_ = 0;
   StreamHolder stream_;
   std::vector<std::unique_ptr<Solver<float, float>>> solvers_;
-  std::vector<UnrolledState> live_;
-  DeviceBuffer start_, checkpoints_, run_states_, run_bars_, grads_, work_, radius_;
+  DeviceBuffer states_[2], bars_, start_, checkpoints_, run_states_, run_bars_, grads_, work_, radius_;
+  DeviceBuffer components_, scalars_;
+  std::vector<UnrolledFrame> scalars_host_;
+  int live_ = 0;
   const float* map_ = nullptr;
   float alpha0_ = 0.0f, alpha1_ = 0.0f;
   bool forwarded_ = false;
