@@ -445,9 +445,11 @@ struct EdgeRoom {
 // a frame (work-groups in a run on each XCD), and the batch's kernel runs tiles of many.
 //
 // dual_run is the work of 4 strips across (strip group `group`) from row `first` to `end`.
-template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes>
+//
+// With MAP, p's ball has a radius per sample, radius_map's (the unrolled solve's).
+template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes, bool MAP = false>
 __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first, int end, int group, Steps<T> steps,
-                                         int keep) {
+                                         int keep, const T* __restrict__ radius_map = nullptr) {
   __shared__ EdgeRoom<T, C> room;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int strip = group * 4 + wave;
@@ -647,7 +649,9 @@ __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first
       }
     }
     const T rho = steps.rho, rest = steps.rest;
-    const T shared_scale = COUPLED ? device::ball_scale(square, steps.radius) : T(1);
+    T radius = steps.radius;
+    if constexpr (MAP) radius = radius_map[index(i)];
+    const T shared_scale = COUPLED ? device::ball_scale(square, radius) : T(1);
     const T shared_tensor_scale = (COUPLED && TGV) ? device::ball_scale(tensor_square, steps.radius0) : T(1);
     const uint32_t at = index(i);
 #pragma unroll
@@ -655,7 +659,7 @@ __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first
       if (writes) {
         T scale = shared_scale;
         if constexpr (!COUPLED) {
-          scale = device::ball_scale(step_p[c][0] * step_p[c][0] + step_p[c][1] * step_p[c][1], steps.radius);
+          scale = device::ball_scale(step_p[c][0] * step_p[c][0] + step_p[c][1] * step_p[c][1], radius);
         }
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
@@ -706,11 +710,12 @@ __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first
   }
 }
 
-template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes>
+template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes, bool MAP = false>
 __device__ __forceinline__ void dual_tile(Fields<T, S> F, int H, int W, int rows, int groups_across,
-                                          Steps<T> steps, int keep, int g) {
+                                          Steps<T> steps, int keep, int g, const T* __restrict__ radius_map = nullptr) {
   const int first = (g / groups_across) * rows;
-  dual_run<T, S, C, TGV, COUPLED, Planes>(F, H, W, first, min(first + rows, H), g % groups_across, steps, keep);
+  dual_run<T, S, C, TGV, COUPLED, Planes, MAP>(F, H, W, first, min(first + rows, H), g % groups_across, steps, keep,
+                                               radius_map);
 }
 
 template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes>

@@ -116,32 +116,31 @@ __device__ __forceinline__ void ascents_at(const UnrolledState& s, const float* 
   r_hat[2] = s.r[2 * N + at] + k.ascent * xy;
 }
 
-// The forward dual step of every frame with a radius per pixel: w, p and r into the next
-// state.
-__global__ __launch_bounds__(256) void unrolled_dual_kernel(UnrolledFrames u, const float* __restrict__ radius,
-                                                            int H, int W) {
-  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int f = static_cast<int>(blockIdx.z);
-  if (i >= H || j >= W) return;
-  const size_t N = static_cast<size_t>(H) * W, at = static_cast<size_t>(i) * W + j;
+// The forward dual step of every frame (frame blockIdx.y) by the frames' dual step, p's
+// ball of radius alpha1 m(i, j) at (i, j): w, p and r of the state into the next state.
+__global__ __launch_bounds__(256) void unrolled_dual_kernel(UnrolledFrames u, const float* __restrict__ radius, int H,
+                                                            int W, int rows, int groups_across) {
+  const int f = static_cast<int>(blockIdx.y);
+  const size_t N = static_cast<size_t>(H) * W;
   float* const now = frame_state(u.state, f, N);
   float* const later = frame_state(u.next, f, N);
-  const UnrolledState s{now, now + N, now + 3 * N, now + 5 * N};
-  const UnrolledState next{later, later + N, later + 3 * N, later + 5 * N};
-  const float* xbar = u.bars + f * N;
+  Fields<float, float> F{};
+  F.xbar = u.bars + f * N;
+  F.w = now + N;
+  F.w_next = F.tw = later + N;
+  F.p = now + 3 * N;
+  F.p_next = F.tp = later + 3 * N;
+  F.r = now + 5 * N;
+  F.r_next = F.tr = later + 5 * N;
   const UnrolledSteps k = u.frames[f].k;
-  float p_hat[2], r_hat[3], w_tilde[2];
-  ascents_at(s, xbar, N, i, j, H, W, k, p_hat, r_hat, w_tilde);
-  const float scale = device::ball_scale(p_hat[0] * p_hat[0] + p_hat[1] * p_hat[1], radius[at]);
-  const float tensor_scale = device::ball_scale(
-      r_hat[0] * r_hat[0] + r_hat[1] * r_hat[1] + 2.0f * (r_hat[2] * r_hat[2]), k.alpha0);
-  const float rest = 1.0f - k.rho;
-  for (int e = 0; e < 2; ++e) {
-    next.p[e * N + at] = k.rho * (p_hat[e] * scale) + rest * s.p[e * N + at];
-    next.w[e * N + at] = k.rho * w_tilde[e] + rest * s.w[e * N + at];
-  }
-  for (int e = 0; e < 3; ++e) next.r[e * N + at] = k.rho * (r_hat[e] * tensor_scale) + rest * s.r[e * N + at];
+  Steps<float> steps{};
+  steps.descent[0] = k.descent;
+  steps.ascent[0] = k.ascent;
+  steps.radius0 = k.alpha0;
+  steps.rho = k.rho;
+  steps.rest = 1.0f - k.rho;
+  dual_tile<float, float, 1, true, false, device::BufferPlanes, true>(F, H, W, rows, groups_across, steps, 0,
+                                                                      device::ordered(blockIdx.x, gridDim.x), radius);
 }
 
 // The adjoint's gradients of one frame: of the state (x, w, p, r), and the planes between
@@ -467,6 +466,12 @@ class Unrolled {
     grads_ = DeviceBuffer(sizeof(float) * B * state * 2);
     work_ = DeviceBuffer(sizeof(float) * B * N_ * 11);
     radius_ = DeviceBuffer(sizeof(float) * N_);
+    // The dual step's rows per wavefront: about 8192 wavefronts over all the frames (one
+    // channel a row: four times the wavefronts of three channels' 2048 for the bytes in
+    // flight), unless the environment says.
+    const long strips = (W_ + kStripColumns - 1) / kStripColumns;
+    while (dual_rows_ < 64 && B * strips * ((H_ + 2 * dual_rows_ - 1) / (2 * dual_rows_)) >= 8192) dual_rows_ *= 2;
+    if (const char* rows = std::getenv("UNROUND_DUAL_ROWS")) dual_rows_ = std::max(1, std::atoi(rows));
     components_ = DeviceBuffer(sizeof(Component) * B);
     scalars_ = DeviceBuffer(sizeof(UnrolledFrame) * B);
     for (int f = 0; f < B; ++f) {
@@ -502,14 +507,15 @@ class Unrolled {
     upload_scalars();
     const int B = frames();
     const size_t all = static_cast<size_t>(B) * 8 * N_;
-    live_ = 0;
-    copy(states_[live_].as<float>(), start_.as<float>(), all);
+    int live = 0;
+    copy(states_[live].as<float>(), start_.as<float>(), all);
     for (int it = 0; it < iterations_; ++it) {
-      if (it % every_ == 0) copy(checkpoint(it / every_), states_[live_].as<float>(), all);
-      step(bars_.as<float>());
+      if (it % every_ == 0) copy(checkpoint(it / every_), states_[live].as<float>(), all);
+      step(states_[live].as<float>(), states_[1 - live].as<float>(), bars_.as<float>());
+      live = 1 - live;
     }
     // x of every frame: the first of its 8 planes.
-    HIP_CHECK(hipMemcpy2DAsync(canvas, sizeof(float) * N_, states_[live_].pointer, sizeof(float) * 8 * N_,
+    HIP_CHECK(hipMemcpy2DAsync(canvas, sizeof(float) * N_, states_[live].pointer, sizeof(float) * 8 * N_,
                                sizeof(float) * N_, static_cast<size_t>(B), hipMemcpyDeviceToDevice, s));
     forwarded_ = true;
     join(s, caller);
@@ -536,11 +542,14 @@ class Unrolled {
     const int runs = (iterations_ + every_ - 1) / every_;
     for (int run = runs - 1; run >= 0; --run) {
       const int first = run * every_, last = std::min(iterations_, first + every_);
-      // Recompute the run, keeping each iteration's state and x-.
-      copy(states_[live_].as<float>(), checkpoint(run), all);
+      // Recompute the run, each iteration's input state into its own slot (the last one's
+      // output, not needed, into a spare state) and its x- beside.
+      float* const kept = run_states_.as<float>();
+      copy(kept, checkpoint(run), all);
       for (int it = first; it < last; ++it) {
-        copy(run_states_.as<float>() + static_cast<size_t>(it - first) * all, states_[live_].as<float>(), all);
-        step(run_bars_.as<float>() + static_cast<size_t>(it - first) * B * N_);
+        float* const next = it + 1 < last ? kept + static_cast<size_t>(it + 1 - first) * all : states_[0].as<float>();
+        step(kept + static_cast<size_t>(it - first) * all, next,
+             run_bars_.as<float>() + static_cast<size_t>(it - first) * B * N_);
       }
       for (int it = last - 1; it >= first; --it) {
         UnrolledFrames u = frames_view();
@@ -623,22 +632,22 @@ class Unrolled {
     copy(to + 5 * N_, F.r, 3 * N_);
   }
 
-  // One forward iteration of every frame: the prox (x relaxed into the other state, x-
-  // into `bars`), then the dual step with the radius per pixel into the other state,
-  // which then becomes the live one.
-  void step(float* bars) {
+  // One forward iteration of every frame from the states at `now` into those at `next`:
+  // the prox (x relaxed, x- into `bars`), then the dual step with the radius per pixel.
+  void step(float* now, float* next, float* bars) {
     hipStream_t s = stream_.stream;
     UnrolledFrames u = frames_view();
-    u.state = states_[live_].as<float>();
-    u.next = states_[1 - live_].as<float>();
+    u.state = now;
+    u.next = next;
     u.bars = bars;
     const unsigned B = static_cast<unsigned>(frames());
     unrolled_prox_kernel<<<dim3(static_cast<unsigned>(solvers_[0]->prox_groups(0)), B), 256, 0, s>>>(u, H_, W_);
     HIP_CHECK(hipGetLastError());
-    const dim3 pixels((W_ + 63) / 64, (H_ + 3) / 4, B);
-    unrolled_dual_kernel<<<pixels, 256, 0, s>>>(u, radius_.as<float>(), H_, W_);
+    const int strips = (W_ + kStripColumns - 1) / kStripColumns, across = (strips + 3) / 4;
+    const int groups = across * ((H_ + dual_rows_ - 1) / dual_rows_);
+    unrolled_dual_kernel<<<dim3(static_cast<unsigned>(groups), B), 256, 0, s>>>(u, radius_.as<float>(), H_, W_,
+                                                                                dual_rows_, across);
     HIP_CHECK(hipGetLastError());
-    live_ = 1 - live_;
   }
 
   // The adjoint of one iteration of every frame, from its input state and x- (u.state,
@@ -671,7 +680,7 @@ class Unrolled {
   DeviceBuffer states_[2], bars_, start_, checkpoints_, run_states_, run_bars_, grads_, work_, radius_;
   DeviceBuffer components_, scalars_;
   std::vector<UnrolledFrame> scalars_host_;
-  int live_ = 0;
+  int dual_rows_ = 8;
   const float* map_ = nullptr;
   float alpha0_ = 0.0f, alpha1_ = 0.0f;
   bool forwarded_ = false;
