@@ -516,7 +516,7 @@ __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first
   if constexpr (TGV) {
     if (first > 0) {
       // w- of the row above, whose right column's r_xx, r_xy come from the wavefront on
-      // the right (its lane 0) through the room's second half.
+      // the right (its lane 0) through the room's slot 1 (the run's first row takes slot 0).
       T r_up[C][3], p_up[C][2], old_up[C][2];
 #pragma unroll
       for (int c = 0; c < C; ++c) {
