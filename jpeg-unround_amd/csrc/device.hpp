@@ -432,11 +432,20 @@ __device__ __forceinline__ int ordered(int g, int total) {
 // planes must span less than 4 GiB. WidePlanes takes 64-bit addresses for larger spans.
 constexpr int kRawBufferWord3 = 0x00020000;  // a raw (untyped) buffer resource on gfx9
 
+// Cache policies of the buffer accesses (the instructions' cache-policy bits).
+constexpr int kTemporal = 0, kStreaming = 2;
+
 template <typename X>
 struct BufferPlanes {
   // A byte offset beyond the resource where its planes span at most 2 GiB: a load there
   // returns zero and a store there is dropped, by the hardware's range check.
   static constexpr uint32_t kOutside = 0x80000000u;
+  // The accesses' cache policy: non-temporal (kStreaming), for a field read or written
+  // once in a launch and not again until the next iteration, a canvas' worth of traffic
+  // later. A store whose samples the very next launch reads (x- of the prox for the dual
+  // step, p of the dual step for the prox) asks for the default (kTemporal): they then
+  // stay in the caches for it.
+  static constexpr int kPolicy = kStreaming;
   __amdgpu_buffer_rsrc_t resource;
   uint32_t plane_bytes;
   __device__ __forceinline__ BufferPlanes(X* base, int count, size_t N)
@@ -449,14 +458,16 @@ struct BufferPlanes {
     return load_at(plane, valid ? index * static_cast<uint32_t>(sizeof(X)) : kOutside);
   }
   // The store where `valid`, dropped elsewhere (spans of at most 2 GiB), with no branch.
+  template <int P = kPolicy>
   __device__ __forceinline__ void store_if(bool valid, int plane, uint32_t index, X value) const {
-    store_at(plane, valid ? index * static_cast<uint32_t>(sizeof(X)) : kOutside, value);
+    store_at<P>(plane, valid ? index * static_cast<uint32_t>(sizeof(X)) : kOutside, value);
   }
   __device__ __forceinline__ X load(int plane, uint32_t index) const {
     return load_at(plane, index * static_cast<uint32_t>(sizeof(X)));
   }
+  template <int P = kPolicy>
   __device__ __forceinline__ void store(int plane, uint32_t index, X value) const {
-    store_at(plane, index * static_cast<uint32_t>(sizeof(X)), value);
+    store_at<P>(plane, index * static_cast<uint32_t>(sizeof(X)), value);
   }
   // Samples index and index + 1 (index even, so that the pair is aligned), zeros where
   // `valid` is false: one load of twice the width.
@@ -465,12 +476,13 @@ struct BufferPlanes {
     const int scalar = static_cast<int>(static_cast<uint32_t>(plane) * plane_bytes);
     if constexpr (sizeof(X) == 2) {
       const unsigned int both = __builtin_bit_cast(unsigned int, __builtin_amdgcn_raw_buffer_load_b32(resource, offset,
-                                                                                                      scalar, 0));
+                                                                                                      scalar, kPolicy));
       first = __builtin_bit_cast(X, static_cast<unsigned short>(both & 0xFFFFu));
       second = __builtin_bit_cast(X, static_cast<unsigned short>(both >> 16));
     } else if constexpr (sizeof(X) == 4) {
       using Pair = unsigned int __attribute__((ext_vector_type(2)));
-      const Pair both = __builtin_bit_cast(Pair, __builtin_amdgcn_raw_buffer_load_b64(resource, offset, scalar, 0));
+      const Pair both =
+          __builtin_bit_cast(Pair, __builtin_amdgcn_raw_buffer_load_b64(resource, offset, scalar, kPolicy));
       // The words go through plain variables: a bit cast of a vector element (`both.y`)
       // is lowered by clang as a load from the vector's first element.
       const unsigned int low = both[0], high = both[1];
@@ -479,52 +491,56 @@ struct BufferPlanes {
     } else {
       using Quad = unsigned int __attribute__((ext_vector_type(4)));
       using Pair = unsigned int __attribute__((ext_vector_type(2)));
-      const Quad both = __builtin_bit_cast(Quad, __builtin_amdgcn_raw_buffer_load_b128(resource, offset, scalar, 0));
+      const Quad both =
+          __builtin_bit_cast(Quad, __builtin_amdgcn_raw_buffer_load_b128(resource, offset, scalar, kPolicy));
       first = __builtin_bit_cast(X, Pair{both.x, both.y});
       second = __builtin_bit_cast(X, Pair{both.z, both.w});
     }
   }
   // The stores of index and index + 1 (index even) where `valid`, dropped elsewhere.
+  template <int P = kPolicy>
   __device__ __forceinline__ void store_pair_if(bool valid, int plane, uint32_t index, X first, X second) const {
     const int offset = static_cast<int>(valid ? index * static_cast<uint32_t>(sizeof(X)) : kOutside);
     const int scalar = static_cast<int>(static_cast<uint32_t>(plane) * plane_bytes);
     if constexpr (sizeof(X) == 2) {
       const unsigned int both = static_cast<unsigned int>(__builtin_bit_cast(unsigned short, first)) |
                                 (static_cast<unsigned int>(__builtin_bit_cast(unsigned short, second)) << 16);
-      __builtin_amdgcn_raw_buffer_store_b32(both, resource, offset, scalar, 0);
+      __builtin_amdgcn_raw_buffer_store_b32(both, resource, offset, scalar, P);
     } else if constexpr (sizeof(X) == 4) {
       using Pair = unsigned int __attribute__((ext_vector_type(2)));
       const Pair both{__builtin_bit_cast(unsigned int, first), __builtin_bit_cast(unsigned int, second)};
-      __builtin_amdgcn_raw_buffer_store_b64(both, resource, offset, scalar, 0);
+      __builtin_amdgcn_raw_buffer_store_b64(both, resource, offset, scalar, P);
     } else {
       using Quad = unsigned int __attribute__((ext_vector_type(4)));
       using Pair = unsigned int __attribute__((ext_vector_type(2)));
       const Pair a = __builtin_bit_cast(Pair, first), b = __builtin_bit_cast(Pair, second);
       const Quad both{a.x, a.y, b.x, b.y};
-      __builtin_amdgcn_raw_buffer_store_b128(both, resource, offset, scalar, 0);
+      __builtin_amdgcn_raw_buffer_store_b128(both, resource, offset, scalar, P);
     }
   }
   __device__ __forceinline__ X load_at(int plane, uint32_t bytes) const {
     const int offset = static_cast<int>(bytes);
     const int scalar = static_cast<int>(static_cast<uint32_t>(plane) * plane_bytes);
     if constexpr (sizeof(X) == 2) {
-      return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b16(resource, offset, scalar, 0));
+      return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b16(resource, offset, scalar, kPolicy));
     } else if constexpr (sizeof(X) == 4) {
-      return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b32(resource, offset, scalar, 0));
+      return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b32(resource, offset, scalar, kPolicy));
     } else {
-      return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b64(resource, offset, scalar, 0));
+      return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b64(resource, offset, scalar, kPolicy));
     }
   }
+  template <int P = kPolicy>
   __device__ __forceinline__ void store_at(int plane, uint32_t bytes, X value) const {
     const int offset = static_cast<int>(bytes);
     const int scalar = static_cast<int>(static_cast<uint32_t>(plane) * plane_bytes);
     if constexpr (sizeof(X) == 2) {
-      __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, value), resource, offset, scalar, 0);
+      __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, value), resource, offset, scalar,
+                                            P);
     } else if constexpr (sizeof(X) == 4) {
-      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, value), resource, offset, scalar, 0);
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, value), resource, offset, scalar, P);
     } else {
       using Pair = unsigned int __attribute__((ext_vector_type(2)));
-      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(Pair, value), resource, offset, scalar, 0);
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(Pair, value), resource, offset, scalar, P);
     }
   }
 };
@@ -535,10 +551,14 @@ struct WidePlanes {
   size_t N;
   __device__ __forceinline__ WidePlanes(X* base_, int, size_t N_) : base(base_), N(N_) {}
   __device__ __forceinline__ X load(int plane, uint32_t index) const { return base[plane * N + index]; }
-  __device__ __forceinline__ void store(int plane, uint32_t index, X value) const { base[plane * N + index] = value; }
+  template <int P = kStreaming>
+  __device__ __forceinline__ void store(int plane, uint32_t index, X value) const {
+    base[plane * N + index] = value;
+  }
   __device__ __forceinline__ X load_if(bool valid, int plane, uint32_t index) const {
     return valid ? base[plane * N + index] : X{};
   }
+  template <int P = kStreaming>
   __device__ __forceinline__ void store_if(bool valid, int plane, uint32_t index, X value) const {
     if (valid) base[plane * N + index] = value;
   }
@@ -546,6 +566,7 @@ struct WidePlanes {
     first = load_if(valid, plane, index);
     second = load_if(valid, plane, index + 1);
   }
+  template <int P = kStreaming>
   __device__ __forceinline__ void store_pair_if(bool valid, int plane, uint32_t index, X first, X second) const {
     store_if(valid, plane, index, first);
     store_if(valid, plane, index + 1, second);

@@ -208,13 +208,13 @@ __device__ __forceinline__ void prox_tile(const Component* __restrict__ componen
       }
       const uint32_t where = at(top + m * DV + a, first_column);
       if constexpr (DH == 2) {
-        xbar.store_pair_if(live, 0, where, bars[0], bars[1]);
+        xbar.template store_pair_if<device::kTemporal>(live, 0, where, bars[0], bars[1]);
         x_next.store_pair_if(live, 0, where, relaxed[0], relaxed[1]);
         if (keep) tilde_x.store_pair_if(live, 0, where, tildes[0], tildes[1]);
       } else {
 #pragma unroll
         for (int bb = 0; bb < DH; ++bb) {
-          xbar.store_if(live, 0, where + bb, bars[bb]);
+          xbar.template store_if<device::kTemporal>(live, 0, where + bb, bars[bb]);
           x_next.store_if(live, 0, where + bb, relaxed[bb]);
           if (keep) tilde_x.store_if(live, 0, where + bb, tildes[bb]);
         }
@@ -665,7 +665,8 @@ __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first
         for (int e = 0; e < 2; ++e) {
           const T projected = step_p[c][e] * scale;
           if (keep) tp.store(2 * c + e, at, device::narrow<S>(projected));
-          p_next.store(2 * c + e, at, device::narrow<S>(rho * projected + rest * p_here[c][e]));
+          p_next.template store<device::kTemporal>(2 * c + e, at,
+                                                   device::narrow<S>(rho * projected + rest * p_here[c][e]));
         }
         if constexpr (TGV) {
 #pragma unroll
