@@ -672,7 +672,7 @@ __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first
 #pragma unroll
           for (int e = 0; e < 2; ++e) {
             if (keep) tw.store(2 * c + e, at, tilde_w[c][e]);
-            w_next.store(2 * c + e, at, rho * tilde_w[c][e] + rest * w_here[c][e]);
+            w_next.template store<device::kTemporal>(2 * c + e, at, rho * tilde_w[c][e] + rest * w_here[c][e]);
           }
           T tensor_scale = shared_tensor_scale;
           if constexpr (!COUPLED) {
