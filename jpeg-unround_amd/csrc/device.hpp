@@ -454,16 +454,18 @@ struct BufferPlanes {
             static_cast<int>(static_cast<uint32_t>(static_cast<size_t>(count) * N * sizeof(X))), kRawBufferWord3)),
         plane_bytes(static_cast<uint32_t>(N * sizeof(X))) {}
   // The sample, or zero where `valid` is false (spans of at most 2 GiB), with no branch.
+  template <int P = kPolicy>
   __device__ __forceinline__ X load_if(bool valid, int plane, uint32_t index) const {
-    return load_at(plane, valid ? index * static_cast<uint32_t>(sizeof(X)) : kOutside);
+    return load_at<P>(plane, valid ? index * static_cast<uint32_t>(sizeof(X)) : kOutside);
   }
   // The store where `valid`, dropped elsewhere (spans of at most 2 GiB), with no branch.
   template <int P = kPolicy>
   __device__ __forceinline__ void store_if(bool valid, int plane, uint32_t index, X value) const {
     store_at<P>(plane, valid ? index * static_cast<uint32_t>(sizeof(X)) : kOutside, value);
   }
+  template <int P = kPolicy>
   __device__ __forceinline__ X load(int plane, uint32_t index) const {
-    return load_at(plane, index * static_cast<uint32_t>(sizeof(X)));
+    return load_at<P>(plane, index * static_cast<uint32_t>(sizeof(X)));
   }
   template <int P = kPolicy>
   __device__ __forceinline__ void store(int plane, uint32_t index, X value) const {
@@ -471,18 +473,19 @@ struct BufferPlanes {
   }
   // Samples index and index + 1 (index even, so that the pair is aligned), zeros where
   // `valid` is false: one load of twice the width.
+  template <int P = kPolicy>
   __device__ __forceinline__ void load_pair_if(bool valid, int plane, uint32_t index, X& first, X& second) const {
     const int offset = static_cast<int>(valid ? index * static_cast<uint32_t>(sizeof(X)) : kOutside);
     const int scalar = static_cast<int>(static_cast<uint32_t>(plane) * plane_bytes);
     if constexpr (sizeof(X) == 2) {
       const unsigned int both = __builtin_bit_cast(unsigned int, __builtin_amdgcn_raw_buffer_load_b32(resource, offset,
-                                                                                                      scalar, kPolicy));
+                                                                                                      scalar, P));
       first = __builtin_bit_cast(X, static_cast<unsigned short>(both & 0xFFFFu));
       second = __builtin_bit_cast(X, static_cast<unsigned short>(both >> 16));
     } else if constexpr (sizeof(X) == 4) {
       using Pair = unsigned int __attribute__((ext_vector_type(2)));
       const Pair both =
-          __builtin_bit_cast(Pair, __builtin_amdgcn_raw_buffer_load_b64(resource, offset, scalar, kPolicy));
+          __builtin_bit_cast(Pair, __builtin_amdgcn_raw_buffer_load_b64(resource, offset, scalar, P));
       // The words go through plain variables: a bit cast of a vector element (`both.y`)
       // is lowered by clang as a load from the vector's first element.
       const unsigned int low = both[0], high = both[1];
@@ -492,7 +495,7 @@ struct BufferPlanes {
       using Quad = unsigned int __attribute__((ext_vector_type(4)));
       using Pair = unsigned int __attribute__((ext_vector_type(2)));
       const Quad both =
-          __builtin_bit_cast(Quad, __builtin_amdgcn_raw_buffer_load_b128(resource, offset, scalar, kPolicy));
+          __builtin_bit_cast(Quad, __builtin_amdgcn_raw_buffer_load_b128(resource, offset, scalar, P));
       first = __builtin_bit_cast(X, Pair{both.x, both.y});
       second = __builtin_bit_cast(X, Pair{both.z, both.w});
     }
@@ -518,15 +521,16 @@ struct BufferPlanes {
       __builtin_amdgcn_raw_buffer_store_b128(both, resource, offset, scalar, P);
     }
   }
+  template <int P = kPolicy>
   __device__ __forceinline__ X load_at(int plane, uint32_t bytes) const {
     const int offset = static_cast<int>(bytes);
     const int scalar = static_cast<int>(static_cast<uint32_t>(plane) * plane_bytes);
     if constexpr (sizeof(X) == 2) {
-      return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b16(resource, offset, scalar, kPolicy));
+      return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b16(resource, offset, scalar, P));
     } else if constexpr (sizeof(X) == 4) {
-      return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b32(resource, offset, scalar, kPolicy));
+      return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b32(resource, offset, scalar, P));
     } else {
-      return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b64(resource, offset, scalar, kPolicy));
+      return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b64(resource, offset, scalar, P));
     }
   }
   template <int P = kPolicy>
@@ -550,11 +554,15 @@ struct WidePlanes {
   X* base;
   size_t N;
   __device__ __forceinline__ WidePlanes(X* base_, int, size_t N_) : base(base_), N(N_) {}
-  __device__ __forceinline__ X load(int plane, uint32_t index) const { return base[plane * N + index]; }
+  template <int P = kStreaming>
+  __device__ __forceinline__ X load(int plane, uint32_t index) const {
+    return base[plane * N + index];
+  }
   template <int P = kStreaming>
   __device__ __forceinline__ void store(int plane, uint32_t index, X value) const {
     base[plane * N + index] = value;
   }
+  template <int P = kStreaming>
   __device__ __forceinline__ X load_if(bool valid, int plane, uint32_t index) const {
     return valid ? base[plane * N + index] : X{};
   }
@@ -562,6 +570,7 @@ struct WidePlanes {
   __device__ __forceinline__ void store_if(bool valid, int plane, uint32_t index, X value) const {
     if (valid) base[plane * N + index] = value;
   }
+  template <int P = kStreaming>
   __device__ __forceinline__ void load_pair_if(bool valid, int plane, uint32_t index, X& first, X& second) const {
     first = load_if(valid, plane, index);
     second = load_if(valid, plane, index + 1);
