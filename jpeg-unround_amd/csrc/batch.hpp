@@ -579,13 +579,13 @@ class Batch {
       HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kernel, 256, 0));
     };
     if (C == 1) {
-      narrow ? held(dual_batch_kernel<T, S, 1, true, false, device::BufferPlanes>)
+      narrow ? held(dual_batch_kernel<T, S, 1, true, false, device::CachedBufferPlanes>)
              : held(dual_batch_kernel<T, S, 1, true, false, device::WidePlanes>);
     } else if (coupled) {
-      narrow ? held(dual_batch_kernel<T, S, 3, true, true, device::BufferPlanes>)
+      narrow ? held(dual_batch_kernel<T, S, 3, true, true, device::CachedBufferPlanes>)
              : held(dual_batch_kernel<T, S, 3, true, true, device::WidePlanes>);
     } else {
-      narrow ? held(dual_batch_kernel<T, S, 3, true, false, device::BufferPlanes>)
+      narrow ? held(dual_batch_kernel<T, S, 3, true, false, device::CachedBufferPlanes>)
              : held(dual_batch_kernel<T, S, 3, true, false, device::WidePlanes>);
     }
     return 2L * std::max(resident, 1) * std::max(units / 8, 1) * 8;
@@ -621,7 +621,7 @@ class Batch {
   void launch_dual(const Launch& launch, const ImageSlot<T, S>* slots, int keep) {
     const auto* tiles = launch.tiles.template as<DualTile>();
     if (launch.key[2]) {
-      dual_batch_kernel<T, S, C, TGV, COUPLED, device::BufferPlanes>
+      dual_batch_kernel<T, S, C, TGV, COUPLED, device::CachedBufferPlanes>
           <<<launch.count, 256, 0, stream_.stream>>>(tiles, slots, state_.active, keep);
     } else {
       dual_batch_kernel<T, S, C, TGV, COUPLED, device::WidePlanes>
