@@ -324,7 +324,7 @@ class Batch {
         r.value_offset[c] = value_at;
         r.conjugate_offset[c] = conjugate_at;
         if (c < v.C_) {
-          value_at += static_cast<int>((64 * frames_[m]->components[c].blocks() + 255) / 256);
+          value_at += static_cast<int>(value_groups_of(frames_[m]->components[c].blocks()));
           conjugate_at += v.prox_groups(c);
         }
       }
@@ -451,7 +451,7 @@ class Batch {
         if (!known) throw GpuError("a batch takes cells of 1 or 2 samples each way");
         auto& list = prox[{static_cast<int>(k.down), static_cast<int>(k.across), buffers_prox}];
         for (int g = 0; g < v.prox_groups(c); ++g) list.push_back({m, c, g});
-        for (int g = 0; g < static_cast<int>((64 * k.blocks() + 255) / 256); ++g) values.push_back({m, c, g});
+        for (int g = 0; g < static_cast<int>(value_groups_of(k.blocks())); ++g) values.push_back({m, c, g});
         for (int g = 0; g < v.prox_groups(c); ++g) {
           conjugates.push_back({m, c, g});
           if (tgv_ && plan.has_partial) partial_conjugates.push_back({m, c, g});

@@ -344,18 +344,25 @@ __global__ __launch_bounds__(256) void norms_kernel(Fields<T, S> F, int C, int H
   norms_tile<T, S, TGV>(F, C, H, W, g0, g1, g2, coupled, partials, gridDim.x, blockIdx.y * gridDim.x + blockIdx.x);
 }
 
-// G's terms at the kept coefficients of one component, a work-group's sum each.
+// G's terms at the kept coefficients of one component, a work-group's sum each: a
+// work-group takes kValueSpan coefficients, each lane every 256th of them.
+constexpr int kValueSpan = 4096;
+inline size_t value_groups_of(size_t blocks) { return (64 * blocks + kValueSpan - 1) / kValueSpan; }
+
 template <typename T>
 __device__ __forceinline__ void value_tile(const Component& K, const T* zeta, double* __restrict__ partials,
                                            int group) {
   __shared__ double room[256];
-  const size_t index = static_cast<size_t>(group) * 256 + threadIdx.x;
   const size_t count = static_cast<size_t>(K.rows) * K.columns * 64;
   double term = 0.0;
-  if (index < count) {
-    const size_t block = index / 64;
-    const int k = static_cast<int>(index % 64);
-    term = device::value_of(K, K.levels[index], K.flat[block] != 0, k, static_cast<double>(zeta[index]));
+#pragma unroll 4
+  for (int run = 0; run < kValueSpan / 256; ++run) {
+    const size_t index = static_cast<size_t>(group) * kValueSpan + static_cast<size_t>(run) * 256 + threadIdx.x;
+    if (index < count) {
+      const size_t block = index / 64;
+      const int k = static_cast<int>(index % 64);
+      term += device::value_of(K, K.levels[index], K.flat[block] != 0, k, static_cast<double>(zeta[index]));
+    }
   }
   const double total = group_reduce<false>(term, room);
   if (threadIdx.x == 0) partials[group] = total;
@@ -788,7 +795,7 @@ class Solver {
     zeta = value_groups = conjugate_groups = 0;
     for (int c = 0; c < C_; ++c) {
       zeta += 64 * frame_.components[c].blocks();
-      value_groups += (64 * frame_.components[c].blocks() + 255) / 256;
+      value_groups += value_groups_of(frame_.components[c].blocks());
       conjugate_groups += static_cast<size_t>(prox_groups(c));
     }
   }
@@ -1135,7 +1142,7 @@ class Solver {
     hipStream_t s = stream();
     size_t offset = 0;
     for (int c = 0; c < C_; ++c) {
-      const size_t groups = (64 * frame_.components[c].blocks() + 255) / 256;
+      const size_t groups = value_groups_of(frame_.components[c].blocks());
       if (groups > 0) {
         value_kernel<T><<<static_cast<unsigned>(groups), 256, 0, s>>>(component(c), fields_.zeta[c],
                                                                       value_partials + offset);
