@@ -103,8 +103,8 @@ __global__ __launch_bounds__(256) void norms_batch_kernel(const BatchTile* __res
   const BatchTile t = tiles[blockIdx.x];
   if (active[t.image] == 0) return;
   const RecordSlot<T, S>& m = slots[t.image];
-  norms_tile<T, S, TGV>(m.F, m.C, m.H, m.W, m.gamma[0], m.gamma[1], m.gamma[2], m.coupled, m.norms, m.norm_across,
-                        t.group);
+  norms_tile<T, S, TGV>(m.F, m.C, m.H, m.W, m.gamma[0], m.gamma[1], m.gamma[2], m.coupled, m.has_partial, m.norms,
+                        m.norm_across, t.group);
 }
 
 template <typename T, typename S>
@@ -198,6 +198,17 @@ __global__ void advance_kernel(unsigned long long* counter, int length) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *counter += static_cast<unsigned long long>(length);
 }
 
+// An event that orders streams and wakes the host, with no timing.
+struct Mark {
+  hipEvent_t event = nullptr;
+  Mark() { HIP_CHECK(hipEventCreateWithFlags(&event, hipEventDisableTiming)); }
+  ~Mark() {
+    if (event != nullptr) (void)hipEventDestroy(event);
+  }
+  Mark(const Mark&) = delete;
+  Mark& operator=(const Mark&) = delete;
+};
+
 // The dual step's rows per wavefront for a batch: the best of a few measured on the batch
 // itself (its tiles change with the rows), kept per shape of batch for the process.
 struct RowsChoice {
@@ -224,11 +235,15 @@ class Batch {
     const int M = static_cast<int>(frames_.size());
     const bool solving = settings_.method == Method::tv || settings_.method == Method::tgv;
     tgv_ = settings_.method == Method::tgv;
+    using Clock = std::chrono::steady_clock;
+    const auto began = Clock::now();
     for (int m = 0; m < M; ++m) {
       solvers_.push_back(std::make_unique<Solver<T, S>>(*frames_[m], settings_, *plans_[m], Timing::none, s));
       solvers_.back()->prepare(nullptr);
     }
     HIP_CHECK(hipStreamSynchronize(s));
+    const auto prepared = Clock::now();
+    auto tabled = prepared, iterated = prepared;
     outs.assign(M, {});
     latencies.assign(M, 0.0);
     most_ = plans_[0]->iterations;
@@ -239,7 +254,9 @@ class Batch {
     if (solving && M > 0) {
       build_state();
       build_tables(choose_rows());
+      tabled = Clock::now();
       iterate_all();
+      iterated = Clock::now();
     }
     for (int m = 0; m < M; ++m) {
       Solver<T, S>& solver = *solvers_[m];
@@ -263,6 +280,14 @@ class Batch {
         latencies[m] = seconds_at(out.iterations);
       }
       solver.finish(out);
+    }
+    if (std::getenv("UNROUND_BATCH_LOG") != nullptr) {
+      const auto seconds = [](Clock::time_point a, Clock::time_point b) {
+        return std::chrono::duration<double>(b - a).count();
+      };
+      std::fprintf(stderr, "batch of %d: prepare %.3f s, tables %.3f s, iterations %.3f s, finish %.3f s\n", M,
+                   seconds(began, prepared), seconds(prepared, tabled), seconds(tabled, iterated),
+                   seconds(iterated, Clock::now()));
     }
   }
 
@@ -644,10 +669,9 @@ class Batch {
     }
   }
 
-  // The records of every frame still iterating and their stopping tests; `length` is the
-  // iterations since the count was last advanced.
-  void launch_records(int length) {
-    hipStream_t s = stream_.stream;
+  // The records of every frame still iterating and their stopping tests on stream `s`, then
+  // the count of iterations advanced by `length`, the iterations since the last record.
+  void launch_records(hipStream_t s, int length) {
     const int M = static_cast<int>(solvers_.size());
     const auto* slots = record_slots_.template as<RecordSlot<T, S>>();
     const auto tiles = [](const Launch& launch) { return launch.tiles.template as<BatchTile>(); };
@@ -680,30 +704,34 @@ class Batch {
     HIP_CHECK(hipGetLastError());
     stop_kernel<T, S><<<(M + 255) / 256, 256, 0, s>>>(slots, M, state_, length);
     HIP_CHECK(hipGetLastError());
-  }
-
-  // `length` iterations from `parity`, the last keeping the tilde point and followed by the
-  // records where `record`, then the count advanced.
-  void launch_run(int length, int parity, bool record) {
-    for (int t = 0; t < length; ++t) {
-      const int p = (parity + t) & 1;
-      const int keep = record && t == length - 1 ? 1 : 0;
-      launch_proxes(p, keep);
-      launch_duals(p, keep);
-    }
-    if (record) launch_records(length);
-    advance_kernel<<<1, 64, 0, stream_.stream>>>(state_.counter, length);
+    advance_kernel<<<1, 64, 0, s>>>(state_.counter, length);
     HIP_CHECK(hipGetLastError());
   }
 
-  hipGraphExec_t graph_of(int length, int parity, bool record) {
-    const Key key{length, parity, record ? 1 : 0};
+  // `length` iterations from `parity`, the last keeping the tilde point where `keep`.
+  void launch_iterations(int length, int parity, bool keep) {
+    for (int t = 0; t < length; ++t) {
+      const int p = (parity + t) & 1;
+      const int kept = keep && t == length - 1 ? 1 : 0;
+      launch_proxes(p, kept);
+      launch_duals(p, kept);
+    }
+  }
+
+  // The graph of `length` iterations (kind 0) on the iterations' stream, or of the records
+  // after `length` iterations (kind 1) on the records' stream: captured once, then replayed.
+  hipGraphExec_t graph_of(int kind, int length, int parity, bool keep) {
+    const Key key{kind, length, parity, keep ? 1 : 0};
     const auto found = graphs_.find(key);
     if (found != graphs_.end()) return found->second;
-    hipStream_t s = stream_.stream;
+    hipStream_t s = kind == 0 ? stream_.stream : records_stream_.stream;
     hipGraph_t graph = nullptr;
     HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    launch_run(length, parity, record);
+    if (kind == 0) {
+      launch_iterations(length, parity, keep);
+    } else {
+      launch_records(s, length);
+    }
     HIP_CHECK(hipStreamEndCapture(s, &graph));
     hipGraphExec_t made = nullptr;
     const hipError_t instantiated = hipGraphInstantiate(&made, graph, nullptr, nullptr, 0);
@@ -713,27 +741,33 @@ class Batch {
     return made;
   }
 
+  // The iterations run on one stream and the records on another, so that a record's kernels
+  // run beside the next run of iterations: the run's last iteration, which keeps the tilde
+  // point anew, waits for the previous records (the readers of the point it overwrites). A
+  // frame that a record stops may take some of the next run's iterations, which change
+  // neither its tilde point nor its records; the host reads which frames still iterate one
+  // record behind, so that the iterations' stream always holds a run.
   void iterate_all() {
-    hipStream_t s = stream_.stream;
+    hipStream_t s = stream_.stream, r = records_stream_.stream;
     const int M = static_cast<int>(solvers_.size());
     EventPair whole;
     HIP_CHECK(hipEventRecord(whole.begin, s));
     const uint64_t most = most_, every = every_;
-    if (every > 0 || most == 0) launch_records(0);
-    std::vector<int> active(M, 1);
+    if (every > 0 || most == 0) launch_records(s, 0);
     int* pinned = nullptr;
-    HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pinned), sizeof(int) * std::max(M, 1), hipHostMallocDefault));
-    std::deque<std::pair<uint64_t, EventPair>> ends;
-    uint64_t iteration = 0;
-    int parity = 0;
-    const auto any_active = [&]() {
-      HIP_CHECK(hipMemcpyAsync(pinned, state_.active, sizeof(int) * M, hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipStreamSynchronize(s));
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pinned), sizeof(int) * 2 * std::max(M, 1),
+                            hipHostMallocDefault));
+    const auto any_of = [M](const int* flags) {
       for (int m = 0; m < M; ++m)
-        if (pinned[m] != 0) return true;
+        if (flags[m] != 0) return true;
       return false;
     };
-    bool going = any_active();
+    std::deque<std::pair<uint64_t, EventPair>> ends;
+    uint64_t iteration = 0, since = 0;
+    int parity = 0, recorded = 0;
+    HIP_CHECK(hipMemcpyAsync(pinned, state_.active, sizeof(int) * M, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    bool going = any_of(pinned);
     while (going && iteration < most) {
       // Up to the next record, in runs of at most kRun iterations.
       constexpr uint64_t kRun = 10;
@@ -741,15 +775,36 @@ class Batch {
       if (every > 0) next_record = std::min(most, (iteration / every + 1) * every);
       const uint64_t length = std::min(kRun, next_record - iteration);
       const bool record = iteration + length == next_record;
-      HIP_CHECK(hipGraphLaunch(graph_of(static_cast<int>(length), parity, record), s));
+      since += length;
+      if (!record) {
+        HIP_CHECK(hipGraphLaunch(graph_of(0, static_cast<int>(length), parity, false), s));
+      } else {
+        const int last = (parity + static_cast<int>(length) - 1) & 1;
+        if (length > 1) HIP_CHECK(hipGraphLaunch(graph_of(0, static_cast<int>(length) - 1, parity, false), s));
+        if (recorded > 0) HIP_CHECK(hipStreamWaitEvent(s, recorded_[(recorded - 1) & 1].event, 0));
+        HIP_CHECK(hipGraphLaunch(graph_of(0, 1, last, true), s));
+        HIP_CHECK(hipEventRecord(kept_.event, s));
+        HIP_CHECK(hipStreamWaitEvent(r, kept_.event, 0));
+        HIP_CHECK(hipGraphLaunch(graph_of(1, static_cast<int>(since), 0, false), r));
+        HIP_CHECK(hipMemcpyAsync(pinned + (recorded & 1) * M, state_.active, sizeof(int) * M,
+                                 hipMemcpyDeviceToHost, r));
+        HIP_CHECK(hipEventRecord(recorded_[recorded & 1].event, r));
+        ++recorded;
+        since = 0;
+      }
       iteration += length;
       parity = (parity + static_cast<int>(length)) & 1;
       ends.emplace_back();
       ends.back().first = iteration;
-      HIP_CHECK(hipEventRecord(ends.back().second.end, s));
-      if (record) going = any_active();
+      HIP_CHECK(hipEventRecord(ends.back().second.end, record ? r : s));
+      if (record && recorded >= 2) {
+        // The flags after the previous record (or a later one: a frame only ever stops).
+        HIP_CHECK(hipEventSynchronize(recorded_[recorded & 1].event));
+        going = any_of(pinned + (recorded & 1) * M);
+      }
     }
     HIP_CHECK(hipStreamSynchronize(s));
+    HIP_CHECK(hipStreamSynchronize(r));
     reached_ = iteration;
     for (const auto& [at, pair] : ends) {
       float ms = 0.0f;
@@ -778,7 +833,8 @@ class Batch {
   bool tgv_ = false;
   uint64_t most_ = 0, every_ = 0, reached_ = 0;
   T rho_ = T(0), rest_ = T(0);
-  StreamHolder stream_;
+  StreamHolder stream_, records_stream_;
+  Mark kept_, recorded_[2];
   std::vector<std::unique_ptr<Solver<T, S>>> solvers_;
   DeviceBuffer state_memory_, record_slots_, first_jobs_, last_jobs_;
   DeviceBuffer slots_[2];

@@ -664,14 +664,14 @@ __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
           const T projected = step_p[c][e] * scale;
-          if (keep) tp.store(2 * c + e, at, device::narrow<S>(projected));
+          if (keep) tp.template store<device::kStreaming>(2 * c + e, at, device::narrow<S>(projected));
           p_next.template store<device::kTemporal>(2 * c + e, at,
                                                    device::narrow<S>(rho * projected + rest * p_here[c][e]));
         }
         if constexpr (TGV) {
 #pragma unroll
           for (int e = 0; e < 2; ++e) {
-            if (keep) tw.store(2 * c + e, at, tilde_w[c][e]);
+            if (keep) tw.template store<device::kStreaming>(2 * c + e, at, tilde_w[c][e]);
             w_next.template store<device::kTemporal>(2 * c + e, at, rho * tilde_w[c][e] + rest * w_here[c][e]);
           }
           T tensor_scale = shared_tensor_scale;
@@ -683,7 +683,7 @@ __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first
 #pragma unroll
           for (int e = 0; e < 3; ++e) {
             const T projected = step_r[c][e] * tensor_scale;
-            if (keep) tr.store(3 * c + e, at, device::narrow<S>(projected));
+            if (keep) tr.template store<device::kStreaming>(3 * c + e, at, device::narrow<S>(projected));
             r_next.store(3 * c + e, at, device::narrow<S>(rho * projected + rest * r_here[c][e]));
           }
         }

@@ -40,6 +40,7 @@
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "device.hpp"
@@ -234,8 +235,9 @@ __device__ double group_reduce(double value, double* room) {
 
 // The norms at one pixel of the tilde point (every channel): the first order (TV's
 // gradient, or TGV's grad x~ - w~), TGV's |E w~|_F, the largest norm of div_2 r~ and the
-// partial gap's sum over the channels of gamma |p~ + div_2 r~|. Every load is made first.
-template <typename T, typename S, bool TGV, int C>
+// partial gap's sum over the channels of gamma |p~ + div_2 r~| (where PARTIAL: a solve with
+// no partial gap leaves it 0 and reads no p~). Every load is made first.
+template <typename T, typename S, bool TGV, int C, bool PARTIAL>
 __device__ __forceinline__ void pixel_norms(const Fields<T, S>& F, int H, int W, int i, int j, const double* gammas,
                                             int coupled, double& first, double& second, double& most,
                                             double& residual) {
@@ -275,8 +277,10 @@ __device__ __forceinline__ void pixel_norms(const Fields<T, S>& F, int H, int W,
       xy[c] = 0.5 * xy[c];
       device::tensor_divergence_loaded<double>(F.tr + (3 * c) * N, F.tr + (3 * c + 1) * N, F.tr + (3 * c + 2) * N, i,
                                                j, H, W, dx[c], dy[c]);
-      pa[c] = static_cast<double>(device::widen(F.tp[(2 * c) * N + at]));
-      pb[c] = static_cast<double>(device::widen(F.tp[(2 * c + 1) * N + at]));
+      if (PARTIAL) {
+        pa[c] = static_cast<double>(device::widen(F.tp[(2 * c) * N + at]));
+        pb[c] = static_cast<double>(device::widen(F.tp[(2 * c + 1) * N + at]));
+      }
     }
     squares = 0.0;
 #pragma unroll
@@ -296,9 +300,11 @@ __device__ __forceinline__ void pixel_norms(const Fields<T, S>& F, int H, int W,
         most = fmax(most, sqrt(squares));
         squares = 0.0;
       }
-      const double ra = pa[c] + 1.0 * dx[c];
-      const double rb = pb[c] + 1.0 * dy[c];
-      residual += gammas[c] * sqrt(1.0 * (ra * ra + rb * rb));
+      if (PARTIAL) {
+        const double ra = pa[c] + 1.0 * dx[c];
+        const double rb = pb[c] + 1.0 * dy[c];
+        residual += gammas[c] * sqrt(1.0 * (ra * ra + rb * rb));
+      }
     }
     if (coupled) most = sqrt(squares);
   }
@@ -306,26 +312,46 @@ __device__ __forceinline__ void pixel_norms(const Fields<T, S>& F, int H, int W,
 
 // The norms of the tilde point's pixels, each work-group's sums and largest into
 // partials[4 group + 0..3] (first order, TGV's second order, the largest |div_2 r~|, the
-// partial-gap residual).
+// partial-gap residual, 0 unless `partial`).
 //
-// Work-group `group` takes 64 columns and 4 rows, groups numbered across, then down, with
-// `across` groups to a row of them.
+// Work-group `group` takes 64 columns and kNormRows rows (a wavefront every fourth row of
+// them, a lane's pixels summed down its column first), groups numbered across, then down,
+// with `across` groups to a row of them.
+constexpr int kNormRows = 8;
+
 template <typename T, typename S, bool TGV>
 __device__ __forceinline__ void norms_tile(const Fields<T, S>& F, int C, int H, int W, double g0, double g1,
-                                           double g2, int coupled, double* __restrict__ partials, int across,
-                                           int group) {
+                                           double g2, int coupled, int partial, double* __restrict__ partials,
+                                           int across, int group) {
   __shared__ double room[256];
   const int j = (group % across) * 64 + (threadIdx.x & 63);
-  const int i = (group / across) * 4 + (threadIdx.x >> 6);
-  const bool here = i < H && j < W;
+  const int top = (group / across) * kNormRows + (threadIdx.x >> 6);
   const double gammas[3] = {g0, g1, g2};
   double first = 0.0, second = 0.0, most = 0.0, residual = 0.0;
+  const auto add = [&](auto channels, auto with_partial) {
+    constexpr int kC = decltype(channels)::value;
+    constexpr bool kPartial = decltype(with_partial)::value;
+#pragma unroll
+    for (int pass = 0; pass < kNormRows / 4; ++pass) {
+      const int i = top + 4 * pass;
+      double f = 0.0, g = 0.0, m = 0.0, r = 0.0;
+      pixel_norms<T, S, TGV, kC, kPartial>(F, H, W, i, j, gammas, coupled, f, g, m, r);
+      const bool here = i < H && j < W;
+      first += here ? f : 0.0;
+      second += here ? g : 0.0;
+      most = fmax(most, here ? m : 0.0);
+      residual += here ? r : 0.0;
+    }
+  };
+  using One = std::integral_constant<int, 1>;
+  using Three = std::integral_constant<int, 3>;
+  using Partial = std::integral_constant<bool, TGV>;
+  using NoPartial = std::integral_constant<bool, false>;
   if (C == 1) {
-    pixel_norms<T, S, TGV, 1>(F, H, W, i, j, gammas, coupled, first, second, most, residual);
+    partial ? add(One{}, Partial{}) : add(One{}, NoPartial{});
   } else {
-    pixel_norms<T, S, TGV, 3>(F, H, W, i, j, gammas, coupled, first, second, most, residual);
+    partial ? add(Three{}, Partial{}) : add(Three{}, NoPartial{});
   }
-  if (!here) first = second = most = residual = 0.0;
   const double s_first = group_reduce<false>(first, room);
   const double s_second = group_reduce<false>(second, room);
   const double s_most = group_reduce<true>(most, room);
@@ -340,8 +366,10 @@ __device__ __forceinline__ void norms_tile(const Fields<T, S>& F, int C, int H, 
 
 template <typename T, typename S, bool TGV>
 __global__ __launch_bounds__(256) void norms_kernel(Fields<T, S> F, int C, int H, int W, double g0, double g1,
-                                                    double g2, int coupled, double* __restrict__ partials) {
-  norms_tile<T, S, TGV>(F, C, H, W, g0, g1, g2, coupled, partials, gridDim.x, blockIdx.y * gridDim.x + blockIdx.x);
+                                                    double g2, int coupled, int partial,
+                                                    double* __restrict__ partials) {
+  norms_tile<T, S, TGV>(F, C, H, W, g0, g1, g2, coupled, partial, partials, gridDim.x,
+                        blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // G's terms at the kept coefficients of one component, a work-group's sum each: a
@@ -883,11 +911,11 @@ class Solver {
       fields_.tr = take_dual(3 * C * N_);
     }
     // Partials of the records: as many as the work-groups of their kernels.
-    const size_t pixel_groups = static_cast<size_t>((W_ + 63) / 64) * ((H_ + 3) / 4);
-    norm_partials_count_ = pixel_groups;
+    const size_t norm_groups = static_cast<size_t>((W_ + 63) / 64) * ((H_ + kNormRows - 1) / kNormRows);
+    norm_partials_count_ = norm_groups;
     value_partials_count_ = value_groups;
     conjugate_partials_count_ = conjugate_groups;
-    partials_ = DeviceBuffer(sizeof(double) * (4 * pixel_groups + value_groups + 2 * conjugate_groups + 16));
+    partials_ = DeviceBuffer(sizeof(double) * (4 * norm_groups + value_groups + 2 * conjugate_groups + 16));
     record_out_ = DeviceBuffer(sizeof(double) * 16);
   }
 
@@ -1101,13 +1129,16 @@ class Solver {
     double* conjugate_partials = value_partials + value_partials_count_;
     double* partial_partials = conjugate_partials + conjugate_partials_count_;
     double* results = record_out_.as<double>();
-    const dim3 pixels((W_ + 63) / 64, (H_ + 3) / 4);
+    const dim3 pixels((W_ + 63) / 64, (H_ + kNormRows - 1) / kNormRows);
     const double g[3] = {plan_.gamma[0], C_ > 1 ? plan_.gamma[1] : 0.0, C_ > 2 ? plan_.gamma[2] : 0.0};
     const int coupled = settings_.coupled ? 1 : 0;
+    const int partial = plan_.has_partial ? 1 : 0;
     if (tgv_) {
-      norms_kernel<T, S, true><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_, g[0], g[1], g[2], coupled, norm_partials);
+      norms_kernel<T, S, true>
+          <<<pixels, 256, 0, s>>>(fields_, C_, H_, W_, g[0], g[1], g[2], coupled, partial, norm_partials);
     } else {
-      norms_kernel<T, S, false><<<pixels, 256, 0, s>>>(fields_, C_, H_, W_, g[0], g[1], g[2], coupled, norm_partials);
+      norms_kernel<T, S, false>
+          <<<pixels, 256, 0, s>>>(fields_, C_, H_, W_, g[0], g[1], g[2], coupled, partial, norm_partials);
     }
     HIP_CHECK(hipGetLastError());
     // results[0..3]: first, second, most, residual.
