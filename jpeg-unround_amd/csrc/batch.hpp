@@ -714,7 +714,7 @@ class Batch {
       const int p = (parity + t) & 1;
       const int kept = keep && t == length - 1 ? 1 : 0;
       launch_proxes(p, kept);
-      launch_duals(p, kept);
+      launch_duals(p, kept ? kKeepTilde | (!tgv_ || plans_[0]->has_partial ? kKeepDualP : 0) : 0);
     }
   }
 

@@ -429,6 +429,10 @@ struct EdgeGather {
   }
 };
 
+// The dual step's `keep`: kKeepTilde keeps w~ and r~ of the step (the tilde point's, which
+// the records read), kKeepDualP p~ too (which only TV's records and the partial gap read).
+constexpr int kKeepTilde = 1, kKeepDualP = 2;
+
 // What each wavefront of a work-group hands to its neighbours every row: lane 63 the
 // samples that the wavefront on its right takes as its left column's (w_x, w_y, p_x, p_y,
 // r_xx, r_yy, r_xy of the row, r_yy, r_xy of the row below), lane 0 those that the one on
@@ -664,7 +668,7 @@ __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
           const T projected = step_p[c][e] * scale;
-          if (keep) tp.template store<device::kStreaming>(2 * c + e, at, device::narrow<S>(projected));
+          if (keep & kKeepDualP) tp.template store<device::kStreaming>(2 * c + e, at, device::narrow<S>(projected));
           p_next.template store<device::kTemporal>(2 * c + e, at,
                                                    device::narrow<S>(rho * projected + rest * p_here[c][e]));
         }

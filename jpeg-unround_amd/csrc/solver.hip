@@ -1079,7 +1079,8 @@ class Solver {
     steps.radius0 = static_cast<T>(settings_.alpha0);
     steps.rho = rho;
     steps.rest = rest;
-    const int k = keep ? 1 : 0;
+    // p~ is read by the records of TV and by the partial gap's only.
+    const int k = keep ? kKeepTilde | (!tgv_ || plan_.has_partial ? kKeepDualP : 0) : 0;
     if (per_kernel) HIP_CHECK(hipEventRecord(dual_events_.back().begin, s));
     if (C_ == 1) {
       tgv_ ? launch_dual<1, true, false>(steps, k) : launch_dual<1, false, false>(steps, k);
