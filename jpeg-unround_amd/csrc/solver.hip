@@ -405,6 +405,21 @@ __global__ __launch_bounds__(256) void value_kernel(const Component* __restrict_
 // div p~ (from_r = 0) or div(-theta div_2 r~) (from_r = 1) of channel c at (i, j), in
 // binary64: the divergence that the dual values take their xi from. Every load is made
 // first (at clamped positions), the cases chosen after.
+// div g at (i, j) for g = -theta div_2 r~, from div_2 r~ at (i, j) (dx, dy), its y part a
+// row up and its x part a column left (each taken as 0 beyond the canvas).
+__device__ __forceinline__ double g_divergence(double theta, double dx, double dy, double up_y, double left_x, int i,
+                                               int j, int H, int W) {
+  const double gx_here = -theta * dx, gy_here = -theta * dy;
+  const double gy_up = i > 0 ? -theta * up_y : 0.0;
+  const double gx_left = j > 0 ? -theta * left_x : 0.0;
+  const bool top = i == 0, bottom = i == H - 1;
+  const double down_part = (!top && !bottom) ? gy_here - gy_up : (!bottom ? gy_here : (!top ? -gy_up : 0.0));
+  const double across_part = W == 1 ? 0.0 : (j == 0 ? gx_here : (j == W - 1 ? -gx_left : gx_here - gx_left));
+  double div = down_part;
+  div += across_part;
+  return div;
+}
+
 template <typename T, typename S>
 __device__ __forceinline__ double dual_divergence_at(const Fields<T, S>& F, int c, int i, int j, int H, int W,
                                                      int from_r, double theta) {
@@ -418,15 +433,9 @@ __device__ __forceinline__ double dual_divergence_at(const Fields<T, S>& F, int 
   device::tensor_divergence_loaded<double>(rxx, ryy, rxy, i, j, H, W, dx, dy);
   device::tensor_divergence_loaded<double>(rxx, ryy, rxy, i - 1, j, H, W, up_x, up_y);
   device::tensor_divergence_loaded<double>(rxx, ryy, rxy, i, j - 1, H, W, left_x, left_y);
-  const double gx_here = -theta * dx, gy_here = -theta * dy;
-  const double gy_up = i > 0 ? -theta * up_y : 0.0;
-  const double gx_left = j > 0 ? -theta * left_x : 0.0;
-  const bool top = i == 0, bottom = i == H - 1;
-  const double down_part = (!top && !bottom) ? gy_here - gy_up : (!bottom ? gy_here : (!top ? -gy_up : 0.0));
-  const double across_part = W == 1 ? 0.0 : (j == 0 ? gx_here : (j == W - 1 ? -gx_left : gx_here - gx_left));
-  double div = down_part;
-  div += across_part;
-  return div;
+  (void)up_x;
+  (void)left_y;
+  return g_divergence(theta, dx, dy, up_y, left_x, i, j, H, W);
 }
 
 // A lane's G* terms of one strip (and the free samples' terms of its cells), laid out as
@@ -452,22 +461,67 @@ __device__ __forceinline__ double conjugate_terms(const Component& K, const Fiel
   const int top = 8 * ty * dv;
   // The free samples' terms (frames.cpp:616-681; docs/math.md:911-949), <zeta, m> +
   // R |zeta|_1: within the blocks R |xi - mean of its cell|, beyond them 128 xi + R |xi|.
+  //
+  // With r~ (TGV) and a cell shape known at compile time, div_2 r~ is taken once a sample:
+  // the row above's y parts carried down the lane's columns, the x part of the column on
+  // the left from the lane on the left (lane 0 loads its own), the same values the
+  // samples' own loads give.
+  constexpr bool kCarry = DV > 0 && DH > 0;
+  const size_t N = static_cast<size_t>(H) * W;
+  const S* rxx = F.tr + (3 * c) * N;
+  const S* ryy = F.tr + (3 * c + 1) * N;
+  const S* rxy = F.tr + (3 * c + 2) * N;
+  double up_y[kAcross];
+  if (kCarry && from_r) {
+#pragma unroll
+    for (int bb = 0; bb < kAcross; ++bb) {
+      double unused;
+      device::tensor_divergence_loaded<double>(rxx, ryy, rxy, top - 1, first_column + bb, H, W, unused, up_y[bb]);
+    }
+  }
   double column[8], free_terms = 0.0;
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     double xis[kDown * kAcross];
+    if (kCarry && from_r) {
+#pragma unroll
+      for (int a = 0; a < kDown; ++a) {
+        const int i = top + m * dv + a;
+        double dx[kAcross], dy[kAcross], own_left, unused;
+#pragma unroll
+        for (int bb = 0; bb < kAcross; ++bb) {
+          device::tensor_divergence_loaded<double>(rxx, ryy, rxy, i, first_column + bb, H, W, dx[bb], dy[bb]);
+        }
+        device::tensor_divergence_loaded<double>(rxx, ryy, rxy, i, first_column - 1, H, W, own_left, unused);
+        const double from_left = __shfl_up(dx[kAcross - 1], 1, 64);
+        const double left = (threadIdx.x & 63) == 0 ? own_left : from_left;
+#pragma unroll
+        for (int bb = 0; bb < kAcross; ++bb) {
+          const double left_x = bb == 0 ? left : dx[bb - 1];
+          xis[kAcross * a + bb] =
+              gamma * g_divergence(theta, dx[bb], dy[bb], up_y[bb], left_x, i, first_column + bb, H, W);
+          up_y[bb] = dy[bb];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int bb = 0; bb < kAcross; ++bb) {
+#pragma unroll
+        for (int a = 0; a < kDown; ++a) {
+          if (a < dv && bb < dh) {
+            const int i = top + m * dv + a, j = first_column + bb;
+            xis[kAcross * a + bb] = gamma * dual_divergence_at(F, c, i, j, H, W, from_r, theta);
+          }
+        }
+      }
+    }
     double sum = 0.0;
 #pragma unroll
     for (int bb = 0; bb < kAcross; ++bb) {
       double part = 0.0;
 #pragma unroll
       for (int a = 0; a < kDown; ++a) {
-        if (a < dv && bb < dh) {
-          const int i = top + m * dv + a, j = first_column + bb;
-          const double xi = gamma * dual_divergence_at(F, c, i, j, H, W, from_r, theta);
-          xis[kAcross * a + bb] = xi;
-          part = a == 0 ? xi : part + xi;
-        }
+        if (a < dv && bb < dh) part = a == 0 ? xis[kAcross * a + bb] : part + xis[kAcross * a + bb];
       }
       if (bb < dh) sum = bb == 0 ? part : sum + part;
     }
