@@ -754,9 +754,11 @@ class Batch {
     HIP_CHECK(hipEventRecord(whole.begin, s));
     const uint64_t most = most_, every = every_;
     if (every > 0 || most == 0) launch_records(s, 0);
-    int* pinned = nullptr;
-    HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pinned), sizeof(int) * 2 * std::max(M, 1),
+    int* flags = nullptr;
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&flags), sizeof(int) * 2 * std::max(M, 1),
                             hipHostMallocDefault));
+    const std::unique_ptr<int, hipError_t (*)(void*)> held(flags, hipHostFree);
+    int* const pinned = held.get();
     const auto any_of = [M](const int* flags) {
       for (int m = 0; m < M; ++m)
         if (flags[m] != 0) return true;
@@ -824,7 +826,6 @@ class Batch {
       HIP_CHECK(hipMemcpy(logs_host_[m].data(), logs_[m].pointer, sizeof(double) * 5 * capacity,
                           hipMemcpyDeviceToHost));
     }
-    (void)hipHostFree(pinned);
   }
 
   const std::vector<const FrameModel*>& frames_;
