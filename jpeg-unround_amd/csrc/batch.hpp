@@ -626,16 +626,16 @@ class Batch {
         kernel<<<launch.count, 256, 0, s>>>(tiles, slots, state_.active, rho_, rest_, keep);
       };
       if (dv == 1 && dh == 1) {
-        buffers ? go(prox_batch_kernel<T, S, 1, 1, device::BufferPlanes>)
+        buffers ? go(prox_batch_kernel<T, S, 1, 1, device::StreamedBufferPlanes>)
                 : go(prox_batch_kernel<T, S, 1, 1, device::WidePlanes>);
       } else if (dv == 2 && dh == 2) {
-        buffers ? go(prox_batch_kernel<T, S, 2, 2, device::BufferPlanes>)
+        buffers ? go(prox_batch_kernel<T, S, 2, 2, device::StreamedBufferPlanes>)
                 : go(prox_batch_kernel<T, S, 2, 2, device::WidePlanes>);
       } else if (dv == 1 && dh == 2) {
-        buffers ? go(prox_batch_kernel<T, S, 1, 2, device::BufferPlanes>)
+        buffers ? go(prox_batch_kernel<T, S, 1, 2, device::StreamedBufferPlanes>)
                 : go(prox_batch_kernel<T, S, 1, 2, device::WidePlanes>);
       } else {
-        buffers ? go(prox_batch_kernel<T, S, 2, 1, device::BufferPlanes>)
+        buffers ? go(prox_batch_kernel<T, S, 2, 1, device::StreamedBufferPlanes>)
                 : go(prox_batch_kernel<T, S, 2, 1, device::WidePlanes>);
       }
       HIP_CHECK(hipGetLastError());

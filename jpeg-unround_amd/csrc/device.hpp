@@ -435,13 +435,14 @@ constexpr int kRawBufferWord3 = 0x00020000;  // a raw (untyped) buffer resource 
 // Cache policies of the buffer accesses (the instructions' cache-policy bits).
 constexpr int kTemporal = 0, kStreaming = 2;
 
-template <typename X, int POLICY>
+template <typename X, int POLICY, int NEXT>
 struct BufferPlanesOf {
   // A byte offset beyond the resource where its planes span at most 2 GiB: a load there
   // returns zero and a store there is dropped, by the hardware's range check.
   static constexpr uint32_t kOutside = 0x80000000u;
-  // The accesses' cache policy where they ask for none.
-  static constexpr int kPolicy = POLICY;
+  // The accesses' cache policy where they ask for none, and the policy of the stores that
+  // the very next launch reads.
+  static constexpr int kPolicy = POLICY, kNext = NEXT;
   __amdgpu_buffer_rsrc_t resource;
   uint32_t plane_bytes;
   __device__ __forceinline__ BufferPlanesOf(X* base, int count, size_t N)
@@ -551,14 +552,21 @@ struct BufferPlanesOf {
 // the dual step) asks for kTemporal and stays in the caches for it. This took the dual
 // kernel from 161 to 147 us on 2048x2048 RGB 4:2:0 (207 to 196 us on 4096x4096 grey).
 template <typename X>
-using BufferPlanes = BufferPlanesOf<X, kStreaming>;
-// The batch's fields: the default policy throughout. Its short runs lean on the caches for
-// the samples their neighbours read too, and the streaming policy measured 5 % slower there.
+using BufferPlanes = BufferPlanesOf<X, kStreaming, kTemporal>;
+// The batch's fields, whose canvases outgrow the caches many times over. Its dual step's
+// accesses take the default policy: its short runs lean on the caches for the samples
+// their neighbours read too (the streaming policy measured 5 % slower there). The stores
+// the next launch reads stream: they cannot stay in the caches until then (C3 300
+// iterations 0.857 to 0.818 s).
 template <typename X>
-using CachedBufferPlanes = BufferPlanesOf<X, kTemporal>;
+using CachedBufferPlanes = BufferPlanesOf<X, kTemporal, kStreaming>;
+// The batch's prox: streaming throughout, x- too (C3 300 iterations 0.819 to 0.816 s).
+template <typename X>
+using StreamedBufferPlanes = BufferPlanesOf<X, kStreaming, kStreaming>;
 
 template <typename X>
 struct WidePlanes {
+  static constexpr int kPolicy = kTemporal, kNext = kTemporal;  // one policy: the default
   X* base;
   size_t N;
   __device__ __forceinline__ WidePlanes(X* base_, int, size_t N_) : base(base_), N(N_) {}

@@ -208,13 +208,13 @@ __device__ __forceinline__ void prox_tile(const Component* __restrict__ componen
       }
       const uint32_t where = at(top + m * DV + a, first_column);
       if constexpr (DH == 2) {
-        xbar.template store_pair_if<device::kTemporal>(live, 0, where, bars[0], bars[1]);
+        xbar.template store_pair_if<Planes<T>::kNext>(live, 0, where, bars[0], bars[1]);
         x_next.store_pair_if(live, 0, where, relaxed[0], relaxed[1]);
         if (keep) tilde_x.store_pair_if(live, 0, where, tildes[0], tildes[1]);
       } else {
 #pragma unroll
         for (int bb = 0; bb < DH; ++bb) {
-          xbar.template store_if<device::kTemporal>(live, 0, where + bb, bars[bb]);
+          xbar.template store_if<Planes<T>::kNext>(live, 0, where + bb, bars[bb]);
           x_next.store_if(live, 0, where + bb, relaxed[bb]);
           if (keep) tilde_x.store_if(live, 0, where + bb, tildes[bb]);
         }
@@ -474,6 +474,8 @@ __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first
   const uint32_t column = static_cast<uint32_t>(min(j, W - 1));
   const size_t N = static_cast<size_t>(H) * W;
   const Planes<T> xbar(F.xbar, C, N);
+  // The next iterate's p and w, which the next launch reads (r streams).
+  constexpr int kNext = Planes<T>::kNext;
   const Planes<S> p(F.p, 2 * C, N), p_next(F.p_next, 2 * C, N), tp(F.tp, 2 * C, N);
   const Planes<T> w(F.w, 2 * C, N), w_next(F.w_next, 2 * C, N), tw(F.tw, 2 * C, N);
   const Planes<S> r(F.r, 3 * C, N), r_next(F.r_next, 3 * C, N), tr(F.tr, 3 * C, N);
@@ -669,14 +671,14 @@ __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first
         for (int e = 0; e < 2; ++e) {
           const T projected = step_p[c][e] * scale;
           if (keep & kKeepDualP) tp.template store<device::kStreaming>(2 * c + e, at, device::narrow<S>(projected));
-          p_next.template store<device::kTemporal>(2 * c + e, at,
+          p_next.template store<kNext>(2 * c + e, at,
                                                    device::narrow<S>(rho * projected + rest * p_here[c][e]));
         }
         if constexpr (TGV) {
 #pragma unroll
           for (int e = 0; e < 2; ++e) {
             if (keep) tw.template store<device::kStreaming>(2 * c + e, at, tilde_w[c][e]);
-            w_next.template store<device::kTemporal>(2 * c + e, at, rho * tilde_w[c][e] + rest * w_here[c][e]);
+            w_next.template store<kNext>(2 * c + e, at, rho * tilde_w[c][e] + rest * w_here[c][e]);
           }
           T tensor_scale = shared_tensor_scale;
           if constexpr (!COUPLED) {
@@ -688,7 +690,7 @@ __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first
           for (int e = 0; e < 3; ++e) {
             const T projected = step_r[c][e] * tensor_scale;
             if (keep) tr.template store<device::kStreaming>(3 * c + e, at, device::narrow<S>(projected));
-            r_next.store(3 * c + e, at, device::narrow<S>(rho * projected + rest * r_here[c][e]));
+            r_next.template store<device::kStreaming>(3 * c + e, at, device::narrow<S>(rho * projected + rest * r_here[c][e]));
           }
         }
       }
