@@ -36,9 +36,10 @@ unround_status unround_solve_start(const unround_input* input, const unround_set
  * iterations and records for all): results[k] is what unround_solve gives for inputs[k]
  * (the same iterates, records and stopping iteration), to be freed with unround_result_free
  * each. Every input's tiles run in the same launches, a run of iterations between two
- * records is one HIP graph, and the records and stopping tests run on the device. No
- * observer. Where latencies is not null, latencies[k] is the seconds from the batch's first
- * iteration to the end of the run of iterations in which input k stopped. On an error no
+ * records is one HIP graph, and the records and stopping tests run on the device (on a
+ * second stream, beside the next run of iterations). No observer. Where latencies is not
+ * null, latencies[k] is the seconds from the batch's first iteration to the end of the
+ * record at which input k stopped (or of the last run of iterations). On an error no
  * result is made. No counterpart in the reference: the batch of the reference's
  * unround_solve calls (rust/capi/include/unround.h:138-140) a caller would otherwise make. */
 unround_status unround_solve_batch(const unround_input* inputs, size_t count, const unround_settings* settings,

@@ -719,7 +719,7 @@ def solve_batch(images: Sequence[jpegio.Image], settings: Settings | list[str] |
     solve() gives for its image (the same iterates, records and stopping iteration), with
     the tiles of every image in the same launches and the stopping tests on the device. Also
     returns each image's latency: the seconds from the batch's first iteration to the end of
-    the run of iterations in which it stopped. No observer."""
+    the record at which it stopped (or of the last run of iterations). No observer."""
     library = _library()
     handle = _Settings(settings if isinstance(settings, list) else options(settings))
     count = len(images)
