@@ -14,11 +14,13 @@
  * Different settings and results can be used on different threads at once, and one can
  * be read by several.
  *
- * unround_decode reads PNG files (the file layer, include/unround_jpegio.h, on zlib) and
- * reconstructs them within the intervals of their rounded samples on the GPU; a JPEG
- * file needs a JPEG reader, which this library has not yet: unround_decode of one, and
- * unround_project, return UNROUND_ERROR_UNSUPPORTED, and so does unround_main (the
- * command line). The writers of TIFF, PNG and PNM, with the rounding and blue-noise
+ * unround_decode reads JPEG files (this library's own reader of baseline, extended and
+ * progressive Huffman files) and reconstructs their coefficients on the GPU, and reads PNG
+ * files (the file layer, include/unround_jpegio.h, on zlib) and reconstructs them within
+ * the intervals of their rounded samples. unround_project projects a picture onto a JPEG
+ * file's intervals on the GPU. Arithmetic-coded, 12-bit, lossless, hierarchical and CMYK
+ * JPEG files are UNROUND_ERROR_UNSUPPORTED, and so is unround_main (the command line,
+ * which returns 2). The writers of TIFF, PNG and PNM, with the rounding and blue-noise
  * dither of docs/math.md 10, and unround_orient are this library's own.
  */
 #ifndef UNROUND_H
@@ -116,10 +118,12 @@ typedef struct unround_result unround_result;
 typedef int32_t unround_format;
 enum { UNROUND_FORMAT_JPEG = 1, UNROUND_FORMAT_PNG = 2 };
 
-/* rust/capi/include/unround.h:128-134: a PNG file (known by its signature) within the
- * intervals of its rounded samples, by TV or TGV with --mu and --slack alone of the data
- * term, its colours together and its alpha alone; a JPEG file is UNROUND_ERROR_UNSUPPORTED
- * (no JPEG reader). */
+/* rust/capi/include/unround.h:128-134: a JPEG file (known by its signature) read to its
+ * levels, tables and sampling and solved as unround_solve solves them; or a PNG file
+ * within the intervals of its rounded samples, by TV or TGV with --mu and --slack alone of
+ * the data term, its colours together and its alpha alone. A JPEG coding this library does
+ * not read (arithmetic, 12-bit, lossless, hierarchical, CMYK) is
+ * UNROUND_ERROR_UNSUPPORTED. */
 unround_status unround_decode(const uint8_t* data, size_t size, const unround_settings* settings,
                               unround_observer observer, void* user, unround_result** result, char* message,
                               size_t message_size);
@@ -129,7 +133,10 @@ unround_status unround_decode(const uint8_t* data, size_t size, const unround_se
 unround_status unround_solve(const unround_input* input, const unround_settings* settings, unround_observer observer,
                              void* user, unround_result** result, char* message, size_t message_size);
 
-/* rust/capi/include/unround.h:142-150: needs a file's reader; unsupported here. */
+/* rust/capi/include/unround.h:142-150: a picture (height x width x channels
+ * interleaved, RGB for a YCbCr file) projected onto the intervals of a JPEG file's
+ * levels on the GPU, each coefficient's distance outside its interval kept as the
+ * result's excess (docs/math.md 12). */
 unround_status unround_project(const uint8_t* data, size_t size, const unround_settings* settings,
                                const double* picture, uint64_t height, uint64_t width, uint64_t channels,
                                unround_result** result, char* message, size_t message_size);

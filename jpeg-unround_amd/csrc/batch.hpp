@@ -468,8 +468,7 @@ class Batch {
       std::swap(odd[m].F.w, odd[m].F.w_next);
       std::swap(odd[m].F.r, odd[m].F.r_next);
       // The tiles of each kind.
-      const int buffers_prox =
-          v.N_ * sizeof(T) <= (size_t{1} << 31) && 2 * v.N_ * sizeof(S) <= (size_t{1} << 31) ? 1 : 0;
+      const int buffers_prox = v.prox_buffers() ? 1 : 0;
       for (int c = 0; c < v.C_; ++c) {
         const ComponentModel& k = frames_[m]->components[c];
         const bool known = (k.down == 1 || k.down == 2) && (k.across == 1 || k.across == 2);

@@ -9,6 +9,7 @@
 #include <array>
 #include <cmath>
 #include <cstdlib>
+#include <cstdint>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -1139,17 +1140,29 @@ unround_status unround_orient(const double* samples, uint64_t height, uint64_t w
   });
 }
 
+// The samples of one plane of height x width, where three planes of them fit in memory's
+// addresses as binary64 (rust/capi/src/lib.rs:1568-1582 refuses what does not fit).
+static bool plane_samples(uint64_t height, uint64_t width, size_t& count) {
+  uint64_t samples = 0, bytes = 0;
+  if (__builtin_mul_overflow(height, width, &samples)) return false;
+  if (__builtin_mul_overflow(samples, uint64_t{3 * sizeof(double)}, &bytes)) return false;
+  if (bytes > static_cast<uint64_t>(PTRDIFF_MAX)) return false;
+  count = static_cast<size_t>(samples);
+  return true;
+}
+
 unround_status unround_to_rgb(const double* planes, uint64_t height, uint64_t width, double* picture) {
-  if (planes == nullptr || picture == nullptr) return UNROUND_ERROR_ARGUMENT;
-  rgb_of(planes, static_cast<size_t>(height * width), picture);
+  size_t count = 0;
+  if (planes == nullptr || picture == nullptr || !plane_samples(height, width, count)) return UNROUND_ERROR_ARGUMENT;
+  rgb_of(planes, count, picture);
   return UNROUND_OK;
 }
 
 unround_status unround_to_ycbcr(const double* picture, uint64_t height, uint64_t width, double* planes) {
-  if (planes == nullptr || picture == nullptr) return UNROUND_ERROR_ARGUMENT;
+  size_t count = 0;
+  if (planes == nullptr || picture == nullptr || !plane_samples(height, width, count)) return UNROUND_ERROR_ARGUMENT;
   // Y = (kR R + kG G) + kB B, Cb = 128 + (250/443)(B - Y), Cr = 128 + (500/701)(R - Y)
   // (docs/math.md:1035-1040).
-  const size_t count = static_cast<size_t>(height * width);
   const double kCbInverse = 250.0 / 443.0, kCrInverse = 500.0 / 701.0;
   for (size_t k = 0; k < count; ++k) {
     const double r = picture[3 * k], g = picture[3 * k + 1], b = picture[3 * k + 2];

@@ -9,6 +9,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "planes.hpp"
+
 namespace unround {
 namespace device {
 
@@ -429,7 +431,9 @@ __device__ __forceinline__ int ordered(int g, int total) {
 // BufferPlanes reads and writes through one buffer resource: the plane's byte offset is a
 // scalar and the sample's a 32-bit lane offset, so that a row of every plane of the kind
 // shares one address register (64-bit addresses per plane cost two registers each). The
-// planes must span less than 4 GiB. WidePlanes takes 64-bit addresses for larger spans.
+// planes must span at most 2^31 bytes (planes::fits, which the host checks before it
+// chooses an instance), so that kOutside lies beyond the resource. WidePlanes takes 64-bit
+// addresses for larger spans.
 constexpr int kRawBufferWord3 = 0x00020000;  // a raw (untyped) buffer resource on gfx9
 
 // Cache policies of the buffer accesses (the instructions' cache-policy bits).
@@ -437,9 +441,9 @@ constexpr int kTemporal = 0, kStreaming = 2;
 
 template <typename X, int POLICY, int NEXT>
 struct BufferPlanesOf {
-  // A byte offset beyond the resource where its planes span at most 2 GiB: a load there
+  // A byte offset beyond the resource (its planes span at most 2^31 bytes): a load there
   // returns zero and a store there is dropped, by the hardware's range check.
-  static constexpr uint32_t kOutside = 0x80000000u;
+  static constexpr uint32_t kOutside = planes::kOutsideOffset;
   // The accesses' cache policy where they ask for none, and the policy of the stores that
   // the very next launch reads.
   static constexpr int kPolicy = POLICY, kNext = NEXT;

@@ -44,6 +44,7 @@
 #include <vector>
 
 #include "device.hpp"
+#include "planes.hpp"
 #include "solver.hpp"
 
 namespace unround {
@@ -1041,12 +1042,11 @@ class Solver {
     return rows;
   }
 
-  // The dual step addresses each kind of field through one buffer resource when its planes
-  // span less than 4 GiB (w: 2C planes of T, r: 3C planes of S), else with 64-bit addresses.
-  bool narrow_spans() const {
-    const size_t limit = size_t{1} << 32;
-    return 2 * C_ * N_ * sizeof(T) < limit && 3 * C_ * N_ * sizeof(S) < limit;
-  }
+  // The dual step addresses each kind of field through one buffer resource where every
+  // kind spans at most 2^31 bytes (planes::dual_fits), else with 64-bit addresses.
+  bool narrow_spans() const { return planes::dual_buffers(static_cast<size_t>(C_), N_, sizeof(T), sizeof(S)); }
+  // The prox step's choice, per component (x: one plane of T, p: two of S).
+  bool prox_buffers() const { return planes::prox_buffers(N_, sizeof(T), sizeof(S)); }
 
   template <int C, bool TGV, bool COUPLED>
   void launch_dual(const Steps<T>& steps, int keep) {
@@ -1095,9 +1095,9 @@ class Solver {
       const auto prox = [&](auto kernel) {
         kernel<<<grid, 256, 0, s>>>(component(c), fields_, c, H_, W_, steps, rho, rest, live);
       };
-      // Through buffer resources where a component's planes span at most 2 GiB (x: one of T,
-      // p: two of S), else with 64-bit addresses.
-      const bool buffers = N_ * sizeof(T) <= (size_t{1} << 31) && 2 * N_ * sizeof(S) <= (size_t{1} << 31);
+      // Through buffer resources where a component's planes span at most 2^31 bytes, else
+      // with 64-bit addresses.
+      const bool buffers = prox_buffers();
       if (k.down == 1 && k.across == 1) {
         buffers ? prox(prox_kernel<T, S, 1, 1, device::BufferPlanes>) : prox(prox_kernel<T, S, 1, 1, device::WidePlanes>);
       } else if (k.down == 2 && k.across == 2) {

@@ -444,6 +444,11 @@ class Unrolled {
           f->components[0].columns * 8 != f->columns)
         throw Refused("the unrolled solve takes frames whose blocks cover them exactly");
       if (f->components[0].costed) throw Refused("the unrolled solve takes no slack cost");
+      // Its kernels address a frame's planes through buffer resources only.
+      const size_t samples = static_cast<size_t>(f->rows) * f->columns;
+      if (!planes::dual_fits(1, samples, sizeof(float), sizeof(float)) ||
+          !planes::prox_fits(samples, sizeof(float), sizeof(float)))
+        throw Refused("the unrolled solve takes frames of at most 2^31 / 12 samples");
     }
     if (settings_.method != Method::tgv) throw Refused("the unrolled solve is TGV's");
     const int B = static_cast<int>(frames_.size());

@@ -199,7 +199,7 @@ def _library() -> ctypes.CDLL:
     for structure, size in _SIZES:
         if ctypes.sizeof(structure) != size:
             raise RuntimeError(f"{structure.__name__} is {ctypes.sizeof(structure)} bytes, the layer's {size}")
-    library = native._library()
+    library = native._lib()
     message = (ctypes.c_char_p, ctypes.c_size_t)
     library.unround_jpegio_abi_version.restype = ctypes.c_int32
     library.unround_jpegio_libjpeg_version.restype = ctypes.c_char_p
@@ -236,10 +236,10 @@ def check_icc(profile: bytes, channels: int) -> str | None:
 
 
 def _options(max_pixels: int, max_scans: int, warnings_are_errors: bool) -> _Options:
-    if not 0 <= max_pixels < 2**64:
-        raise ValueError(f"max_pixels is {max_pixels}, not 0 to 2**64 - 1")
-    if not 0 <= max_scans < 2**31:
-        raise ValueError(f"max_scans is {max_scans}, not 0 to 2**31 - 1")
+    if max_pixels < 0 or max_pixels >= 1 << 64:
+        raise ValueError(f"max_pixels must fit an unsigned 64-bit count; got {max_pixels}")
+    if max_scans < 0 or max_scans >= 1 << 31:
+        raise ValueError(f"max_scans must fit a non-negative 32-bit int; got {max_scans}")
     return _Options(max_pixels, max_scans, 1 if warnings_are_errors else 0)
 
 

@@ -1,22 +1,36 @@
-"""JPEG-Unround on an AMD Instinct GPU, through its C interface (include/unround.h).
+"""JPEG-Unround on an AMD Instinct GPU, reached through its C interface (include/unround.h).
 
-The same functions, types and errors as the reference's python/src/unround/native.py for
-the reconstruction of components given as arrays: options() and held_options() turn the
-settings into the command line's options, solve() reconstructs an Image on the GPU, and
-its Decoded result, records and errors read as the reference's. The library is the one at
-the path in UNROUND_LIBRARY, or else unround/_native/libunround_capi.so, which
-jpeg-unround_amd/build.py writes; a missing library raises LibraryNotFoundError, and no
-computation falls back to the CPU. ctypes releases the GIL while the library computes.
+The public names, signatures and result types are the reference package's
+(python/src/unround/native.py: options, held_options, decode, project, solve, the writers,
+to_rgb and to_ycbcr, and the Decoded, Problem, History, Solution and Record types), so that
+its callers carry over; this edition adds solve_batch, solve_samples and KernelTimes.
+
+How this module is put together:
+
+* the C layouts of the interface, and its entry points declared from one text table
+  (_ENTRY_POINTS), each line a function's name, its result and its arguments in a small
+  vocabulary of C types (_C_TYPES);
+* _Call, one call of a function that reports a status and a message: it gives the message
+  buffer and turns a failing status into UnroundError;
+* _Marshalled, an Image's components laid out as the interface's input, the arrays kept
+  alive for as long as the object is;
+* _ResultReader, one result handle read into arrays and the result types;
+* the option writer, which turns Settings into the command line's options.
+
+The library is the file that UNROUND_LIBRARY names, or else unround/_native/
+libunround_capi.so, which jpeg-unround_amd/build.py writes. Without it every function
+raises LibraryNotFoundError: nothing here computes on the CPU. ctypes lets go of the GIL
+while the library runs.
 """
 
 from __future__ import annotations
 
 import ctypes
 import enum
-import functools
 import os
+import threading
 import warnings
-from collections.abc import Callable
+from collections.abc import Callable, Iterable
 from dataclasses import dataclass
 from pathlib import Path
 from typing import Final, Sequence
@@ -42,6 +56,7 @@ __all__ = [
     "UnroundWarning",
     "decode",
     "held_options",
+    "library_path",
     "options",
     "oriented",
     "png_bytes",
@@ -50,8 +65,8 @@ __all__ = [
     "solve",
     "solve_batch",
     "solve_samples",
-    "to_rgb",
     "tiff_bytes",
+    "to_rgb",
     "to_ycbcr",
     "version",
 ]
@@ -60,13 +75,16 @@ Array = npt.NDArray[np.float64]
 Observer = Callable[["Record"], "bool | None"]
 
 ABI_VERSION: Final = 5
-
-_MESSAGE_SIZE: Final = 4096
-_FORMAT_PNG: Final = 2
-_BLOCK_SIZE: Final = 64
+MESSAGE_BYTES: Final = 4096
+_PNG_FORMAT: Final = 2
 
 
-class _Record(ctypes.Structure):
+# ---------------------------------------------------------------------------------------
+# The interface's structures (include/unround.h; sizes checked when the library loads).
+
+class CRecord(ctypes.Structure):
+    """unround_record: what the observer is given at each record."""
+
     _fields_ = (
         ("iteration", ctypes.c_uint64),
         ("gap", ctypes.c_double),
@@ -79,28 +97,34 @@ class _Record(ctypes.Structure):
     )
 
 
-class _Component(ctypes.Structure):
+class CComponent(ctypes.Structure):
+    """unround_component: one component's levels, blocks, sampling and table."""
+
     _fields_ = (
         ("levels", ctypes.POINTER(ctypes.c_int16)),
         ("rows", ctypes.c_uint32),
         ("columns", ctypes.c_uint32),
         ("h_samp_factor", ctypes.c_int32),
         ("v_samp_factor", ctypes.c_int32),
-        ("quant_table", ctypes.c_uint16 * _BLOCK_SIZE),
+        ("quant_table", ctypes.c_uint16 * 64),
     )
 
 
-class _Input(ctypes.Structure):
+class CInput(ctypes.Structure):
+    """unround_input: a picture's size, color space and components."""
+
     _fields_ = (
         ("height", ctypes.c_uint32),
         ("width", ctypes.c_uint32),
         ("color_space", ctypes.c_int32),
         ("count", ctypes.c_uint32),
-        ("components", ctypes.POINTER(_Component)),
+        ("components", ctypes.POINTER(CComponent)),
     )
 
 
-class _Start(ctypes.Structure):
+class CStart(ctypes.Structure):
+    """unround_start (include/unround_ext.h): a state to resume from, null where not given."""
+
     _fields_ = (
         ("coefficients", ctypes.POINTER(ctypes.POINTER(ctypes.c_double))),
         ("p", ctypes.POINTER(ctypes.c_double)),
@@ -109,13 +133,16 @@ class _Start(ctypes.Structure):
     )
 
 
-_SIZES: Final = ((_Record, 64), (_Component, 152), (_Input, 24))
+LAYOUT_BYTES: Final = {CRecord: 64, CComponent: 152, CInput: 24}
 
-_OBSERVER: Final = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.POINTER(_Record))
+OBSERVER_TYPE: Final = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.POINTER(CRecord))
 
+
+# ---------------------------------------------------------------------------------------
+# Errors and enumerations.
 
 class Status(enum.IntEnum):
-    """What a function of the interface reports: unround_status."""
+    """What a function of the interface reports (unround_status)."""
 
     OK = 0
     ARGUMENT = 1
@@ -136,7 +163,7 @@ class Stop(enum.IntEnum):
 
 
 class UnroundError(Exception):
-    """Arrays that were not reconstructed: the status of the interface, and its message."""
+    """A call the library refused or could not finish: its status and its message."""
 
     def __init__(self, status: Status, message: str) -> None:
         super().__init__(f"{message} ({status.name.lower()})")
@@ -148,219 +175,312 @@ class UnroundError(Exception):
 
 
 class LibraryNotFoundError(RuntimeError):
-    """The library is not where this module looks for it, or is not of its version."""
+    """No library where this module looks for one, or one of another ABI version."""
 
 
 class UnroundWarning(UserWarning):
-    """A warning of the library."""
+    """A warning the library gave with a result."""
 
 
-def _declare(library: ctypes.CDLL, name: str, result: object, *arguments: object) -> None:
-    function = getattr(library, name)
-    function.argtypes = arguments
-    function.restype = result
+def status_error(status: int, text: str) -> UnroundError:
+    """The error for a status other than OK; a status outside the enumeration is INTERNAL."""
+    known = {member.value for member in Status}
+    return UnroundError(Status(status) if status in known else Status.INTERNAL, text)
 
 
-def _declare_all(library: ctypes.CDLL) -> None:
-    size, handle, u64, i32 = ctypes.c_size_t, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32
-    doubles, u64_out, i32_out = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64), ctypes.POINTER(i32)
-    strings = ctypes.POINTER(ctypes.c_char_p)
-    message = (ctypes.c_char_p, size)
-    made = (_OBSERVER, handle, ctypes.POINTER(handle), *message)
-    _declare(library, "unround_abi_version", i32)
-    _declare(library, "unround_version", ctypes.c_char_p)
-    _declare(library, "unround_settings_new", handle, size, strings, *message)
-    _declare(library, "unround_settings_free", None, handle)
-    _declare(library, "unround_settings_options", size, handle, ctypes.c_char_p, size)
-    _declare(library, "unround_decode", i32, ctypes.c_char_p, size, handle, *made)
-    _declare(library, "unround_solve", i32, ctypes.POINTER(_Input), handle, *made)
-    _declare(library, "unround_solve_start", i32, ctypes.POINTER(_Input), handle, ctypes.POINTER(_Start), *made)
-    _declare(library, "unround_solve_batch", i32, ctypes.POINTER(_Input), size, handle, ctypes.POINTER(handle), doubles,
-             *message)
-    _declare(library, "unround_solve_samples", i32, ctypes.POINTER(ctypes.c_uint16), u64, u64, u64, ctypes.c_uint32,
-             handle, *made)
-    _declare(library, "unround_result_alpha_solver", i32, handle, u64_out, i32_out)
-    _declare(library, "unround_result_alpha_record_iterations", u64_out, handle, u64_out)
-    _declare(library, "unround_result_alpha_record_values", doubles, handle, i32, u64_out)
-    _declare(library, "unround_result_free", None, handle)
-    _declare(library, "unround_result_picture", doubles, handle, u64_out, u64_out, u64_out)
-    _declare(library, "unround_result_planes", doubles, handle, u64_out)
-    _declare(library, "unround_result_color_space", i32, handle)
-    _declare(library, "unround_result_canvas", doubles, handle, u64_out, u64_out, u64_out)
-    _declare(library, "unround_result_components", u64, handle)
-    _declare(library, "unround_result_coefficients", doubles, handle, u64, u64_out)
-    _declare(library, "unround_result_problem", doubles, handle, u64, i32, u64_out)
-    _declare(library, "unround_result_component", i32, handle, u64, u64_out, u64_out, u64_out, u64_out)
-    _declare(library, "unround_result_solver", i32, handle, u64_out, i32_out)
-    _declare(library, "unround_result_record_iterations", u64_out, handle, u64_out)
-    _declare(library, "unround_result_record_values", doubles, handle, i32, u64_out)
-    _declare(library, "unround_result_kernel_times", i32, handle, doubles, size)
-    _declare(library, "unround_result_exif_orientation", i32, handle)
-    _declare(library, "unround_result_icc_profile", ctypes.POINTER(ctypes.c_uint8), handle, u64_out)
-    _declare(library, "unround_result_excess", doubles, handle, u64, u64_out)
-    _declare(library, "unround_project", i32, ctypes.c_char_p, size, handle, doubles, u64, u64, u64,
-             ctypes.POINTER(handle), *message)
-    _declare(library, "unround_result_format", i32, handle)
-    _declare(library, "unround_result_bits", i32, handle)
-    _declare(library, "unround_result_warning", ctypes.c_char_p, handle)
-    bytes_out = (ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8)), u64_out, *message)
-    icc = (ctypes.c_char_p, u64)
-    _declare(library, "unround_tiff", i32, doubles, u64, u64, u64, i32, *icc, *bytes_out)
-    _declare(library, "unround_png", i32, doubles, u64, u64, u64, i32, i32, i32, *icc, *bytes_out)
-    _declare(library, "unround_pnm", i32, doubles, u64, u64, u64, i32, i32, *bytes_out)
-    _declare(library, "unround_bytes_free", None, ctypes.POINTER(ctypes.c_uint8), u64)
-    _declare(library, "unround_orient", i32, doubles, u64, u64, u64, i32, doubles, u64_out, u64_out, *message)
-    _declare(library, "unround_to_rgb", i32, doubles, u64, u64, doubles)
-    _declare(library, "unround_to_ycbcr", i32, doubles, u64, u64, doubles)
+# ---------------------------------------------------------------------------------------
+# The entry points.
+
+_C_TYPES: Final[dict[str, object]] = {
+    "void": None,
+    "i32": ctypes.c_int32,
+    "u32": ctypes.c_uint32,
+    "u64": ctypes.c_uint64,
+    "size": ctypes.c_size_t,
+    "f64": ctypes.c_double,
+    "text": ctypes.c_char_p,
+    "texts": ctypes.POINTER(ctypes.c_char_p),
+    "handle": ctypes.c_void_p,
+    "handle*": ctypes.POINTER(ctypes.c_void_p),
+    "address": ctypes.c_void_p,
+    "f64*": ctypes.POINTER(ctypes.c_double),
+    "u64*": ctypes.POINTER(ctypes.c_uint64),
+    "i32*": ctypes.POINTER(ctypes.c_int32),
+    "u8*": ctypes.POINTER(ctypes.c_uint8),
+    "u8**": ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8)),
+    "u16*": ctypes.POINTER(ctypes.c_uint16),
+    "input*": ctypes.POINTER(CInput),
+    "start*": ctypes.POINTER(CStart),
+    "observer": OBSERVER_TYPE,
+}
+
+# name, result, arguments; "message" stands for the pair (char* message, size_t size).
+_ENTRY_POINTS: Final = """
+unround_abi_version                     i32
+unround_version                         text
+unround_settings_new                    handle  size texts message
+unround_settings_free                   void    handle
+unround_settings_options                size    handle text size
+unround_decode                          i32     text size handle observer handle handle* message
+unround_solve                           i32     input* handle observer handle handle* message
+unround_solve_start                     i32     input* handle start* observer handle handle* message
+unround_solve_batch                     i32     input* size handle handle* f64* message
+unround_solve_samples                   i32     u16* u64 u64 u64 u32 handle observer handle handle* message
+unround_project                         i32     text size handle f64* u64 u64 u64 handle* message
+unround_result_free                     void    handle
+unround_result_picture                  f64*    handle u64* u64* u64*
+unround_result_planes                   f64*    handle u64*
+unround_result_canvas                   f64*    handle u64* u64* u64*
+unround_result_color_space              i32     handle
+unround_result_components               u64     handle
+unround_result_component                i32     handle u64 u64* u64* u64* u64*
+unround_result_coefficients             f64*    handle u64 u64*
+unround_result_problem                  f64*    handle u64 i32 u64*
+unround_result_excess                   f64*    handle u64 u64*
+unround_result_solver                   i32     handle u64* i32*
+unround_result_record_iterations        u64*    handle u64*
+unround_result_record_values            f64*    handle i32 u64*
+unround_result_alpha_solver             i32     handle u64* i32*
+unround_result_alpha_record_iterations  u64*    handle u64*
+unround_result_alpha_record_values      f64*    handle i32 u64*
+unround_result_kernel_times             i32     handle f64* size
+unround_result_exif_orientation         i32     handle
+unround_result_icc_profile              u8*     handle u64*
+unround_result_format                   i32     handle
+unround_result_bits                     i32     handle
+unround_result_warning                  text    handle
+unround_tiff                            i32     f64* u64 u64 u64 i32 text u64 u8** u64* message
+unround_png                             i32     f64* u64 u64 u64 i32 i32 i32 text u64 u8** u64* message
+unround_pnm                             i32     f64* u64 u64 u64 i32 i32 u8** u64* message
+unround_bytes_free                      void    u8* u64
+unround_orient                          i32     f64* u64 u64 u64 i32 f64* u64* u64* message
+unround_to_rgb                          i32     f64* u64 u64 f64*
+unround_to_ycbcr                        i32     f64* u64 u64 f64*
+"""
 
 
-@functools.cache
-def _library() -> ctypes.CDLL:
-    for structure, wanted in _SIZES:
-        if ctypes.sizeof(structure) != wanted:
-            raise RuntimeError(f"{structure.__name__} is {ctypes.sizeof(structure)} bytes, and the interface's is {wanted}")
-    path = _loading.configured("UNROUND_LIBRARY") or _loading.bundled(_loading.LIBRARY)
-    if path is None or not path.is_file():
-        where = f"UNROUND_LIBRARY names {path}, where there is no file" if path else "the package holds no library"
-        raise LibraryNotFoundError(f"{where}: build it with python jpeg-unround_amd/build.py")
-    library = ctypes.CDLL(str(path))
-    _declare_all(library)
-    found = int(library.unround_abi_version())
-    if found != ABI_VERSION:
-        raise LibraryNotFoundError(f"{path} is of ABI version {found}, and this module of version {ABI_VERSION}")
-    return library
+def declare(library: ctypes.CDLL, table: str) -> None:
+    """Declares every function of a table written as _ENTRY_POINTS is."""
+    for line in table.strip().splitlines():
+        name, result, *arguments = line.split()
+        words: list[str] = []
+        for argument in arguments:
+            words += ["text", "size"] if argument == "message" else [argument]
+        function = getattr(library, name)
+        function.restype = _C_TYPES[result]
+        function.argtypes = tuple(_C_TYPES[word] for word in words)
+
+
+_loaded: ctypes.CDLL | None = None
+_loading_lock = threading.Lock()
+
+
+def _lib() -> ctypes.CDLL:
+    """The library, loaded and declared at the first call; checked against this module's
+    layouts and ABI version."""
+    global _loaded
+    with _loading_lock:
+        if _loaded is not None:
+            return _loaded
+        for layout, wanted in LAYOUT_BYTES.items():
+            if ctypes.sizeof(layout) != wanted:
+                raise RuntimeError(f"this Python's {layout.__name__} takes {ctypes.sizeof(layout)} bytes; "
+                                   f"the C layout takes {wanted}")
+        path = _loading.configured("UNROUND_LIBRARY") or _loading.bundled(_loading.LIBRARY)
+        if path is None:
+            raise LibraryNotFoundError("no libunround_capi.so in unround/_native and no UNROUND_LIBRARY: "
+                                       "run python jpeg-unround_amd/build.py")
+        if not path.is_file():
+            raise LibraryNotFoundError(f"UNROUND_LIBRARY is {path}, which is not a file")
+        library = ctypes.CDLL(str(path))
+        declare(library, _ENTRY_POINTS)
+        abi = int(library.unround_abi_version())
+        if abi != ABI_VERSION:
+            raise LibraryNotFoundError(f"{path} speaks ABI {abi}; this module speaks ABI {ABI_VERSION}")
+        _loaded = library
+        return library
 
 
 def library_path() -> str:
-    """The path of the library this module loaded."""
-    return str(_library()._name)
+    """The file of the library in use."""
+    return str(_lib()._name)
 
 
 def version() -> str:
-    """The version of JPEG-Unround and of the implementation, as the library says them."""
-    return _library().unround_version().decode("utf-8")
+    """The library's own description of itself."""
+    return _lib().unround_version().decode("utf-8")
 
 
-def _number(value: float) -> str:
+class _Call:
+    """One call of a function that reports a status, with a message buffer of its own."""
+
+    def __init__(self) -> None:
+        self.buffer = ctypes.create_string_buffer(MESSAGE_BYTES)
+
+    @property
+    def message(self) -> tuple[ctypes.Array, int]:
+        return self.buffer, MESSAGE_BYTES
+
+    @property
+    def text(self) -> str:
+        return self.buffer.value.decode("utf-8", "replace")
+
+    def check(self, status: int) -> None:
+        if int(status) != Status.OK:
+            raise status_error(int(status), self.text)
+
+
+# ---------------------------------------------------------------------------------------
+# Settings as options.
+
+def _shortest(value: float) -> str:
+    """The shortest decimal that reads back as the same double."""
     return repr(float(value))
 
 
-def _data_options(data: DataTerm | tuple[DataTerm, ...]) -> list[str]:
-    """The options of G: one value where every component has it, or one for each
-    (python/src/unround/native.py:245-271)."""
-    terms = (data,) if isinstance(data, DataTerm) else data
-    fields: tuple[tuple[str, Callable[[DataTerm], str]], ...] = (
-        ("--mu", lambda term: "rule" if term.mu is None else _number(term.mu)),
-        ("--mu-scale", lambda term: _number(term.mu_scale)),
-        ("--mu-power", lambda term: _number(term.mu_power)),
-        ("--mu-chroma", lambda term: _number(term.mu_chroma)),
-        ("--weight-power", lambda term: _number(term.power)),
-        ("--dc-weight", lambda term: _number(term.dc_weight)),
-        ("--centres", lambda term: term.centres),
-        ("--slack", lambda term: _number(term.slack)),
-        ("--slack-cost", lambda term: _number(term.slack_cost)),
-        ("--flat-weight", lambda term: _number(term.flat_weight)),
-        ("--flat-levels", lambda term: str(term.flat_levels)),
-        ("--flat-dc", lambda term: "any" if term.flat_dc is None else str(term.flat_dc)),
-    )
-    result = []
-    for name, value in fields:
-        values = [value(term) for term in terms]
-        result += [name, values[0] if len(set(values)) == 1 else ",".join(values)]
-    return result
+def _maybe(value: float | None) -> str | None:
+    return None if value is None else _shortest(value)
 
 
-def _pdhg_options(solver: PdhgOptions) -> list[str]:
-    result = [
-        "--relative-tolerance", _number(solver.relative_tolerance),
-        "--partial-tolerance", _number(solver.partial_tolerance),
-        "--step-product", _number(solver.step_product),
-        "--record-every", str(solver.record_every),
-        "--free-radius", _number(solver.free_radius),
-        "--threads", str(solver.threads),
-        "--precision", solver.precision,
-    ]
-    if solver.iterations is not None:
-        result += ["--iterations", str(solver.iterations)]
-    for name, value in (
-        ("--tolerance", solver.tolerance),
-        ("--step-ratio", solver.step_ratio),
-        ("--relaxation", solver.relaxation),
-        ("--norm-squared", solver.norm_squared),
-        ("--partial-radius", solver.partial_radius),
-    ):
+# Each option of G, and how a DataTerm gives its value. Several terms (one per component)
+# write one value where they agree and a comma-separated list where they do not.
+_DATA_TERM_OPTIONS: Final[tuple[tuple[str, Callable[[DataTerm], str]], ...]] = (
+    ("--mu", lambda t: "rule" if t.mu is None else _shortest(t.mu)),
+    ("--mu-scale", lambda t: _shortest(t.mu_scale)),
+    ("--mu-power", lambda t: _shortest(t.mu_power)),
+    ("--mu-chroma", lambda t: _shortest(t.mu_chroma)),
+    ("--weight-power", lambda t: _shortest(t.power)),
+    ("--dc-weight", lambda t: _shortest(t.dc_weight)),
+    ("--centres", lambda t: t.centres),
+    ("--slack", lambda t: _shortest(t.slack)),
+    ("--slack-cost", lambda t: _shortest(t.slack_cost)),
+    ("--flat-weight", lambda t: _shortest(t.flat_weight)),
+    ("--flat-levels", lambda t: str(int(t.flat_levels))),
+    ("--flat-dc", lambda t: "any" if t.flat_dc is None else str(int(t.flat_dc))),
+)
+
+# The primal-dual method's options: always written, or written where not None.
+_PDHG_ALWAYS: Final[tuple[tuple[str, Callable[[PdhgOptions], str]], ...]] = (
+    ("--relative-tolerance", lambda o: _shortest(o.relative_tolerance)),
+    ("--partial-tolerance", lambda o: _shortest(o.partial_tolerance)),
+    ("--step-product", lambda o: _shortest(o.step_product)),
+    ("--record-every", lambda o: str(int(o.record_every))),
+    ("--free-radius", lambda o: _shortest(o.free_radius)),
+    ("--threads", lambda o: str(int(o.threads))),
+    ("--precision", lambda o: o.precision),
+)
+_PDHG_GIVEN: Final[tuple[tuple[str, Callable[[PdhgOptions], str | None]], ...]] = (
+    ("--iterations", lambda o: None if o.iterations is None else str(int(o.iterations))),
+    ("--tolerance", lambda o: _maybe(o.tolerance)),
+    ("--step-ratio", lambda o: _maybe(o.step_ratio)),
+    ("--relaxation", lambda o: _maybe(o.relaxation)),
+    ("--norm-squared", lambda o: _maybe(o.norm_squared)),
+    ("--partial-radius", lambda o: _maybe(o.partial_radius)),
+)
+_SUBGRADIENT_OPTIONS: Final[tuple[tuple[str, Callable[[SubgradientOptions], str]], ...]] = (
+    ("--subgradient-iterations", lambda o: str(int(o.iterations))),
+    ("--subgradient-record-every", lambda o: str(int(o.record_every))),
+    ("--subgradient-step", lambda o: _shortest(o.step)),
+    ("--subgradient-decay", lambda o: _shortest(o.decay)),
+)
+
+
+def _pairs(items: Iterable[tuple[str, str | None]]) -> list[str]:
+    out: list[str] = []
+    for name, value in items:
         if value is not None:
-            result += [name, _number(value)]
+            out += [name, value]
+    return out
+
+
+def _data_term_options(data: DataTerm | tuple[DataTerm, ...]) -> list[str]:
+    terms = (data,) if isinstance(data, DataTerm) else tuple(data)
+    written = []
+    for name, value_of in _DATA_TERM_OPTIONS:
+        values = [value_of(term) for term in terms]
+        written.append((name, values[0] if all(v == values[0] for v in values) else ",".join(values)))
+    return _pairs(written)
+
+
+def _settings_options(settings: Settings) -> list[str]:
+    model: TV | TGV = settings.tgv if settings.method == "tgv" else settings.tv
+    out = _pairs((("--method", settings.method), ("--alpha", _shortest(settings.tv.alpha)),
+                  ("--alpha1", _shortest(settings.tgv.alpha1)), ("--alpha0", _shortest(settings.tgv.alpha0))))
+    if model.channel_weights is not None:
+        out += ["--channel-weights", ",".join(_shortest(weight) for weight in model.channel_weights)]
+    out += ["--channels", "coupled" if model.coupled else "apart"]
+    out += _data_term_options(settings.data)
+    solver = settings.pdhg
+    out += _pairs((name, value_of(solver)) for name, value_of in _PDHG_ALWAYS)
+    out += _pairs((name, value_of(solver)) for name, value_of in _PDHG_GIVEN)
     if not solver.scale_with_weight:
-        result.append("--no-weight-scaling")
-    return result
+        out.append("--no-weight-scaling")
+    out += _pairs((name, value_of(settings.subgradient)) for name, value_of in _SUBGRADIENT_OPTIONS)
+    if not settings.subgradient.momentum:
+        out.append("--no-momentum")
+    return out
 
 
-def _subgradient_options(method: SubgradientOptions) -> list[str]:
-    result = [
-        "--subgradient-iterations", str(method.iterations),
-        "--subgradient-record-every", str(method.record_every),
-        "--subgradient-step", _number(method.step),
-        "--subgradient-decay", _number(method.decay),
-    ]
-    if not method.momentum:
-        result.append("--no-momentum")
-    return result
-
-
-def options(settings: Settings | None = None, *, max_pixels: int = 0, max_scans: int = 0,
+def options(settings: Settings | list[str] | None = None, *, max_pixels: int = 0, max_scans: int = 0,
             warnings_are_errors: bool = False) -> list[str]:
-    """The command line's options that are the settings (python/src/unround/native.py:321-353),
-    with the limits of reading a file."""
-    reading = ([f"--max-pixels={max_pixels}"] if max_pixels else []) + (
-        [f"--max-scans={max_scans}"] if max_scans else []) + (["--warnings-are-errors"] if warnings_are_errors else [])
+    """The command line's options that are these settings, then the limits of reading a
+    file (0: the library's own). The channels' weights and coupling written are the
+    method's model's: TGV's for TGV, TV's otherwise. A list of options is taken as it is."""
+    if max_pixels < 0 or max_scans < 0:
+        raise ValueError(f"limits of reading cannot be negative: max_pixels={max_pixels}, max_scans={max_scans}")
     if isinstance(settings, list):
-        return settings + reading
-    settings = Settings() if settings is None else settings
-    weights: TV | TGV = settings.tgv if settings.method == "tgv" else settings.tv
-    result = ["--method", settings.method, "--alpha", _number(settings.tv.alpha)]
-    result += ["--alpha1", _number(settings.tgv.alpha1), "--alpha0", _number(settings.tgv.alpha0)]
-    if weights.channel_weights is not None:
-        result += ["--channel-weights", ",".join(_number(value) for value in weights.channel_weights)]
-    result += ["--channels", "coupled" if weights.coupled else "apart"]
-    result += _data_options(settings.data)
-    result += _pdhg_options(settings.pdhg)
-    result += _subgradient_options(settings.subgradient)
-    return result + reading
+        out = list(settings)
+    else:
+        out = _settings_options(Settings() if settings is None else settings)
+    if max_pixels:
+        out += ["--max-pixels", str(int(max_pixels))]
+    if max_scans:
+        out += ["--max-scans", str(int(max_scans))]
+    if warnings_are_errors:
+        out.append("--warnings-are-errors")
+    return out
 
 
-class _Settings:
-    """Settings of the interface, made from options, and freed with the object."""
+class SettingsHandle:
+    """The library's settings made from options; freed with this object."""
 
-    def __init__(self, given: list[str]) -> None:
-        self._library = _library()
-        encoded = [option.encode("utf-8") for option in given]
-        array = (ctypes.c_char_p * max(len(encoded), 1))(*encoded)
-        message = ctypes.create_string_buffer(_MESSAGE_SIZE)
-        self.handle: int | None = self._library.unround_settings_new(len(encoded), array, message, _MESSAGE_SIZE)
+    def __init__(self, given: Sequence[str]) -> None:
+        self._library = _lib()
+        words = [word.encode("utf-8") for word in given]
+        array = (ctypes.c_char_p * max(1, len(words)))(*words)
+        call = _Call()
+        self.handle = self._library.unround_settings_new(len(words), array, *call.message)
         if not self.handle:
-            raise UnroundError(Status.OPTIONS, message.value.decode("utf-8", errors="replace"))
+            raise UnroundError(Status.OPTIONS, call.text)
+
+    @classmethod
+    def of(cls, settings: Settings | list[str] | None) -> "SettingsHandle":
+        return cls(options(settings))
+
+    def held(self) -> list[str]:
+        size = int(self._library.unround_settings_options(self.handle, None, 0))
+        room = ctypes.create_string_buffer(size + 1)
+        self._library.unround_settings_options(self.handle, room, size + 1)
+        return room.value.decode("ascii").split(" ")
 
     def __del__(self) -> None:
-        handle = getattr(self, "handle", None)
-        if handle:
-            self._library.unround_settings_free(handle)
+        if getattr(self, "handle", None):
+            self._library.unround_settings_free(self.handle)
             self.handle = None
 
 
 def held_options(settings: Settings | list[str] | None = None) -> list[str]:
-    """The settings as the library holds them: every value, as --name=value options."""
-    library = _library()
-    handle = _Settings(settings if isinstance(settings, list) else options(settings))
-    length = int(library.unround_settings_options(handle.handle, None, 0))
-    buffer = ctypes.create_string_buffer(length + 1)
-    library.unround_settings_options(handle.handle, buffer, length + 1)
-    return buffer.value.decode("ascii").split(" ")
+    """Every value the library holds for these settings, as --name=value options."""
+    return SettingsHandle.of(settings).held()
 
+
+# ---------------------------------------------------------------------------------------
+# Result types.
 
 @dataclass(frozen=True, slots=True, eq=False)
 class Record:
-    """A record of the solver, as an observer sees it (gap per sample; canvas (C, H, W))."""
+    """A record as an observer receives it: the gap per sample, the primal and dual values,
+    and a copy of the canvas (C, H, W)."""
 
     iteration: int
     gap: float
@@ -371,7 +491,8 @@ class Record:
 
 @dataclass(frozen=True, slots=True, eq=False)
 class Problem:
-    """A component's model: cell ratio, interval ends and centres (rows, columns, 8, 8), steps and weights (8, 8)."""
+    """One component's model: the canvas's samples per component sample (down, across), the
+    intervals' ends and the centres (rows, columns, 8, 8), the steps and weights (8, 8)."""
 
     ratio: tuple[int, int]
     lower: Array
@@ -383,7 +504,8 @@ class Problem:
 
 @dataclass(frozen=True, slots=True, eq=False)
 class History:
-    """The solver's records: iterations, seconds, primal and dual values, TGV's scaling and partial gap."""
+    """The records: the iteration of each, the seconds to it, the primal and dual values,
+    TGV's scaling theta and the partial gap (NaN where not taken)."""
 
     iterations: npt.NDArray[np.int64]
     seconds: Array
@@ -394,12 +516,13 @@ class History:
 
     @property
     def gap(self) -> Array:
+        """Primal less dual at each record."""
         return np.asarray(self.primal - self.dual, dtype=np.float64)
 
 
 @dataclass(frozen=True, slots=True, eq=False)
 class Solution:
-    """What the solver did: its iterations, why it stopped, and its records."""
+    """The solver's iterations, the reason it stopped, and its records."""
 
     iterations: int
     stop: Stop
@@ -407,13 +530,14 @@ class Solution:
 
     @property
     def converged(self) -> bool:
+        """True where a tolerance stopped the solver."""
         return self.stop is Stop.CONVERGED
 
 
 @dataclass(frozen=True, slots=True)
 class KernelTimes:
-    """HIP-event durations of a solve's kernels: UNROUND_TIMING=1 during the solve times the
-    runs of iterations between records only, UNROUND_TIMING=2 also every launch."""
+    """HIP-event durations of a solve's kernels. UNROUND_TIMING=1 during the solve times the
+    runs of iterations between records; UNROUND_TIMING=2 also every launch."""
 
     primal_seconds: float
     primal_launches: int
@@ -425,7 +549,11 @@ class KernelTimes:
 
 @dataclass(frozen=True, slots=True, eq=False)
 class Decoded:
-    """Arrays reconstructed on the GPU, as the reference's Decoded (python/src/unround/native.py:520-560)."""
+    """A reconstruction. picture: (height, width) or (height, width, C), binary64, JFIF's
+    RGB of a YCbCr file; planes: (C, height, width); canvas: the solver's whole (C, H, W);
+    coefficients and problems per component (none of a PNG file); result: the solver's
+    (None for the decoder of the centres or a projection); alpha_result: a PNG file's
+    alpha's; excess: a projection's distances outside the intervals, in steps."""
 
     picture: Array
     planes: Array
@@ -443,423 +571,433 @@ class Decoded:
     kernel_times: KernelTimes | None = None
 
 
-def _copy(pointer: object, shape: tuple[int, ...]) -> Array:
-    count = int(np.prod(shape, dtype=np.int64))
-    if count == 0:
+def _floats(pointer: object, shape: tuple[int, ...], what: str, count: int | None = None) -> Array:
+    """A copy of the doubles at a pointer of the library as an array of `shape`; `count` is
+    how many the library says it holds."""
+    size = int(np.prod(shape, dtype=np.int64))
+    if count is not None and int(count) != size:
+        raise RuntimeError(f"{what}: the library holds {count} values, and shape {shape} needs {size}")
+    if size == 0:
         return np.zeros(shape, dtype=np.float64)
     if not pointer:
-        raise RuntimeError("the library gave no values where it has some")
-    values = np.ctypeslib.as_array(pointer, shape=(count,))
-    return np.array(values, dtype=np.float64, copy=True).reshape(shape)
+        raise RuntimeError(f"{what}: the library returned no pointer for {size} values")
+    return np.ctypeslib.as_array(pointer, shape=(size,)).astype(np.float64, copy=True).reshape(shape)
 
 
-def _checked(pointer: object, count: ctypes.c_uint64, shape: tuple[int, ...]) -> Array:
-    if int(count.value) != int(np.prod(shape, dtype=np.int64)):
-        raise RuntimeError(f"the library gave {count.value} values where {shape} holds {int(np.prod(shape))}")
-    return _copy(pointer, shape)
+class _ResultReader:
+    """One result handle of the library, read into this module's types."""
 
+    def __init__(self, library: ctypes.CDLL, handle: ctypes.c_void_p) -> None:
+        self.lib = library
+        self.handle = handle
 
-def _problem(library: ctypes.CDLL, result: ctypes.c_void_p, index: int) -> tuple[Array, Problem]:
-    rows, columns, down, across = (ctypes.c_uint64() for _ in range(4))
-    status = library.unround_result_component(result, index, ctypes.byref(rows), ctypes.byref(columns),
-                                              ctypes.byref(down), ctypes.byref(across))
-    if status != Status.OK:
-        raise RuntimeError(f"the result has no component {index}")
-    shape = (int(rows.value), int(columns.value), 8, 8)
-    count = ctypes.c_uint64()
-    coefficients = _checked(library.unround_result_coefficients(result, index, ctypes.byref(count)), count, shape)
-    arrays = []
-    for which, wanted in ((0, shape), (1, shape), (2, shape), (3, (8, 8)), (4, (8, 8))):
-        arrays.append(_checked(library.unround_result_problem(result, index, which, ctypes.byref(count)), count, wanted))
-    lower, upper, centres, steps, weights = arrays
-    return coefficients, Problem((int(down.value), int(across.value)), lower, upper, centres, steps, weights)
-
-
-def _solution(library: ctypes.CDLL, result: ctypes.c_void_p, *, alpha: bool = False) -> Solution | None:
-    """The solver of the result, or of its alpha (python/src/unround/native.py:548-575)."""
-    solver, recorded_at, values_of = (
-        (library.unround_result_alpha_solver, library.unround_result_alpha_record_iterations,
-         library.unround_result_alpha_record_values) if alpha else
-        (library.unround_result_solver, library.unround_result_record_iterations, library.unround_result_record_values))
-    iterations, stop = ctypes.c_uint64(), ctypes.c_int32()
-    if not solver(result, ctypes.byref(iterations), ctypes.byref(stop)):
-        return None
-    count = ctypes.c_uint64()
-    recorded = recorded_at(result, ctypes.byref(count))
-    size = int(count.value)
-    steps = np.array(np.ctypeslib.as_array(recorded, shape=(size,)), dtype=np.int64) if size else np.zeros(0, np.int64)
-    values = [_checked(values_of(result, which, ctypes.byref(count)), count, (size,)) for which in range(5)]
-    seconds, primal, dual, scaling, partial_gap = values
-    return Solution(int(iterations.value), Stop(int(stop.value)), History(steps, seconds, primal, dual, scaling,
-                                                                          partial_gap))
-
-
-def _times(library: ctypes.CDLL, result: ctypes.c_void_p) -> KernelTimes | None:
-    values = (ctypes.c_double * 6)()
-    if not library.unround_result_kernel_times(result, values, 6):
-        return None
-    return KernelTimes(values[0], int(values[1]), values[2], int(values[3]), values[4], int(values[5]))
-
-
-def _decoded(library: ctypes.CDLL, result: ctypes.c_void_p) -> Decoded:
-    height, width, channels = (ctypes.c_uint64() for _ in range(3))
-    samples = library.unround_result_picture(result, ctypes.byref(height), ctypes.byref(width), ctypes.byref(channels))
-    rows, columns, depth = int(height.value), int(width.value), int(channels.value)
-    picture = _copy(samples, (rows, columns) if depth == 1 else (rows, columns, depth))
-    planes_count, canvas_channels, canvas_height, canvas_width = (ctypes.c_uint64() for _ in range(4))
-    canvas_pointer = library.unround_result_canvas(result, ctypes.byref(canvas_channels), ctypes.byref(canvas_height),
-                                                   ctypes.byref(canvas_width))
-    planes_pointer = library.unround_result_planes(result, ctypes.byref(planes_count))
-    count = int(canvas_channels.value)
-    planes = _checked(planes_pointer, planes_count, (count, rows, columns))
-    canvas = _copy(canvas_pointer, (count, int(canvas_height.value), int(canvas_width.value)))
-    components = [_problem(library, result, index) for index in range(int(library.unround_result_components(result)))]
-    excess = []
-    for index, (coefficients, _) in enumerate(components):
+    def _counted(self, function: Callable, *arguments: object) -> tuple[object, int]:
         count = ctypes.c_uint64()
-        pointer = library.unround_result_excess(result, index, ctypes.byref(count))
-        if not pointer:
-            break
-        excess.append(_checked(pointer, count, coefficients.shape))
-    profile_size = ctypes.c_uint64()
-    profile = library.unround_result_icc_profile(result, ctypes.byref(profile_size))
-    warning = (library.unround_result_warning(result) or b"").decode("utf-8", "replace")
-    if warning:
-        warnings.warn(warning, UnroundWarning, stacklevel=4)
-    return Decoded(
-        picture=picture,
-        planes=planes,
-        canvas=canvas,
-        color_space=jpegio.ColorSpace(int(library.unround_result_color_space(result))),
-        coefficients=tuple(coefficients for coefficients, _ in components),
-        problems=tuple(problem for _, problem in components),
-        result=_solution(library, result),
-        icc_profile=ctypes.string_at(profile, int(profile_size.value)) if profile else None,
-        exif_orientation=int(library.unround_result_exif_orientation(result)),
-        input_format="png" if int(library.unround_result_format(result)) == _FORMAT_PNG else "jpeg",
-        bits=int(library.unround_result_bits(result)),
-        alpha_result=_solution(library, result, alpha=True),
-        excess=tuple(excess) if components and len(excess) == len(components) else None,
-        kernel_times=_times(library, result),
-    )
+        pointer = function(self.handle, *arguments, ctypes.byref(count))
+        return pointer, int(count.value)
+
+    def picture(self) -> Array:
+        sizes = [ctypes.c_uint64() for _ in range(3)]
+        pointer = self.lib.unround_result_picture(self.handle, *map(ctypes.byref, sizes))
+        height, width, channels = (int(size.value) for size in sizes)
+        shape = (height, width) if channels == 1 else (height, width, channels)
+        return _floats(pointer, shape, "the picture")
+
+    def canvas_and_planes(self, height: int, width: int) -> tuple[Array, Array]:
+        sizes = [ctypes.c_uint64() for _ in range(3)]
+        pointer = self.lib.unround_result_canvas(self.handle, *map(ctypes.byref, sizes))
+        channels, rows, columns = (int(size.value) for size in sizes)
+        canvas = _floats(pointer, (channels, rows, columns), "the canvas")
+        planes_pointer, planes_count = self._counted(self.lib.unround_result_planes)
+        planes = _floats(planes_pointer, (channels, height, width), "the planes", planes_count)
+        return canvas, planes
+
+    def component(self, index: int) -> tuple[Array, Problem]:
+        sizes = [ctypes.c_uint64() for _ in range(4)]
+        if self.lib.unround_result_component(self.handle, index, *map(ctypes.byref, sizes)) != Status.OK:
+            raise RuntimeError(f"the library has no component {index} in this result")
+        rows, columns, down, across = (int(size.value) for size in sizes)
+        blocks = (rows, columns, 8, 8)
+        pointer, count = self._counted(self.lib.unround_result_coefficients, index)
+        coefficients = _floats(pointer, blocks, f"coefficients of component {index}", count)
+        arrays = []
+        for which, (what, shape) in enumerate((("lower ends", blocks), ("upper ends", blocks), ("centres", blocks),
+                                               ("steps", (8, 8)), ("weights", (8, 8)))):
+            pointer, count = self._counted(self.lib.unround_result_problem, index, which)
+            arrays.append(_floats(pointer, shape, f"{what} of component {index}", count))
+        return coefficients, Problem((down, across), *arrays)
+
+    def solution(self, alpha: bool = False) -> Solution | None:
+        prefix = "unround_result_alpha_" if alpha else "unround_result_"
+        iterations, stop = ctypes.c_uint64(), ctypes.c_int32()
+        if not getattr(self.lib, prefix + "solver")(self.handle, ctypes.byref(iterations), ctypes.byref(stop)):
+            return None
+        pointer, count = self._counted(getattr(self.lib, prefix + "record_iterations"))
+        recorded = (np.ctypeslib.as_array(pointer, shape=(count,)).astype(np.int64, copy=True) if count
+                    else np.zeros(0, dtype=np.int64))
+        columns = []
+        for which, what in enumerate(("seconds", "primal values", "dual values", "scalings", "partial gaps")):
+            values, values_count = self._counted(getattr(self.lib, prefix + "record_values"), which)
+            columns.append(_floats(values, (count,), f"the records' {what}", values_count))
+        return Solution(int(iterations.value), Stop(int(stop.value)), History(recorded, *columns))
+
+    def excess(self, shapes: list[tuple[int, ...]]) -> tuple[Array, ...] | None:
+        found = []
+        for index, shape in enumerate(shapes):
+            pointer, count = self._counted(self.lib.unround_result_excess, index)
+            if not pointer:
+                return None
+            found.append(_floats(pointer, shape, f"excess of component {index}", count))
+        return tuple(found) if found else None
+
+    def kernel_times(self) -> KernelTimes | None:
+        values = (ctypes.c_double * 6)()
+        if not self.lib.unround_result_kernel_times(self.handle, values, 6):
+            return None
+        return KernelTimes(values[0], int(values[1]), values[2], int(values[3]), values[4], int(values[5]))
+
+    def icc_profile(self) -> bytes | None:
+        pointer, count = self._counted(self.lib.unround_result_icc_profile)
+        return ctypes.string_at(pointer, count) if pointer else None
+
+    def decoded(self) -> Decoded:
+        picture = self.picture()
+        canvas, planes = self.canvas_and_planes(picture.shape[0], picture.shape[1])
+        parts = [self.component(index) for index in range(int(self.lib.unround_result_components(self.handle)))]
+        said = self.lib.unround_result_warning(self.handle)
+        if said:
+            warnings.warn(said.decode("utf-8", "replace"), UnroundWarning, stacklevel=4)
+        return Decoded(
+            picture=picture,
+            planes=planes,
+            canvas=canvas,
+            color_space=jpegio.ColorSpace(int(self.lib.unround_result_color_space(self.handle))),
+            coefficients=tuple(coefficients for coefficients, _ in parts),
+            problems=tuple(problem for _, problem in parts),
+            result=self.solution(),
+            icc_profile=self.icc_profile(),
+            exif_orientation=int(self.lib.unround_result_exif_orientation(self.handle)),
+            input_format="png" if int(self.lib.unround_result_format(self.handle)) == _PNG_FORMAT else "jpeg",
+            bits=int(self.lib.unround_result_bits(self.handle)),
+            alpha_result=self.solution(alpha=True),
+            excess=self.excess([coefficients.shape for coefficients, _ in parts]),
+            kernel_times=self.kernel_times(),
+        )
 
 
-class _Watcher:
-    """The observer as the interface calls it: each record made a Record, an exception kept."""
+def read_result(handle: ctypes.c_void_p) -> Decoded:
+    """The Decoded of a result handle, which is then freed."""
+    library = _lib()
+    try:
+        return _ResultReader(library, handle).decoded()
+    finally:
+        library.unround_result_free(handle)
+
+
+# ---------------------------------------------------------------------------------------
+# Calls that make one result.
+
+class _ObserverRelay:
+    """The observer as the library calls it. Each record becomes a Record; an exception the
+    observer raises stops the solver and is raised again once the library returns."""
 
     def __init__(self, observer: Observer | None) -> None:
-        self._observer = observer
-        self.error: BaseException | None = None
-        self.callback = _OBSERVER() if observer is None else _OBSERVER(self._call)
+        self.observer = observer
+        self.raised: BaseException | None = None
+        self.function = OBSERVER_TYPE(self) if observer is not None else OBSERVER_TYPE()
 
-    def _call(self, _user: int | None, pointer: "ctypes._Pointer[_Record]") -> int:
-        if self._observer is None or self.error is not None:
+    def __call__(self, _user: object, pointer: "ctypes._Pointer[CRecord]") -> int:
+        if self.raised is not None or self.observer is None:
             return 1
         try:
-            record = pointer.contents
-            shape = (int(record.channels), int(record.height), int(record.width))
-            seen = Record(int(record.iteration), float(record.gap), float(record.primal), float(record.dual),
-                          _copy(record.canvas, shape))
-            return 1 if self._observer(seen) else 0
-        except BaseException as error:  # noqa: BLE001 -- raised again once the library has returned
-            self.error = error
+            c = pointer.contents
+            record = Record(int(c.iteration), float(c.gap), float(c.primal), float(c.dual),
+                            _floats(c.canvas, (int(c.channels), int(c.height), int(c.width)), "the record's canvas"))
+            return 1 if self.observer(record) else 0
+        except BaseException as error:  # noqa: BLE001 -- raised again after the library returns
+            self.raised = error
             return 1
 
 
-def _reconstruct(call: Callable, observer: Observer | None) -> Decoded:
-    library = _library()
-    watcher = _Watcher(observer)
-    result = ctypes.c_void_p()
-    message = ctypes.create_string_buffer(_MESSAGE_SIZE)
-    status = call(watcher.callback, ctypes.byref(result), message)
-    try:
-        if watcher.error is not None:
-            raise watcher.error
-        if status != Status.OK:
-            known = status in Status.__members__.values()
-            raise UnroundError(Status(status) if known else Status.INTERNAL, message.value.decode("utf-8", "replace"))
-        return _decoded(library, result)
-    finally:
-        library.unround_result_free(result)
+def _one_result(run: Callable[[object, ctypes.c_void_p, _Call], int], observer: Observer | None) -> Decoded:
+    """Runs a call that makes one result (observer, result out, message) and reads it."""
+    relay = _ObserverRelay(observer)
+    handle = ctypes.c_void_p()
+    call = _Call()
+    status = int(run(relay.function, handle, call))
+    if relay.raised is not None:
+        if handle:
+            _lib().unround_result_free(handle)
+        raise relay.raised
+    if status != Status.OK:
+        if handle:
+            _lib().unround_result_free(handle)
+        raise status_error(status, call.text)
+    return read_result(handle)
+
+
+class Marshalled:
+    """An Image's components as the interface's unround_input; the level arrays and the
+    component table live as long as this object."""
+
+    def __init__(self, image: jpegio.Image) -> None:
+        self.levels = []
+        self.table = (CComponent * max(1, len(image.components)))()
+        for index, component in enumerate(image.components):
+            levels = np.ascontiguousarray(component.coefficients, dtype=np.int16)
+            if levels.ndim != 4 or levels.shape[2:] != (8, 8):
+                raise ValueError(f"component {index}: levels must be (rows, columns, 8, 8); got {levels.shape}")
+            self.levels.append(levels)
+            entry = self.table[index]
+            entry.levels = levels.ctypes.data_as(ctypes.POINTER(ctypes.c_int16))
+            entry.rows, entry.columns = levels.shape[0], levels.shape[1]
+            entry.h_samp_factor, entry.v_samp_factor = int(component.h_samp_factor), int(component.v_samp_factor)
+            steps = np.asarray(component.quant_table, dtype=np.uint16).ravel()
+            entry.quant_table[:] = [int(step) for step in steps]
+        self.input = CInput(int(image.height), int(image.width), int(image.color_space), len(image.components),
+                            self.table)
+
+
+class _StartState:
+    """The state solve() resumes from, as unround_start, its arrays kept alive."""
+
+    def __init__(self, start: dict) -> None:
+        self.struct = CStart()
+        self.kept: list[object] = []
+        doubles = ctypes.POINTER(ctypes.c_double)
+        coefficients = start.get("coefficients")
+        if coefficients is not None:
+            arrays = [np.ascontiguousarray(values, dtype=np.float64) for values in coefficients]
+            table = (doubles * len(arrays))(*(array.ctypes.data_as(doubles) for array in arrays))
+            self.kept += [arrays, table]
+            self.struct.coefficients = ctypes.cast(table, ctypes.POINTER(doubles))
+        for name in ("p", "w", "r"):
+            if start.get(name) is not None:
+                array = np.ascontiguousarray(start[name], dtype=np.float64)
+                self.kept.append(array)
+                setattr(self.struct, name, array.ctypes.data_as(doubles))
+
+
+def _file_bytes(source: bytes | str | os.PathLike) -> bytes:
+    return Path(source).read_bytes() if isinstance(source, (str, os.PathLike)) else bytes(source)
 
 
 def decode(source: bytes | str | os.PathLike | jpegio.Image, settings: Settings | list[str] | None = None, *,
            observer: Observer | None = None, max_pixels: int = 0, max_scans: int = 0,
            warnings_are_errors: bool = False) -> Decoded:
-    """Reconstructs a JPEG file or a PNG file, its bytes or its path (python/src/unround/
-    native.py:677-716): a PNG file within the intervals of its rounded samples, by TV or TGV
-    with mu and slack alone of the data term, its colours together and its alpha alone; any
-    other as a JPEG file, read by the library's reader with these limits of reading. An Image
-    is reconstructed as solve() does. A warning of the reading is said as an UnroundWarning."""
+    """Reconstructs a file (its bytes or its path) on the GPU: a PNG file within the
+    intervals of its rounded samples (TV or TGV, mu and slack alone of the data term, the
+    colours together and the alpha alone); any other file as a JPEG file, read with these
+    limits. An Image is solved as solve() solves it. A warning of the reader is an
+    UnroundWarning."""
     if isinstance(source, jpegio.Image):
         if max_pixels or max_scans or warnings_are_errors:
-            raise ValueError("an image read already takes no limits of reading")
+            raise ValueError("limits of reading apply to files; this Image has been read already")
         return solve(source, settings, observer=observer)
-    library = _library()
-    handle = _Settings(options(settings, max_pixels=max_pixels, max_scans=max_scans,
-                               warnings_are_errors=warnings_are_errors))
-    buffer = Path(source).read_bytes() if isinstance(source, (str, os.PathLike)) else bytes(source)
+    library = _lib()
+    held = SettingsHandle(options(settings, max_pixels=max_pixels, max_scans=max_scans,
+                                  warnings_are_errors=warnings_are_errors))
+    data = _file_bytes(source)
+    return _one_result(lambda relay, handle, call: library.unround_decode(
+        data, len(data), held.handle, relay, None, ctypes.byref(handle), *call.message), observer)
 
-    def call(callback: object, result: object, message: ctypes.Array) -> int:
-        return int(library.unround_decode(buffer, len(buffer), handle.handle, callback, None, result, message,
-                                          _MESSAGE_SIZE))
 
-    return _reconstruct(call, observer)
+def _picture_array(picture: npt.ArrayLike) -> tuple[Array, int, int, int]:
+    samples = np.ascontiguousarray(picture, dtype=np.float64)
+    if samples.ndim not in (2, 3):
+        raise ValueError(f"a picture has 2 or 3 axes (height, width[, channels]); got shape {samples.shape}")
+    channels = 1 if samples.ndim == 2 else int(samples.shape[2])
+    return samples, int(samples.shape[0]), int(samples.shape[1]), channels
+
+
+def _doubles(array: np.ndarray) -> object:
+    return array.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
 def project(picture: npt.ArrayLike, source: bytes | str | os.PathLike, settings: Settings | list[str] | None = None,
             *, max_pixels: int = 0, max_scans: int = 0, warnings_are_errors: bool = False) -> Decoded:
-    """Measures a picture made elsewhere against a JPEG file's intervals and projects it onto
-    them on the GPU (docs/math.md, 12; python/src/unround/native.py:718-755): picture (height,
-    width) of a grey file or (height, width, 3) RGB of a file in colour, as the file stores
-    it; the result is the projection as a reconstruction without a solver (result None), and
-    its excess how far each coefficient lay outside its interval, in steps."""
-    library = _library()
-    samples, height, width, channels = _samples(picture)
-    handle = _Settings(options(settings, max_pixels=max_pixels, max_scans=max_scans,
-                               warnings_are_errors=warnings_are_errors))
-    buffer = Path(source).read_bytes() if isinstance(source, (str, os.PathLike)) else bytes(source)
-    pointer = samples.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-
-    def call(_callback: object, result: object, message: ctypes.Array) -> int:
-        return int(library.unround_project(buffer, len(buffer), handle.handle, pointer, height, width, channels,
-                                           result, message, _MESSAGE_SIZE))
-
-    projected = _reconstruct(call, None)
-    del samples
-    return projected
-
-
-def _input_of(image: jpegio.Image) -> tuple[_Input, list[np.ndarray], object]:
-    count = len(image.components)
-    levels = [np.ascontiguousarray(component.coefficients, dtype=np.int16) for component in image.components]
-    raw = (_Component * max(count, 1))()
-    for index, (component, values) in enumerate(zip(image.components, levels, strict=True)):
-        if values.ndim != 4 or values.shape[2:] != (8, 8):
-            raise ValueError(f"the levels of component {index} are (rows, columns, 8, 8), not {values.shape}")
-        raw[index].levels = values.ctypes.data_as(ctypes.POINTER(ctypes.c_int16))
-        raw[index].rows, raw[index].columns = int(values.shape[0]), int(values.shape[1])
-        raw[index].h_samp_factor = int(component.h_samp_factor)
-        raw[index].v_samp_factor = int(component.v_samp_factor)
-        table = np.asarray(component.quant_table, dtype=np.uint16).reshape(_BLOCK_SIZE)
-        raw[index].quant_table = (ctypes.c_uint16 * _BLOCK_SIZE)(*(int(step) for step in table))
-    return _Input(int(image.height), int(image.width), int(image.color_space), count, raw), levels, raw
+    """A picture made elsewhere, (height, width) grey or (height, width, 3) RGB, measured
+    against a JPEG file's intervals and projected onto them on the GPU (docs/math.md 12):
+    a Decoded without a solver whose excess holds how far each coefficient lay outside its
+    interval, in steps."""
+    library = _lib()
+    samples, height, width, channels = _picture_array(picture)
+    held = SettingsHandle(options(settings, max_pixels=max_pixels, max_scans=max_scans,
+                                  warnings_are_errors=warnings_are_errors))
+    data = _file_bytes(source)
+    return _one_result(lambda _relay, handle, call: library.unround_project(
+        data, len(data), held.handle, _doubles(samples), height, width, channels, ctypes.byref(handle),
+        *call.message), None)
 
 
 def solve(image: jpegio.Image, settings: Settings | list[str] | None = None, *, observer: Observer | None = None,
           start: dict | None = None) -> Decoded:
-    """Reconstructs the components of an image on the GPU (python/src/unround/native.py:758-788).
+    """Reconstructs an Image's components on the GPU. settings may be the command line's
+    options themselves. start (this edition's) resumes from a state: 'coefficients', one
+    (rows, columns, 8, 8) array per component, 'p' and 'w' (C, 2, H, W), 'r' (C, 3, H, W);
+    any of them may be left out."""
+    library = _lib()
+    held = SettingsHandle.of(settings)
+    marshalled = Marshalled(image)
+    state = None if start is None else _StartState(start)
 
-    settings may also be the command line's options themselves. start, this edition's,
-    resumes from a state: 'coefficients' (one (rows, columns, 8, 8) array per component),
-    'p' and 'w' (C, 2, H, W) and 'r' (C, 3, H, W), any of them left out.
-    """
-    library = _library()
-    handle = _Settings(settings if isinstance(settings, list) else options(settings))
-    given, levels, raw = _input_of(image)
-    kept: list = [levels, raw]
-    start_struct = None
-    if start is not None:
-        start_struct = _Start()
-        coefficient_list = start.get("coefficients")
-        if coefficient_list is not None:
-            arrays = [np.ascontiguousarray(values, dtype=np.float64) for values in coefficient_list]
-            pointers = (ctypes.POINTER(ctypes.c_double) * len(arrays))(
-                *(array.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for array in arrays))
-            kept += [arrays, pointers]
-            start_struct.coefficients = ctypes.cast(pointers, ctypes.POINTER(ctypes.POINTER(ctypes.c_double)))
-        for name in ("p", "w", "r"):
-            if start.get(name) is not None:
-                array = np.ascontiguousarray(start[name], dtype=np.float64)
-                kept.append(array)
-                setattr(start_struct, name, array.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    def run(relay: object, handle: ctypes.c_void_p, call: _Call) -> int:
+        if state is None:
+            return library.unround_solve(ctypes.byref(marshalled.input), held.handle, relay, None,
+                                         ctypes.byref(handle), *call.message)
+        return library.unround_solve_start(ctypes.byref(marshalled.input), held.handle, ctypes.byref(state.struct),
+                                           relay, None, ctypes.byref(handle), *call.message)
 
-    def call(callback: object, result: object, message: ctypes.Array) -> int:
-        if start_struct is None:
-            return int(library.unround_solve(ctypes.byref(given), handle.handle, callback, None, result, message,
-                                             _MESSAGE_SIZE))
-        return int(library.unround_solve_start(ctypes.byref(given), handle.handle, ctypes.byref(start_struct),
-                                               callback, None, result, message, _MESSAGE_SIZE))
-
-    decoded = _reconstruct(call, observer)
-    del kept
-    return decoded
+    return _one_result(run, observer)
 
 
 def solve_samples(levels: npt.ArrayLike, bits: int = 8, settings: Settings | list[str] | None = None, *,
                   observer: Observer | None = None) -> Decoded:
-    """Reconstructs a picture of rounded samples on the GPU, as decode() does a PNG file
-    (rust/unround/src/rounded.rs:71-170): levels (height, width) or (height, width, channels)
-    of `bits` bits, channels 1 grey, 2 grey and alpha, 3 RGB, 4 RGB and alpha; TV or TGV
-    within the intervals of the levels with --mu and --slack alone of the data term, the
-    colours together and the alpha alone. The Decoded is a PNG file's: input_format "png",
-    the bits, no components, and alpha_result where there is alpha."""
-    library = _library()
-    handle = _Settings(settings if isinstance(settings, list) else options(settings))
+    """Reconstructs a picture of rounded samples on the GPU as decode() does a PNG file
+    (rust/unround/src/rounded.rs:71-170): levels (height, width) or (height, width,
+    channels) of `bits` bits, channels 1 grey, 2 grey and alpha, 3 RGB, 4 RGB and alpha.
+    The Decoded is a PNG file's: no components, alpha_result where there is alpha."""
+    library = _lib()
+    held = SettingsHandle.of(settings)
     given = np.asarray(levels)
     if given.ndim not in (2, 3):
-        raise ValueError(f"levels are (height, width) or (height, width, channels), not {given.shape}")
-    if given.size and (np.min(given) < 0 or np.max(given) > 65535):
-        raise ValueError("levels lie in 0..65535")
+        raise ValueError(f"levels have 2 or 3 axes (height, width[, channels]); got shape {given.shape}")
+    if given.size and not (0 <= np.min(given) and np.max(given) <= 0xFFFF):
+        raise ValueError("every level must fit 16 bits (0 to 65535)")
     values = np.ascontiguousarray(given, dtype=np.uint16)
     height, width = int(values.shape[0]), int(values.shape[1])
     channels = 1 if values.ndim == 2 else int(values.shape[2])
-
-    def call(callback: object, result: object, message: ctypes.Array) -> int:
-        return int(library.unround_solve_samples(values.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), height, width,
-                                                 channels, bits, handle.handle, callback, None, result, message,
-                                                 _MESSAGE_SIZE))
-
-    return _reconstruct(call, observer)
+    pointer = values.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))
+    return _one_result(lambda relay, handle, call: library.unround_solve_samples(
+        pointer, height, width, channels, bits, held.handle, relay, None, ctypes.byref(handle), *call.message),
+        observer)
 
 
 def solve_batch(images: Sequence[jpegio.Image], settings: Settings | list[str] | None = None
                 ) -> tuple[list[Decoded], list[float]]:
-    """Reconstructs many images together on the GPU, this edition's: each result is what
-    solve() gives for its image (the same iterates, records and stopping iteration), with
-    the tiles of every image in the same launches and the stopping tests on the device. Also
-    returns each image's latency: the seconds from the batch's first iteration to the end of
-    the record at which it stopped (or of the last run of iterations). No observer."""
-    library = _library()
-    handle = _Settings(settings if isinstance(settings, list) else options(settings))
-    count = len(images)
-    inputs = (_Input * max(count, 1))()
-    kept: list = []
-    for index, image in enumerate(images):
-        given, levels, raw = _input_of(image)
-        inputs[index] = given
-        kept += [levels, raw]
-    results = (ctypes.c_void_p * max(count, 1))()
-    latencies = np.zeros(max(count, 1))
-    message = ctypes.create_string_buffer(_MESSAGE_SIZE)
-    status = int(library.unround_solve_batch(inputs, count, handle.handle, results,
-                                             latencies.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), message,
-                                             _MESSAGE_SIZE))
-    if status != Status.OK:
-        known = status in Status.__members__.values()
-        raise UnroundError(Status(status) if known else Status.INTERNAL, message.value.decode("utf-8", "replace"))
-    decoded: list[Decoded] = []
+    """Reconstructs many images together on the GPU (this edition's): each Decoded is what
+    solve() gives for its image, with every image's tiles in shared launches and the stopping
+    tests on the device. Also gives each image's latency: the seconds from the batch's first
+    iteration to the end of the run of iterations in which it stopped. No observer."""
+    library = _lib()
+    held = SettingsHandle.of(settings)
+    marshalled = [Marshalled(image) for image in images]
+    count = len(marshalled)
+    inputs = (CInput * max(1, count))(*(m.input for m in marshalled))
+    handles = (ctypes.c_void_p * max(1, count))()
+    latencies = np.zeros(max(1, count), dtype=np.float64)
+    call = _Call()
+    call.check(library.unround_solve_batch(inputs, count, held.handle, handles, _doubles(latencies), *call.message))
+    results: list[Decoded] = []
     try:
         for index in range(count):
-            decoded.append(_decoded(library, ctypes.c_void_p(results[index])))
+            results.append(_ResultReader(library, ctypes.c_void_p(handles[index])).decoded())
     finally:
         for index in range(count):
-            library.unround_result_free(ctypes.c_void_p(results[index]))
-    del kept
-    return decoded, [float(value) for value in latencies[:count]]
+            library.unround_result_free(ctypes.c_void_p(handles[index]))
+    return results, [float(seconds) for seconds in latencies[:count]]
 
 
-def _samples(picture: npt.ArrayLike) -> tuple[Array, int, int, int]:
-    samples = np.ascontiguousarray(picture, dtype=np.float64)
-    if samples.ndim == 2:
-        return samples, int(samples.shape[0]), int(samples.shape[1]), 1
-    if samples.ndim == 3:
-        return samples, int(samples.shape[0]), int(samples.shape[1]), int(samples.shape[2])
-    raise ValueError(f"a picture is (height, width) or (height, width, channels), not {samples.shape}")
+# ---------------------------------------------------------------------------------------
+# Writers and conversions.
 
-
-def _written(name: str, picture: npt.ArrayLike, *arguments: object) -> bytes:
-    """The file a writer of the library makes of the picture, with the arguments after its size;
-    its warning, where it has one, as an UnroundWarning (python/src/unround/native.py:801-822)."""
-    library = _library()
-    samples, height, width, channels = _samples(picture)
-    pointer = ctypes.POINTER(ctypes.c_uint8)()
+def _file_of(function: str, picture: npt.ArrayLike, *arguments: object) -> bytes:
+    """The bytes a writer of the library makes of a picture (its arguments after the
+    picture's size); its warning, if any, as an UnroundWarning."""
+    library = _lib()
+    samples, height, width, channels = _picture_array(picture)
+    made = ctypes.POINTER(ctypes.c_uint8)()
     size = ctypes.c_uint64()
-    message = ctypes.create_string_buffer(_MESSAGE_SIZE)
-    status = getattr(library, name)(samples.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), height, width, channels,
-                                    *arguments, ctypes.byref(pointer), ctypes.byref(size), message, _MESSAGE_SIZE)
-    text = message.value.decode("utf-8", "replace")
-    if status != Status.OK:
-        known = status in Status.__members__.values()
-        raise UnroundError(Status(status) if known else Status.INTERNAL, text)
+    call = _Call()
+    status = getattr(library, function)(_doubles(samples), height, width, channels, *arguments, ctypes.byref(made),
+                                        ctypes.byref(size), *call.message)
+    call.check(status)
     try:
-        data = ctypes.string_at(pointer, int(size.value))
+        data = ctypes.string_at(made, int(size.value))
     finally:
-        library.unround_bytes_free(pointer, size)
-    if text:
-        warnings.warn(text, UnroundWarning, stacklevel=3)
+        library.unround_bytes_free(made, size)
+    if call.text:
+        warnings.warn(call.text, UnroundWarning, stacklevel=3)
     return data
 
 
-def _profile(icc_profile: bytes | None) -> tuple[bytes | None, int]:
-    return (None, 0) if icc_profile is None else (bytes(icc_profile), len(icc_profile))
+def _icc(profile: bytes | None) -> tuple[bytes | None, int]:
+    return (None, 0) if profile is None else (bytes(profile), len(profile))
 
 
 def tiff_bytes(picture: npt.ArrayLike, *, ycbcr: bool = False, icc_profile: bytes | None = None) -> bytes:
-    """A TIFF of the picture's samples in binary64, bit for bit (python/src/unround/native.py:829-844):
-    (height, width) or (height, width, channels), 1 to 4, alpha last. With ycbcr the three samples of
-    a pixel are JFIF's Y, Cb and Cr, and the file says so. The ICC profile is embedded where it goes
-    with the picture; otherwise it is left out, with an UnroundWarning."""
-    return _written("unround_tiff", picture, 1 if ycbcr else 0, *_profile(icc_profile))
+    """A TIFF holding the picture's samples as binary64, bit for bit: (height, width) or
+    (height, width, channels), 1 to 4 channels, alpha last. With ycbcr the three samples of
+    a pixel are JFIF's Y, Cb and Cr. An ICC profile that does not suit the picture is left
+    out, with an UnroundWarning."""
+    return _file_of("unround_tiff", picture, int(bool(ycbcr)), *_icc(icc_profile))
 
 
-_DITHERS: Final = {("none", None): 0, ("blue-noise", None): 1, ("blue-noise", "apart"): 1,
-                   ("blue-noise", "together"): 2}
+_DITHER_CODES: Final = {"none": {None: 0}, "blue-noise": {None: 1, "apart": 1, "together": 2}}
 
 
-def _dither(dither: str, channels: str | None) -> int:
-    found = _DITHERS.get((dither, channels))
-    if found is None:
-        raise ValueError(f"the dither is blue-noise (channels apart or together) or none, not {dither}"
-                         + ("" if channels is None else f" with the channels {channels}"))
-    return found
+def _dither_code(dither: str, channels: str | None) -> int:
+    code = _DITHER_CODES.get(dither, {}).get(channels)
+    if code is None:
+        raise ValueError(f"unknown dither {dither!r} with channels {channels!r}: use 'none', or 'blue-noise' "
+                         "with channels 'apart' (the default) or 'together'")
+    return code
 
 
 def png_bytes(picture: npt.ArrayLike, *, sixteen: bool = False, dither: str = "blue-noise",
               dither_channels: str | None = None, compression: int | None = None,
               icc_profile: bytes | None = None) -> bytes:
-    """A PNG file of the picture's samples, of 8 or 16 bits (python/src/unround/native.py:858-879):
-    dither "blue-noise" (the ordered dither of the blue-noise matrix; dither_channels "apart", the
-    default, or "together") or "none" (each sample rounded, halves away from 0) (docs/math.md, 10);
-    compression zlib's level 0 to 9, None for zlib's default; the ICC profile as tiff_bytes takes it."""
-    level = -1 if compression is None else compression
-    if not -1 <= level <= 9:
-        raise ValueError(f"zlib's level is 0 to 9, or None, not {compression}")
-    return _written("unround_png", picture, 1 if sixteen else 0, _dither(dither, dither_channels), level,
-                    *_profile(icc_profile))
+    """A PNG file of the picture, 8 or 16 bits a sample (docs/math.md 10): dither
+    "blue-noise" (ordered by the blue-noise ranks; dither_channels "apart", the default, or
+    "together") or "none" (each sample rounded, halves away from 0); compression zlib's
+    level 0 to 9, None for its default; the ICC profile as tiff_bytes takes it."""
+    level = -1 if compression is None else int(compression)
+    if level < -1 or level > 9:
+        raise ValueError(f"compression is a zlib level from 0 to 9 (or None); got {compression}")
+    return _file_of("unround_png", picture, int(bool(sixteen)), _dither_code(dither, dither_channels), level,
+                    *_icc(icc_profile))
 
 
 def pnm_bytes(picture: npt.ArrayLike, *, sixteen: bool = False, dither: str = "blue-noise",
               dither_channels: str | None = None) -> bytes:
-    """A PNM file (P5 grey, P6 RGB) of the picture's samples, made as png_bytes makes them."""
-    return _written("unround_pnm", picture, 1 if sixteen else 0, _dither(dither, dither_channels))
+    """A PNM file (P5 grey, P6 RGB) of the picture, its samples made as png_bytes makes them."""
+    return _file_of("unround_pnm", picture, int(bool(sixteen)), _dither_code(dither, dither_channels))
 
 
 def oriented(picture: npt.ArrayLike, orientation: int) -> Array:
-    """The picture, (height, width) or (height, width, channels), turned upright as EXIF's
-    orientation (1 to 8, 0 for none) says; from 5 on the rows become columns. Every sample moves
-    bit for bit (python/src/unround/native.py:889-919)."""
-    library = _library()
-    samples, height, width, channels = _samples(picture)
-    if not -(2**31) <= orientation < 2**31:
-        raise ValueError(f"an EXIF orientation is 1 to 8, or 0 for none, not {orientation}")
+    """The picture turned upright as an EXIF orientation (1 to 8; 0 for none) says; from 5
+    on, rows and columns trade places. Samples move bit for bit."""
+    library = _lib()
+    samples, height, width, channels = _picture_array(picture)
+    if not -(2**31) <= int(orientation) < 2**31:
+        raise ValueError(f"an EXIF orientation is 0 to 8; got {orientation}")
     turned = np.empty(samples.size, dtype=np.float64)
     rows, columns = ctypes.c_uint64(), ctypes.c_uint64()
-    message = ctypes.create_string_buffer(_MESSAGE_SIZE)
-    status = library.unround_orient(samples.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), height, width, channels,
-                                    orientation, turned.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                    ctypes.byref(rows), ctypes.byref(columns), message, _MESSAGE_SIZE)
-    if status != Status.OK:
-        raise UnroundError(Status(status), message.value.decode("utf-8", "replace"))
+    call = _Call()
+    call.check(library.unround_orient(_doubles(samples), height, width, channels, int(orientation), _doubles(turned),
+                                      ctypes.byref(rows), ctypes.byref(columns), *call.message))
     shape = (int(rows.value), int(columns.value)) + (() if samples.ndim == 2 else (channels,))
     return turned.reshape(shape)
 
 
-def to_rgb(planes: npt.ArrayLike) -> Array:
-    """JFIF's RGB (height, width, 3) of Y, Cb and Cr planes (3, height, width) (docs/math.md, 8)."""
-    values = np.ascontiguousarray(planes, dtype=np.float64)
-    _, height, width = values.shape
-    out = np.zeros((height, width, 3))
-    _library().unround_to_rgb(values.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), height, width,
-                              out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+def _convert(function: str, given: npt.ArrayLike, axes: str) -> Array:
+    values = np.ascontiguousarray(given, dtype=np.float64)
+    if values.ndim != 3 or (values.shape[0] if axes == "planes" else values.shape[2]) != 3:
+        wanted = "(3, height, width)" if axes == "planes" else "(height, width, 3)"
+        raise ValueError(f"expected {axes} of shape {wanted}; got {values.shape}")
+    height, width = (values.shape[1], values.shape[2]) if axes == "planes" else values.shape[:2]
+    out = np.empty((height, width, 3) if axes == "planes" else (3, height, width))
+    status = getattr(_lib(), function)(_doubles(values), height, width, _doubles(out))
+    if status != Status.OK:
+        raise status_error(int(status), f"{function} refused a picture of {height} x {width}")
     return out
+
+
+def to_rgb(planes: npt.ArrayLike) -> Array:
+    """JFIF's RGB, (height, width, 3), of Y, Cb and Cr planes (3, height, width) (docs/math.md 8)."""
+    return _convert("unround_to_rgb", planes, "planes")
 
 
 def to_ycbcr(picture: npt.ArrayLike) -> Array:
-    """JFIF's Y, Cb and Cr planes (3, height, width) of an RGB picture (height, width, 3)."""
-    values = np.ascontiguousarray(picture, dtype=np.float64)
-    height, width, _ = values.shape
-    out = np.zeros((3, height, width))
-    _library().unround_to_ycbcr(values.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), height, width,
-                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
-    return out
+    """JFIF's Y, Cb and Cr planes, (3, height, width), of an RGB picture (height, width, 3)."""
+    return _convert("unround_to_ycbcr", picture, "picture")

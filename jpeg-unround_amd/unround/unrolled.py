@@ -27,38 +27,38 @@ import torch
 
 from . import jpegio, native
 
-_SIZE = ctypes.c_size_t
-_HANDLE = ctypes.c_void_p
 _DOUBLES = ctypes.POINTER(ctypes.c_double)
-_U64 = ctypes.POINTER(ctypes.c_uint64)
+
+# The unrolled solve's entry points (include/unround_ext.h), in native.declare's table form.
+_ENTRY_POINTS = """
+unround_unrolled_new        i32   input* size handle u32 u32 handle* message
+unround_unrolled_free       void  handle
+unround_unrolled_shape      i32   handle u64* u64* u64*
+unround_unrolled_constants  i32   handle f64*
+unround_unrolled_start      i32   handle f64* message
+unround_unrolled_problem    i32   handle u64 f64*
+unround_unrolled_forward    i32   handle address f64 f64 address address message
+unround_unrolled_backward   i32   handle address address f64* address message
+"""
+
+_declared_library: ctypes.CDLL | None = None
 
 
 def _declared() -> ctypes.CDLL:
-    library = native._library()
-    if not getattr(library, "_unrolled_declared", False):
-        message = (ctypes.c_char_p, _SIZE)
-        declare = native._declare
-        i32 = ctypes.c_int32
-        declare(library, "unround_unrolled_new", i32, ctypes.POINTER(native._Input), _SIZE, _HANDLE, ctypes.c_uint32,
-                ctypes.c_uint32, ctypes.POINTER(_HANDLE), *message)
-        declare(library, "unround_unrolled_free", None, _HANDLE)
-        declare(library, "unround_unrolled_shape", i32, _HANDLE, _U64, _U64, _U64)
-        declare(library, "unround_unrolled_constants", i32, _HANDLE, _DOUBLES)
-        declare(library, "unround_unrolled_start", i32, _HANDLE, _DOUBLES, *message)
-        declare(library, "unround_unrolled_problem", i32, _HANDLE, ctypes.c_uint64, _DOUBLES)
-        declare(library, "unround_unrolled_forward", i32, _HANDLE, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
-                ctypes.c_void_p, ctypes.c_void_p, *message)
-        declare(library, "unround_unrolled_backward", i32, _HANDLE, ctypes.c_void_p, ctypes.c_void_p, _DOUBLES,
-                ctypes.c_void_p, *message)
-        library._unrolled_declared = True
-    return library
+    global _declared_library
+    if _declared_library is None:
+        library = native._lib()
+        native.declare(library, _ENTRY_POINTS)
+        _declared_library = library
+    return _declared_library
 
 
-def _checked(status: int, message: ctypes.Array) -> None:
+def _call(function, *arguments: object) -> None:
+    """A call of an entry point whose last arguments are the message buffer and its size."""
+    room = ctypes.create_string_buffer(native.MESSAGE_BYTES)
+    status = int(function(*arguments, room, native.MESSAGE_BYTES))
     if status != native.Status.OK:
-        known = status in native.Status.__members__.values()
-        raise native.UnroundError(native.Status(status) if known else native.Status.INTERNAL,
-                                  message.value.decode("utf-8", "replace"))
+        raise native.status_error(status, room.value.decode("utf-8", "replace"))
 
 
 class UnrolledTGV:
@@ -68,19 +68,13 @@ class UnrolledTGV:
                  iterations: int = 100, checkpoint_every: int = 10) -> None:
         library = _declared()
         self._library = library
-        self._settings = native._Settings(settings if isinstance(settings, list) else native.options(settings))
-        count = len(images)
-        inputs = (native._Input * max(count, 1))()
-        kept: list = []
-        for index, image in enumerate(images):
-            given, levels, raw = native._input_of(image)
-            inputs[index] = given
-            kept += [levels, raw]
-        handle = _HANDLE()
-        message = ctypes.create_string_buffer(native._MESSAGE_SIZE)
-        status = library.unround_unrolled_new(inputs, count, self._settings.handle, iterations, checkpoint_every,
-                                              ctypes.byref(handle), message, native._MESSAGE_SIZE)
-        _checked(int(status), message)
+        self._settings = native.SettingsHandle.of(settings)
+        marshalled = [native.Marshalled(image) for image in images]
+        count = len(marshalled)
+        inputs = (native.CInput * max(count, 1))(*(m.input for m in marshalled))
+        handle = ctypes.c_void_p()
+        _call(library.unround_unrolled_new, inputs, count, self._settings.handle, iterations, checkpoint_every,
+              ctypes.byref(handle))
         self._handle = handle
         shape = [ctypes.c_uint64() for _ in range(3)]
         library.unround_unrolled_shape(handle, *(ctypes.byref(value) for value in shape))
@@ -91,7 +85,7 @@ class UnrolledTGV:
     def close(self) -> None:
         if getattr(self, "_handle", None) is not None and self._handle.value is not None:
             self._library.unround_unrolled_free(self._handle)
-            self._handle = _HANDLE()
+            self._handle = ctypes.c_void_p()
 
     def __del__(self) -> None:
         self.close()
@@ -105,10 +99,7 @@ class UnrolledTGV:
     def start_state(self) -> np.ndarray:
         """The start, (count, 8, height, width): x, w_x, w_y, p_x, p_y, r_xx, r_yy, r_xy."""
         out = np.zeros((self.count, 8, self.height, self.width))
-        message = ctypes.create_string_buffer(native._MESSAGE_SIZE)
-        status = self._library.unround_unrolled_start(self._handle, out.ctypes.data_as(_DOUBLES), message,
-                                                      native._MESSAGE_SIZE)
-        _checked(int(status), message)
+        _call(self._library.unround_unrolled_start, self._handle, out.ctypes.data_as(_DOUBLES))
         return out
 
     def problem(self, frame: int) -> np.ndarray:
@@ -125,21 +116,17 @@ class UnrolledTGV:
 
     def _forward(self, weight_map: torch.Tensor, alpha0: float, alpha1: float) -> torch.Tensor:
         canvas = torch.empty((self.count, self.height, self.width), device=weight_map.device, dtype=torch.float32)
-        message = ctypes.create_string_buffer(native._MESSAGE_SIZE)
         stream = torch.cuda.current_stream(weight_map.device).cuda_stream
-        status = self._library.unround_unrolled_forward(self._handle, weight_map.data_ptr(), alpha0, alpha1,
-                                                        canvas.data_ptr(), stream, message, native._MESSAGE_SIZE)
-        _checked(int(status), message)
+        _call(self._library.unround_unrolled_forward, self._handle, weight_map.data_ptr(), alpha0, alpha1,
+              canvas.data_ptr(), stream)
         return canvas
 
     def _backward(self, grad_canvas: torch.Tensor) -> tuple[float, float, torch.Tensor]:
         grad_map = torch.empty((self.height, self.width), device=grad_canvas.device, dtype=torch.float32)
         alphas = (ctypes.c_double * 2)()
-        message = ctypes.create_string_buffer(native._MESSAGE_SIZE)
         stream = torch.cuda.current_stream(grad_canvas.device).cuda_stream
-        status = self._library.unround_unrolled_backward(self._handle, grad_canvas.data_ptr(), grad_map.data_ptr(),
-                                                         alphas, stream, message, native._MESSAGE_SIZE)
-        _checked(int(status), message)
+        _call(self._library.unround_unrolled_backward, self._handle, grad_canvas.data_ptr(), grad_map.data_ptr(),
+              alphas, stream)
         return float(alphas[0]), float(alphas[1]), grad_map
 
 

@@ -33,13 +33,13 @@ def test_library_exports_every_declared_function():
 
 
 def test_versions_and_layouts():
-    assert native._library().unround_abi_version() == native.ABI_VERSION == 5
+    assert native._lib().unround_abi_version() == native.ABI_VERSION == 5
     assert "MI355X" in native.version()
-    assert ctypes.sizeof(native._Record) == 64
-    assert ctypes.sizeof(native._Component) == 152
-    assert native._Component.quant_table.offset == 24
-    assert ctypes.sizeof(native._Input) == 24
-    assert native._Input.components.offset == 16
+    assert ctypes.sizeof(native.CRecord) == 64
+    assert ctypes.sizeof(native.CComponent) == 152
+    assert native.CComponent.quant_table.offset == 24
+    assert ctypes.sizeof(native.CInput) == 24
+    assert native.CInput.components.offset == 16
 
 
 def test_settings_round_trip_through_the_library():
@@ -87,10 +87,10 @@ def test_every_refusal_is_named_at_once():
 
 
 def test_null_arguments_and_unsupported_paths_are_statuses():
-    library = native._library()
+    library = native._lib()
     message = ctypes.create_string_buffer(256)
     result = ctypes.c_void_p()
-    callback = native._OBSERVER()
+    callback = native.OBSERVER_TYPE()
     status = library.unround_solve(None, None, callback, None, ctypes.byref(result), message, 256)
     assert status == native.Status.ARGUMENT
     assert b"null" in message.value
@@ -110,7 +110,7 @@ def test_null_arguments_and_unsupported_paths_are_statuses():
 
 
 def test_batch_refusals_come_before_the_gpu():
-    library = native._library()
+    library = native._lib()
     message = ctypes.create_string_buffer(256)
     results = (ctypes.c_void_p * 2)()
     status = library.unround_solve_batch(None, 2, None, results, None, message, 256)

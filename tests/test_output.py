@@ -301,7 +301,7 @@ def test_tiff_holds_the_samples_bit_for_bit(channels, ycbcr):
 
 
 def test_the_icc_check_names_what_it_refuses():
-    library = native._library()
+    library = native._lib()
     library.unround_jpeg_check_icc.restype = ctypes.c_int32
     library.unround_jpeg_check_icc.argtypes = (ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int32, ctypes.c_char_p,
                                                ctypes.c_size_t)
