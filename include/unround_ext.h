@@ -98,6 +98,28 @@ unround_status unround_solve_samples(const uint16_t* levels, uint64_t height, ui
                                      uint32_t bits, const unround_settings* settings, unround_observer observer,
                                      void* user, unround_result** result, char* message, size_t message_size);
 
+/* The bytes of the device region behind unround_solve_scratch, held on each GPU. */
+#define UNROUND_SCRATCH_BYTES (64u << 20)
+
+/* No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * unround_solve of one input from the default start, with no observer and no result
+ * object, allocating no device memory: every buffer of the solve is a stretch of a
+ * region of UNROUND_SCRATCH_BYTES that the library holds on each GPU from its first use
+ * to the end of the process (a file-scope device array). The final canvas cut to the
+ * picture, channel by channel (what unround_result_planes gives), is written to `planes`,
+ * host memory of `count` = components x height x width doubles; iterations and stop (1
+ * converged, 2 iterations; 0 for the decoder of the centres) as unround_result_solver
+ * gives them, where not null. An input whose solve needs more than the region (about 110
+ * bytes a canvas sample in binary32 TGV: 512 x 512 grey fits) is refused with
+ * UNROUND_ERROR_ARGUMENT before anything runs. The region keeps the last call's contents
+ * and is cleared where a solve needs zeros; calls that overlap on one GPU would race for
+ * it, so the library makes them take turns (a lock per GPU). The work runs on the calling
+ * thread's default stream. */
+unround_status unround_solve_scratch(const unround_input* input, const unround_settings* settings, double* planes,
+                                     uint64_t count, uint64_t* iterations, int32_t* stop, char* message,
+                                     size_t message_size);
+
 /* The durations of the solve's kernels, where the environment variable UNROUND_TIMING was
  * set during the solve (HIP events on the solve's stream): values[0] the seconds of the
  * primal kernels and [1] their launches, [2] and [3] those of the dual kernels, [4] the

@@ -685,6 +685,36 @@ unround_status unround_solve_samples(const uint16_t* levels, uint64_t height, ui
   });
 }
 
+unround_status unround_solve_scratch(const unround_input* input, const unround_settings* settings, double* planes,
+                                     uint64_t count, uint64_t* iterations, int32_t* stop, char* message,
+                                     size_t message_size) {
+  return guarded(message, message_size, [&](std::string& text) -> unround_status {
+    if (planes == nullptr) {
+      text = "the input, the settings and the planes are not null pointers";
+      return UNROUND_ERROR_ARGUMENT;
+    }
+    unround_result made;
+    unround::Plan plan;
+    const unround_status checked = frame_of(input, settings, made, plan, text);
+    if (checked != UNROUND_OK) return checked;
+    const unround::FrameModel& frame = made.frame;
+    const uint64_t wanted = static_cast<uint64_t>(frame.channels()) * frame.height * frame.width;
+    if (count != wanted) {
+      text = "the planes hold " + std::to_string(wanted) + " doubles (" + std::to_string(frame.channels()) + " x " +
+             std::to_string(frame.height) + " x " + std::to_string(frame.width) + "), not " + std::to_string(count);
+      return UNROUND_ERROR_ARGUMENT;
+    }
+    uint64_t taken = 0;
+    int32_t reason = 0, status = UNROUND_OK;
+    if (!unround::solve_in_scratch(frame, settings->settings, plan, planes, taken, reason, text, status)) {
+      return static_cast<unround_status>(status);
+    }
+    if (iterations != nullptr) *iterations = taken;
+    if (stop != nullptr) *stop = reason;
+    return UNROUND_OK;
+  });
+}
+
 unround_status unround_solve_batch(const unround_input* inputs, size_t count, const unround_settings* settings,
                                    unround_result** results, double* latencies, char* message,
                                    size_t message_size) {

@@ -689,25 +689,60 @@ __global__ __launch_bounds__(256) void final_kernel(const Component* __restrict_
 // ---------------------------------------------------------------------------------------
 // The host driver.
 
+// Device memory: its own (hipMalloc'd, freed with it), or a stretch of an Arena that it
+// only points into.
 struct DeviceBuffer {
   void* pointer = nullptr;
+  bool owned = false;
   DeviceBuffer() = default;
   explicit DeviceBuffer(size_t bytes) {
-    if (bytes > 0) HIP_CHECK(hipMalloc(&pointer, bytes));
+    if (bytes > 0) {
+      HIP_CHECK(hipMalloc(&pointer, bytes));
+      owned = true;
+    }
+  }
+  static DeviceBuffer within(void* stretch) {
+    DeviceBuffer buffer;
+    buffer.pointer = stretch;
+    return buffer;
   }
   DeviceBuffer(const DeviceBuffer&) = delete;
   DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-  DeviceBuffer(DeviceBuffer&& other) noexcept : pointer(other.pointer) { other.pointer = nullptr; }
+  DeviceBuffer(DeviceBuffer&& other) noexcept : pointer(other.pointer), owned(other.owned) {
+    other.pointer = nullptr;
+    other.owned = false;
+  }
   DeviceBuffer& operator=(DeviceBuffer&& other) noexcept {
     std::swap(pointer, other.pointer);
+    std::swap(owned, other.owned);
     return *this;
   }
   ~DeviceBuffer() {
-    if (pointer != nullptr) (void)hipFree(pointer);
+    if (owned && pointer != nullptr) (void)hipFree(pointer);
   }
   template <typename T>
   T* as() const {
     return static_cast<T*>(pointer);
+  }
+};
+
+// A fixed region of device memory handed out in order, each stretch 256-byte aligned: the
+// buffers of a solve that may not allocate. Nothing is given back; the region is the
+// caller's to reuse once the solve is done.
+constexpr size_t kArenaAlign = 256;
+inline size_t arena_stretch(size_t bytes) { return bytes == 0 ? 0 : (bytes + kArenaAlign - 1) / kArenaAlign * kArenaAlign; }
+
+struct Arena {
+  char* base = nullptr;
+  size_t capacity = 0;
+  size_t used = 0;
+  void* take(size_t bytes) {
+    if (bytes == 0) return nullptr;
+    const size_t stretch = arena_stretch(bytes);
+    if (stretch > capacity - used) throw GpuError("the solve needs more device memory than its region holds");
+    void* at = base + used;
+    used += stretch;
+    return at;
   }
 };
 
@@ -827,19 +862,58 @@ class Solver {
     finish(out);
   }
 
-  // Everything before the iterations: the device, the fields, the components and the start
-  // (x from the coefficients given or the centres, w = grad x unless given, p and r given
-  // or 0, projected onto their balls). Returns whether the method iterates at all.
-  bool prepare(const StartState* start) {
-    upload_basis();
+  // The frame's sizes and the method's kind.
+  void shape() {
     C_ = static_cast<int>(frame_.channels());
     H_ = static_cast<int>(frame_.rows);
     W_ = static_cast<int>(frame_.columns);
     N_ = static_cast<size_t>(H_) * W_;
     tgv_ = settings_.method == Method::tgv;
-    const bool solving = settings_.method == Method::tv || settings_.method == Method::tgv;
     coupled_ = settings_.coupled && C_ > 1;
     if (C_ != 1 && C_ != 3) throw GpuError("a frame of 1 or 3 channels");
+  }
+
+  // The device memory a solve from the default start takes, stretch by stretch as
+  // allocate(), upload_components() and project_tilde() take it (each rounded up to an
+  // Arena's alignment); after shape().
+  size_t device_bytes() const {
+    const size_t C = static_cast<size_t>(C_);
+    size_t t_count = 3 * C * N_, s_count = 3 * 2 * C * N_;
+    if (tgv_) {
+      t_count += 3 * 2 * C * N_;
+      s_count += 3 * 3 * C * N_;
+    }
+    size_t zeta_count = 0, value_groups = 0, conjugate_groups = 0;
+    data_sizes(zeta_count, value_groups, conjugate_groups);
+    const size_t norm_groups = static_cast<size_t>((W_ + 63) / 64) * ((H_ + kNormRows - 1) / kNormRows);
+    size_t total = arena_stretch(sizeof(T) * (t_count + zeta_count) + sizeof(S) * s_count + 256);
+    total += arena_stretch(sizeof(double) * (4 * norm_groups + value_groups + 2 * conjugate_groups + 16));
+    total += arena_stretch(sizeof(double) * 16);
+    for (int c = 0; c < C_; ++c) {
+      const ComponentModel& k = frame_.components[c];
+      total += arena_stretch(sizeof(int16_t) * std::max<size_t>(1, k.levels.size()));
+      total += arena_stretch(std::max<size_t>(1, k.flat.size()));
+      total += arena_stretch(sizeof(double) * std::max<size_t>(1, 64 * k.blocks()));
+    }
+    total += arena_stretch(sizeof(Component) * C);
+    total += arena_stretch(sizeof(double) * C * N_);
+    return total;
+  }
+
+  // Buffers from `arena` rather than hipMalloc from now on (null: hipMalloc again).
+  void use_arena(Arena* arena) { arena_ = arena; }
+  // The final projection's planes (the canvas cut to the picture, channel by channel,
+  // binary64) copied into host memory at `planes` instead of into the output's canvas and
+  // coefficients.
+  void planes_into(double* planes) { planes_into_ = planes; }
+
+  // Everything before the iterations: the device, the fields, the components and the start
+  // (x from the coefficients given or the centres, w = grad x unless given, p and r given
+  // or 0, projected onto their balls). Returns whether the method iterates at all.
+  bool prepare(const StartState* start) {
+    upload_basis();
+    shape();
+    const bool solving = settings_.method == Method::tv || settings_.method == Method::tgv;
     dual_rows_ = dual_rows_for(H_, W_);
     if (const char* rows = std::getenv("UNROUND_DUAL_ROWS")) dual_rows_ = std::max(1, std::atoi(rows));
     hipStream_t s = stream();
@@ -896,7 +970,7 @@ class Solver {
       double* device_given = nullptr;
       if (coefficients != nullptr) {
         const size_t bytes = sizeof(double) * 64 * frame_.components[c].blocks();
-        given[c] = DeviceBuffer(bytes);
+        given[c] = buffer(bytes);
         HIP_CHECK(hipMemcpyAsync(given[c].pointer, coefficients, bytes, hipMemcpyHostToDevice, s));
         device_given = given[c].as<double>();
       }
@@ -921,6 +995,10 @@ class Solver {
 
   const Component* component(int c) const { return components_.as<Component>() + c; }
 
+  DeviceBuffer buffer(size_t bytes) {
+    return arena_ != nullptr ? DeviceBuffer::within(arena_->take(bytes)) : DeviceBuffer(bytes);
+  }
+
   void allocate() {
     const size_t C = static_cast<size_t>(C_);
     size_t t_count = 3 * C * N_;      // x, xbar, tx
@@ -933,7 +1011,7 @@ class Solver {
     data_sizes(zeta_count, value_groups, conjugate_groups);
     const size_t t_bytes = sizeof(T) * (t_count + zeta_count);
     const size_t s_bytes = sizeof(S) * s_count;
-    state_ = DeviceBuffer(t_bytes + s_bytes + 256);
+    state_ = buffer(t_bytes + s_bytes + 256);
     HIP_CHECK(hipMemsetAsync(state_.pointer, 0, t_bytes + s_bytes + 256, stream()));
     T* at = state_.as<T>();
     const auto take = [&at](size_t n) {
@@ -970,8 +1048,8 @@ class Solver {
     norm_partials_count_ = norm_groups;
     value_partials_count_ = value_groups;
     conjugate_partials_count_ = conjugate_groups;
-    partials_ = DeviceBuffer(sizeof(double) * (4 * norm_groups + value_groups + 2 * conjugate_groups + 16));
-    record_out_ = DeviceBuffer(sizeof(double) * 16);
+    partials_ = buffer(sizeof(double) * (4 * norm_groups + value_groups + 2 * conjugate_groups + 16));
+    record_out_ = buffer(sizeof(double) * 16);
   }
 
   void upload_components() {
@@ -983,8 +1061,8 @@ class Solver {
       if (k.down > static_cast<uint32_t>(kMaxCell) || k.across > static_cast<uint32_t>(kMaxCell)) {
         throw GpuError("a cell larger than 4 x 4");
       }
-      levels_.emplace_back(sizeof(int16_t) * std::max<size_t>(1, k.levels.size()));
-      flats_.emplace_back(std::max<size_t>(1, k.flat.size()));
+      levels_.push_back(buffer(sizeof(int16_t) * std::max<size_t>(1, k.levels.size())));
+      flats_.push_back(buffer(std::max<size_t>(1, k.flat.size())));
       HIP_CHECK(hipMemcpyAsync(levels_.back().pointer, k.levels.data(), sizeof(int16_t) * k.levels.size(),
                                hipMemcpyHostToDevice, stream()));
       HIP_CHECK(hipMemcpyAsync(flats_.back().pointer, k.flat.data(), k.flat.size(), hipMemcpyHostToDevice,
@@ -1010,7 +1088,7 @@ class Solver {
         d.costs[q] = k.costs[q];
       }
     }
-    components_ = DeviceBuffer(sizeof(Component) * C_);
+    components_ = buffer(sizeof(Component) * C_);
     HIP_CHECK(hipMemcpyAsync(components_.pointer, host.data(), sizeof(Component) * C_, hipMemcpyHostToDevice,
                              stream()));
     HIP_CHECK(hipStreamSynchronize(stream()));
@@ -1021,7 +1099,7 @@ class Solver {
   void upload_field(const double* given, F* field, int entries) {
     if (given == nullptr) return;
     const size_t count = static_cast<size_t>(entries) * C_ * N_;
-    DeviceBuffer staging(sizeof(double) * count);
+    DeviceBuffer staging = buffer(sizeof(double) * count);
     HIP_CHECK(hipMemcpyAsync(staging.pointer, given, sizeof(double) * count, hipMemcpyHostToDevice, stream()));
     convert_kernel<F><<<static_cast<unsigned>((count + 255) / 256), 256, 0, stream()>>>(staging.as<double>(),
                                                                                              field, count);
@@ -1298,18 +1376,28 @@ class Solver {
   // The final projection of x~ into out (and, where asked, the excess of each coefficient).
   void project_tilde(SolveOutput& out, double margin, std::vector<std::vector<double>>* excess) {
     hipStream_t s = stream();
-    DeviceBuffer canvas(sizeof(double) * C_ * N_);
+    DeviceBuffer canvas = buffer(sizeof(double) * C_ * N_);
     std::vector<DeviceBuffer> coefficients, excesses;
-    out.coefficients.assign(C_, {});
+    out.coefficients.assign(planes_into_ != nullptr ? 0 : C_, {});
     for (int c = 0; c < C_; ++c) {
       const size_t count = std::max<size_t>(1, 64 * frame_.components[c].blocks());
-      coefficients.emplace_back(sizeof(double) * count);
-      excesses.emplace_back(excess != nullptr ? sizeof(double) * count : 0);
+      coefficients.push_back(buffer(sizeof(double) * count));
+      excesses.push_back(buffer(excess != nullptr ? sizeof(double) * count : 0));
       final_kernel<T><<<(tiles(c) + 3) / 4, 256, 0, s>>>(component(c), fields_.tx + c * N_, W_, margin,
                                                          canvas.as<double>() + c * N_,
                                                          coefficients.back().as<double>(),
                                                          excesses.back().as<double>());
       HIP_CHECK(hipGetLastError());
+    }
+    if (planes_into_ != nullptr) {
+      // The picture's rows of each channel, straight into the caller's planes.
+      const size_t h = frame_.height, w = frame_.width;
+      for (int c = 0; c < C_; ++c) {
+        HIP_CHECK(hipMemcpy2DAsync(planes_into_ + c * h * w, sizeof(double) * w, canvas.as<double>() + c * N_,
+                                   sizeof(double) * W_, sizeof(double) * w, h, hipMemcpyDeviceToHost, s));
+      }
+      HIP_CHECK(hipStreamSynchronize(s));
+      return;
     }
     if (excess != nullptr) {
       excess->assign(C_, {});
@@ -1347,11 +1435,72 @@ class Solver {
   std::deque<EventPair> primal_events_, dual_events_, launches_;
   // Rows a wavefront of the dual kernel walks down.
   int dual_rows_ = 32;
+  Arena* arena_ = nullptr;
+  double* planes_into_ = nullptr;
 };
 
 #include "batch.hpp"
 #include "samples.hpp"
 #include "unrolled.hpp"
+
+// ---------------------------------------------------------------------------------------
+// The region behind the solve that allocates nothing (solve_in_scratch): every GPU the
+// library runs on holds its own copy, from the code object's load to the end of the
+// process. In host code `scratch` names a host-side stand-in, not the device's copy: its
+// device address is taken once per device with hipGetSymbolAddress, with that device
+// current, and only that address is handed to kernels and copies.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+struct ScratchOfDevice {
+  std::mutex use;  // one solve at a time holds the region
+  unsigned char* address = nullptr;
+};
+
+// The current device's region: its address and lock, looked up once per device.
+ScratchOfDevice& scratch_of_current_device() {
+  static std::mutex guard;
+  static std::map<int, std::unique_ptr<ScratchOfDevice>> devices;
+  int device = 0;
+  HIP_CHECK(hipGetDevice(&device));
+  const std::lock_guard<std::mutex> lock(guard);
+  std::unique_ptr<ScratchOfDevice>& slot = devices[device];
+  if (slot == nullptr) {
+    auto made = std::make_unique<ScratchOfDevice>();
+    void* address = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&address, HIP_SYMBOL(scratch)));
+    if (address == nullptr) throw GpuError("the scratch region has no device address");
+    made->address = static_cast<unsigned char*>(address);
+    slot = std::move(made);
+  }
+  return *slot;
+}
+
+template <typename T, typename S>
+void solve_in_region(const FrameModel& frame, const Settings& settings, const Plan& plan, double* planes,
+                     uint64_t& iterations, int32_t& stop, std::string& message, int32_t& status, bool& done) {
+  // The solve's own stream would be an allocation of the runtime's: it takes the calling
+  // thread's default stream.
+  Solver<T, S> solver(frame, settings, plan, Timing::none, hipStreamPerThread);
+  solver.shape();
+  const size_t needed = solver.device_bytes();
+  if (needed > kScratchBytes) {
+    status = 1;
+    message = "this input's solve needs " + std::to_string(needed) + " bytes of device memory, more than the " +
+              std::to_string(kScratchBytes) + " of the region that unround_solve_scratch holds";
+    done = false;
+    return;
+  }
+  ScratchOfDevice& region = scratch_of_current_device();
+  const std::lock_guard<std::mutex> hold(region.use);
+  Arena arena{reinterpret_cast<char*>(region.address), kScratchBytes, 0};
+  solver.use_arena(&arena);
+  solver.planes_into(planes);
+  SolveOutput out;
+  solver.run(nullptr, Observer{}, out);
+  iterations = out.iterations;
+  stop = out.stop;
+  done = true;
+}
 
 class UnrolledOnGpu final : public UnrolledSolve {
  public:
@@ -1474,6 +1623,32 @@ bool solve_samples_on_gpu(const SamplesModel& model, const Settings& settings, c
       solver.run(nullptr, observer, out);
     }
     return true;
+  } catch (const GpuError& error) {
+    status = 6;
+    message = std::string("a GPU error: ") + error.what();
+    return false;
+  }
+}
+
+bool solve_in_scratch(const FrameModel& frame, const Settings& settings, const Plan& plan, double* planes,
+                      uint64_t& iterations, int32_t& stop, std::string& message, int32_t& status) {
+  try {
+    int devices = 0;
+    if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) {
+      status = 6;
+      message = "no HIP device: this library computes on an AMD GPU";
+      return false;
+    }
+    upload_basis();
+    bool done = false;
+    if (settings.precision == Precision::dual) {
+      solve_in_region<double, double>(frame, settings, plan, planes, iterations, stop, message, status, done);
+    } else if (settings.precision == Precision::mixed) {
+      solve_in_region<float, bf16>(frame, settings, plan, planes, iterations, stop, message, status, done);
+    } else {
+      solve_in_region<float, float>(frame, settings, plan, planes, iterations, stop, message, status, done);
+    }
+    return done;
   } catch (const GpuError& error) {
     status = 6;
     message = std::string("a GPU error: ") + error.what();

@@ -115,6 +115,19 @@ std::unique_ptr<UnrolledSolve> make_unrolled(const std::vector<const FrameModel*
                                              const std::vector<const Plan*>& plans, const Settings& settings,
                                              int iterations, int every, std::string& message, int32_t& status);
 
+// The bytes of the fixed device region behind solve_in_scratch, on each GPU.
+constexpr size_t kScratchBytes = size_t{64} << 20;
+
+// solve_on_gpu from the default start with no observer, allocating no device memory:
+// every buffer of the solve is a stretch of a fixed region of kScratchBytes that the
+// library holds on each GPU (one solve at a time holds it: a lock per GPU orders them),
+// the work on the calling thread's default stream, and the final projection's planes (the
+// canvas cut to the picture, channel by channel, binary64) copied into `planes`, host
+// memory of channels x height x width doubles. A frame whose solve needs more than the
+// region is refused with status 1 before anything runs on the GPU.
+bool solve_in_scratch(const FrameModel& frame, const Settings& settings, const Plan& plan, double* planes,
+                      uint64_t& iterations, int32_t& stop, std::string& message, int32_t& status);
+
 bool solve_batch_on_gpu(const std::vector<const FrameModel*>& frames, const std::vector<const Plan*>& plans,
                         const Settings& settings, std::vector<SolveOutput>& outs, std::vector<double>& latencies,
                         std::string& message, int32_t& status);

@@ -12,7 +12,7 @@ How this module is put together:
   vocabulary of C types (_C_TYPES);
 * _Call, one call of a function that reports a status and a message: it gives the message
   buffer and turns a failing status into UnroundError;
-* _Marshalled, an Image's components laid out as the interface's input, the arrays kept
+* Marshalled, an Image's components laid out as the interface's input, the arrays kept
   alive for as long as the object is;
 * _ResultReader, one result handle read into arrays and the result types;
 * the option writer, which turns Settings into the command line's options.
@@ -225,6 +225,7 @@ unround_decode                          i32     text size handle observer handle
 unround_solve                           i32     input* handle observer handle handle* message
 unround_solve_start                     i32     input* handle start* observer handle handle* message
 unround_solve_batch                     i32     input* size handle handle* f64* message
+unround_solve_scratch                   i32     input* handle f64* u64 u64* i32* message
 unround_solve_samples                   i32     u16* u64 u64 u64 u32 handle observer handle handle* message
 unround_project                         i32     text size handle f64* u64 u64 u64 handle* message
 unround_result_free                     void    handle
