@@ -28,6 +28,18 @@ __device__ __forceinline__ float basis<float>(int k, int n) {
   return basis32[8 * k + n];
 }
 
+// a b + c rounded once. The kernels' arithmetic on the iteration's path writes its fused
+// multiply-adds out and lets the compiler fuse nothing else (`fp contract(off)`), so that
+// every copy of a computation (a kernel's own, a fused kernel's, a batch's) rounds alike.
+template <typename T>
+__device__ __forceinline__ T fused(T a, T b, T c) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_fmaf(a, b, c);
+  } else {
+    return __builtin_fma(a, b, c);
+  }
+}
+
 // A bfloat16 in memory: the high half of a binary32.
 struct bf16 {
   uint16_t bits;
@@ -154,6 +166,7 @@ __device__ __forceinline__ T clamp_to(T value, T low, T high) {
 // a row of coefficients runs without branches.
 template <typename Constants, typename T>
 __device__ __forceinline__ T prox_of(const Constants& K, int16_t q, bool flat, int k, T e, T step) {
+#pragma clang fp contract(off)
   const double frequency_step = K.steps[k];
   const double weight_of = (flat ? K.flat_weights : K.weights)[k];
   const double shrink = K.shrink[k];
@@ -163,7 +176,7 @@ __device__ __forceinline__ T prox_of(const Constants& K, int16_t q, bool flat, i
   const T weight = static_cast<T>(weight_of);
   const T t = step * weight;
   const T r = T(1) / (T(1) + t);
-  const T z = (e + t * static_cast<T>(centre)) * r;
+  const T z = fused(t, static_cast<T>(centre), e) * r;
   const T h = step * static_cast<T>(cost) * r;
   const T high_own = static_cast<T>(own_upper), low_own = static_cast<T>(own_lower);
   const T above = high_own > z - h ? high_own : z - h;
@@ -212,12 +225,21 @@ __device__ inline double value_of(const Component& K, int16_t q, bool flat, int 
 }
 
 // ---------------------------------------------------------------------------------------
+// b0 v0 + b1 v1 + b2 v2 + b3 v3, fused from the first term on (docs/math.md:108-122).
+template <typename T>
+__device__ __forceinline__ T sum_of_four(T b0, T v0, T b1, T v1, T b2, T v2, T b3, T v3) {
+#pragma clang fp contract(off)
+  return fused(b3, v3, fused(b2, v2, fused(b1, v1, b0 * v0)));
+}
+
+// ---------------------------------------------------------------------------------------
 // 8-point transforms by their even and odd halves (docs/math.md:108-122): each output a
 // sum of four products from its first term on.
 
 // y = C x of eight values in registers.
 template <typename T>
 __device__ __forceinline__ void forward8(const T (&x)[8], T (&y)[8]) {
+#pragma clang fp contract(off)
   T s[4], d[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -226,19 +248,20 @@ __device__ __forceinline__ void forward8(const T (&x)[8], T (&y)[8]) {
   }
 #pragma unroll
   for (int k = 0; k < 8; k += 2) {
-    y[k] = basis<T>(k, 0) * s[0] + basis<T>(k, 1) * s[1] + basis<T>(k, 2) * s[2] + basis<T>(k, 3) * s[3];
-    y[k + 1] =
-        basis<T>(k + 1, 0) * d[0] + basis<T>(k + 1, 1) * d[1] + basis<T>(k + 1, 2) * d[2] + basis<T>(k + 1, 3) * d[3];
+    y[k] = sum_of_four(basis<T>(k, 0), s[0], basis<T>(k, 1), s[1], basis<T>(k, 2), s[2], basis<T>(k, 3), s[3]);
+    y[k + 1] = sum_of_four(basis<T>(k + 1, 0), d[0], basis<T>(k + 1, 1), d[1], basis<T>(k + 1, 2), d[2],
+                           basis<T>(k + 1, 3), d[3]);
   }
 }
 
 // x = C^T y of eight values in registers: even and odd parts, their sum and difference.
 template <typename T>
 __device__ __forceinline__ void inverse8(const T (&y)[8], T (&x)[8]) {
+#pragma clang fp contract(off)
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
-    const T even = basis<T>(0, n) * y[0] + basis<T>(2, n) * y[2] + basis<T>(4, n) * y[4] + basis<T>(6, n) * y[6];
-    const T odd = basis<T>(1, n) * y[1] + basis<T>(3, n) * y[3] + basis<T>(5, n) * y[5] + basis<T>(7, n) * y[7];
+    const T even = sum_of_four(basis<T>(0, n), y[0], basis<T>(2, n), y[2], basis<T>(4, n), y[4], basis<T>(6, n), y[6]);
+    const T odd = sum_of_four(basis<T>(1, n), y[1], basis<T>(3, n), y[3], basis<T>(5, n), y[5], basis<T>(7, n), y[7]);
     x[n] = even + odd;
     x[7 - n] = even - odd;
   }

@@ -71,18 +71,36 @@ __device__ __forceinline__ LevelRow level_row(const Component& K, int ty, int bx
 //
 // prox_tile is work-group g's part of one component's step; prox_kernel runs a component
 // (work-groups in a run on each XCD), and the batch's kernel runs tiles of many images.
-template <typename T, typename S, int DV, int DH, template <typename> class Planes>
-__device__ __forceinline__ void prox_tile(const Component* __restrict__ component, Fields<T, S> F, int c,
-                                          T* __restrict__ zeta_kept, T* x_out, int H, int W, T descent, T step, T rho,
-                                          T rest, int keep, int g) {
-  __shared__ T forward_room[4][8][65];
-  __shared__ T inverse_room[4][8][65];
-  __shared__ device::StagedArrays staged_arrays;
-  const Component& K = *component;
+// The LDS a prox tile transposes its blocks through, a wavefront's own part each.
+template <typename T>
+struct ProxRooms {
+  T forward[4][8][65];
+  T inverse[4][8][65];
+};
+
+// Where a prox tile takes the samples of p beside its own that another work-group writes
+// in the same launch (the fused sweep's): p_x of the column left of the tile, row by row
+// from `left` (left[(i - left_top) * left_stride]), and p_y of the row above the tile from
+// `up` (up[j - up_first]), each null to read them from p in memory.
+template <typename T>
+struct ProxEdges {
+  const T* left = nullptr;
+  int left_stride = 1;
+  int left_top = 0;
+  const T* up = nullptr;
+  int up_first = 0;
+};
+
+// The tile of this wavefront in tile row ty and strip `strip` (8 tiles across) of
+// component c. staging() gives the component's constants in LDS: called once the tile's
+// loads are issued, and read after the tile's first barrier.
+template <typename T, typename S, int DV, int DH, template <typename> class Planes, typename Staging>
+__device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& staging, ProxRooms<T>& rooms,
+                                             Fields<T, S> F, int c, T* __restrict__ zeta_kept, T* x_out, int H,
+                                             int W, T descent, T step, T rho, T rest, int keep, int ty, int strip,
+                                             const ProxEdges<T>& edges) {
+#pragma clang fp contract(off)
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int strips = (K.tiles_x + 7) >> 3;
-  const int strip = g % strips;
-  const int ty = (g / strips) * 4 + wave;  // the wave's row of tiles (blocks)
   const int b = lane >> 3, n = lane & 7;
   const int tx = strip * 8 + b;  // the tile of the lane's column
   const bool live = (ty < K.tiles_y) & (tx < K.tiles_x);
@@ -111,7 +129,12 @@ __device__ __forceinline__ void prox_tile(const Component* __restrict__ componen
       for (int bb = 0; bb < DH; ++bb) out[bb] = device::widen(planes.load_if(valid, plane, where + bb));
     }
   };
-  load_row(p, live && top > 0, 1, at(top - 1, first_column), up);
+  if (edges.up != nullptr) {
+#pragma unroll
+    for (int bb = 0; bb < DH; ++bb) up[bb] = (live && top > 0) ? edges.up[first_column + bb - edges.up_first] : T(0);
+  } else {
+    load_row(p, live && top > 0, 1, at(top - 1, first_column), up);
+  }
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
 #pragma unroll
@@ -120,14 +143,18 @@ __device__ __forceinline__ void prox_tile(const Component* __restrict__ componen
       load_row(x, live, 0, at(i, first_column), here[m][a]);
       load_row(p, live, 0, at(i, first_column), across[m][a]);
       load_row(p, live, 1, at(i, first_column), down[m][a]);
-      edge[m][a] = device::widen(p.load_if(live && lane == 0 && first_column > 0, 0, at(i, first_column - 1)));
+      if (edges.left != nullptr) {
+        edge[m][a] = edges.left[(i - edges.left_top) * edges.left_stride];
+      } else {
+        edge[m][a] = device::widen(p.load_if(live && lane == 0 && first_column > 0, 0, at(i, first_column - 1)));
+      }
     }
   }
-  // In the coefficient phase a lane takes row n of block b of the strip: its levels and the
-  // constants, loaded while the samples are in flight.
+  // In the coefficient phase a lane takes row n of block b of the strip: its levels, loaded
+  // while the samples are in flight.
   const bool inside = (ty < K.rows) & (tx < K.columns);
   const LevelRow own = level_row(K, ty, tx, n, inside);
-  const device::Staged staged = device::stage(K, staged_arrays);
+  const device::Staged staged = staging();
 
   T vs[8][DV * DH], column[8];
 #pragma unroll
@@ -146,7 +173,7 @@ __device__ __forceinline__ void prox_tile(const Component* __restrict__ componen
         const T before = bb == 0 ? left : across[m][a][bb - 1];
         T div = backward_down_of(down[m][a][bb], up[bb], i, H);
         div += backward_across_of(across[m][a][bb], before, j, W);
-        const T v = here[m][a][bb] + descent * div;
+        const T v = device::fused(descent, div, here[m][a][bb]);
         up[bb] = down[m][a][bb];
         vs[m][a * DH + bb] = v;
         // The cell's sum: the rows of a column first, then the columns.
@@ -161,12 +188,12 @@ __device__ __forceinline__ void prox_tile(const Component* __restrict__ componen
   T y[8];
   device::forward8(column, y);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) forward_room[wave][k][lane] = y[k];
+  for (int k = 0; k < 8; ++k) rooms.forward[wave][k][lane] = y[k];
   __syncthreads();
   // A lane now takes row n of block b of the strip.
   T row[8], coefficient[8], zeta[8];
 #pragma unroll
-  for (int u = 0; u < 8; ++u) row[u] = forward_room[wave][n][8 * b + u];
+  for (int u = 0; u < 8; ++u) row[u] = rooms.forward[wave][n][8 * b + u];
   device::forward8(row, coefficient);
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
@@ -181,11 +208,11 @@ __device__ __forceinline__ void prox_tile(const Component* __restrict__ componen
   T back_row[8];
   device::inverse8(zeta, back_row);
 #pragma unroll
-  for (int v = 0; v < 8; ++v) inverse_room[wave][n][8 * b + v] = back_row[v];
+  for (int v = 0; v < 8; ++v) rooms.inverse[wave][n][8 * b + v] = back_row[v];
   __syncthreads();
   T back_column[8], back[8];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) back_column[m] = inverse_room[wave][m][lane];
+  for (int m = 0; m < 8; ++m) back_column[m] = rooms.inverse[wave][m][lane];
   device::inverse8(back_column, back);
   // The lane's column of tiles is one of the blocks' (a tile beyond them keeps v): the
   // lane's block in the coefficient phase is the one of its column here (b = lane / 8).
@@ -203,7 +230,7 @@ __device__ __forceinline__ void prox_tile(const Component* __restrict__ componen
         const T v = vs[m][a * DH + bb], old = here[m][a][bb];
         const T tilde = own_inside ? (DV * DH > 1 ? v + change : back[m]) : v;
         bars[bb] = T(2) * tilde - old;
-        relaxed[bb] = rho * tilde + rest * old;
+        relaxed[bb] = device::fused(rho, tilde, rest * old);
         tildes[bb] = tilde;
       }
       const uint32_t where = at(top + m * DV + a, first_column);
@@ -221,6 +248,22 @@ __device__ __forceinline__ void prox_tile(const Component* __restrict__ componen
       }
     }
   }
+}
+
+// Work-group g's part of one component's step: 4 tile rows of one strip, its constants
+// staged here.
+template <typename T, typename S, int DV, int DH, template <typename> class Planes>
+__device__ __forceinline__ void prox_tile(const Component* __restrict__ component, Fields<T, S> F, int c,
+                                          T* __restrict__ zeta_kept, T* x_out, int H, int W, T descent, T step, T rho,
+                                          T rest, int keep, int g) {
+  __shared__ ProxRooms<T> rooms;
+  __shared__ device::StagedArrays staged_arrays;
+  const Component& K = *component;
+  const int wave = threadIdx.x >> 6;
+  const int strips = (K.tiles_x + 7) >> 3;
+  prox_tile_at<T, S, DV, DH, Planes>(
+      K, [&]() { return device::stage(K, staged_arrays); }, rooms, F, c, zeta_kept, x_out, H, W, descent, step, rho,
+      rest, keep, (g / strips) * 4 + wave, g % strips, ProxEdges<T>{});
 }
 
 // The wavefronts per SIMD a prox kernel is compiled for: as many as its registers allow
@@ -376,14 +419,59 @@ __device__ __forceinline__ T lane_value(T v, int source) {
   }
 }
 
+// The arithmetic of the dual step at one sample, shared by every path that makes it (the
+// row walk, and the fused sweep's making of a neighbour's p once more), with its fused
+// multiply-adds written out (device::fused) and no other contraction, so that every path
+// gives the same bits (docs/math.md:599-611; primal_dual.hpp:121-182, 276-327).
+using device::fused;
+// w~ = w + tau (p + div_2 r) and w- = 2 w~ - w at a sample from w, p and div_2 r (dx, dy).
+template <typename T>
+__device__ __forceinline__ void w_extrapolation(const T old[2], const T q[2], T dx, T dy, T descent, T tilde[2],
+                                                T bar[2]) {
+#pragma clang fp contract(off)
+  tilde[0] = fused(descent, q[0] + dx, old[0]);
+  tilde[1] = fused(descent, q[1] + dy, old[1]);
+  bar[0] = T(2) * tilde[0] - old[0];
+  bar[1] = T(2) * tilde[1] - old[1];
+}
+
+// p + sigma gamma (grad x- - w-) (w- zero for TV), one entry.
+template <typename T>
+__device__ __forceinline__ T p_ascent(T p, T ascent, T difference, T bar_w) {
+#pragma clang fp contract(off)
+  return fused(ascent, difference - bar_w, p);
+}
+
+// |q|^2 of two entries, and of the three of a symmetric tensor (the off-diagonal twice).
+template <typename T>
+__device__ __forceinline__ T square_of(T a, T b) {
+#pragma clang fp contract(off)
+  return fused(b, b, a * a);
+}
+template <typename T>
+__device__ __forceinline__ T tensor_square_of(T xx, T yy, T xy) {
+#pragma clang fp contract(off)
+  return fused(T(2) * xy, xy, fused(yy, yy, xx * xx));
+}
+
+// rho q + (1 - rho) old.
+template <typename T>
+__device__ __forceinline__ T relaxed_of(T rho, T projected, T rest, T old) {
+#pragma clang fp contract(off)
+  return fused(rho, projected, rest * old);
+}
+
 // The gather of a work-group's edge samples for one row: the left ones for its first
 // wavefront (`left`), the right ones for its last (`right`); the strips inside the
-// work-group hand theirs over through LDS (EdgeRoom). Lanes 3c + e of the T gather hold
-// w_x, w_y on the left (e = 0, 1) and x- on the right (e = 2) of channel c; lanes 9c + e
-// of the S gather hold p_x, p_y, r_xx, r_yy, r_xy on the left (e = 0..4), r_yy and r_xy on
-// the left one row down (e = 5, 6), and r_xx, r_xy on the right (e = 7, 8).
-template <typename T, typename S, int C, bool TGV>
+// work-group hand theirs over through LDS (EdgeRoom). Lanes kT c + e of the T gather hold
+// w_x, w_y on the left (e = 0, 1), x- on the right (e = 2) and, for the fused sweep, x- on
+// the left (e = 3) of channel c; lanes 9c + e of the S gather hold p_x, p_y, r_xx, r_yy,
+// r_xy on the left (e = 0..4), r_yy and r_xy on the left one row down (e = 5, 6), and r_xx,
+// r_xy on the right (e = 7, 8); TV gathers p_x and p_y on the left for the sweep only.
+template <typename T, typename S, int C, bool TGV, bool SWEEP = false>
 struct EdgeGather {
+  static constexpr int kT = SWEEP ? 4 : 3;
+  static constexpr bool kS = TGV || SWEEP;
   const T* t_source;
   const S* s_source;
   int t_down = 0, s_down = 0;
@@ -395,21 +483,24 @@ struct EdgeGather {
     const size_t N = static_cast<size_t>(H) * W;
     const size_t left = static_cast<size_t>(max(j0 - 1, 0));
     const size_t right = static_cast<size_t>(min(j0 + kStripColumns, W - 1));
-    if (lane < 3 * C) {
-      const int c = lane / 3, e = lane % 3;
+    if (lane < kT * C) {
+      const int c = lane / kT, e = lane % kT;
       if (e == 2 && right_side) {
         t_source = F.xbar + c * N;
         t_column = right;
+      } else if (e == 3 && left_side) {
+        t_source = F.xbar + c * N;
+        t_column = left;
       } else if (TGV && e < 2 && left_side) {
         t_source = F.w + (2 * c + e) * N;
         t_column = left;
       }
     }
-    if (TGV && lane < 9 * C) {
+    if (kS && lane < 9 * C) {
       const int c = lane / 9, e = lane % 9;
       // The entry of p (e < 2) or r for each e, two bits each: 0, 1, 0, 1, 2, 1, 2, 0, 2.
       const int entry = (0x22644 >> (2 * e)) & 3;
-      if (e < 7 ? left_side : right_side) {
+      if ((TGV || e < 2) && (e < 7 ? left_side : right_side)) {
         s_source = (e < 2 ? F.p + (2 * c) * N : F.r + (3 * c) * N) + entry * N;
         s_down = (e == 5 || e == 6) ? 1 : 0;
         s_column = e < 7 ? left : right;
@@ -425,7 +516,7 @@ struct EdgeGather {
   __device__ __forceinline__ void load(int i, int H, int W, T& t, S& s) const {
     if (!gathers) return;
     t = t_source[static_cast<size_t>(min(i + t_down, H - 1)) * W + t_column];
-    if constexpr (TGV) s = s_source[static_cast<size_t>(min(i + s_down, H - 1)) * W + s_column];
+    if constexpr (kS) s = s_source[static_cast<size_t>(min(i + s_down, H - 1)) * W + s_column];
   }
 };
 
@@ -444,105 +535,189 @@ struct EdgeRoom {
   T right[2][4][C][3];
 };
 
+// What the fused sweep takes from the dual step of a band for its prox step: p_x of the
+// column left of each strip of the work-group, row by row (slot s: the column left of
+// strip s; slot 0 made once more by the first wavefront, the others handed over by each
+// strip's lane 63), and p_y of the row above a run, for the work-group's 256 columns.
+template <typename T, int C>
+struct SweepRoom {
+  T left[5][16][C];
+  T up[C][4 * kStripColumns];
+};
+
 //
-// dual_tile is work-group g's part: 4 strips across and `rows` rows down; dual_kernel runs
-// a frame (work-groups in a run on each XCD), and the batch's kernel runs tiles of many.
+// The dual step of every channel, with TGV's step of w (primal_dual.hpp:547-557, 572-735),
+// from the buffers of the iteration's start into the other ones, walked row by row: a
+// wavefront takes a strip of 64 columns, a lane per column, so that each row of each field
+// is one aligned 256-byte load. Per lane and channel it carries x- and r of the row and the
+// next one and w- of the row above, and it loads the row after next while it works on the
+// row. A neighbour in the strip comes from the neighbouring lane; the samples beyond the
+// strip's two edges (x-, r_xx, r_xy on the right for lane 63; w, p and r on the left, from
+// which lane 0 makes w- of the column on its left) come through LDS from the wavefronts
+// beside, or for the work-group's outer strips from one gather per row, a lane per sample,
+// read back as scalars.
 //
-// dual_run is the work of 4 strips across (strip group `group`) from row `first` to `end`.
+// DualWalk is the walk of the 4 strips of strip group `group` from row `first` to `end`:
+// begin() sets it up, step(i) makes row i (rows in order from `first`). Every wavefront of
+// the work-group makes the same barriers, those beyond the canvas too.
 //
 // With MAP, p's ball has a radius per sample, radius_map's (the unrolled solve's).
-template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes, bool MAP = false>
-__device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first, int end, int group, Steps<T> steps,
-                                         int keep, const T* __restrict__ radius_map = nullptr) {
-  __shared__ EdgeRoom<T, C> room;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int strip = group * 4 + wave;
-  const int j0 = strip * kStripColumns;
-  if (j0 >= W) {
-    // The whole wavefront beyond the canvas: it keeps the work-group's barriers only.
-    if (TGV && first > 0) __syncthreads();
-    for (int i = first; i < end; ++i) __syncthreads();
-    return;
+//
+// With SWEEP (the fused sweep, whose prox step follows each band of rows in the same
+// launch), the walk also hands the sweep what its prox step reads of p beside the
+// work-group's own (SweepRoom): begin() makes p_y of row first - 1 once more (the run
+// above writes it in the same launch), and step() gives p_x of the column left of each
+// strip, made once more by the first wavefront for the work-group's left edge.
+template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes, bool MAP = false,
+          bool SWEEP = false>
+struct DualWalk {
+  static_assert(!(MAP && SWEEP), "the fused sweep takes one radius");
+  using Gather = EdgeGather<T, S, C, TGV, SWEEP>;
+  static constexpr int kT = Gather::kT;
+  static constexpr int kNext = Planes<T>::kNext;
+
+  EdgeRoom<T, C>& room;
+  SweepRoom<T, C>* sweep;
+  Fields<T, S> F;
+  int H, W, first, end;
+  Steps<T> steps;
+  int keep;
+  const T* radius_map;
+  int wave, lane, j0, j;
+  bool outside, left_gathered, right_gathered, writes, right;
+  uint32_t column;
+  Planes<T> xbar;
+  Planes<S> p, p_next, tp;
+  Planes<T> w, w_next, tw;
+  Planes<S> r, r_next, tr;
+  Gather edges;
+  // Carried from row to row.
+  T bar[C], bar_next[C], r_here[C][3], r_below[C][3], p_here[C][2], w_here[C][2], w_up[C][2];
+  T edge_t = T(0);
+  S edge_s{};
+
+  __device__ __forceinline__ DualWalk(EdgeRoom<T, C>& room_, SweepRoom<T, C>* sweep_, const Fields<T, S>& F_, int H_,
+                                      int W_, int first_, int end_, int group, const Steps<T>& steps_, int keep_,
+                                      const T* radius_map_)
+      : room(room_),
+        sweep(sweep_),
+        F(F_),
+        H(H_),
+        W(W_),
+        first(first_),
+        end(end_),
+        steps(steps_),
+        keep(keep_),
+        radius_map(radius_map_),
+        wave(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6))),
+        lane(threadIdx.x & 63),
+        j0((group * 4 + wave) * kStripColumns),
+        j(j0 + (threadIdx.x & 63)),
+        outside(j0 >= W_),
+        left_gathered(wave == 0),
+        right_gathered(wave == 3),
+        writes(j < W_),
+        right(j + 1 < W_),
+        column(static_cast<uint32_t>(min(j, W_ - 1))),
+        xbar(F_.xbar, C, static_cast<size_t>(H_) * W_),
+        p(F_.p, 2 * C, static_cast<size_t>(H_) * W_),
+        p_next(F_.p_next, 2 * C, static_cast<size_t>(H_) * W_),
+        tp(F_.tp, 2 * C, static_cast<size_t>(H_) * W_),
+        w(F_.w, 2 * C, static_cast<size_t>(H_) * W_),
+        w_next(F_.w_next, 2 * C, static_cast<size_t>(H_) * W_),
+        tw(F_.tw, 2 * C, static_cast<size_t>(H_) * W_),
+        r(F_.r, 3 * C, static_cast<size_t>(H_) * W_),
+        r_next(F_.r_next, 3 * C, static_cast<size_t>(H_) * W_),
+        tr(F_.tr, 3 * C, static_cast<size_t>(H_) * W_),
+        edges(F_, threadIdx.x & 63, H_, W_, j0, wave == 0, wave == 3) {}
+
+  // The index of the lane's sample in row i, i clamped into the canvas (a clamped load only
+  // feeds differences that the canvas' edges take as zero).
+  __device__ __forceinline__ uint32_t index(int i) const {
+    return static_cast<uint32_t>(min(i, H - 1)) * static_cast<uint32_t>(W) + column;
   }
-  // The edge samples of the strips beside: through LDS from the wavefronts beside, or for
-  // the work-group's outer strips, from memory. A neighbour beyond the canvas is never
-  // read (its column is the canvas' edge, where the differences are zero).
-  const bool left_gathered = wave == 0, right_gathered = wave == 3;
-  const int j = j0 + lane;
-  const bool writes = j < W;
-  const bool right = j + 1 < W;
-  const uint32_t column = static_cast<uint32_t>(min(j, W - 1));
-  const size_t N = static_cast<size_t>(H) * W;
-  const Planes<T> xbar(F.xbar, C, N);
-  // The next iterate's p and w, which the next launch reads (r streams).
-  constexpr int kNext = Planes<T>::kNext;
-  const Planes<S> p(F.p, 2 * C, N), p_next(F.p_next, 2 * C, N), tp(F.tp, 2 * C, N);
-  const Planes<T> w(F.w, 2 * C, N), w_next(F.w_next, 2 * C, N), tw(F.tw, 2 * C, N);
-  const Planes<S> r(F.r, 3 * C, N), r_next(F.r_next, 3 * C, N), tr(F.tr, 3 * C, N);
-  const EdgeGather<T, S, C, TGV> edges(F, lane, H, W, j0, left_gathered, right_gathered);
-  // The index of the lane's sample in row i, i clamped into the canvas (a clamped load
-  // only feeds differences that the canvas' edges take as zero).
-  const auto index = [&](int i) { return static_cast<uint32_t>(min(i, H - 1)) * static_cast<uint32_t>(W) + column; };
+
   // w~ and w- at a sample from w, p and r there, r of the row below and r_xx, r_xy of the
-  // column on the right: w~ = w + tau (p + div_2 r).
-  const auto w_step = [&](const T old[2], const T q[2], const T here[3], const T next[3], T right_xx, T right_xy,
-                          bool across, bool below, T descent, T tilde[2], T bar[2]) {
+  // column on the right.
+  __device__ __forceinline__ void w_step(const T old[2], const T q[2], const T here[3], const T next[3], T right_xx,
+                                         T right_xy, bool across, bool below, T descent, T tilde[2], T bar_w[2]) const {
     const T dx = (across ? right_xx - here[0] : T(0)) + (below ? next[2] - here[2] : T(0));
     const T dy = (across ? right_xy - here[2] : T(0)) + (below ? next[1] - here[1] : T(0));
-    tilde[0] = old[0] + descent * (q[0] + dx);
-    tilde[1] = old[1] + descent * (q[1] + dy);
-    bar[0] = T(2) * tilde[0] - old[0];
-    bar[1] = T(2) * tilde[1] - old[1];
-  };
+    w_extrapolation(old, q, dx, dy, descent, tilde, bar_w);
+  }
+
+  // The scale of p's ball for the steps of every channel (coupled), or of channel c.
+  __device__ __forceinline__ T coupled_scale(const T step_p[C][2], T radius) const {
+    T square = T(0);
+#pragma unroll
+    for (int c = 0; c < C; ++c) square += square_of(step_p[c][0], step_p[c][1]);
+    return device::ball_scale(square, radius);
+  }
 
   // The rows of the start: x- and r of the first two rows, p and w of the first, the edge
-  // samples of the first row, and w- of the row above.
-  T bar[C], bar_next[C], r_here[C][3], r_below[C][3], p_here[C][2], w_here[C][2], w_up[C][2];
-  T edge_t = T(0), up_t = T(0);
-  S edge_s{}, up_s{};
-  edges.load(first, H, W, edge_t, edge_s);
-  if (TGV && first > 0) edges.load(first - 1, H, W, up_t, up_s);
-#pragma unroll
-  for (int c = 0; c < C; ++c) {
-    bar[c] = xbar.load(c, index(first));
-    bar_next[c] = xbar.load(c, index(first + 1));
-#pragma unroll
-    for (int e = 0; e < 2; ++e) p_here[c][e] = device::widen(p.load(2 * c + e, index(first)));
-    if constexpr (TGV) {
-#pragma unroll
-      for (int e = 0; e < 3; ++e) {
-        r_here[c][e] = device::widen(r.load(3 * c + e, index(first)));
-        r_below[c][e] = device::widen(r.load(3 * c + e, index(first + 1)));
-      }
-#pragma unroll
-      for (int e = 0; e < 2; ++e) w_here[c][e] = w.load(2 * c + e, index(first));
-      w_up[c][0] = w_up[c][1] = T(0);
+  // samples of the first row, and w- of the row above (and for the sweep p_y of the row
+  // above, made as the run above makes it).
+  __device__ __forceinline__ void begin() { begin_stretch(true); }
+
+  // up_row: for the sweep, p_y of the row above is made once more (a run's first stretch;
+  // the sweep walks each band as a stretch of its own).
+  __device__ __forceinline__ void begin_stretch(bool up_row) {
+    if (outside) {
+      // The whole wavefront beyond the canvas: it keeps the work-group's barriers only.
+      if ((TGV || SWEEP) && first > 0) __syncthreads();
+      return;
     }
-  }
-  if constexpr (TGV) {
-    if (first > 0) {
-      // w- of the row above, whose right column's r_xx, r_xy come from the wavefront on
-      // the right (its lane 0) through the room's slot 1 (the run's first row takes slot 0).
-      T r_up[C][3], p_up[C][2], old_up[C][2];
+    T up_t = T(0);
+    S up_s{};
+    edges.load(first, H, W, edge_t, edge_s);
+    if ((TGV || SWEEP) && first > 0) edges.load(first - 1, H, W, up_t, up_s);
 #pragma unroll
-      for (int c = 0; c < C; ++c) {
+    for (int c = 0; c < C; ++c) {
+      bar[c] = xbar.load(c, index(first));
+      bar_next[c] = xbar.load(c, index(first + 1));
+#pragma unroll
+      for (int e = 0; e < 2; ++e) p_here[c][e] = device::widen(p.load(2 * c + e, index(first)));
+      if constexpr (TGV) {
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          r_here[c][e] = device::widen(r.load(3 * c + e, index(first)));
+          r_below[c][e] = device::widen(r.load(3 * c + e, index(first + 1)));
+        }
+#pragma unroll
+        for (int e = 0; e < 2; ++e) w_here[c][e] = w.load(2 * c + e, index(first));
+        w_up[c][0] = w_up[c][1] = T(0);
+      }
+    }
+    if (!((TGV || SWEEP) && first > 0)) return;
+    // The row above: w- of it (TGV), whose right column's r_xx, r_xy come from the wavefront
+    // on the right (its lane 0) through the room's slot 1 (the run's first row takes slot
+    // 0); for the sweep also x- of it and of its right column, and p there.
+    T r_up[C][3], p_up[C][2], old_up[C][2], bar_up[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+#pragma unroll
+      for (int e = 0; e < 2; ++e) p_up[c][e] = device::widen(p.load(2 * c + e, index(first - 1)));
+      if constexpr (TGV) {
 #pragma unroll
         for (int e = 0; e < 3; ++e) r_up[c][e] = device::widen(r.load(3 * c + e, index(first - 1)));
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
-          p_up[c][e] = device::widen(p.load(2 * c + e, index(first - 1)));
-          old_up[c][e] = w.load(2 * c + e, index(first - 1));
-        }
-        if (lane == 0) {
+        for (int e = 0; e < 2; ++e) old_up[c][e] = w.load(2 * c + e, index(first - 1));
+      }
+      if constexpr (SWEEP) bar_up[c] = xbar.load(c, index(first - 1));
+      if (lane == 0) {
+        if constexpr (SWEEP) room.right[1][wave][c][0] = bar_up[c];
+        if constexpr (TGV) {
           room.right[1][wave][c][1] = r_up[c][0];
           room.right[1][wave][c][2] = r_up[c][2];
         }
       }
-      __syncthreads();
+    }
+    __syncthreads();
+    const T up_wide = device::widen(up_s);
 #pragma unroll
-      for (int c = 0; c < C; ++c) {
+    for (int c = 0; c < C; ++c) {
+      if constexpr (TGV) {
         T tilde_up[2];
-        const T up_wide = device::widen(up_s);
         const T edge_xx = right_gathered ? lane_value(up_wide, 9 * c + 7) : room.right[1][wave + 1][c][1];
         const T edge_xy = right_gathered ? lane_value(up_wide, 9 * c + 8) : room.right[1][wave + 1][c][2];
         const T shifted_xx = from_right(r_up[c][0]), shifted_xy = from_right(r_up[c][2]);
@@ -551,18 +726,47 @@ __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first
                w_up[c]);
       }
     }
+    if (SWEEP && up_row) {
+      // p of row first - 1 made as the run above makes it (step() below), its p_y kept for
+      // the prox step of this run's first band.
+      T step_p[C][2];
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const T edge_bar = right_gathered ? lane_value(up_t, kT * c + 2) : room.right[1][wave + 1][c][0];
+        const T shifted_bar = from_right(bar_up[c]);
+        const T bar_right = lane == 63 ? edge_bar : shifted_bar;
+        const T g1 = right ? bar_right - bar_up[c] : T(0);
+        const T g2 = bar[c] - bar_up[c];  // row first lies in the canvas
+        const T wx = TGV ? w_up[c][0] : T(0), wy = TGV ? w_up[c][1] : T(0);
+        step_p[c][0] = p_ascent(p_up[c][0], steps.ascent[c], g1, wx);
+        step_p[c][1] = p_ascent(p_up[c][1], steps.ascent[c], g2, wy);
+      }
+      const T shared_scale = COUPLED ? coupled_scale(step_p, steps.radius) : T(1);
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        T scale = shared_scale;
+        if constexpr (!COUPLED) scale = device::ball_scale(square_of(step_p[c][0], step_p[c][1]), steps.radius);
+        const S stored = device::narrow<S>(relaxed_of(steps.rho, step_p[c][1] * scale, steps.rest, p_up[c][1]));
+        sweep->up[c][wave * kStripColumns + lane] = device::widen(stored);
+      }
+    }
   }
 
-#pragma unroll 1
-  for (int i = first; i < end; ++i) {
+  // Row i, rows in order from `first`; `band_row` is its row within the sweep's band.
+  __device__ __forceinline__ void step(int i, int band_row = 0) {
+    if (outside) {
+      __syncthreads();
+      return;
+    }
     const bool below = i + 1 < H;
-    // The loads of the next iteration: x- and r two rows down, p and w one row down, and the
-    // edge samples one row down; none in the run's last row, whose next rows belong to the
-    // run below (an unwanted buffer load reads nothing from memory).
+    // The loads of the next row: x- and r two rows down, p and w one row down, and the edge
+    // samples one row down; none in the run's last row, whose next rows belong to the run
+    // below (an unwanted buffer load reads nothing from memory), but for the sweep's edge
+    // samples, whose x- of the left column one row down the row itself takes.
     const bool ahead = i + 1 < end;
     T bar_ahead[C], r_ahead[C][3], p_ahead[C][2], w_ahead[C][2], edge_t_ahead = T(0);
     S edge_s_ahead{};
-    if (ahead) edges.load(i + 1, H, W, edge_t_ahead, edge_s_ahead);
+    if (ahead || SWEEP) edges.load(i + 1, H, W, edge_t_ahead, edge_s_ahead);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       bar_ahead[c] = xbar.load_if(ahead, c, index(i + 2));
@@ -599,15 +803,14 @@ __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first
     // and the right column's (x-, r_xx, r_xy).
     const T edge_wide = device::widen(edge_s);
     const auto left_of = [&](int c, int e) {
-      if (left_gathered) return e >= 7 ? lane_value(edge_t, 3 * c + (e - 7)) : lane_value(edge_wide, 9 * c + e);
+      if (left_gathered) return e >= 7 ? lane_value(edge_t, kT * c + (e - 7)) : lane_value(edge_wide, 9 * c + e);
       return room.left[slot][wave - 1][c][e < 7 ? e + 2 : e - 7];
     };
     const auto right_of = [&](int c, int e) {
-      if (right_gathered) return e == 0 ? lane_value(edge_t, 3 * c + 2) : lane_value(edge_wide, 9 * c + 6 + e);
+      if (right_gathered) return e == 0 ? lane_value(edge_t, kT * c + 2) : lane_value(edge_wide, 9 * c + 6 + e);
       return room.right[slot][wave + 1][c][e];
     };
-    T step_p[C][2], step_r[C][3], tilde_w[C][2], bar_w[C][2];
-    T square = T(0), tensor_square = T(0);
+    T step_p[C][2], step_r[C][3], tilde_w[C][2], bar_w[C][2], bar_left[C][2];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const T a = steps.ascent[c];
@@ -627,74 +830,110 @@ __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first
         const T p_left[2] = {left_of(c, 0), left_of(c, 1)};
         const T r_left[3] = {left_of(c, 2), left_of(c, 3), left_of(c, 4)};
         const T r_left_below[3] = {T(0), left_of(c, 5), left_of(c, 6)};
-        T tilde_left[2], bar_left[2];
+        T tilde_left[2];
         w_step(old_left, p_left, r_left, r_left_below, r_here[c][0], r_here[c][2], true, below, steps.descent[c],
-               tilde_left, bar_left);
+               tilde_left, bar_left[c]);
         const T shifted_x = from_left(bar_w[c][0]), shifted_y = from_left(bar_w[c][1]);
-        const T left_x = lane == 0 ? bar_left[0] : shifted_x, left_y = lane == 0 ? bar_left[1] : shifted_y;
+        const T left_x = lane == 0 ? bar_left[c][0] : shifted_x, left_y = lane == 0 ? bar_left[c][1] : shifted_y;
         // E w- (docs/math.md:355): backward differences.
         const T across_x = backward_across_of(bar_w[c][0], left_x, j, W);
         const T down_y = backward_down_of(bar_w[c][1], w_up[c][1], i, H);
         const T exy =
             T(0.5) * (backward_down_of(bar_w[c][0], w_up[c][0], i, H) + backward_across_of(bar_w[c][1], left_y, j, W));
-        step_p[c][0] = p_here[c][0] + a * (g1 - bar_w[c][0]);
-        step_p[c][1] = p_here[c][1] + a * (g2 - bar_w[c][1]);
-        step_r[c][0] = r_here[c][0] + a * across_x;
-        step_r[c][1] = r_here[c][1] + a * down_y;
-        step_r[c][2] = r_here[c][2] + a * exy;
+        step_p[c][0] = p_ascent(p_here[c][0], a, g1, bar_w[c][0]);
+        step_p[c][1] = p_ascent(p_here[c][1], a, g2, bar_w[c][1]);
+        step_r[c][0] = p_ascent(r_here[c][0], a, across_x, T(0));
+        step_r[c][1] = p_ascent(r_here[c][1], a, down_y, T(0));
+        step_r[c][2] = p_ascent(r_here[c][2], a, exy, T(0));
       } else {
-        step_p[c][0] = p_here[c][0] + a * g1;
-        step_p[c][1] = p_here[c][1] + a * g2;
-      }
-      if constexpr (COUPLED) {
-        square += step_p[c][0] * step_p[c][0] + step_p[c][1] * step_p[c][1];
-        if constexpr (TGV) {
-          tensor_square +=
-              step_r[c][0] * step_r[c][0] + step_r[c][1] * step_r[c][1] + T(2) * (step_r[c][2] * step_r[c][2]);
-        }
+        step_p[c][0] = p_ascent(p_here[c][0], a, g1, T(0));
+        step_p[c][1] = p_ascent(p_here[c][1], a, g2, T(0));
       }
     }
     const T rho = steps.rho, rest = steps.rest;
     T radius = steps.radius;
     if constexpr (MAP) radius = radius_map[index(i)];
-    const T shared_scale = COUPLED ? device::ball_scale(square, radius) : T(1);
-    const T shared_tensor_scale = (COUPLED && TGV) ? device::ball_scale(tensor_square, steps.radius0) : T(1);
+    const T shared_scale = COUPLED ? coupled_scale(step_p, radius) : T(1);
+    T shared_tensor_scale = T(1);
+    if constexpr (COUPLED && TGV) {
+      T tensor_square = T(0);
+#pragma unroll
+      for (int c = 0; c < C; ++c) tensor_square += tensor_square_of(step_r[c][0], step_r[c][1], step_r[c][2]);
+      shared_tensor_scale = device::ball_scale(tensor_square, steps.radius0);
+    }
     const uint32_t at = index(i);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      if (writes) {
-        T scale = shared_scale;
-        if constexpr (!COUPLED) {
-          scale = device::ball_scale(step_p[c][0] * step_p[c][0] + step_p[c][1] * step_p[c][1], radius);
-        }
+      T scale = shared_scale;
+      if constexpr (!COUPLED) scale = device::ball_scale(square_of(step_p[c][0], step_p[c][1]), radius);
+      S stored_x{};
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
-          const T projected = step_p[c][e] * scale;
+      for (int e = 0; e < 2; ++e) {
+        const T projected = step_p[c][e] * scale;
+        const S stored = device::narrow<S>(relaxed_of(rho, projected, rest, p_here[c][e]));
+        if (e == 0) stored_x = stored;
+        if (writes) {
           if (keep & kKeepDualP) tp.template store<device::kStreaming>(2 * c + e, at, device::narrow<S>(projected));
-          p_next.template store<kNext>(2 * c + e, at,
-                                                   device::narrow<S>(rho * projected + rest * p_here[c][e]));
+          p_next.template store<kNext>(2 * c + e, at, stored);
         }
-        if constexpr (TGV) {
+      }
+      if constexpr (SWEEP) {
+        // p_x of the strip's last column, the left column of the strip on the right.
+        if (lane == 63) sweep->left[wave + 1][band_row][c] = device::widen(stored_x);
+      }
+      if constexpr (TGV) {
+        if (writes) {
 #pragma unroll
           for (int e = 0; e < 2; ++e) {
             if (keep) tw.template store<device::kStreaming>(2 * c + e, at, tilde_w[c][e]);
-            w_next.template store<kNext>(2 * c + e, at, rho * tilde_w[c][e] + rest * w_here[c][e]);
+            w_next.template store<kNext>(2 * c + e, at, relaxed_of(rho, tilde_w[c][e], rest, w_here[c][e]));
           }
-          T tensor_scale = shared_tensor_scale;
-          if constexpr (!COUPLED) {
-            tensor_scale = device::ball_scale(step_r[c][0] * step_r[c][0] + step_r[c][1] * step_r[c][1] +
-                                                  T(2) * (step_r[c][2] * step_r[c][2]),
-                                              steps.radius0);
-          }
+        }
+        T tensor_scale = shared_tensor_scale;
+        if constexpr (!COUPLED) {
+          tensor_scale =
+              device::ball_scale(tensor_square_of(step_r[c][0], step_r[c][1], step_r[c][2]), steps.radius0);
+        }
+        if (writes) {
 #pragma unroll
           for (int e = 0; e < 3; ++e) {
             const T projected = step_r[c][e] * tensor_scale;
             if (keep) tr.template store<device::kStreaming>(3 * c + e, at, device::narrow<S>(projected));
-            r_next.template store<device::kStreaming>(3 * c + e, at, device::narrow<S>(rho * projected + rest * r_here[c][e]));
+            r_next.template store<device::kStreaming>(3 * c + e,
+                                                      at, device::narrow<S>(relaxed_of(rho, projected, rest,
+                                                                                       r_here[c][e])));
           }
         }
       }
-      // The next row's values become the current ones.
+    }
+    if constexpr (SWEEP) {
+      if (left_gathered) {
+        // p_x of the column left of the work-group, made as the work-group on the left
+        // makes it (above), from the edge samples: lane 0's own x- is its right neighbour.
+        T left_step[C][2];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const T x_left = lane_value(edge_t, kT * c + 3), x_left_below = lane_value(edge_t_ahead, kT * c + 3);
+          const T x_right = lane_value(bar[c], 0);
+          const T g1 = x_right - x_left;  // the column left of a strip has one on its right
+          const T g2 = below ? x_left_below - x_left : T(0);
+          const T wx = TGV ? bar_left[c][0] : T(0), wy = TGV ? bar_left[c][1] : T(0);
+          left_step[c][0] = p_ascent(left_of(c, 0), steps.ascent[c], g1, wx);
+          left_step[c][1] = p_ascent(left_of(c, 1), steps.ascent[c], g2, wy);
+        }
+        const T left_shared = COUPLED ? coupled_scale(left_step, radius) : T(1);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          T scale = left_shared;
+          if constexpr (!COUPLED) scale = device::ball_scale(square_of(left_step[c][0], left_step[c][1]), radius);
+          const S stored = device::narrow<S>(relaxed_of(rho, left_step[c][0] * scale, rest, left_of(c, 0)));
+          if (lane == 0) sweep->left[0][band_row][c] = device::widen(stored);
+        }
+      }
+    }
+    // The next row's values become the current ones.
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
       bar[c] = bar_next[c];
       bar_next[c] = bar_ahead[c];
 #pragma unroll
@@ -715,8 +954,22 @@ __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first
     edge_t = edge_t_ahead;
     edge_s = edge_s_ahead;
   }
+};
+
+// The work of 4 strips across (strip group `group`) from row `first` to `end`.
+template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes, bool MAP = false>
+__device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first, int end, int group, Steps<T> steps,
+                                         int keep, const T* __restrict__ radius_map = nullptr) {
+  __shared__ EdgeRoom<T, C> room;
+  DualWalk<T, S, C, TGV, COUPLED, Planes, MAP> walk(room, nullptr, F, H, W, first, end, group, steps, keep,
+                                                    radius_map);
+  walk.begin();
+#pragma unroll 1
+  for (int i = first; i < end; ++i) walk.step(i);
 }
 
+// dual_tile is work-group g's part: 4 strips across and `rows` rows down; dual_kernel runs
+// a frame (work-groups in a run on each XCD), and the batch's kernel runs tiles of many.
 template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes, bool MAP = false>
 __device__ __forceinline__ void dual_tile(Fields<T, S> F, int H, int W, int rows, int groups_across,
                                           Steps<T> steps, int keep, int g, const T* __restrict__ radius_map = nullptr) {

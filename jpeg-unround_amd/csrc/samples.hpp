@@ -200,6 +200,9 @@ class SamplesSolver : private SamplesShape, public Solver<T, S> {
     HIP_CHECK(hipStreamSynchronize(s));
   }
 
+  // The fused sweep's proximal step is a JPEG frame's.
+  bool fusable() const override { return false; }
+
   void primal_step(bool keep) override {
     hipStream_t s = this->stream();
     const bool per_kernel = this->timing_ == Timing::kernels;

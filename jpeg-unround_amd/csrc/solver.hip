@@ -77,6 +77,7 @@ template <typename T, typename S>
 struct Fields {
   T* x;
   T* xbar;
+  T* xbar_next;  // the other buffer of x-, which the fused sweep writes
   T* tx;
   S* p;
   S* p_next;
@@ -104,6 +105,7 @@ struct Steps {
 constexpr int kMaxCell = 4;
 
 #include "iterate.hpp"
+#include "sweep.hpp"
 
 // ---------------------------------------------------------------------------------------
 // The start (primal_dual.hpp:401-432; frames.cpp:569-593): x = the canvas of the
@@ -842,7 +844,11 @@ class Solver {
           HIP_CHECK(hipEventRecord(launches_.back().begin, s));
           segment_open = true;
         }
-        iterate(keep);
+        // The dual step of iteration `next` takes the proximal step of the one after it into
+        // the same launch where the records do not read between them.
+        const uint64_t after = next + 1;
+        const bool keep_after = (every > 0 && after % every == 0) || after == most;
+        iterate(keep, next < most && !keep, keep_after);
         if (timing_ != Timing::none) launches_.back().iterations += 1;
         if (timing_ == Timing::kernels) HIP_CHECK(hipEventRecord(launches_.back().end, s));
         iteration = next;
@@ -859,6 +865,7 @@ class Solver {
     }
     out.iterations = iteration;
     out.stop = solving ? stop : 0;
+    primal_done_ = false;
     finish(out);
   }
 
@@ -878,7 +885,7 @@ class Solver {
   // Arena's alignment); after shape().
   size_t device_bytes() const {
     const size_t C = static_cast<size_t>(C_);
-    size_t t_count = 3 * C * N_, s_count = 3 * 2 * C * N_;
+    size_t t_count = 4 * C * N_, s_count = 3 * 2 * C * N_;
     if (tgv_) {
       t_count += 3 * 2 * C * N_;
       s_count += 3 * 3 * C * N_;
@@ -915,6 +922,7 @@ class Solver {
     shape();
     const bool solving = settings_.method == Method::tv || settings_.method == Method::tgv;
     dual_rows_ = dual_rows_for(H_, W_);
+    fusable_ = fusable();
     if (const char* rows = std::getenv("UNROUND_DUAL_ROWS")) dual_rows_ = std::max(1, std::atoi(rows));
     hipStream_t s = stream();
     allocate();
@@ -1001,7 +1009,7 @@ class Solver {
 
   void allocate() {
     const size_t C = static_cast<size_t>(C_);
-    size_t t_count = 3 * C * N_;      // x, xbar, tx
+    size_t t_count = 4 * C * N_;      // x, xbar, xbar_next, tx
     size_t s_count = 3 * 2 * C * N_;  // p, p_next, tp
     if (tgv_) {
       t_count += 3 * 2 * C * N_;  // w, w_next, tw
@@ -1022,6 +1030,7 @@ class Solver {
     fields_ = Fields<T, S>{};
     fields_.x = take(C * N_);
     fields_.xbar = take(C * N_);
+    fields_.xbar_next = take(C * N_);
     fields_.tx = take(C * N_);
     if (tgv_) {
       fields_.w = take(2 * C * N_);
@@ -1196,11 +1205,31 @@ class Solver {
     }
   }
 
-  void iterate(bool keep) {
+  // Whether the fused sweep takes this frame: asked for (UNROUND_SWEEP=1), and a layout it
+  // has: one component, or three whose first has cells of one sample and the other two
+  // cells of 1 x 1 or 2 x 2 (4:4:4, 4:2:0); binary32 iterates; buffer-resource planes. It
+  // makes the same values as the two kernels, but measured slower (its tiles' registers
+  // leave 2 or 3 wavefronts a SIMD), so the two kernels are the default.
+  virtual bool fusable() const {
+    if (sizeof(T) != sizeof(float)) return false;
+    const char* sweep = std::getenv("UNROUND_SWEEP");
+    if (sweep == nullptr || std::strcmp(sweep, "1") != 0) return false;
+    if (!(narrow_spans() && prox_buffers())) return false;
+    if (C_ == 1) return frame_.components[0].down == 1 && frame_.components[0].across == 1;
+    const ComponentModel &y = frame_.components[0], &b = frame_.components[1], &r = frame_.components[2];
+    if (y.down != 1 || y.across != 1 || b.down != r.down || b.across != r.across) return false;
+    return (b.down == 1 && b.across == 1) || (b.down == 2 && b.across == 2);
+  }
+
+  // One iteration: its proximal step (unless the last launch made it), then its dual step,
+  // alone or fused with the proximal step of the next iteration (`fuse`, the next keeping
+  // its tilde point where keep_after).
+  void iterate(bool keep, bool fuse, bool keep_after) {
     hipStream_t s = stream();
     const T rho = static_cast<T>(plan_.relaxation), rest = static_cast<T>(1.0 - plan_.relaxation);
     const bool per_kernel = timing_ == Timing::kernels;
-    primal_step(keep);
+    if (!primal_done_) primal_step(keep);
+    fuse = fuse && fusable_;
     if (per_kernel) dual_events_.emplace_back();
     Steps<T> steps{};
     for (int c = 0; c < C_; ++c) {
@@ -1214,7 +1243,18 @@ class Solver {
     // p~ is read by the records of TV and by the partial gap's only.
     const int k = keep ? kKeepTilde | (!tgv_ || plan_.has_partial ? kKeepDualP : 0) : 0;
     if (per_kernel) HIP_CHECK(hipEventRecord(dual_events_.back().begin, s));
-    if (C_ == 1) {
+    if (fuse) {
+      if constexpr (sizeof(T) == sizeof(float)) {
+        const int kept = keep_after ? 1 : 0;
+        if (C_ == 1) {
+          tgv_ ? launch_sweep<1, true, false>(steps, k, kept) : launch_sweep<1, false, false>(steps, k, kept);
+        } else if (coupled_) {
+          tgv_ ? launch_sweep<3, true, true>(steps, k, kept) : launch_sweep<3, false, true>(steps, k, kept);
+        } else {
+          tgv_ ? launch_sweep<3, true, false>(steps, k, kept) : launch_sweep<3, false, false>(steps, k, kept);
+        }
+      }
+    } else if (C_ == 1) {
       tgv_ ? launch_dual<1, true, false>(steps, k) : launch_dual<1, false, false>(steps, k);
     } else if (coupled_) {
       tgv_ ? launch_dual<3, true, true>(steps, k) : launch_dual<3, false, true>(steps, k);
@@ -1226,6 +1266,36 @@ class Solver {
     std::swap(fields_.p, fields_.p_next);
     std::swap(fields_.w, fields_.w_next);
     std::swap(fields_.r, fields_.r_next);
+    if (fuse) std::swap(fields_.xbar, fields_.xbar_next);
+    primal_done_ = fuse;
+  }
+
+  // The fused sweep: the dual step of this iteration and the proximal step of the next
+  // (keep_prox: that one's tilde point kept), in runs of whole bands.
+  template <int C, bool TGV, bool COUPLED>
+  void launch_sweep(const Steps<T>& steps, int keep_dual, int keep_prox) {
+    ProxSteps<T> prox{};
+    for (int c = 0; c < C_; ++c) {
+      prox.descent[c] = static_cast<T>(plan_.tau * plan_.gamma[c]);
+      prox.step[c] = static_cast<T>(plan_.tau / static_cast<double>(frame_.components[c].cells()));
+    }
+    const bool halves = C == 3 && frame_.components[1].down == 2;
+    const int band = halves ? 16 : 8;
+    const int rows = std::max(dual_rows_, band) / band * band;
+    const int strips = (W_ + kStripColumns - 1) / kStripColumns;
+    const int across = (strips + 3) / 4, down = (H_ + rows - 1) / rows;
+    const dim3 grid(across * down);
+    const auto go = [&](auto kernel) {
+      kernel<<<grid, 256, 0, stream()>>>(component(0), fields_, H_, W_, rows, across, steps, prox, keep_dual,
+                                         keep_prox);
+    };
+    if constexpr (C == 1) {
+      go(sweep_kernel<T, S, 1, TGV, COUPLED, 1, 1, device::BufferPlanes>);
+    } else if (halves) {
+      go(sweep_kernel<T, S, 3, TGV, COUPLED, 2, 2, device::BufferPlanes>);
+    } else {
+      go(sweep_kernel<T, S, 3, TGV, COUPLED, 1, 1, device::BufferPlanes>);
+    }
   }
 
   // The durations of the launches since the last collection, then their events freed.
@@ -1437,6 +1507,10 @@ class Solver {
   int dual_rows_ = 32;
   Arena* arena_ = nullptr;
   double* planes_into_ = nullptr;
+  // Whether the last launch made the next iteration's proximal step (the fused sweep), and
+  // whether this frame takes the sweep at all.
+  bool primal_done_ = false;
+  bool fusable_ = false;
 };
 
 #include "batch.hpp"

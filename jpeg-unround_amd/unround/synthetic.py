@@ -138,14 +138,30 @@ def encode_grey(picture: np.ndarray, quality: int) -> jpegio.Image:
     return jpegio.make_image(height, width, jpegio.ColorSpace.GRAYSCALE, (component,))
 
 
-def encode_colour420(picture: np.ndarray, quality: int) -> jpegio.Image:
-    """An RGB picture as JFIF YCbCr 4:2:0 quantized at a quality with the scaled tables."""
-    height, width, _ = picture.shape
+def _ycbcr_planes(picture: np.ndarray) -> list[np.ndarray]:
+    """JFIF's Y, Cb and Cr of an RGB picture, rounded to 8 bits as an encoder stores them."""
     rgb = picture.astype(np.float64)
     y = 0.299 * rgb[..., 0] + 0.587 * rgb[..., 1] + 0.114 * rgb[..., 2]
     cb = 128.0 + (rgb[..., 2] - y) / 1.772
     cr = 128.0 + (rgb[..., 0] - y) / 1.402
-    planes = [np.clip(np.rint(p), 0, 255) for p in (y, cb, cr)]
+    return [np.clip(np.rint(p), 0, 255) for p in (y, cb, cr)]
+
+
+def encode_colour444(picture: np.ndarray, quality: int) -> jpegio.Image:
+    """An RGB picture as JFIF YCbCr without subsampling, quantized at a quality with the
+    scaled luma table for Y and chroma table for Cb and Cr."""
+    height, width, _ = picture.shape
+    rows, columns = -(-height // 8) * 8, -(-width // 8) * 8
+    tables = (ijg_table(LUMA, quality), ijg_table(CHROMA, quality), ijg_table(CHROMA, quality))
+    components = tuple(jpegio.make_component(_levels(_padded(plane, rows, columns), table), table, 1, 1)
+                       for plane, table in zip(_ycbcr_planes(picture), tables))
+    return jpegio.make_image(height, width, jpegio.ColorSpace.YCBCR, components)
+
+
+def encode_colour420(picture: np.ndarray, quality: int) -> jpegio.Image:
+    """An RGB picture as JFIF YCbCr 4:2:0 quantized at a quality with the scaled tables."""
+    height, width, _ = picture.shape
+    planes = _ycbcr_planes(picture)
     rows, columns = -(-height // 16) * 16, -(-width // 16) * 16
     luma = _padded(planes[0], rows, columns)
     chroma = []
