@@ -24,8 +24,22 @@ scaled by a copy kernel's reading), where profiles/traffic.json holds them for t
 workload. iteration_achieved is the same for a whole iteration (2 x the state's bytes +
 2 x the coefficients per sample, each int16 level read once).
 
-cpu_baseline: the CPU oracle (binary64, one thread) on a bounded sample of the same kind
-of workload, timed on this host.
+accuracy: the timed solve's result against the same solve in binary64 on the GPU (largest
+difference of the canvases, PSNR against the uncompressed picture, requantization).
+
+cpu_baseline: the CPU oracle (binary64; its iterations threaded over bands of rows, the
+same bits on any count of threads) on the workload itself, as the reference's throughput
+harness times its solvers (rust/unround/tests/throughput.rs:21-61): fixed iterations with
+no tolerance and no records between them, the least of 5 single iterations, setup and
+records excluded; on 1 thread and on every core this process may use (value), with nproc
+and the CPU model.
+
+batch (the default run, and --config c3): the mixed batch of 64 pictures (C3) solved
+together (unround_solve_batch) and one at a time (unround_solve), TGV to a relative gap of
+1e-4 checked every 10 iterations with the reference's default cap of 10000 TGV iterations
+(whatever --steps says): images/s both ways, per-image latency, how many pictures the gap
+stopped, whether both ways give the same canvases, and PSNR against binary64 on every
+eighth picture. --no-batch leaves it out.
 
 --config png: the same measurement for a picture of rounded samples (unround_solve_samples,
 the solve behind a PNG file): 2048 x 2048 RGB levels of 8 bits, TGV in binary32; its
@@ -91,57 +105,70 @@ def field_bytes(method: str, precision: str) -> float:
     return iterate + 2 * dual if method == "tv" else 3 * iterate + 5 * dual
 
 
-def cpu_baseline(method: str, budget_seconds: float = 12.0, samples: bool = False) -> dict:
-    """The oracle on a 512 x 512 sample of the workload's kind, about budget_seconds of work."""
+def cpu_description() -> dict:
+    """nproc, the cores this process may use, and the CPU's model name."""
+    model = ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    limit = os.environ.get("OMP_NUM_THREADS")
+    if limit and limit.isdigit() and int(limit) > 0:
+        usable = min(usable, int(limit))
+    return {"nproc": os.cpu_count(), "usable": usable, "model": model}
+
+
+def cpu_baseline(image, method: str, samples: bool = False, iterations: int = 5) -> dict:
+    """The oracle on the workload itself, timed as the reference's throughput harness times its
+    solvers: `iterations` iterations with no tolerance and no records between them, the least
+    of single iterations (setup and the final record excluded), on 1 thread and on every usable
+    core; Mpixel-iterations/s."""
     import numpy as np
 
     from oracle import oracle
-    from unround import synthetic
 
+    cpu = cpu_description()
+    options = ["--method", method, "--iterations", str(iterations), "--tolerance", "0", "--record-every", "0"]
     if samples:
-        levels = np.ascontiguousarray(np.moveaxis(samples_levels(512, 512), -1, 0))
+        levels = np.ascontiguousarray(np.moveaxis(image, -1, 0))
+        count = int(levels.size)
 
-        def run_samples(iterations: int) -> tuple[float, int]:
-            options = ["--method", method, "--iterations", str(iterations), "--tolerance", "0", "--record-every", "0"]
-            began = time.perf_counter()
-            result = oracle.solve_samples(levels, 8, options)
-            return time.perf_counter() - began, int(np.prod(result.picture.shape))
-
-        short, _ = run_samples(2)
-        longer, _ = run_samples(6)
-        iterations = max(4, int(budget_seconds / max((longer - short) / 4, 1e-6)))
-        seconds, count = run_samples(iterations)
-        return {"value": round(count * iterations / seconds / 1e6, 3), "unit": "Mpixel-iters/s", "cores": 1,
-                "kind": "port", "sample": f"512x512 RGB 8-bit rounded samples TGV, {iterations} iterations in "
-                                          f"{seconds:.1f} s (records and setup included)"}
-    if method == "tv":
-        picture = synthetic.grey_picture(512, 512, 0)
-        image = synthetic.encode_grey(picture, 50)
-        kind = "512x512 grey q50 TV"
+        def run() -> float:
+            return oracle.solve_samples(levels, 8, options).least_iteration_seconds
     else:
-        picture = synthetic.colour_picture(512, 512, 0)
-        image = synthetic.encode_colour420(picture, 75)
-        kind = "512x512 RGB YCbCr 4:2:0 q75 TGV"
-    components = [oracle.Component(np.asarray(c.coefficients), np.asarray(c.quant_table).reshape(64),
-                                   c.h_samp_factor, c.v_samp_factor) for c in image.components]
+        components = [oracle.Component(np.asarray(c.coefficients), np.asarray(c.quant_table).reshape(64),
+                                       c.h_samp_factor, c.v_samp_factor) for c in image.components]
+        count = 0
 
-    def run(iterations: int) -> tuple[float, int]:
-        options = ["--method", method, "--iterations", str(iterations), "--tolerance", "0", "--record-every", "0"]
-        began = time.perf_counter()
-        result = oracle.solve(image.height, image.width, int(image.color_space), components, options)
-        return time.perf_counter() - began, int(np.prod(result.canvas.shape))
+        def run() -> float:
+            nonlocal count
+            result = oracle.solve(int(image.height), int(image.width), int(image.color_space), components, options)
+            count = int(np.prod(result.canvas.shape))
+            return result.least_iteration_seconds
 
-    short, _ = run(2)
-    longer, _ = run(6)
-    per_iteration = max((longer - short) / 4, 1e-6)
-    iterations = max(4, int(budget_seconds / per_iteration))
-    seconds, samples = run(iterations)
+    rates = {}
+    for threads in (1, cpu["usable"]):
+        oracle.set_threads(threads)
+        least = run()
+        rates[threads] = count * 1.0 / least / 1e6
+        log(f"cpu oracle: {threads} thread(s), least iteration {least:.4f} s, {rates[threads]:.2f} Mpixel-it/s")
+    oracle.set_threads(0)
     return {
-        "value": round(samples * iterations / seconds / 1e6, 3),
+        "value": round(rates[cpu["usable"]], 3),
         "unit": "Mpixel-iters/s",
-        "cores": 1,
+        "cores": cpu["usable"],
         "kind": "port",
-        "sample": f"{kind}, {iterations} iterations in {seconds:.1f} s (records and setup included)",
+        "single_thread": round(rates[1], 3),
+        "nproc": cpu["nproc"],
+        "cpu_model": cpu["model"],
+        "sample": f"the workload itself ({count} canvas samples), least of {iterations} single iterations on 1 "
+                  f"and on {cpu['usable']} threads, binary64, setup and records excluded",
+        "reference_published": "TGV colour 1024x1024 4:2:0, Rust, 1 thread: 94 Mpixel-it/s; 4 threads: 183-210 "
+                               "(i7-1360P, experiments/results/performance.md:25,33)",
     }
 
 
@@ -202,9 +229,13 @@ def psnr_of(decoded, original) -> float:
     return float(10 * np.log10(255.0 ** 2 / max(error, 1e-30)))
 
 
-def batch_main(args) -> None:
-    """C3: the mixed batch, solved together (unround_solve_batch) and one image at a time
-    (unround_solve), TGV to a relative gap of 1e-4 checked every 10 iterations."""
+BATCH_CAP = 10000  # the reference's default cap of TGV iterations (cpp/include/unround/pdhg.hpp:45)
+
+
+def batch_measure(precision: str, cap: int = BATCH_CAP) -> dict:
+    """C3: the mixed batch solved together (unround_solve_batch) and one image at a time
+    (unround_solve), TGV to a relative gap of 1e-4 checked every 10 iterations, at most `cap`
+    iterations; with PSNR against the same pictures solved in binary64 (every eighth)."""
     import numpy as np
     import torch
 
@@ -212,70 +243,87 @@ def batch_main(args) -> None:
 
     images, originals, kinds = batch_workload()
     options = ["--method", "tgv", "--tolerance", "0", "--relative-tolerance", "1e-4", "--record-every", "10",
-               "--iterations", str(args.steps), "--precision", args.precision]
+               "--iterations", str(cap), "--precision", precision]
     log(f"C3: {len(images)} images, {sum(k == 'colour' for k in kinds)} 4:2:0 colour; options {' '.join(options)}")
-    # Warm-up: the library, the kernels and the batch's choice of rows.
-    native.solve_batch(images[:4], options)
-    native.solve_batch(images, options[:-4] + ["--iterations", "20", "--precision", args.precision])
+    # Warm-up: the library, the kernels and the batch's choice of rows for this shape.
+    native.solve_batch(images[:4], options[:-4] + ["--iterations", "20", "--precision", precision])
+    native.solve_batch(images, options[:-4] + ["--iterations", "20", "--precision", precision])
     torch.cuda.synchronize()
     began = time.perf_counter()
     together, latencies = native.solve_batch(images, options)
     torch.cuda.synchronize()
     batch_seconds = time.perf_counter() - began
+    log(f"C3 batched: {batch_seconds:.3f} s")
     began = time.perf_counter()
     alone = [native.solve(image, options) for image in images]
     torch.cuda.synchronize()
     plain_seconds = time.perf_counter() - began
+    log(f"C3 one at a time: {plain_seconds:.3f} s")
     same = all(np.array_equal(a.canvas, b.canvas) for a, b in zip(together, alone))
-    reference = None
-    if args.precision != "double":
-        reference, _ = native.solve_batch(images, options[:-2] + ["--precision", "double"])
+    picked = list(range(0, len(images), 8))
     psnrs = [psnr_of(d, o) for d, o in zip(together, originals)]
-    deltas = ([abs(p - psnr_of(r, o)) for p, r, o in zip(psnrs, reference, originals)]
-              if reference is not None else [0.0])
+    deltas = [0.0]
+    if precision != "double":
+        reference, _ = native.solve_batch([images[k] for k in picked], options[:-2] + ["--precision", "double"])
+        deltas = [abs(psnrs[k] - psnr_of(r, originals[k])) for k, r in zip(picked, reference)]
     iterations = [d.result.iterations for d in together]
+    by_gap = sum(d.result.stop == native.Stop.CONVERGED for d in together)
     channel_samples = [int(np.prod(d.canvas.shape)) for d in together]
-    per_sample = 2 * field_bytes("tgv", args.precision)
+    per_sample = 2 * field_bytes("tgv", precision) + 1  # the state each way and the levels
     moved = sum(n * k * per_sample for n, k in zip(channel_samples, iterations))
     log(f"batched {batch_seconds:.3f} s, one at a time {plain_seconds:.3f} s; identical canvases {same}; "
-        f"iterations {min(iterations)}..{max(iterations)} (mean {np.mean(iterations):.0f}); "
-        f"PSNR {min(psnrs):.2f}..{max(psnrs):.2f} dB, largest |dPSNR| vs binary64 {max(deltas):.4f} dB")
-    value = len(images) / batch_seconds
+        f"iterations {min(iterations)}..{max(iterations)} (mean {np.mean(iterations):.0f}), {by_gap} stopped by the "
+        f"gap; PSNR {min(psnrs):.2f}..{max(psnrs):.2f} dB, largest |dPSNR| vs binary64 {max(deltas):.5f} dB")
+    return {
+        "workload": "C3: 64 images, sides U{96..1920 step 8}, q U[30,90], grey or 4:2:0, seeds 1..64, TGV to "
+                    "relative gap 1e-4 (every 10 iterations)",
+        "images": len(images),
+        "colour_images": sum(k == "colour" for k in kinds),
+        "samples": int(sum(channel_samples)),
+        "iteration_cap": cap,
+        "precision": precision,
+        "images_per_s": round(len(images) / batch_seconds, 3),
+        "plain_images_per_s": round(len(images) / plain_seconds, 3),
+        "speedup_vs_plain": round(plain_seconds / batch_seconds, 3),
+        "batch_seconds": round(batch_seconds, 3),
+        "plain_seconds": round(plain_seconds, 3),
+        "latency_p50_s": round(float(np.percentile(latencies, 50)), 4),
+        "latency_p99_s": round(float(np.percentile(latencies, 99)), 4),
+        "stopped_by_gap": int(by_gap),
+        "iterations_min_mean_max": [int(min(iterations)), round(float(np.mean(iterations)), 1), int(max(iterations))],
+        "identical_to_plain": bool(same),
+        "psnr_delta_vs_f64_max_db": round(max(deltas), 6),
+        "psnr_reference_images": len(picked),
+        "whole_solve_achieved_gbs": round(moved / batch_seconds / 1e9, 1),
+        "whole_solve_frac": round(moved / batch_seconds / 1e9 / HBM_PEAK_GBS, 4),
+    }
+
+
+def batch_main(args) -> None:
+    """--config c3: the batch's measurement as the line's value (images/s)."""
+    batch = batch_measure(args.precision)
     line = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": batch["images_per_s"],
         "unit": "images/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": batch["iteration_cap"],
         "warmup": 1,
-        "ms_per_step": round(batch_seconds / len(images) * 1e3, 3),
+        "ms_per_step": round(batch["batch_seconds"] / batch["images"] * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"single": "f32", "mixed": "f32+bf16", "double": "f64"}[args.precision],
         "data": "synthetic",
-        "config": {
-            "workload": "C3: 64 images, sides U{96..1920 step 8}, q U[30,90], grey or 4:2:0, seeds 1..64, TGV to "
-                        "relative gap 1e-4 (every 10 iterations)",
-            "images": len(images),
-            "colour_images": sum(k == "colour" for k in kinds),
-            "samples": int(sum(channel_samples)),
-            "iteration_cap": args.steps,
-            "precision": args.precision,
-        },
-        "plain_images_per_s": round(len(images) / plain_seconds, 3),
-        "speedup_vs_plain": round(plain_seconds / batch_seconds, 3),
-        "latency_p50_s": round(float(np.percentile(latencies, 50)), 4),
-        "latency_p99_s": round(float(np.percentile(latencies, 99)), 4),
-        "identical_to_plain": bool(same),
-        "psnr_delta_vs_f64_max_db": round(max(deltas), 5),
+        "config": {"workload": batch["workload"], "images": batch["images"], "precision": args.precision},
+        "batch": batch,
         "roofline": {
             "bound": "hbm",
             "kernel": "the batch's iterations (prox and dual launches over all images), whole solve",
-            "achieved": round(moved / batch_seconds / 1e9, 1),
+            "achieved": batch["whole_solve_achieved_gbs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(moved / batch_seconds / 1e9 / HBM_PEAK_GBS, 4),
+            "frac": batch["whole_solve_frac"],
             "traffic": None,
         },
         "cpu_baseline": None,
@@ -357,6 +405,7 @@ def main() -> None:
     parser.add_argument("--config", default="c2")
     parser.add_argument("--precision", default="single", choices=["single", "mixed", "double"])
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--no-batch", action="store_true", help="leave the C3 batch out of the default run")
     args = parser.parse_args()
     if args.gpus != 1:
         raise SystemExit("this benchmark runs on one GPU: the path shards by picture across GPUs with no "
@@ -453,7 +502,7 @@ def main() -> None:
             "iteration_achieved": round(iteration_achieved, 1),
             "iteration_frac": round(iteration_achieved / HBM_PEAK_GBS, 4),
         },
-        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(method, samples=rounded_samples),
+        "cpu_baseline": None,
     }
     if rounded_samples:
         # The per-sample proximal step reads x, p_x, p_y and the level and writes x- and x.
@@ -465,15 +514,19 @@ def main() -> None:
                                       "launch_us": round(primal_launch * 1e6, 2),
                                       "achieved": round(primal_achieved, 1),
                                       "frac": round(primal_achieved / HBM_PEAK_GBS, 4)}
-    if args.config == "c4" and args.precision != "double":
-        line["accuracy"] = accuracy_against_binary64(image, decoded, base, args.steps)
+    if args.config in ("c2", "c4") and args.precision != "double":
+        line["accuracy"] = accuracy_against_binary64(args.config, image, decoded, base, args.steps)
+    if args.config == "c2" and not args.no_batch:
+        line["batch"] = batch_measure(args.precision)
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(image, method, samples=rounded_samples)
     print(json.dumps(line), flush=True)
 
 
-def accuracy_against_binary64(image, decoded, base: list[str], steps: int) -> dict:
+def accuracy_against_binary64(config: str, image, decoded, base: list[str], steps: int) -> dict:
     """The timed solve's result against the same solve in binary64 on the GPU: the largest
-    difference of the canvases, of PSNR against the uncompressed picture, and whether every
-    block of both requantizes to the file's levels."""
+    difference of the canvases, PSNR against the uncompressed picture (RGB of a colour one),
+    and whether every block of both requantizes to the file's levels."""
     import numpy as np
 
     from unround import native, synthetic
@@ -481,23 +534,30 @@ def accuracy_against_binary64(image, decoded, base: list[str], steps: int) -> di
     options = [value for value in base]
     options[options.index("--precision") + 1] = "double"
     reference = native.solve(image, options + ["--iterations", str(steps), "--record-every", "0"])
-    original = synthetic.grey_picture(int(image.height), int(image.width), 0)
     h, w = int(image.height), int(image.width)
+    colour = len(image.components) == 3
+    original = (synthetic.colour_picture(h, w, 0) if colour else synthetic.grey_picture(h, w, 0)).astype(np.float64)
 
-    def psnr(planes):
-        error = np.mean((np.clip(np.round(planes[0, :h, :w]), 0, 255) - original) ** 2)
+    def psnr(result):
+        picture = native.to_rgb(result.planes) if colour else result.planes[0]
+        error = np.mean((np.clip(np.round(picture), 0, 255) - original) ** 2)
         return float(10 * np.log10(255.0**2 / error))
 
     def exact(result):
-        return bool(np.array_equal(synthetic.requantized(result.canvas[0], image, 0), image.components[0].coefficients))
+        for index, problem in enumerate(result.problems):
+            if not np.array_equal(synthetic.requantized(result.canvas[index], image, index, problem.ratio),
+                                  np.asarray(image.components[index].coefficients)):
+                return False
+        return True
 
+    ours, theirs = psnr(decoded), psnr(reference)
     return {
         "reference": "the same solve in binary64 on the GPU",
         "iterations": steps,
         "max_abs_canvas": float(np.max(np.abs(decoded.canvas - reference.canvas))),
-        "psnr": round(psnr(decoded.planes), 5),
-        "psnr_binary64": round(psnr(reference.planes), 5),
-        "psnr_delta_db": round(psnr(decoded.planes) - psnr(reference.planes), 5),
+        "psnr": round(ours, 5),
+        "psnr_binary64": round(theirs, 5),
+        "psnr_delta_db": round(ours - theirs, 6),
         "requantizes_exactly": exact(decoded),
         "requantizes_exactly_binary64": exact(reference),
     }

@@ -119,6 +119,12 @@ def library() -> ctypes.CDLL:
         lib.oracle_result_array.restype = dp
         lib.oracle_result_array.argtypes = (ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, u64p)
         lib.oracle_result_solver.argtypes = (ctypes.c_void_p, dp, dp, u64p, i32p)
+        lib.oracle_result_iteration_seconds.restype = ctypes.c_double
+        lib.oracle_result_iteration_seconds.argtypes = (ctypes.c_void_p,)
+        lib.oracle_result_least_iteration_seconds.restype = ctypes.c_double
+        lib.oracle_result_least_iteration_seconds.argtypes = (ctypes.c_void_p,)
+        lib.oracle_set_threads.argtypes = (ctypes.c_int32,)
+        lib.oracle_threads.restype = ctypes.c_int32
         lib.oracle_result_record_iterations.restype = u64p
         lib.oracle_result_record_iterations.argtypes = (ctypes.c_void_p, u64p)
         lib.oracle_result_record_values.restype = dp
@@ -279,6 +285,8 @@ class OracleResult:
     dual: np.ndarray
     theta: np.ndarray
     partial: np.ndarray
+    iteration_seconds: float = 0.0
+    least_iteration_seconds: float = 0.0
 
 
 def solve(height: int, width: int, color_space: int, components: list[Component], options: list[str],
@@ -355,10 +363,20 @@ def solve(height: int, width: int, color_space: int, components: list[Component]
             steps=arrays[4], weights=arrays[5], flat_weights=arrays[6], flat=arrays[7], cells=cells,
             tau=tau.value, sigma=sigma.value, iterations=iterations.value, stop=stop.value,
             record_iterations=record_iterations, primal=values[0], dual=values[1], theta=values[2],
-            partial=values[3],
+            partial=values[3], iteration_seconds=float(lib.oracle_result_iteration_seconds(handle)),
+            least_iteration_seconds=float(lib.oracle_result_least_iteration_seconds(handle)),
         )
     finally:
         lib.oracle_result_free(handle)
+
+
+def set_threads(count: int) -> None:
+    """The threads the oracle's iterations take (0: every core); results do not depend on it."""
+    library().oracle_set_threads(int(count))
+
+
+def threads() -> int:
+    return int(library().oracle_threads())
 
 
 def basis() -> np.ndarray:
@@ -459,6 +477,7 @@ class SamplesResult:
     dual: np.ndarray
     theta: np.ndarray
     partial: np.ndarray
+    least_iteration_seconds: float = 0.0
 
 
 def solve_samples(levels, bits: int, options: list[str]) -> SamplesResult:
@@ -492,6 +511,6 @@ def solve_samples(levels, bits: int, options: list[str]) -> SamplesResult:
             p = lib.oracle_result_record_values(handle, which, ctypes.byref(n))
             values.append(np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else np.zeros(0))
         return SamplesResult(picture, tau.value, sigma.value, iterations.value, stop.value, record_iterations,
-                             *values)
+                             *values, least_iteration_seconds=float(lib.oracle_result_least_iteration_seconds(handle)))
     finally:
         lib.oracle_result_free(handle)

@@ -7,6 +7,8 @@
  * is computed from the same operands by the same operations either way
  * (cpp/src/primal_dual.hpp:8-16, docs/math.md 5.1).
  */
+#include <omp.h>
+#include <time.h>
 #include "unround_oracle.h"
 
 #include <math.h>
@@ -470,6 +472,17 @@ static int frame_free(const frame* f, size_t c) {
          8u * k->pr.columns * k->across < f->columns;
 }
 
+/* dst = src, count doubles, in bands over the threads. */
+static void copy_doubles(double* dst, const double* src, size_t count) {
+  const size_t band = 1u << 16;
+  const long bands = (long)((count + band - 1) / band);
+#pragma omp parallel for schedule(static)
+  for (long b = 0; b < bands; ++b) {
+    const size_t at = (size_t)b * band;
+    memcpy(dst + at, src + at, sizeof(double) * (count - at < band ? count - at : band));
+  }
+}
+
 /* The cells of the first 8 by x 8 bx samples summed (or their means): rows of a cell
  * added down first, then the samples across (frames.cpp:244-339). */
 static void cells_of(const frame* f, size_t c, const double* x, double* out, int divide) {
@@ -477,30 +490,36 @@ static void cells_of(const frame* f, size_t c, const double* x, double* out, int
   const size_t rows = 8u * k->pr.rows, columns = 8u * k->pr.columns;
   const size_t span = columns * k->across;
   const double cells = (double)(k->down * k->across);
-  double* sums = (double*)malloc(sizeof(double) * (span ? span : 1));
-  for (size_t i = 0; i < rows; ++i) {
-    memcpy(sums, x + (k->down * i) * f->columns, sizeof(double) * span);
-    for (size_t a = 1; a < k->down; ++a) {
-      const double* next = x + (k->down * i + a) * f->columns;
-      for (size_t j = 0; j < span; ++j) sums[j] = sums[j] + next[j];
+#pragma omp parallel
+  {
+    double* sums = (double*)malloc(sizeof(double) * (span ? span : 1));
+#pragma omp for schedule(static)
+    for (size_t i = 0; i < rows; ++i) {
+      memcpy(sums, x + (k->down * i) * f->columns, sizeof(double) * span);
+      for (size_t a = 1; a < k->down; ++a) {
+        const double* next = x + (k->down * i + a) * f->columns;
+        for (size_t j = 0; j < span; ++j) sums[j] = sums[j] + next[j];
+      }
+      for (size_t j = 0; j < columns; ++j) {
+        double sum = sums[k->across * j];
+        for (size_t b = 1; b < k->across; ++b) sum += sums[k->across * j + b];
+        out[i * columns + j] = divide ? sum / cells : sum;
+      }
     }
-    for (size_t j = 0; j < columns; ++j) {
-      double sum = sums[k->across * j];
-      for (size_t b = 1; b < k->across; ++b) sum += sums[k->across * j + b];
-      out[i * columns + j] = divide ? sum / cells : sum;
-    }
+    free(sums);
   }
-  free(sums);
 }
 
 /* The coefficients of a grid of 8 rows x 8 columns blocks, pitch `pitch`. */
 static void forward_grid(const double* grid, size_t pitch, size_t brows, size_t bcols, double* coefficients) {
+#pragma omp parallel for schedule(static)
   for (size_t by = 0; by < brows; ++by)
     for (size_t bx = 0; bx < bcols; ++bx)
       forward_block(grid + 8u * by * pitch + 8u * bx, pitch, coefficients + 64u * (by * bcols + bx));
 }
 
 static void inverse_grid(const double* coefficients, size_t brows, size_t bcols, double* grid, size_t pitch) {
+#pragma omp parallel for schedule(static)
   for (size_t by = 0; by < brows; ++by)
     for (size_t bx = 0; bx < bcols; ++bx)
       inverse_block(coefficients + 64u * (by * bcols + bx), grid + 8u * by * pitch + 8u * bx, pitch);
@@ -515,9 +534,10 @@ static void frame_prox(const frame* f, size_t c, const double* v, double tau, do
   double* at = (double*)malloc(sizeof(double) * 64u * (blocks ? blocks : 1));
   if (k->down * k->across == 1u) {
     forward_grid(v, f->columns, pr->rows, pr->columns, at);
+#pragma omp parallel for schedule(static)
     for (size_t b = 0; b < blocks; ++b)
       for (int q = 0; q < 64; ++q) kept[64u * b + q] = prox_one(pr, b, q, at[64u * b + q], step);
-    memcpy(out, v, sizeof(double) * f->rows * f->columns);
+    copy_doubles(out, v, f->rows * f->columns);
     inverse_grid(kept, pr->rows, pr->columns, out, f->columns);
     free(at);
     return;
@@ -527,13 +547,16 @@ static void frame_prox(const frame* f, size_t c, const double* v, double tau, do
   double* change = (double*)malloc(sizeof(double) * crows * ccols);
   cells_of(f, c, v, means, 1);
   forward_grid(means, ccols, pr->rows, pr->columns, at);
+#pragma omp parallel for schedule(static)
   for (size_t b = 0; b < blocks; ++b)
     for (int q = 0; q < 64; ++q) kept[64u * b + q] = prox_one(pr, b, q, at[64u * b + q], step);
   inverse_grid(kept, pr->rows, pr->columns, change, ccols);
+#pragma omp parallel for schedule(static)
   for (size_t e = 0; e < crows * ccols; ++e) change[e] -= means[e];
   /* out = v + the change spread over each cell, v beyond the blocks (frames.cpp:445-476). */
-  memcpy(out, v, sizeof(double) * f->rows * f->columns);
+  copy_doubles(out, v, f->rows * f->columns);
   const size_t span = ccols * k->across;
+#pragma omp parallel for schedule(static)
   for (size_t i = 0; i < crows; ++i) {
     for (size_t a = 0; a < k->down; ++a) {
       const size_t row = k->down * i + a;
@@ -613,6 +636,7 @@ static double frame_conjugate(const frame* f, size_t c, const double* xi, double
 
 /* Forward differences, 0 out of the grid. */
 static void gradient(const double* f, size_t H, size_t W, double* gx, double* gy) {
+#pragma omp parallel for schedule(static)
   for (size_t i = 0; i < H; ++i) {
     for (size_t j = 0; j < W; ++j) {
       const double here = f[i * W + j];
@@ -648,6 +672,7 @@ static double forward_down(const double* f, size_t H, size_t W, size_t i, size_t
 
 /* div p = delta_y p_y, then + delta_x p_x. */
 static void divergence(const double* px, const double* py, size_t H, size_t W, double* out) {
+#pragma omp parallel for schedule(static)
   for (size_t i = 0; i < H; ++i)
     for (size_t j = 0; j < W; ++j) {
       double value = backward_down(py, H, W, i, j);
@@ -659,6 +684,7 @@ static void divergence(const double* px, const double* py, size_t H, size_t W, d
 /* E w = (delta_x w_x, delta_y w_y, (delta_y w_x + delta_x w_y) / 2). */
 static void symmetrized_gradient(const double* wx, const double* wy, size_t H, size_t W, double* xx, double* yy,
                                  double* xy) {
+#pragma omp parallel for schedule(static)
   for (size_t i = 0; i < H; ++i)
     for (size_t j = 0; j < W; ++j) {
       xx[i * W + j] = backward_across(wx, W, i, j);
@@ -672,6 +698,7 @@ static void symmetrized_gradient(const double* wx, const double* wy, size_t H, s
 /* div_2 r = (d_x r_xx + d_y r_xy, d_x r_xy + d_y r_yy). */
 static void tensor_divergence(const double* rxx, const double* ryy, const double* rxy, size_t H, size_t W, double* x,
                               double* y) {
+#pragma omp parallel for schedule(static)
   for (size_t i = 0; i < H; ++i)
     for (size_t j = 0; j < W; ++j) {
       double a = forward_across(rxx, W, i, j);
@@ -701,6 +728,7 @@ struct oracle_result {
   double tau, sigma;
   uint64_t iterations;
   int32_t stop;
+  double iteration_seconds, least_iteration_seconds;
   uint64_t records;
   uint64_t* record_iterations;
   double* record_values[4];
@@ -736,6 +764,14 @@ const double* oracle_result_array(const oracle_result* r, uint32_t c, int32_t wh
   if (count) *count = r->counts[c][which];
   return r->arrays[c][which];
 }
+
+double oracle_result_iteration_seconds(const oracle_result* r) { return r->iteration_seconds; }
+
+double oracle_result_least_iteration_seconds(const oracle_result* r) { return r->least_iteration_seconds; }
+
+void oracle_set_threads(int32_t count) { omp_set_num_threads(count > 0 ? count : omp_get_num_procs()); }
+
+int32_t oracle_threads(void) { return omp_get_max_threads(); }
 
 void oracle_result_solver(const oracle_result* r, double* tau, double* sigma, uint64_t* iterations, int32_t* stop) {
   if (tau) *tau = r->tau;
@@ -886,6 +922,7 @@ static void samples_prox(const samples_model* m, size_t c, const double* v, doub
   const size_t N = m->H * m->W, at = c * N;
   const double t = tau * m->mu;
   const double r = 1.0 / (1.0 + t);
+#pragma omp parallel for schedule(static)
   for (size_t e = 0; e < N; ++e) out[e] = clamp_to(fma(t, m->centre[at + e], v[e]) * r, m->low[at + e], m->high[at + e]);
 }
 
@@ -966,6 +1003,12 @@ typedef struct machine {
   double *x, *p, *w, *r;
   double *tx, *tp, *tw, *tr;
   double* zeta[3];
+  /* The iteration's work fields, made at its first iteration and kept (v, three
+   * differences, x-, w- and the steps). */
+  double* work;
+  /* Wall seconds inside the iterations (records and set-up excluded), and the least of
+   * one iteration. */
+  double iteration_seconds, least_iteration_seconds;
 } machine;
 
 #define FIELD(base, c, e, m) ((base) + ((c) * (m) + (e)) * N)
@@ -1016,17 +1059,19 @@ static void project_duals(machine* M, double* p, double* r) {
 /* One iteration (primal_dual.hpp:525-735). */
 static void iterate(machine* M) {
   const size_t C = M->C, H = M->H, W = M->W, N = M->N;
-  double* v = (double*)malloc(sizeof(double) * N);
-  double* one = (double*)malloc(sizeof(double) * N);
-  double* two = (double*)malloc(sizeof(double) * N);
-  double* three = (double*)malloc(sizeof(double) * N);
-  double* bar = (double*)malloc(sizeof(double) * C * N);
-  double* barw = (double*)malloc(sizeof(double) * 2 * C * N);
-  double* steps = (double*)malloc(sizeof(double) * 5 * C * N);
+  if (M->work == NULL) M->work = (double*)malloc(sizeof(double) * (4 + 8 * C) * N);
+  double* v = M->work;
+  double* one = v + N;
+  double* two = one + N;
+  double* three = two + N;
+  double* bar = three + N;
+  double* barw = bar + C * N;
+  double* steps = barw + 2 * C * N;
   for (size_t c = 0; c < C; ++c) {
     const double descent = M->tau * M->gamma[c];
     divergence(FIELD(M->p, c, 0, 2), FIELD(M->p, c, 1, 2), H, W, one);
     const double* x = FIELD(M->x, c, 0, 1);
+#pragma omp parallel for schedule(static)
     for (size_t e = 0; e < N; ++e) v[e] = x[e] + descent * one[e];
     if (M->sm != NULL) {
       samples_prox(M->sm, c, v, M->tau, FIELD(M->tx, c, 0, 1));
@@ -1035,6 +1080,7 @@ static void iterate(machine* M) {
     }
     if (M->tgv) {
       tensor_divergence(FIELD(M->r, c, 0, 3), FIELD(M->r, c, 1, 3), FIELD(M->r, c, 2, 3), H, W, one, two);
+#pragma omp parallel for schedule(static)
       for (size_t e = 0; e < N; ++e) {
         FIELD(M->tw, c, 0, 2)[e] = FIELD(M->w, c, 0, 2)[e] + descent * (FIELD(M->p, c, 0, 2)[e] + one[e]);
         FIELD(M->tw, c, 1, 2)[e] = FIELD(M->w, c, 1, 2)[e] + descent * (FIELD(M->p, c, 1, 2)[e] + two[e]);
@@ -1047,14 +1093,17 @@ static void iterate(machine* M) {
     double* x = FIELD(M->x, c, 0, 1);
     const double* tx = FIELD(M->tx, c, 0, 1);
     double* b = bar + c * N;
+#pragma omp parallel for schedule(static)
     for (size_t e = 0; e < N; ++e) b[e] = 2.0 * tx[e] - x[e];
     gradient(b, H, W, one, two);
+#pragma omp parallel for schedule(static)
     for (size_t e = 0; e < N; ++e) x[e] = M->rho * tx[e] + M->rest * x[e];
     double* sx = steps + (5 * c + 0) * N;
     double* sy = steps + (5 * c + 1) * N;
     double* px = FIELD(M->p, c, 0, 2);
     double* py = FIELD(M->p, c, 1, 2);
     if (!M->tgv) {
+#pragma omp parallel for schedule(static)
       for (size_t e = 0; e < N; ++e) {
         sx[e] = px[e] + ascent * one[e];
         sy[e] = py[e] + ascent * two[e];
@@ -1067,6 +1116,7 @@ static void iterate(machine* M) {
     double* wy = FIELD(M->w, c, 1, 2);
     const double* twx = FIELD(M->tw, c, 0, 2);
     const double* twy = FIELD(M->tw, c, 1, 2);
+#pragma omp parallel for schedule(static)
     for (size_t e = 0; e < N; ++e) {
       bwx[e] = 2.0 * twx[e] - wx[e];
       bwy[e] = 2.0 * twy[e] - wy[e];
@@ -1079,6 +1129,7 @@ static void iterate(machine* M) {
     double* sxx = steps + (5 * c + 2) * N;
     double* syy = steps + (5 * c + 3) * N;
     double* sxy = steps + (5 * c + 4) * N;
+#pragma omp parallel for schedule(static)
     for (size_t e = 0; e < N; ++e) {
       sxx[e] = FIELD(M->r, c, 0, 3)[e] + ascent * one[e];
       syy[e] = FIELD(M->r, c, 1, 3)[e] + ascent * two[e];
@@ -1086,6 +1137,7 @@ static void iterate(machine* M) {
     }
   }
   const double radius = M->tgv ? M->alpha1 : M->alpha;
+#pragma omp parallel for schedule(static)
   for (size_t e = 0; e < N; ++e) {
     double scale_all = 1.0, tscale_all = 1.0;
     if (M->coupled) {
@@ -1129,13 +1181,6 @@ static void iterate(machine* M) {
       }
     }
   }
-  free(v);
-  free(one);
-  free(two);
-  free(three);
-  free(bar);
-  free(barw);
-  free(steps);
 }
 
 /* The record of the tilde point (primal_dual.hpp:446-501, 791-895). */
@@ -1305,13 +1350,21 @@ static void run_machine(machine* M, oracle_result* result, double tolerance, dou
   } while (0)
   if (every > 0 || most == 0) TAKE(0);
   while (!stopped && iteration < most) {
+    struct timespec began, ended;
+    clock_gettime(CLOCK_MONOTONIC, &began);
     iterate(M);
+    clock_gettime(CLOCK_MONOTONIC, &ended);
+    const double seconds = (double)(ended.tv_sec - began.tv_sec) + 1e-9 * (double)(ended.tv_nsec - began.tv_nsec);
+    M->iteration_seconds += seconds;
+    if (iteration == 0 || seconds < M->least_iteration_seconds) M->least_iteration_seconds = seconds;
     iteration += 1;
     if ((every > 0 && iteration % every == 0) || iteration == most) TAKE(iteration);
   }
 #undef TAKE
   result->iterations = iteration;
   result->stop = stopped ? 1 : 2;
+  result->iteration_seconds = M->iteration_seconds;
+  result->least_iteration_seconds = M->least_iteration_seconds;
 }
 
 oracle_result* oracle_solve(uint32_t height, uint32_t width, int32_t color_space, uint32_t count,
@@ -1520,6 +1573,7 @@ oracle_result* oracle_solve(uint32_t height, uint32_t width, int32_t color_space
     result->arrays[c][0] = M.zeta[c];
     M.zeta[c] = NULL;
   }
+  free(M.work);
   free(M.x);
   free(M.tx);
   free(M.p);
@@ -1607,6 +1661,7 @@ oracle_result* oracle_samples_solve(const uint16_t* levels, uint32_t channels, u
   if (tgv) memcpy(M.tw, M.w, sizeof(double) * 2 * C * N);
   run_machine(&M, result, tolerance, o->relative_tolerance, o->partial_tolerance, o->record_every, most, 1);
   result->canvas = copy_of(M.tx, C * N);
+  free(M.work);
   free(M.x);
   free(M.tx);
   free(M.p);

@@ -107,6 +107,15 @@ void oracle_result_component(const oracle_result* result, uint32_t c, uint64_t* 
  * 6 flat weights (64); 7 flat marks as doubles (blocks). */
 const double* oracle_result_array(const oracle_result* result, uint32_t c, int32_t which, uint64_t* count);
 /* tau, sigma, iterations, stop (1 converged, 2 iterations, 0 no solver). */
+/* Wall seconds spent in the iterations of a solve (records and set-up excluded). */
+double oracle_result_iteration_seconds(const oracle_result* result);
+/* The least wall seconds of one iteration of the solve. */
+double oracle_result_least_iteration_seconds(const oracle_result* result);
+/* The threads the iterations take (0: every core), as bands of rows and blocks; each sample
+ * is made by one thread alone, in the same operations whatever the count, so results do
+ * not depend on it (cpp/src/workers.hpp:21-86 does the same with its strips). */
+void oracle_set_threads(int32_t count);
+int32_t oracle_threads(void);
 void oracle_result_solver(const oracle_result* result, double* tau, double* sigma, uint64_t* iterations,
                           int32_t* stop);
 /* Records: iterations, and which 0 primal, 1 dual, 2 theta (NaN for TV), 3 partial gap
