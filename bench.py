@@ -194,28 +194,12 @@ def measured_traffic(kernel: str, workload_name: str, precision: str) -> float |
 
 
 def batch_workload(count: int = 64):
-    """The mixed batch: image k (seeds 1..count) of height and width uniform on the multiples of
-    8 in [96, 1920], quality uniform in [30, 90], grey or 4:2:0 colour, each drawn from its
-    own seed; with the uncompressed pictures for PSNR."""
-    import numpy as np
-
+    """The mixed batch of synthetic.batch_pictures: sides uniform on the multiples of 8 in
+    [96, 1920], quality uniform in [30, 90], grey or 4:2:0 colour, each drawn from its own
+    seed; with the uncompressed pictures for PSNR."""
     from unround import synthetic
 
-    images, originals, kinds = [], [], []
-    for seed in range(1, count + 1):
-        draw = np.random.default_rng(1000 + seed)
-        height, width = (int(v) * 8 for v in draw.integers(12, 241, size=2))
-        quality = int(draw.integers(30, 91))
-        if draw.random() < 0.5:
-            picture = synthetic.grey_picture(height, width, seed)
-            images.append(synthetic.encode_grey(picture, quality))
-            kinds.append("grey")
-        else:
-            picture = synthetic.colour_picture(height, width, seed)
-            images.append(synthetic.encode_colour420(picture, quality))
-            kinds.append("colour")
-        originals.append(picture)
-    return images, originals, kinds
+    return synthetic.batch_pictures(count)
 
 
 def psnr_of(decoded, original) -> float:

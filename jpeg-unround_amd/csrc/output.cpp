@@ -196,19 +196,100 @@ void put_chunk(std::vector<uint8_t>& out, const char* type, const uint8_t* body,
   put_big_endian(out, static_cast<uint32_t>(crc));
 }
 
-// A zlib stream of the bytes at the level (-1: zlib's default).
-bool compressed(const uint8_t* data, size_t size, int level, std::vector<uint8_t>& out) {
-  z_stream z{};
-  if (deflateInit(&z, level) != Z_OK) return false;
-  out.resize(deflateBound(&z, static_cast<uLong>(size)) + 64);
-  z.next_in = const_cast<Bytef*>(data);
-  z.avail_in = static_cast<uInt>(size);
-  z.next_out = out.data();
-  z.avail_out = static_cast<uInt>(out.size());
-  const int result = deflate(&z, Z_FINISH);
-  out.resize(z.total_out);
-  deflateEnd(&z);
-  return result == Z_STREAM_END;
+// A zlib stream made as libpng 1.6 makes those of the files the reference writes
+// (c/jpegio/src/png.c:250-276 leaves every zlib setting to libpng but the level), so that
+// the bytes are libpng's for the same zlib: memory level 8; the window 2^15 bytes, or, for
+// at most 16384 bytes of input, the least power of two (at least 2^9) that holds the input
+// and the 262 bytes deflate looks ahead; the window the header names then trimmed to the
+// least that holds the input itself, which is all inflate needs (FCHECK made again). The
+// input goes to deflate in the pieces libpng gives it and the output into buffers of
+// libpng's sizes: a stored stream's (level 0) blocks end where those do.
+class LibpngStream {
+ public:
+  // `first` and `next`: the sizes of the first output buffer and of the others.
+  LibpngStream(size_t input_size, int level, int strategy, size_t first, size_t next)
+      : input_size_(input_size), next_(next) {
+    int window_bits = 15;
+    if (input_size <= kSmall) {
+      while (window_bits > 9 && input_size + kLookAhead <= (size_t{1} << (window_bits - 1))) --window_bits;
+    }
+    ready_ = deflateInit2(&z_, level, Z_DEFLATED, window_bits, 8, strategy) == Z_OK;
+    buffer_.resize(first);
+    z_.next_out = buffer_.data();
+    z_.avail_out = static_cast<uInt>(first);
+  }
+  ~LibpngStream() {
+    if (ready_) deflateEnd(&z_);
+  }
+  LibpngStream(const LibpngStream&) = delete;
+  LibpngStream& operator=(const LibpngStream&) = delete;
+
+  // Deflates `size` bytes with no flush, handing every full buffer to `emit`.
+  template <typename Emit>
+  bool feed(const uint8_t* data, size_t size, Emit&& emit) {
+    if (!ready_) return false;
+    z_.next_in = const_cast<Bytef*>(data);
+    z_.avail_in = static_cast<uInt>(size);
+    for (;;) {
+      const int result = deflate(&z_, Z_NO_FLUSH);
+      if (result != Z_OK) return false;
+      if (z_.avail_out == 0) hand_over(buffer_.size(), emit);
+      if (z_.avail_in == 0) return true;
+    }
+  }
+
+  // Ends the stream, handing over the rest, the last buffer as far as it is filled.
+  template <typename Emit>
+  bool finish(Emit&& emit) {
+    if (!ready_) return false;
+    z_.next_in = nullptr;
+    z_.avail_in = 0;
+    for (;;) {
+      const int result = deflate(&z_, Z_FINISH);
+      if (result == Z_STREAM_END) {
+        hand_over(buffer_.size() - z_.avail_out, emit);
+        return true;
+      }
+      if (result != Z_OK || z_.avail_out != 0) return false;
+      hand_over(buffer_.size(), emit);
+    }
+  }
+
+ private:
+  static constexpr size_t kLookAhead = 262, kSmall = 16384;
+
+  template <typename Emit>
+  void hand_over(size_t filled, Emit&& emit) {
+    // The header: CMF = CINFO << 4 | 8 (deflate), CINFO = log2(window) - 8; FLG's low five
+    // bits make CMF * 256 + FLG a multiple of 31.
+    if (!handed_ && input_size_ <= kSmall && filled >= 2 && (buffer_[0] & 0x0F) == 8 && (buffer_[0] >> 4) <= 7) {
+      int info = buffer_[0] >> 4;
+      while (info > 0 && input_size_ <= (size_t{1} << (info + 7))) --info;
+      const unsigned cmf = static_cast<unsigned>(info << 4) | 8u;
+      const unsigned flags = buffer_[1] & 0xE0u;
+      buffer_[0] = static_cast<uint8_t>(cmf);
+      buffer_[1] = static_cast<uint8_t>(flags + 31u - (cmf * 256u + flags) % 31u);
+    }
+    handed_ = true;
+    if (filled > 0) emit(buffer_.data(), filled);
+    buffer_.resize(next_);
+    z_.next_out = buffer_.data();
+    z_.avail_out = static_cast<uInt>(next_);
+  }
+
+  z_stream z_{};
+  bool ready_ = false, handed_ = false;
+  size_t input_size_, next_;
+  std::vector<uint8_t> buffer_;
+};
+
+// A profile in zlib as libpng compresses text (the whole input at once, into a first
+// buffer of 1024 bytes and then of 8192).
+bool compressed_text(const uint8_t* data, size_t size, int level, std::vector<uint8_t>& out) {
+  out.clear();
+  LibpngStream stream(size, level, Z_DEFAULT_STRATEGY, 1024, 8192);
+  const auto keep = [&out](const uint8_t* bytes, size_t count) { out.insert(out.end(), bytes, bytes + count); };
+  return stream.feed(data, size, keep) && stream.finish(keep);
 }
 
 uint8_t paeth_of(int a, int b, int c) {
@@ -380,11 +461,11 @@ bool png_bytes(const void* samples, uint32_t width, uint32_t height, int32_t cha
     std::vector<uint8_t> body(kName, kName + sizeof(kName));  // with the NUL
     body.push_back(0);
     std::vector<uint8_t> packed;
-    if (!compressed(icc, static_cast<size_t>(icc_size), compression, packed)) {
+    if (!compressed_text(icc, static_cast<size_t>(icc_size), compression, packed)) {
       message = "zlib could not compress the ICC profile";
       return false;
     }
-    if (body.size() + packed.size() < 92 && !compressed(icc, static_cast<size_t>(icc_size), 0, packed)) {
+    if (body.size() + packed.size() < 92 && !compressed_text(icc, static_cast<size_t>(icc_size), 0, packed)) {
       message = "zlib could not store the ICC profile";
       return false;
     }
@@ -392,9 +473,12 @@ bool png_bytes(const void* samples, uint32_t width, uint32_t height, int32_t cha
     put_chunk(file, "iCCP", body.data(), body.size());
   }
   // The rows, each by the filter of the least sum of its bytes taken as signed (the
-  // usual heuristic of PNG encoders), 16-bit samples most significant byte first.
-  std::vector<uint8_t> raw(static_cast<size_t>(height) * (1 + row_bytes));
-  std::vector<uint8_t> row(row_bytes), previous(row_bytes), trial(1 + row_bytes);
+  // usual heuristic of PNG encoders, the first of equal sums), 16-bit samples most
+  // significant byte first; each filtered row deflated as libpng deflates it, with no flush,
+  // into IDAT chunks of 8192 bytes (its output buffer's), the last shorter.
+  LibpngStream stream((1 + row_bytes) * static_cast<size_t>(height), compression, Z_FILTERED, 8192, 8192);
+  const auto chunk = [&file](const uint8_t* bytes, size_t count) { put_chunk(file, "IDAT", bytes, count); };
+  std::vector<uint8_t> row(row_bytes), previous(row_bytes), trial(1 + row_bytes), best(1 + row_bytes);
   for (size_t y = 0; y < height; ++y) {
     if (bits == 8) {
       std::memcpy(row.data(), static_cast<const uint8_t*>(samples) + y * row_bytes, row_bytes);
@@ -406,26 +490,25 @@ bool png_bytes(const void* samples, uint32_t width, uint32_t height, int32_t cha
       }
     }
     uint64_t best_sum = std::numeric_limits<uint64_t>::max();
-    uint8_t* out = raw.data() + y * (1 + row_bytes);
     for (int kind = 0; kind < 5; ++kind) {
       filter_row(kind, row.data(), y > 0 ? previous.data() : nullptr, row_bytes, step, trial.data());
       uint64_t sum = 0;
       for (size_t i = 1; i <= row_bytes; ++i) sum += static_cast<uint64_t>(std::abs(static_cast<int8_t>(trial[i])));
       if (sum < best_sum) {
         best_sum = sum;
-        std::memcpy(out, trial.data(), 1 + row_bytes);
+        std::swap(best, trial);
       }
+    }
+    if (!stream.feed(best.data(), best.size(), chunk)) {
+      message = "zlib could not compress the picture";
+      return false;
     }
     std::swap(row, previous);
   }
-  std::vector<uint8_t> stream;
-  if (!compressed(raw.data(), raw.size(), compression, stream)) {
+  if (!stream.finish(chunk)) {
     message = "zlib could not compress the picture";
     return false;
   }
-  constexpr size_t kChunk = size_t{1} << 20;
-  for (size_t at = 0; at < stream.size(); at += kChunk)
-    put_chunk(file, "IDAT", stream.data() + at, std::min(kChunk, stream.size() - at));
   put_chunk(file, "IEND", nullptr, 0);
   status = 0;
   return true;

@@ -176,3 +176,25 @@ def encode_colour420(picture: np.ndarray, quality: int) -> jpegio.Image:
         jpegio.make_component(_levels(chroma[1], chroma_table), chroma_table, 1, 1),
     )
     return jpegio.make_image(height, width, jpegio.ColorSpace.YCBCR, components)
+
+
+def batch_pictures(count: int = 64) -> tuple[list[jpegio.Image], list[np.ndarray], list[str]]:
+    """The mixed batch: picture k (seeds 1..count) of height and width uniform on the multiples
+    of 8 in [96, 1920], quality uniform in [30, 90], grey or 4:2:0 colour with even odds, each
+    drawn from its own generator (seed 1000 + k); with the uncompressed pictures and the kinds
+    ("grey" or "colour")."""
+    images, originals, kinds = [], [], []
+    for seed in range(1, count + 1):
+        draw = np.random.default_rng(1000 + seed)
+        height, width = (int(v) * 8 for v in draw.integers(12, 241, size=2))
+        quality = int(draw.integers(30, 91))
+        if draw.random() < 0.5:
+            picture = grey_picture(height, width, seed)
+            images.append(encode_grey(picture, quality))
+            kinds.append("grey")
+        else:
+            picture = colour_picture(height, width, seed)
+            images.append(encode_colour420(picture, quality))
+            kinds.append("colour")
+        originals.append(picture)
+    return images, originals, kinds

@@ -115,6 +115,8 @@ def library() -> ctypes.CDLL:
         lib.oracle_result_free.argtypes = (ctypes.c_void_p,)
         lib.oracle_result_canvas.restype = dp
         lib.oracle_result_canvas.argtypes = (ctypes.c_void_p, u64p, u64p, u64p)
+        lib.oracle_result_iterate.restype = dp
+        lib.oracle_result_iterate.argtypes = (ctypes.c_void_p,)
         lib.oracle_result_component.argtypes = (ctypes.c_void_p, ctypes.c_uint32, u64p, u64p, u64p, u64p)
         lib.oracle_result_array.restype = dp
         lib.oracle_result_array.argtypes = (ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, u64p)
@@ -287,6 +289,7 @@ class OracleResult:
     partial: np.ndarray
     iteration_seconds: float = 0.0
     least_iteration_seconds: float = 0.0
+    iterate: np.ndarray | None = None  # the last x, (C, H, W); canvas is the last x~
 
 
 def solve(height: int, width: int, color_space: int, components: list[Component], options: list[str],
@@ -329,6 +332,7 @@ def solve(height: int, width: int, color_space: int, components: list[Component]
         pointer = lib.oracle_result_canvas(handle, ctypes.byref(C), ctypes.byref(H), ctypes.byref(W))
         shape = (C.value, H.value, W.value)
         canvas = np.ctypeslib.as_array(pointer, shape=(int(np.prod(shape)),)).copy().reshape(shape)
+        iterate = np.ctypeslib.as_array(lib.oracle_result_iterate(handle), shape=(int(np.prod(shape)),)).copy()
         arrays: list[list[np.ndarray]] = [[] for _ in range(8)]
         cells = []
         for c in range(count):
@@ -365,6 +369,7 @@ def solve(height: int, width: int, color_space: int, components: list[Component]
             record_iterations=record_iterations, primal=values[0], dual=values[1], theta=values[2],
             partial=values[3], iteration_seconds=float(lib.oracle_result_iteration_seconds(handle)),
             least_iteration_seconds=float(lib.oracle_result_least_iteration_seconds(handle)),
+            iterate=iterate.reshape(shape),
         )
     finally:
         lib.oracle_result_free(handle)

@@ -722,6 +722,7 @@ static double ball_scale(double square, double radius) {
 struct oracle_result {
   size_t channels, rows, columns;
   double* canvas;
+  double* iterate;
   size_t block_rows[3], block_columns[3], down[3], across[3];
   double* arrays[3][8];
   uint64_t counts[3][8];
@@ -737,6 +738,7 @@ struct oracle_result {
 void oracle_result_free(oracle_result* result) {
   if (result == NULL) return;
   free(result->canvas);
+  free(result->iterate);
   for (int c = 0; c < 3; ++c)
     for (int k = 0; k < 8; ++k) free(result->arrays[c][k]);
   free(result->record_iterations);
@@ -750,6 +752,8 @@ const double* oracle_result_canvas(const oracle_result* r, uint64_t* channels, u
   if (columns) *columns = r->columns;
   return r->canvas;
 }
+
+const double* oracle_result_iterate(const oracle_result* r) { return r->iterate; }
 
 void oracle_result_component(const oracle_result* r, uint32_t c, uint64_t* block_rows, uint64_t* block_columns,
                              uint64_t* down, uint64_t* across) {
@@ -1569,6 +1573,7 @@ oracle_result* oracle_solve(uint32_t height, uint32_t width, int32_t color_space
     result->stop = 0;
   }
   result->canvas = copy_of(M.tx, C * N);
+  result->iterate = copy_of(M.x, C * N);
   for (size_t c = 0; c < C; ++c) {
     result->arrays[c][0] = M.zeta[c];
     M.zeta[c] = NULL;
@@ -1661,6 +1666,7 @@ oracle_result* oracle_samples_solve(const uint16_t* levels, uint32_t channels, u
   if (tgv) memcpy(M.tw, M.w, sizeof(double) * 2 * C * N);
   run_machine(&M, result, tolerance, o->relative_tolerance, o->partial_tolerance, o->record_every, most, 1);
   result->canvas = copy_of(M.tx, C * N);
+  result->iterate = copy_of(M.x, C * N);
   free(M.work);
   free(M.x);
   free(M.tx);

@@ -100,6 +100,9 @@ void oracle_result_free(oracle_result* result);
 /* The canvas, channels x rows x columns. */
 const double* oracle_result_canvas(const oracle_result* result, uint64_t* channels, uint64_t* rows,
                                    uint64_t* columns);
+/* The last iterate x (the canvas is the last tilde point x~, projected into the
+ * intervals), the canvas's shape. */
+const double* oracle_result_iterate(const oracle_result* result);
 /* A component's blocks, and its cell (rows, columns of the canvas per sample). */
 void oracle_result_component(const oracle_result* result, uint32_t c, uint64_t* block_rows,
                              uint64_t* block_columns, uint64_t* down, uint64_t* across);

@@ -8,52 +8,12 @@ import numpy as np
 import pytest
 
 import helpers
-import unround
 from unround import native, synthetic
 
 pytestmark = pytest.mark.gpu
 
 
-def check(image, original, options, precision="single", ratio_limit=1e-3, psnr_limit=0.01):
-    print(f"parity {image.height}x{image.width} space {int(image.color_space)} {' '.join(options)} {precision}",
-          flush=True)
-    expected = helpers.oracle_solve(image, options)
-    decoded = native.solve(image, options + ["--precision", precision])
-    C, H, W = expected.canvas.shape
-    assert decoded.canvas.shape == (C, H, W)
-    h, w = int(image.height), int(image.width)
-    ours = decoded.planes if C == 1 else decoded.planes
-    theirs = expected.canvas[:, :h, :w]
-    a, b = helpers.rounded(ours), helpers.rounded(theirs)
-    differing = float(np.mean(a != b))
-    largest = int(np.max(np.abs(a - b)))
-    worst = float(np.max(np.abs(ours - theirs)))
-    print(f"  max |diff| {worst:.3g}, 8-bit differing {differing:.5f}, largest {largest}")
-    assert largest <= 1
-    assert differing <= ratio_limit
-    if original is not None:
-        if C == 1:
-            ours_psnr, theirs_psnr = helpers.psnr(ours[0], original), helpers.psnr(theirs[0], original)
-        else:
-            ours_rgb = unround.to_rgb(ours)
-            theirs_rgb = unround.to_rgb(theirs)
-            ours_psnr, theirs_psnr = helpers.psnr(ours_rgb, original), helpers.psnr(theirs_rgb, original)
-        print(f"  PSNR {ours_psnr:.4f} vs {theirs_psnr:.4f}")
-        assert abs(ours_psnr - theirs_psnr) < psnr_limit
-    widened = "--slack" in options
-    for c, problem in enumerate(decoded.problems):
-        rows, columns = decoded.coefficients[c].shape[:2]
-        assert np.all(decoded.coefficients[c] > problem.lower) and np.all(decoded.coefficients[c] < problem.upper)
-        if widened:
-            continue  # a slack widens the intervals beyond the file's own: no requantization
-        levels = helpers.requantized(decoded.canvas[c], image, c, problem.ratio, rows, columns)
-        assert np.array_equal(levels, helpers.kept_levels(image, c, rows, columns)), f"component {c} requantizes"
-    history = decoded.result.history
-    assert list(history.iterations) == list(expected.record_iterations)
-    relative = np.abs(history.primal - expected.primal) / np.abs(expected.primal)
-    print(f"  records: primal rel {np.max(relative):.3g}")
-    assert np.max(relative) < 1e-4
-    return decoded, expected
+check = helpers.parity_check
 
 
 def test_tv_grey_256_q50_200_iterations():

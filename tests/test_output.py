@@ -22,6 +22,7 @@ import numpy as np
 import pytest
 from PIL import Image
 
+import helpers
 from unround import native
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
@@ -253,19 +254,7 @@ def test_png_refusals():
         native.png_bytes(np.array([[math.nan]]))
 
 
-def tiff_tags(data: bytes) -> tuple[dict, int]:
-    assert data[:4] == b"II*\x00"
-    ifd = struct.unpack("<I", data[4:8])[0]
-    count = struct.unpack("<H", data[ifd:ifd + 2])[0]
-    tags = {}
-    previous = -1
-    for k in range(count):
-        tag, kind, n, value = struct.unpack("<HHII", data[ifd + 2 + 12 * k:ifd + 14 + 12 * k])
-        assert tag > previous
-        previous = tag
-        tags[tag] = (kind, n, value, data[ifd + 10 + 12 * k:ifd + 14 + 12 * k])
-    assert struct.unpack("<I", data[ifd + 2 + 12 * count:ifd + 6 + 12 * count])[0] == 0
-    return tags, ifd
+tiff_tags = helpers.tiff_tags
 
 
 @pytest.mark.parametrize("channels,ycbcr", [(1, False), (2, False), (3, False), (3, True), (4, False)])
