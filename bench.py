@@ -351,6 +351,34 @@ def unrolled_main(args) -> None:
     ended = time.perf_counter()
     forward, backward = middle - began, ended - middle
     samples = count * size * size
+    # The launches' own durations: a handle made with UNROUND_TIMING=2 (events around every
+    # launch on its stream), one forward and backward.
+    os.environ["UNROUND_TIMING"] = "2"
+    timed = UnrolledTGV(images, ["--method", "tgv"], iterations=iterations, checkpoint_every=every)
+    os.environ.pop("UNROUND_TIMING")
+    alpha0.grad = alpha1.grad = weight_map.grad = None
+    canvas = timed(alpha0, alpha1, weight_map)
+    ((canvas - target) ** 2).mean().backward()
+    torch.cuda.synchronize()
+    times = timed.kernel_times()
+    timed.close()
+    # Algorithmic bytes a sample: the dual step's adjoint reads w, p, r, x-, tau g grad v, the
+    # radius and grad w, grad p, grad r after (68 B) and writes grad x- and grad w, grad p,
+    # grad r before (32 B); the prox's adjoint reads grad x-, grad x after and a clip mark and
+    # writes grad x and tau g grad v (17 B); the forward dual step 64 B (60 B and the radius),
+    # the forward prox 23 B (x, p, the level; x, x-, the mark). The backward: the two adjoints
+    # every iteration and the forward again for every run but the last (the forward keeps it).
+    back_bytes, prox_back_bytes, dual_bytes_per, prox_bytes = 100, 17, 64, 23
+    recomputed = (iterations - 1) // every * every  # the iterations of every run but the last
+    backward_bytes = samples * (iterations * (back_bytes + prox_back_bytes) + recomputed * (dual_bytes_per + prox_bytes))
+    dual_back_s, dual_back_n = times["dual_back"]
+    prox_back_s, prox_back_n = times["prox_back"]
+    dual_s, dual_n = times["dual"]
+    launch = dual_back_s / dual_back_n
+    achieved = back_bytes * samples / launch / 1e9
+    log(f"C5 launches: dual step's adjoint {1e6 * launch:.1f} us x {dual_back_n}, prox's adjoint "
+        f"{1e6 * prox_back_s / prox_back_n:.1f} us x {prox_back_n}, forward dual step {1e6 * dual_s / dual_n:.1f} us x "
+        f"{dual_n}; backward {backward_bytes / samples / iterations:.1f} algorithmic bytes a sample-iteration")
     log(f"C5: loss {loss:.4f}; forward {forward * 1e3:.1f} ms, backward {backward * 1e3:.1f} ms; "
         f"dL/dalpha0 {float(alpha0.grad):.6g}, dL/dalpha1 {float(alpha1.grad):.6g}, "
         f"|dL/dm| {float(weight_map.grad.norm()):.6g}")
@@ -375,7 +403,23 @@ def unrolled_main(args) -> None:
         "grad_alpha0": float(alpha0.grad),
         "grad_alpha1": float(alpha1.grad),
         "grad_map_norm": float(weight_map.grad.norm()),
-        "roofline": None,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "unrolled_dual_back_kernel",
+            "bytes_per_launch": int(back_bytes * samples),
+            "launch_us": round(1e6 * launch, 2),
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "prox_back_us": round(1e6 * prox_back_s / prox_back_n, 2),
+            "prox_back_frac": round(prox_back_bytes * samples / (prox_back_s / prox_back_n) / 1e9 / HBM_PEAK_GBS, 4),
+            "backward_bytes_per_sample_iteration": round(backward_bytes / samples / iterations, 1),
+            "backward_achieved": round(backward_bytes / backward / 1e9, 1),
+            "backward_frac": round(backward_bytes / backward / 1e9 / HBM_PEAK_GBS, 4),
+            "backward_over_forward": round(backward / forward, 3),
+        },
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

@@ -80,6 +80,14 @@ unround_status unround_unrolled_forward(unround_unrolled* handle, const float* w
  * [height][width] (device) and grad_alphas = {dL/d alpha0, dL/d alpha1} (host). */
 unround_status unround_unrolled_backward(unround_unrolled* handle, const float* grad_canvas, float* grad_map,
                                          double* grad_alphas, void* stream, char* message, size_t message_size);
+/* The durations of the handle's launches since its last forward began, where the
+ * environment variable UNROUND_TIMING was 2 when the handle was made (HIP events around
+ * every launch on the handle's stream): values[0] the seconds of the dual step's adjoint and
+ * [1] its launches, [2] and [3] those of the prox's adjoint, [4] and [5] those of the
+ * forward dual step (the forward's and the backward's recomputation); at most count values
+ * are written. Waits for the handle's stream. Returns 1 where launches were timed, 0
+ * otherwise. */
+int32_t unround_unrolled_kernel_times(unround_unrolled* handle, double* values, size_t count);
 
 /* A picture of rounded samples given as arrays, reconstructed as unround_decode does a PNG
  * file (rust/capi/include/unround.h:128-136; rust/unround/src/rounded.rs:71-170): levels of

@@ -748,7 +748,7 @@ unround_status unround_unrolled_new(const unround_input* inputs, size_t count, c
     }
     int32_t status = UNROUND_OK;
     handle->solve = unround::make_unrolled(frames, plans, settings->settings, static_cast<int>(iterations),
-                                           static_cast<int>(checkpoint_every), text, status);
+                                           static_cast<int>(checkpoint_every), timing_asked(), text, status);
     if (handle->solve == nullptr) return static_cast<unround_status>(status);
     *made = handle.release();
     return UNROUND_OK;
@@ -828,6 +828,18 @@ unround_status unround_unrolled_backward(unround_unrolled* handle, const float* 
     handle->solve->backward(grad_canvas, grad_map, grad_alphas, stream);
     return UNROUND_OK;
   });
+}
+
+int32_t unround_unrolled_kernel_times(unround_unrolled* handle, double* values, size_t count) {
+  if (handle == nullptr || values == nullptr) return 0;
+  double all[6] = {};
+  try {
+    if (!handle->solve->kernel_times(all)) return 0;
+  } catch (const std::exception&) {
+    return 0;
+  }
+  for (size_t k = 0; k < count && k < 6; ++k) values[k] = all[k];
+  return 1;
 }
 
 unround_status unround_project(const uint8_t* data, size_t size, const unround_settings* settings,

@@ -163,9 +163,10 @@ __device__ __forceinline__ T clamp_to(T value, T low, T high) {
 // The proximal map of one coefficient of level q (docs/math.md:457-491;
 // cpp/src/model.cpp:62-75): the quadratic's minimizer, clipped to the interval. The
 // frequency's constants are read first and every case is computed and selected, so that
-// a row of coefficients runs without branches.
+// a row of coefficients runs without branches. `within`: whether the quadratic's
+// minimizer (with no slack cost) lay inside the interval.
 template <typename Constants, typename T>
-__device__ __forceinline__ T prox_of(const Constants& K, int16_t q, bool flat, int k, T e, T step) {
+__device__ __forceinline__ T prox_of(const Constants& K, int16_t q, bool flat, int k, T e, T step, bool& within) {
 #pragma clang fp contract(off)
   const double frequency_step = K.steps[k];
   const double weight_of = (flat ? K.flat_weights : K.weights)[k];
@@ -183,7 +184,13 @@ __device__ __forceinline__ T prox_of(const Constants& K, int16_t q, bool flat, i
   const T below = low_own < z + h ? low_own : z + h;
   const T inside = z < low_own ? below : z;
   const T costed = z > high_own ? above : inside;
+  within = z >= static_cast<T>(lower) && z <= static_cast<T>(upper);
   return clamp_to(K.costed ? costed : z, static_cast<T>(lower), static_cast<T>(upper));
+}
+template <typename Constants, typename T>
+__device__ __forceinline__ T prox_of(const Constants& K, int16_t q, bool flat, int k, T e, T step) {
+  bool within;
+  return prox_of(K, q, flat, k, e, step, within);
 }
 
 // g*(s) of one coefficient, in binary64 (cpp/src/model.cpp:80-109).

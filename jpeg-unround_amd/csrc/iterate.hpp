@@ -3,18 +3,37 @@
 //
 // Both kernels give each lane of a wavefront one column of a 64-column strip and walk
 // down rows, so that every load of a row is one coalesced 256-byte stretch (binary32).
-// A value of the neighbouring column comes from the neighbouring lane (a shuffle); of the
+// A value of the neighbouring column comes from the neighbouring lane (a DPP move); of the
 // row above, from the lane's own registers; only the lanes at a strip's two edges read a
 // column beyond it from memory. Each field is then read about once per iteration.
 
-// The value of the lane on the left (lane 0: its own) and on the right (lane 63: its own).
+// The value of the lane on the left (lane 0: its own) and on the right (lane 63: its own):
+// DPP's whole-wavefront shifts by one lane (wave_shr:1, wave_shl:1), a VALU move of each
+// 32-bit word with no trip through LDS; a lane with no source keeps its own word.
+template <int CONTROL, typename T>
+__device__ __forceinline__ T lane_shifted(T value) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8);
+  if constexpr (sizeof(T) == 4) {
+    const int word = __builtin_bit_cast(int, value);
+    return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(word, word, CONTROL, 0xF, 0xF, false));
+  } else {
+    struct Words {
+      int low, high;
+    };
+    Words w = __builtin_bit_cast(Words, value);
+    w.low = __builtin_amdgcn_update_dpp(w.low, w.low, CONTROL, 0xF, 0xF, false);
+    w.high = __builtin_amdgcn_update_dpp(w.high, w.high, CONTROL, 0xF, 0xF, false);
+    return __builtin_bit_cast(T, w);
+  }
+}
+constexpr int kWaveShiftRight = 0x138, kWaveShiftLeft = 0x130;  // DPP controls wave_shr:1, wave_shl:1
 template <typename T>
 __device__ __forceinline__ T from_left(T value) {
-  return __shfl_up(value, 1, 64);
+  return lane_shifted<kWaveShiftRight>(value);
 }
 template <typename T>
 __device__ __forceinline__ T from_right(T value) {
-  return __shfl_down(value, 1, 64);
+  return lane_shifted<kWaveShiftLeft>(value);
 }
 
 // delta_x f at column j from f there and at j - 1 (docs/math.md:337).
@@ -81,7 +100,10 @@ struct ProxRooms {
 // Where a prox tile takes the samples of p beside its own that another work-group writes
 // in the same launch (the fused sweep's): p_x of the column left of the tile, row by row
 // from `left` (left[(i - left_top) * left_stride]), and p_y of the row above the tile from
-// `up` (up[j - up_first]), each null to read them from p in memory.
+// `up` (up[j - up_first]), each null to read them from p in memory. Where `inside` is not
+// null, the tile also writes there, a byte per coefficient in the layout of the kept
+// coefficients, 1 where the prox's minimizer lay inside the interval (the clip's slope is
+// the quadratic's there, 0 elsewhere: the unrolled solve's adjoint reads it).
 template <typename T>
 struct ProxEdges {
   const T* left = nullptr;
@@ -89,6 +111,7 @@ struct ProxEdges {
   int left_top = 0;
   const T* up = nullptr;
   int up_first = 0;
+  uint8_t* inside = nullptr;
 };
 
 // The tile of this wavefront in tile row ty and strip `strip` (8 tiles across) of
@@ -195,10 +218,14 @@ __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& stagi
 #pragma unroll
   for (int u = 0; u < 8; ++u) row[u] = rooms.forward[wave][n][8 * b + u];
   device::forward8(row, coefficient);
+  uint8_t* const marks =
+      (edges.inside != nullptr && inside) ? edges.inside + 64 * (static_cast<size_t>(ty) * K.columns + tx) + 8 * n : nullptr;
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
-    const T z = device::prox_of(staged, own.q[u], own.flat, 8 * n + u, coefficient[u], step);
+    bool within = false;
+    const T z = device::prox_of(staged, own.q[u], own.flat, 8 * n + u, coefficient[u], step, within);
     zeta[u] = inside ? z : T(0);
+    if (marks != nullptr) marks[u] = within;
   }
   if (keep && inside) {
     T* kept = zeta_kept + 64 * (static_cast<size_t>(ty) * K.columns + tx) + 8 * n;
@@ -255,15 +282,17 @@ __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& stagi
 template <typename T, typename S, int DV, int DH, template <typename> class Planes>
 __device__ __forceinline__ void prox_tile(const Component* __restrict__ component, Fields<T, S> F, int c,
                                           T* __restrict__ zeta_kept, T* x_out, int H, int W, T descent, T step, T rho,
-                                          T rest, int keep, int g) {
+                                          T rest, int keep, int g, uint8_t* inside = nullptr) {
   __shared__ ProxRooms<T> rooms;
   __shared__ device::StagedArrays staged_arrays;
   const Component& K = *component;
   const int wave = threadIdx.x >> 6;
   const int strips = (K.tiles_x + 7) >> 3;
+  ProxEdges<T> edges;
+  edges.inside = inside;
   prox_tile_at<T, S, DV, DH, Planes>(
       K, [&]() { return device::stage(K, staged_arrays); }, rooms, F, c, zeta_kept, x_out, H, W, descent, step, rho,
-      rest, keep, (g / strips) * 4 + wave, g % strips, ProxEdges<T>{});
+      rest, keep, (g / strips) * 4 + wave, g % strips, edges);
 }
 
 // The wavefronts per SIMD a prox kernel is compiled for: as many as its registers allow

@@ -1579,8 +1579,8 @@ void solve_in_region(const FrameModel& frame, const Settings& settings, const Pl
 class UnrolledOnGpu final : public UnrolledSolve {
  public:
   UnrolledOnGpu(const std::vector<const FrameModel*>& frames, const std::vector<const Plan*>& plans,
-                const Settings& settings, int iterations, int every)
-      : solve_(frames, plans, settings, iterations, every) {}
+                const Settings& settings, int iterations, int every, Timing timing)
+      : solve_(frames, plans, settings, iterations, every, timing == Timing::kernels) {}
   int frames() const override { return solve_.frames(); }
   int height() const override { return solve_.height(); }
   int width() const override { return solve_.width(); }
@@ -1598,6 +1598,7 @@ class UnrolledOnGpu final : public UnrolledSolve {
   void backward(const float* grad_canvas, float* grad_map, double* grad_alphas, void* stream) override {
     solve_.backward(grad_canvas, grad_map, grad_alphas, static_cast<hipStream_t>(stream));
   }
+  bool kernel_times(double* out) override { return solve_.kernel_times(out); }
 
  private:
   Unrolled solve_;
@@ -1607,7 +1608,8 @@ class UnrolledOnGpu final : public UnrolledSolve {
 
 std::unique_ptr<UnrolledSolve> make_unrolled(const std::vector<const FrameModel*>& frames,
                                              const std::vector<const Plan*>& plans, const Settings& settings,
-                                             int iterations, int every, std::string& message, int32_t& status) {
+                                             int iterations, int every, Timing timing, std::string& message,
+                                             int32_t& status) {
   try {
     int devices = 0;
     if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) {
@@ -1615,7 +1617,7 @@ std::unique_ptr<UnrolledSolve> make_unrolled(const std::vector<const FrameModel*
       message = "no HIP device: this library computes on an AMD GPU";
       return nullptr;
     }
-    return std::make_unique<UnrolledOnGpu>(frames, plans, settings, iterations, every);
+    return std::make_unique<UnrolledOnGpu>(frames, plans, settings, iterations, every, timing);
   } catch (const Refused& refusal) {
     status = 3;
     message = refusal.what();

@@ -107,13 +107,19 @@ class UnrolledSolve {
   // Of the last forward: grad_canvas [frame][H][W] to grad_map [H][W] (device) and
   // grad_alphas[0] = dL/d alpha0, [1] = dL/d alpha1 (host).
   virtual void backward(const float* grad_canvas, float* grad_map, double* grad_alphas, void* stream) = 0;
+  // Where launches are timed (timing == Timing::kernels at construction), the seconds and
+  // launches since the last forward began of: the dual step's adjoint, the prox's adjoint,
+  // the forward dual step (the forward's and the backward's recomputation); false where
+  // nothing is timed.
+  virtual bool kernel_times(double* out) = 0;
 };
 
 // Null with the reason and a status where the frames or the settings are refused or the
 // GPU fails. The frames and plans must outlive the result.
 std::unique_ptr<UnrolledSolve> make_unrolled(const std::vector<const FrameModel*>& frames,
                                              const std::vector<const Plan*>& plans, const Settings& settings,
-                                             int iterations, int every, std::string& message, int32_t& status);
+                                             int iterations, int every, Timing timing, std::string& message,
+                                             int32_t& status);
 
 // The bytes of the fixed device region behind solve_in_scratch, on each GPU.
 constexpr size_t kScratchBytes = size_t{64} << 20;

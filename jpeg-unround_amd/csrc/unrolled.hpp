@@ -12,15 +12,15 @@
 //   p^ = p + sigma g (grad x- - w-);  p~ = P_{alpha1 m}(p^);  p+ = rho p~ + (1 - rho) p
 //   r^ = r + sigma g E w-;  r~ = P_{alpha0}(r^) (|r|^2 = xx^2 + yy^2 + 2 xy^2);  r+ likewise
 //
-// The adjoint of one iteration runs in five kernels, each the transpose of a piece of the
-// forward, recomputing what it needs from the iteration's input state and x-:
-//   1. the balls' Jacobians (p^, r^ and w- recomputed), the radius and alpha0 sensitivities;
-//   2. grad x- (the transpose of grad) and grad w- (of -1 and of E), grad w~, grad w, the
-//      part of grad p known there;
-//   3. grad r (div_2's transpose);
-//   4. the data term's block: grad x~ to grad zeta by the DCT, the clip's slope, back by
-//      the inverse DCT to grad v; grad x;
-//   5. grad p's last part (div's transpose of tau g grad v).
+// The adjoint of one iteration runs in two kernels, as the forward does, each recomputing
+// what it needs from the iteration's input state and x-:
+//   1. the dual step's: the balls' Jacobians (p^, r^ and w- made again), the radius and
+//      alpha0 sensitivities, grad x-, grad w, grad r and grad p but div's part, as a walk
+//      down the rows; grad p after the iteration completed on the way in with div^T of the
+//      last prox adjoint's tau g grad v;
+//   2. the data term's block: grad x~ to grad zeta by the DCT, the clip's slope (from the
+//      marks the run's recomputed prox left, a byte a coefficient), back by the inverse
+//      DCT to grad v; grad x, and tau g grad v for the next dual adjoint.
 // The forward keeps the state every `every` iterations; the backward recomputes each run of
 // iterations from its checkpoint and walks it in reverse.
 
@@ -82,40 +82,6 @@ __device__ __forceinline__ float* frame_state(float* base, int f, size_t N) {
   return base + static_cast<size_t>(f) * 8 * N;
 }
 
-// w- at (i, j) from w, p and r (r at (i, j + 1) and (i + 1, j) too); w~ into `tilde`.
-__device__ __forceinline__ void w_bar_at(const float* w, const float* p, const float* r, size_t N, int i, int j, int H,
-                                         int W, float descent, float tilde[2], float bar[2]) {
-  float dx, dy;
-  device::tensor_divergence_at<float>(r, r + N, r + 2 * N, i, j, H, W, dx, dy);
-  const size_t at = static_cast<size_t>(i) * W + j;
-  const float wx = w[at], wy = w[N + at];
-  tilde[0] = wx + descent * (p[at] + dx);
-  tilde[1] = wy + descent * (p[N + at] + dy);
-  bar[0] = 2.0f * tilde[0] - wx;
-  bar[1] = 2.0f * tilde[1] - wy;
-}
-
-// p^ and r^ at (i, j) from the state and x-.
-__device__ __forceinline__ void ascents_at(const UnrolledState& s, const float* xbar, size_t N, int i, int j, int H,
-                                           int W, const UnrolledSteps& k, float p_hat[2], float r_hat[3],
-                                           float w_tilde[2]) {
-  float bar[2], left[2] = {0.0f, 0.0f}, up[2] = {0.0f, 0.0f}, unused[2];
-  w_bar_at(s.w, s.p, s.r, N, i, j, H, W, k.descent, w_tilde, bar);
-  if (j > 0) w_bar_at(s.w, s.p, s.r, N, i, j - 1, H, W, k.descent, unused, left);
-  if (i > 0) w_bar_at(s.w, s.p, s.r, N, i - 1, j, H, W, k.descent, unused, up);
-  const size_t at = static_cast<size_t>(i) * W + j;
-  const float g1 = device::forward_across<float>(xbar, i, j, W);
-  const float g2 = device::forward_down<float>(xbar, i, j, H, W);
-  p_hat[0] = s.p[at] + k.ascent * (g1 - bar[0]);
-  p_hat[1] = s.p[N + at] + k.ascent * (g2 - bar[1]);
-  const float across_x = backward_across_of(bar[0], left[0], j, W);
-  const float down_y = backward_down_of(bar[1], up[1], i, H);
-  const float xy = 0.5f * (backward_down_of(bar[0], up[0], i, H) + backward_across_of(bar[1], left[1], j, W));
-  r_hat[0] = s.r[at] + k.ascent * across_x;
-  r_hat[1] = s.r[N + at] + k.ascent * down_y;
-  r_hat[2] = s.r[2 * N + at] + k.ascent * xy;
-}
-
 // The forward dual step of every frame (frame blockIdx.y) by the frames' dual step, p's
 // ball of radius alpha1 m(i, j) at (i, j): w, p and r of the state into the next state.
 __global__ __launch_bounds__(256) void unrolled_dual_kernel(UnrolledFrames u, const float* __restrict__ radius, int H,
@@ -143,22 +109,19 @@ __global__ __launch_bounds__(256) void unrolled_dual_kernel(UnrolledFrames u, co
                                                                       device::ordered(blockIdx.x, gridDim.x), radius);
 }
 
-// The adjoint's gradients of one frame: of the state (x, w, p, r), and the planes between
-// its kernels.
+// The adjoint's gradients of one frame: of the state (x, w, p, r).
 struct UnrolledGrads {
   float* x;
   float* w;
   float* p;
   float* r;
 };
+// The planes between the adjoint's two kernels, and the sensitivities it adds up.
 struct UnrolledWork {
-  float* p_hat;    // 2 planes
-  float* r_hat;    // 3 planes
-  float* xbar;     // 1
-  float* w_tilde;  // 2
-  float* v;        // 1: tau g grad v
-  float* radius;   // += the radius's sensitivity
-  float* alpha0;   // += alpha0's
+  float* xbar;     // grad x-, from the dual step's adjoint to the prox's
+  float* v;        // tau g grad v, from the prox's adjoint to the next dual adjoint (div's transpose)
+  float* radius;   // += the radius's sensitivity, per sample
+  float* alpha0;   // += alpha0's, per run of rows and column
 };
 
 // Frame f's views for the adjoint's kernels.
@@ -173,206 +136,306 @@ __device__ __forceinline__ UnrolledBack back_of(const UnrolledFrames& u, int f, 
   float* const kept = frame_state(u.state, f, N);
   float* const a = frame_state(u.after, f, N);
   float* const b = frame_state(u.before, f, N);
-  float* const w = u.work + static_cast<size_t>(f) * 9 * N;
+  float* const w = u.work + static_cast<size_t>(f) * 2 * N;
   UnrolledBack v;
   v.s = UnrolledState{kept, kept + N, kept + 3 * N, kept + 5 * N};
   v.xbar = u.bars + f * N;
   v.after = UnrolledGrads{a, a + N, a + 3 * N, a + 5 * N};
   v.before = UnrolledGrads{b, b + N, b + 3 * N, b + 5 * N};
-  v.work = UnrolledWork{w,         w + 2 * N, w + 5 * N, w + 6 * N, w + 8 * N, u.sensitivity + f * N,
-                        u.sensitivity + (static_cast<size_t>(u.count) + f) * N};
+  v.work = UnrolledWork{w, w + N, u.sensitivity + f * N, u.sensitivity + (static_cast<size_t>(u.count) + f) * N};
   v.k = u.frames[f].k;
   return v;
 }
 
-// 1. The balls' Jacobians: grad p^ and grad r^ from the gradients after the iteration
-// (`after`), and the radius and alpha0 sensitivities added up.
-__global__ __launch_bounds__(256) void unrolled_balls_back_kernel(UnrolledFrames u, const float* __restrict__ radius,
-                                                                  int H, int W) {
-  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (i >= H || j >= W) return;
-  const size_t N = static_cast<size_t>(H) * W, at = static_cast<size_t>(i) * W + j;
-  const UnrolledBack v = back_of(u, static_cast<int>(blockIdx.z), N);
-  const UnrolledState& s = v.s;
-  const float* xbar = v.xbar;
-  const UnrolledGrads& after = v.after;
-  const UnrolledWork& work = v.work;
-  const UnrolledSteps& k = v.k;
-  float p_hat[2], r_hat[3], w_tilde[2];
-  ascents_at(s, xbar, N, i, j, H, W, k, p_hat, r_hat, w_tilde);
-  // p~ = P(p^): where outside the ball, p~ = R p^ / n.
-  const float gp[2] = {k.rho * after.p[at], k.rho * after.p[N + at]};
-  const float R = radius[at];
-  const float n = sqrt(p_hat[0] * p_hat[0] + p_hat[1] * p_hat[1]);
-  if (n > R) {
-    const float dot = p_hat[0] * gp[0] + p_hat[1] * gp[1];
-    for (int e = 0; e < 2; ++e) work.p_hat[e * N + at] = (R / n) * (gp[e] - p_hat[e] * (dot / (n * n)));
-    work.radius[at] += dot / n;
-  } else {
-    for (int e = 0; e < 2; ++e) work.p_hat[e * N + at] = gp[e];
-  }
-  // r~ = P(r^) in the norm with xy counted twice.
-  const float gr[3] = {k.rho * after.r[at], k.rho * after.r[N + at], k.rho * after.r[2 * N + at]};
-  const float nr = sqrt(r_hat[0] * r_hat[0] + r_hat[1] * r_hat[1] + 2.0f * (r_hat[2] * r_hat[2]));
-  if (nr > k.alpha0) {
-    const float dot = r_hat[0] * gr[0] + r_hat[1] * gr[1] + r_hat[2] * gr[2];
-    const float weight[3] = {1.0f, 1.0f, 2.0f};
-    for (int e = 0; e < 3; ++e) {
-      work.r_hat[e * N + at] = (k.alpha0 / nr) * (gr[e] - weight[e] * r_hat[e] * (dot / (nr * nr)));
-    }
-    work.alpha0[at] += dot / nr;
-  } else {
-    for (int e = 0; e < 3; ++e) work.r_hat[e * N + at] = gr[e];
-  }
-}
+// A wavefront's own columns in the dual step's adjoint: its 64 lanes take columns
+// 60 s - 2 .. 60 s + 61 of strip s and write 60 s .. 60 s + 59; lanes 0-1 and 62-63 make
+// again what the neighbouring strips make, the two columns each side its own need.
+constexpr int kBackColumns = 60;
 
-// 2. grad x- (sigma g grad^T grad p^), grad w- (-sigma g grad p^ + sigma g E^T grad r^),
-// then grad w~, grad w, and grad p less div's part.
-__global__ __launch_bounds__(256) void unrolled_bars_back_kernel(UnrolledFrames u, int H, int W) {
-  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (i >= H || j >= W) return;
-  const size_t N = static_cast<size_t>(H) * W, at = static_cast<size_t>(i) * W + j;
-  const UnrolledBack v = back_of(u, static_cast<int>(blockIdx.z), N);
-  const UnrolledGrads& after = v.after;
-  const UnrolledGrads& before = v.before;
-  const UnrolledWork& work = v.work;
-  const UnrolledSteps& k = v.k;
-  const auto g = [&](const float* plane, int di, int dj) {
-    const int a = i + di, b = j + dj;
-    return (a >= 0 && a < H && b >= 0 && b < W) ? plane[static_cast<size_t>(a) * W + b] : 0.0f;
-  };
-  const float* px = work.p_hat;
-  const float* py = work.p_hat + N;
-  const float* rxx = work.r_hat;
-  const float* ryy = work.r_hat + N;
-  const float* rxy = work.r_hat + 2 * N;
-  // p^ = ... + sigma g (F_x x-, F_y x-).
-  work.xbar[at] =
-      k.ascent * (forward_transpose(g(px, 0, -1), g(px, 0, 0), j, W) + forward_transpose(g(py, -1, 0), g(py, 0, 0), i, H));
-  // r^ = ... + sigma g (B_x w-_x, B_y w-_y, (B_y w-_x + B_x w-_y) / 2).
-  const float e_x = backward_transpose(g(rxx, 0, 0), g(rxx, 0, 1), j, W) +
-                    0.5f * backward_transpose(g(rxy, 0, 0), g(rxy, 1, 0), i, H);
-  const float e_y = backward_transpose(g(ryy, 0, 0), g(ryy, 1, 0), i, H) +
-                    0.5f * backward_transpose(g(rxy, 0, 0), g(rxy, 0, 1), j, W);
-  const float bar[2] = {-k.ascent * px[at] + k.ascent * e_x, -k.ascent * py[at] + k.ascent * e_y};
-  const float rest = 1.0f - k.rho;
-  for (int e = 0; e < 2; ++e) {
-    const float tilde = k.rho * after.w[e * N + at] + 2.0f * bar[e];
-    work.w_tilde[e * N + at] = tilde;
-    before.w[e * N + at] = rest * after.w[e * N + at] - bar[e] + tilde;
-    before.p[e * N + at] = rest * after.p[e * N + at] + work.p_hat[e * N + at] + k.descent * tilde;
-  }
-}
-
-// 3. grad r: (1 - rho) grad r+ + grad r^ + div_2^T (tau g grad w~).
-__global__ __launch_bounds__(256) void unrolled_r_back_kernel(UnrolledFrames u, int H, int W) {
-  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (i >= H || j >= W) return;
-  const size_t N = static_cast<size_t>(H) * W, at = static_cast<size_t>(i) * W + j;
-  const UnrolledBack v = back_of(u, static_cast<int>(blockIdx.z), N);
-  const UnrolledGrads& after = v.after;
-  const UnrolledGrads& before = v.before;
-  const UnrolledWork& work = v.work;
-  const UnrolledSteps& k = v.k;
-  const auto g = [&](const float* plane, int di, int dj) {
-    const int a = i + di, b = j + dj;
-    return (a >= 0 && a < H && b >= 0 && b < W) ? k.descent * plane[static_cast<size_t>(a) * W + b] : 0.0f;
-  };
-  const float* dx = work.w_tilde;
-  const float* dy = work.w_tilde + N;
-  // div_2 r = (F_x r_xx + F_y r_xy, F_x r_xy + F_y r_yy).
-  const float xx = forward_transpose(g(dx, 0, -1), g(dx, 0, 0), j, W);
-  const float yy = forward_transpose(g(dy, -1, 0), g(dy, 0, 0), i, H);
-  const float xy = forward_transpose(g(dx, -1, 0), g(dx, 0, 0), i, H) + forward_transpose(g(dy, 0, -1), g(dy, 0, 0), j, W);
-  const float rest = 1.0f - k.rho;
-  const float parts[3] = {xx, yy, xy};
-  for (int e = 0; e < 3; ++e) before.r[e * N + at] = rest * after.r[e * N + at] + work.r_hat[e * N + at] + parts[e];
-}
-
-// 4. The data term's block: one wavefront per block, a lane per sample (and per
-// coefficient). grad x~ = 2 grad x- + rho grad x+; grad zeta = DCT grad x~; through the
-// clip's slope; grad v = IDCT; grad x = grad v - grad x- + (1 - rho) grad x+.
-__global__ __launch_bounds__(256) void unrolled_prox_back_kernel(UnrolledFrames u, int H, int W) {
-  __shared__ float first[4][64];
-  __shared__ float second[4][64];
+// 1. The adjoint of the dual step of one iteration, for one frame (blockIdx.y): a wavefront
+// walks rows first .. first + rows - 1 of its strip, a lane per column. Per row i it makes
+// the forward's w-, p^ and r^ of row i + 1 again (from the iteration's input state and x-,
+// loading rows i + 1 and i + 2), the balls' Jacobians there (grad p^, grad r^ from the
+// gradients after the iteration, the radius and alpha0 sensitivities added up), and then
+// row i's gradients: grad x- = sigma g grad^T grad p^, grad w- = sigma g (E^T grad r^ -
+// grad p^), grad w~ = rho grad w+ + 2 grad w-, and grad w, grad p (less div's part) and
+// grad r = (1 - rho) grad r+ + grad r^ + div_2^T (tau g grad w~) before the iteration. The
+// gradient of p after the iteration is completed on the way in with div^T of the last prox
+// adjoint's tau g grad v. Neighbours in the row come from the neighbouring lanes; the walk
+// starts two rows above its first so that the rows it carries are whole there.
+__global__ __launch_bounds__(64) void unrolled_dual_back_kernel(UnrolledFrames u, const float* __restrict__ radius,
+                                                                int H, int W, int rows, int strips) {
   const int f = static_cast<int>(blockIdx.y);
-  const UnrolledBack views = back_of(u, f, static_cast<size_t>(H) * W);
-  const UnrolledState& s = views.s;
-  const UnrolledGrads& after = views.after;
-  const UnrolledGrads& before = views.before;
-  const UnrolledWork& work = views.work;
+  const int strip = static_cast<int>(blockIdx.x) % strips, run = static_cast<int>(blockIdx.x) / strips;
+  const int lane = static_cast<int>(threadIdx.x);
+  const int j = strip * kBackColumns - 2 + lane;
+  const bool column = j >= 0 && j < W;
+  const bool own = column && lane >= 2 && lane < 2 + kBackColumns;
+  const int first = run * rows, end = min(first + rows, H);
+  const size_t N = static_cast<size_t>(H) * W;
+  const UnrolledBack v = back_of(u, f, N);
+  const UnrolledSteps k = v.k;
+  const float rest = 1.0f - k.rho;
+  // Every kind of field through a buffer resource of its planes (at most 2^31 bytes each,
+  // as the solve's checks hold), addressed by one offset a row for all of them; a load
+  // beyond the frame reads 0 and a store there is dropped. The default cache policy
+  // throughout: the neighbouring strips' and runs' walks read the edge columns and rows
+  // again.
+  using Planes = device::BufferPlanesOf<float, device::kTemporal, device::kTemporal>;
+  const Planes w(v.s.w, 2, N), p(v.s.p, 2, N), r(v.s.r, 3, N), xbar(const_cast<float*>(v.xbar), 1, N);
+  const Planes tgv(v.work.v, 1, N);
+  const Planes after_w(v.after.w, 2, N), after_p(v.after.p, 2, N), after_r(v.after.r, 3, N);
+  const Planes map(const_cast<float*>(radius), 1, N);
+  const Planes before_w(v.before.w, 2, N), before_p(v.before.p, 2, N), before_r(v.before.r, 3, N);
+  const Planes grad_bar(v.work.xbar, 1, N), radius_sensitivity(v.work.radius, 1, N);
+  const auto index = [&](int i) { return static_cast<uint32_t>(i) * static_cast<uint32_t>(W) + static_cast<uint32_t>(j); };
+  const auto inside = [&](int i) { return column && i >= 0 && i < H; };
+  // w- of row i from w, p and r of the row and r of the row below.
+  const auto w_bar = [&](int i, float w0, float w1, float p0, float p1, const float (&r)[3], const float (&below)[3],
+                         float (&bar)[2]) {
+    const float xx_right = from_right(r[0]), xy_right = from_right(r[2]);
+    float dx = j + 1 < W ? xx_right - r[0] : 0.0f;
+    dx += i + 1 < H ? below[2] - r[2] : 0.0f;
+    float dy = j + 1 < W ? xy_right - r[2] : 0.0f;
+    dy += i + 1 < H ? below[1] - r[1] : 0.0f;
+    const float t0 = w0 + k.descent * (p0 + dx), t1 = w1 + k.descent * (p1 + dy);
+    bar[0] = 2.0f * t0 - w0;
+    bar[1] = 2.0f * t1 - w1;
+  };
+
+  // The rows the walk carries: r, x- and tau g grad v of row i + 1; w- of row i; grad p^,
+  // grad r^, grad p and grad r after the iteration of row i; grad p^'s y of row i - 1 and
+  // tau g grad w~ of row i - 1.
+  const int top = first - 2;
+  const bool in1 = inside(top + 1), in0 = inside(top);
+  const uint32_t at1 = index(top + 1), at0 = index(top);
+  float r1[3] = {r.load_if(in1, 0, at1), r.load_if(in1, 1, at1), r.load_if(in1, 2, at1)};
+  float xb1 = xbar.load_if(in1, 0, at1), v1 = tgv.load_if(in1, 0, at1);
+  float wb0[2];
+  {
+    const float r0[3] = {r.load_if(in0, 0, at0), r.load_if(in0, 1, at0), r.load_if(in0, 2, at0)};
+    w_bar(top, w.load_if(in0, 0, at0), w.load_if(in0, 1, at0), p.load_if(in0, 0, at0), p.load_if(in0, 1, at0), r0,
+          r1, wb0);
+  }
+  float gp0[2] = {0.0f, 0.0f}, gr0[3] = {0.0f, 0.0f, 0.0f}, ap0[2] = {0.0f, 0.0f}, ar0[3] = {0.0f, 0.0f, 0.0f};
+  float gp_up = 0.0f, d_up[2] = {0.0f, 0.0f};
+  float alpha0_sum = 0.0f;
+  // The loads of the step at row i (r, x- and tau g grad v of row i + 2; w, p, grad p, grad
+  // r and the radius of row i + 1; grad w of row i), issued a step ahead of their use.
+  struct Loads {
+    float r[3], xb, v, w[2], p[2], ap[2], ar[3], R, aw[2];
+  };
+  const auto fetch = [&](int i) {
+    Loads l;
+    const bool in2 = inside(i + 2), in1 = inside(i + 1), in0 = inside(i);
+    const uint32_t at2 = index(i + 2), at1 = index(i + 1), at0 = index(i);
+    for (int e = 0; e < 3; ++e) l.r[e] = r.load_if(in2, e, at2);
+    l.xb = xbar.load_if(in2, 0, at2);
+    l.v = tgv.load_if(in2, 0, at2);
+    for (int e = 0; e < 2; ++e) {
+      l.w[e] = w.load_if(in1, e, at1);
+      l.p[e] = p.load_if(in1, e, at1);
+      l.ap[e] = after_p.load_if(in1, e, at1);
+      l.aw[e] = after_w.load_if(in0, e, at0);
+    }
+    for (int e = 0; e < 3; ++e) l.ar[e] = after_r.load_if(in1, e, at1);
+    l.R = map.load_if(in1, 0, at1);
+    return l;
+  };
+  Loads next = fetch(top);
+#pragma unroll 1
+  for (int i = top; i < end; ++i) {
+    const int n1 = i + 1;
+    const Loads l = next;
+    if (i + 1 < end) next = fetch(i + 1);
+    const float(&r2)[3] = l.r;
+    const float xb2 = l.xb, v2 = l.v;
+    const float w1x = l.w[0], w1y = l.w[1], p1x = l.p[0], p1y = l.p[1];
+    float ap1[2] = {l.ap[0], l.ap[1]};
+    const float(&ar1)[3] = l.ar;
+    const float R = l.R;
+    const float(&aw0)[2] = l.aw;
+    // The forward's w-, p^ and r^ of row i + 1.
+    float wb1[2];
+    w_bar(n1, w1x, w1y, p1x, p1y, r1, r2, wb1);
+    const float xb_right = from_right(xb1);
+    const float g1 = j + 1 < W ? xb_right - xb1 : 0.0f;
+    const float g2 = n1 + 1 < H ? xb2 - xb1 : 0.0f;
+    const float ph[2] = {p1x + k.ascent * (g1 - wb1[0]), p1y + k.ascent * (g2 - wb1[1])};
+    const float left[2] = {from_left(wb1[0]), from_left(wb1[1])};
+    const float across_x = backward_across_of(wb1[0], left[0], j, W);
+    const float down_y = backward_down_of(wb1[1], wb0[1], n1, H);
+    const float xy = 0.5f * (backward_down_of(wb1[0], wb0[0], n1, H) + backward_across_of(wb1[1], left[1], j, W));
+    const float rh[3] = {r1[0] + k.ascent * across_x, r1[1] + k.ascent * down_y, r1[2] + k.ascent * xy};
+    // grad p after the iteration, with div^T of tau g grad v: div p = B_x p_x + B_y p_y.
+    const float v_right = from_right(v1);
+    ap1[0] += backward_transpose(v1, v_right, j, W);
+    ap1[1] += backward_transpose(v1, v2, n1, H);
+    // The balls' Jacobians at row i + 1: where p^ is outside, p~ = R p^ / n.
+    const bool counted = own && n1 >= first && n1 < end;
+    float gp1[2], gr1[3];
+    {
+      const float gp[2] = {k.rho * ap1[0], k.rho * ap1[1]};
+      // One reciprocal of the norm (hardware, 1 ulp) for the three quotients: the gradients
+      // are held to 1e-3 of binary64's, far from binary32's last places.
+      const float n = sqrt(ph[0] * ph[0] + ph[1] * ph[1]);
+      if (n > R) {
+        const float inverse = __builtin_amdgcn_rcpf(n);
+        const float dot = ph[0] * gp[0] + ph[1] * gp[1];
+        const float along = dot * inverse, scale = R * inverse;
+        for (int e = 0; e < 2; ++e) gp1[e] = scale * (gp[e] - ph[e] * (along * inverse));
+        if (counted) radius_sensitivity.store(0, index(n1), radius_sensitivity.load(0, index(n1)) + along);
+      } else {
+        gp1[0] = gp[0];
+        gp1[1] = gp[1];
+      }
+      // r~ = P(r^) in the norm with xy counted twice.
+      const float gr[3] = {k.rho * ar1[0], k.rho * ar1[1], k.rho * ar1[2]};
+      const float nr = sqrt(rh[0] * rh[0] + rh[1] * rh[1] + 2.0f * (rh[2] * rh[2]));
+      if (nr > k.alpha0) {
+        const float inverse = __builtin_amdgcn_rcpf(nr);
+        const float dot = rh[0] * gr[0] + rh[1] * gr[1] + rh[2] * gr[2];
+        const float along = dot * inverse, scale = k.alpha0 * inverse;
+        const float weight[3] = {1.0f, 1.0f, 2.0f};
+        for (int e = 0; e < 3; ++e) gr1[e] = scale * (gr[e] - weight[e] * rh[e] * (along * inverse));
+        if (counted) alpha0_sum += along;
+      } else {
+        gr1[0] = gr[0];
+        gr1[1] = gr[1];
+        gr1[2] = gr[2];
+      }
+    }
+    // Row i's gradients. p^ = ... + sigma g (F_x x-, F_y x-).
+    const float gpx_left = from_left(gp0[0]);
+    const float gxbar = k.ascent * (forward_transpose(gpx_left, gp0[0], j, W) + forward_transpose(gp_up, gp0[1], i, H));
+    // r^ = ... + sigma g (B_x w-_x, B_y w-_y, (B_y w-_x + B_x w-_y) / 2).
+    const float grxx_right = from_right(gr0[0]), grxy_right = from_right(gr0[2]);
+    const float e_x = backward_transpose(gr0[0], grxx_right, j, W) + 0.5f * backward_transpose(gr0[2], gr1[2], i, H);
+    const float e_y = backward_transpose(gr0[1], gr1[1], i, H) + 0.5f * backward_transpose(gr0[2], grxy_right, j, W);
+    const float bar[2] = {-k.ascent * gp0[0] + k.ascent * e_x, -k.ascent * gp0[1] + k.ascent * e_y};
+    float d0[2], bw[2], bp[2];
+    for (int e = 0; e < 2; ++e) {
+      const float tilde = k.rho * aw0[e] + 2.0f * bar[e];
+      bw[e] = rest * aw0[e] - bar[e] + tilde;
+      bp[e] = rest * ap0[e] + gp0[e] + k.descent * tilde;
+      d0[e] = k.descent * tilde;
+    }
+    // div_2 r = (F_x r_xx + F_y r_xy, F_x r_xy + F_y r_yy).
+    const float dx_left = from_left(d0[0]), dy_left = from_left(d0[1]);
+    const float xx = forward_transpose(dx_left, d0[0], j, W);
+    const float yy = forward_transpose(d_up[1], d0[1], i, H);
+    const float txy = forward_transpose(d_up[0], d0[0], i, H) + forward_transpose(dy_left, d0[1], j, W);
+    const bool writes = own && i >= first;
+    const uint32_t at = index(i);
+    grad_bar.store_if<Planes::kNext>(writes, 0, at, gxbar);  // the prox adjoint reads it next
+    for (int e = 0; e < 2; ++e) {
+      before_w.store_if(writes, e, at, bw[e]);
+      before_p.store_if(writes, e, at, bp[e]);
+    }
+    before_r.store_if(writes, 0, at, rest * ar0[0] + gr0[0] + xx);
+    before_r.store_if(writes, 1, at, rest * ar0[1] + gr0[1] + yy);
+    before_r.store_if(writes, 2, at, rest * ar0[2] + gr0[2] + txy);
+    gp_up = gp0[1];
+    d_up[0] = d0[0];
+    d_up[1] = d0[1];
+    for (int e = 0; e < 2; ++e) {
+      gp0[e] = gp1[e];
+      ap0[e] = ap1[e];
+      wb0[e] = wb1[e];
+    }
+    for (int e = 0; e < 3; ++e) {
+      gr0[e] = gr1[e];
+      ar0[e] = ar1[e];
+      r1[e] = r2[e];
+    }
+    xb1 = xb2;
+    v1 = v2;
+  }
+  // alpha0's sensitivity of the lane's column over the run's rows, into the run's slot.
+  if (own) v.work.alpha0[static_cast<size_t>(run) * W + j] += alpha0_sum;
+}
+
+// 2. The adjoint of the prox of one iteration, for one frame (blockIdx.y), tiled as the
+// forward's prox: a wavefront takes a row of 8 blocks (64 columns, a lane per column, 8
+// rows in registers), the 8 x 8 transforms across lanes through LDS. The clip's slope is
+// r = 1 / (1 + t) where the forward's prox marked the coefficient inside its interval
+// (`inside`, [frame][N] bytes in the kept coefficients' layout), 0 elsewhere: grad x~ =
+// 2 grad x- + rho grad x+, grad zeta = DCT grad x~, grad e = slope grad zeta, grad v =
+// IDCT grad e; grad x = grad v - grad x- + (1 - rho) grad x+ before the iteration, and
+// tau g grad v for the next dual adjoint's div^T.
+__global__ __launch_bounds__(256) void unrolled_prox_back_kernel(UnrolledFrames u, const uint8_t* __restrict__ inside,
+                                                                 int H, int W, int strips) {
+#pragma clang fp contract(off)
+  __shared__ float room[4][8][65];
+  const int f = static_cast<int>(blockIdx.y);
+  const size_t N = static_cast<size_t>(H) * W;
+  const UnrolledBack views = back_of(u, f, N);
   const UnrolledSteps& k = views.k;
   const float step = u.frames[f].step;
   const Component& K = u.components[f];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane >> 3, n = lane & 7;
-  const int tile = blockIdx.x * 4 + wave;
-  const bool active = tile < K.rows * K.columns;
-  const int ty = active ? tile / K.columns : 0, tx = active ? tile % K.columns : 0;
-  const size_t N = static_cast<size_t>(H) * W;
-  const int i = 8 * ty + m, j = 8 * tx + n;
-  const size_t at = static_cast<size_t>(i) * W + j;
-  // The forward's coefficient, its clip and slope.
-  const float v = active ? s.x[at] + k.descent * device::divergence_at<float>(s.p, s.p + N, i, j, H, W) : 0.0f;
-  first[wave][lane] = v;
-  __syncthreads();
-  second[wave][lane] = device::forward_down(first[wave], m, n);
-  __syncthreads();
-  const float e = device::forward_along(second[wave], m, n);
-  float slope = 0.0f;
-  if (active) {
-    const size_t block = static_cast<size_t>(ty) * K.columns + tx;
-    const int16_t q = K.levels[64 * block + lane];
-    const bool flat = K.flat[block] != 0;
-    double lower, upper, centre, own_lower, own_upper;
-    device::ends_of(K, q, lane, lower, upper, centre, own_lower, own_upper);
-    const float t = step * static_cast<float>(flat ? K.flat_weights[lane] : K.weights[lane]);
-    const float r = 1.0f / (1.0f + t);
-    const float z = (e + t * static_cast<float>(centre)) * r;
-    slope = (z >= static_cast<float>(lower) && z <= static_cast<float>(upper)) ? r : 0.0f;
+  const int wave = static_cast<int>(threadIdx.x) >> 6, lane = static_cast<int>(threadIdx.x) & 63;
+  const int strip = static_cast<int>(blockIdx.x) % strips;
+  const int ty = (static_cast<int>(blockIdx.x) / strips) * 4 + wave;
+  const int b = lane >> 3, n = lane & 7;
+  const int tx = strip * 8 + b;
+  const bool live = ty < K.rows && tx < K.columns;
+  const int j = 8 * tx + n, top = 8 * ty;
+  const auto at = [&](int i) { return static_cast<size_t>(i) * W + j; };
+  // grad x~ of the lane's column, to grad zeta (the inverse DCT's transpose is the DCT).
+  float grad_bar[8], after_x[8], column[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    grad_bar[m] = live ? views.work.xbar[at(top + m)] : 0.0f;
+    after_x[m] = live ? views.after.x[at(top + m)] : 0.0f;
+    column[m] = 2.0f * grad_bar[m] + k.rho * after_x[m];
   }
+  // A lane's row n of block b: the marks and the slopes.
+  const size_t block = live ? static_cast<size_t>(ty) * K.columns + tx : 0;
+  const uint64_t marks = live ? *reinterpret_cast<const uint64_t*>(inside + f * N + 64 * block + 8 * n) : 0;
+  const bool flat = live && K.flat[block] != 0;
+  float slope[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const float t = step * static_cast<float>((flat ? K.flat_weights : K.weights)[8 * n + c]);
+    slope[c] = ((marks >> (8 * c)) & 0xFFu) != 0 ? 1.0f / (1.0f + t) : 0.0f;
+  }
+  float y[8], row[8];
+  device::forward8(column, y);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) room[wave][q][lane] = y[q];
   __syncthreads();
-  // grad x~ to grad zeta (the inverse DCT's transpose is the DCT).
-  const float grad_tilde = active ? 2.0f * work.xbar[at] + k.rho * after.x[at] : 0.0f;
-  first[wave][lane] = grad_tilde;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) row[c] = room[wave][n][8 * b + c];
   __syncthreads();
-  second[wave][lane] = device::forward_down(first[wave], m, n);
-  __syncthreads();
-  const float grad_e = slope * device::forward_along(second[wave], m, n);
-  __syncthreads();
+  float grad_zeta[8], grad_e[8];
+  device::forward8(row, grad_zeta);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) grad_e[c] = slope[c] * grad_zeta[c];
   // grad e to grad v (the DCT's transpose is the inverse DCT).
-  first[wave][lane] = grad_e;
+  device::inverse8(grad_e, row);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) room[wave][n][8 * b + c] = row[c];
   __syncthreads();
-  second[wave][lane] = device::inverse_along(first[wave], m, n);
-  __syncthreads();
-  const float grad_v = device::inverse_down(second[wave], m, n);
-  if (!active) return;
-  before.x[at] = grad_v - work.xbar[at] + (1.0f - k.rho) * after.x[at];
-  work.v[at] = k.descent * grad_v;
-}
-
-// 5. grad p += div^T (tau g grad v): div p = B_x p_x + B_y p_y.
-__global__ __launch_bounds__(256) void unrolled_div_back_kernel(UnrolledFrames u, int H, int W) {
-  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (i >= H || j >= W) return;
-  const size_t N = static_cast<size_t>(H) * W, at = static_cast<size_t>(i) * W + j;
-  const UnrolledBack v = back_of(u, static_cast<int>(blockIdx.z), N);
-  const UnrolledGrads& before = v.before;
-  const UnrolledWork& work = v.work;
-  const float here = work.v[at];
-  const float right = j + 1 < W ? work.v[at + 1] : 0.0f;
-  const float below = i + 1 < H ? work.v[at + W] : 0.0f;
-  before.p[at] += backward_transpose(here, right, j, W);
-  before.p[N + at] += backward_transpose(here, below, i, H);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) y[q] = room[wave][q][lane];
+  float grad_v[8];
+  device::inverse8(y, grad_v);
+  if (!live) return;
+  const float rest = 1.0f - k.rho;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    views.before.x[at(top + m)] = grad_v[m] - grad_bar[m] + rest * after_x[m];
+    views.work.v[at(top + m)] = k.descent * grad_v[m];
+  }
 }
 
 // The forward prox of every frame (frame blockIdx.y): x of the state relaxed into the next
-// state, x- into `bars`; no tilde copies.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kProxWaves<float, 1, 1>))) void
-unrolled_prox_kernel(UnrolledFrames u, int H, int W) {
+// state, x- into `bars`; no tilde copies. MARKS: the clip's marks of every frame's
+// coefficients also go to `inside` ([frame][N] bytes) for the adjoint (an instance of its
+// own, with the registers the marks take).
+template <bool MARKS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MARKS ? 3 : kProxWaves<float, 1, 1>))) void
+unrolled_prox_kernel(UnrolledFrames u, int H, int W, uint8_t* inside) {
   const int f = static_cast<int>(blockIdx.y);
   const size_t N = static_cast<size_t>(H) * W;
   const UnrolledFrame scalars = u.frames[f];
@@ -384,7 +447,8 @@ unrolled_prox_kernel(UnrolledFrames u, int H, int W) {
   F.p = now + 3 * N;
   prox_tile<float, float, 1, 1, device::BufferPlanes>(u.components + f, F, 0, nullptr, frame_state(u.next, f, N), H,
                                                       W, scalars.k.descent, scalars.step, scalars.k.rho,
-                                                      1.0f - scalars.k.rho, 0, device::ordered(blockIdx.x, gridDim.x));
+                                                      1.0f - scalars.k.rho, 0, device::ordered(blockIdx.x, gridDim.x),
+                                                      MARKS ? inside + f * N : nullptr);
 }
 
 __global__ void scale_kernel(const float* __restrict__ map, float factor, float* __restrict__ out, size_t count) {
@@ -429,8 +493,9 @@ __global__ __launch_bounds__(256) void unrolled_parameters_kernel(const float* _
 class Unrolled {
  public:
   Unrolled(const std::vector<const FrameModel*>& frames, const std::vector<const Plan*>& plans,
-           const Settings& settings, int iterations, int every)
-      : frames_(frames), plans_(plans), settings_(settings), iterations_(iterations), every_(std::max(1, every)) {
+           const Settings& settings, int iterations, int every, bool timed = false)
+      : frames_(frames), plans_(plans), settings_(settings), iterations_(iterations), every_(std::max(1, every)),
+        timed_(timed) {
     if (frames_.empty()) throw Refused("an unrolled solve of no frame");
     H_ = static_cast<int>(frames_[0]->rows);
     W_ = static_cast<int>(frames_[0]->columns);
@@ -458,8 +523,9 @@ class Unrolled {
       solvers_.back()->prepare(nullptr);
     }
     // Every frame's state side by side (two of them, the iteration's input and output);
-    // x-; the start, kept; checkpoints; a run's states and x-; gradients; work planes. The
-    // frames' components side by side, their level tables staying their solvers'.
+    // x-; the start, kept; checkpoints; a run's states, x- and clip marks; gradients; work
+    // planes. The frames' components side by side, their level tables staying their
+    // solvers'.
     const size_t state = 8 * N_;
     states_[0] = DeviceBuffer(sizeof(float) * B * state);
     states_[1] = DeviceBuffer(sizeof(float) * B * state);
@@ -468,8 +534,9 @@ class Unrolled {
     checkpoints_ = DeviceBuffer(sizeof(float) * B * state * (iterations_ / every_ + 1));
     run_states_ = DeviceBuffer(sizeof(float) * B * state * every_);
     run_bars_ = DeviceBuffer(sizeof(float) * B * N_ * every_);
+    run_inside_ = DeviceBuffer(static_cast<size_t>(B) * N_ * every_);
     grads_ = DeviceBuffer(sizeof(float) * B * state * 2);
-    work_ = DeviceBuffer(sizeof(float) * B * N_ * 11);
+    work_ = DeviceBuffer(sizeof(float) * B * N_ * 4);
     radius_ = DeviceBuffer(sizeof(float) * N_);
     // The dual step's rows per wavefront: about 8192 wavefronts over all the frames (one
     // channel a row: four times the wavefronts of three channels' 2048 for the bytes in
@@ -477,6 +544,7 @@ class Unrolled {
     const long strips = (W_ + kStripColumns - 1) / kStripColumns;
     while (dual_rows_ < 64 && B * strips * ((H_ + 2 * dual_rows_ - 1) / (2 * dual_rows_)) >= 8192) dual_rows_ *= 2;
     if (const char* rows = std::getenv("UNROUND_DUAL_ROWS")) dual_rows_ = std::max(1, std::atoi(rows));
+    if (const char* rows = std::getenv("UNROUND_BACK_ROWS")) back_rows_ = std::max(1, std::atoi(rows));
     components_ = DeviceBuffer(sizeof(Component) * B);
     scalars_ = DeviceBuffer(sizeof(UnrolledFrame) * B);
     for (int f = 0; f < B; ++f) {
@@ -504,6 +572,7 @@ class Unrolled {
   void forward(const float* map, double alpha0, double alpha1, float* canvas, hipStream_t caller) {
     hipStream_t s = stream_.stream;
     join(caller, s);
+    release_timings();
     map_ = map;
     alpha0_ = static_cast<float>(alpha0);
     alpha1_ = static_cast<float>(alpha1);
@@ -514,11 +583,20 @@ class Unrolled {
     const size_t all = static_cast<size_t>(B) * 8 * N_;
     int live = 0;
     copy(states_[live].as<float>(), start_.as<float>(), all);
+    // The runs before the last, keeping their first states; the last run as the backward
+    // recomputes a run, its states, x- and clip marks kept for it.
+    const int last_run = (iterations_ - 1) / every_;
     for (int it = 0; it < iterations_; ++it) {
       if (it % every_ == 0) copy(checkpoint(it / every_), states_[live].as<float>(), all);
+      if (it == last_run * every_) {
+        run_into_slots(it, iterations_, states_[live].as<float>(), states_[1 - live].as<float>());
+        live = 1 - live;
+        break;
+      }
       step(states_[live].as<float>(), states_[1 - live].as<float>(), bars_.as<float>());
       live = 1 - live;
     }
+    last_run_kept_ = true;
     // x of every frame: the first of its 8 planes.
     HIP_CHECK(hipMemcpy2DAsync(canvas, sizeof(float) * N_, states_[live].pointer, sizeof(float) * 8 * N_,
                                sizeof(float) * N_, static_cast<size_t>(B), hipMemcpyDeviceToDevice, s));
@@ -540,22 +618,17 @@ class Unrolled {
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpy2DAsync(g, sizeof(float) * state, grad_canvas, sizeof(float) * N_, sizeof(float) * N_,
                                static_cast<size_t>(B), hipMemcpyDeviceToDevice, s));
-    float* sensitivities = work_.as<float>() + B * N_ * 9;  // radius [B][N], then alpha0 [B][N]
-    fill_kernel<<<static_cast<unsigned>((2 * B * N_ + 255) / 256), 256, 0, s>>>(sensitivities, 0.0f, 2 * B * N_);
+    // The work planes: grad x- and tau g grad v per frame, 0 for the last iteration's div^T;
+    // then the sensitivities, the radius's [B][N] and alpha0's [B][runs of rows][W].
+    fill_kernel<<<static_cast<unsigned>((4 * B * N_ + 255) / 256), 256, 0, s>>>(work_.as<float>(), 0.0f, 4 * B * N_);
     HIP_CHECK(hipGetLastError());
+    float* sensitivities = work_.as<float>() + 2 * B * N_;
     int parity = 0;  // which half of grads_ holds the gradients after the current iteration
     const int runs = (iterations_ + every_ - 1) / every_;
     for (int run = runs - 1; run >= 0; --run) {
       const int first = run * every_, last = std::min(iterations_, first + every_);
-      // Recompute the run, each iteration's input state into its own slot (the last one's
-      // output, not needed, into a spare state) and its x- beside.
-      float* const kept = run_states_.as<float>();
-      copy(kept, checkpoint(run), all);
-      for (int it = first; it < last; ++it) {
-        float* const next = it + 1 < last ? kept + static_cast<size_t>(it + 1 - first) * all : states_[0].as<float>();
-        step(kept + static_cast<size_t>(it - first) * all, next,
-             run_bars_.as<float>() + static_cast<size_t>(it - first) * B * N_);
-      }
+      // Recompute the run (the forward kept the last one's, for the first backward after it).
+      if (run != runs - 1 || !last_run_kept_) run_into_slots(first, last, checkpoint(run), states_[0].as<float>());
       for (int it = last - 1; it >= first; --it) {
         UnrolledFrames u = frames_view();
         u.state = run_states_.as<float>() + static_cast<size_t>(it - first) * all;
@@ -564,10 +637,11 @@ class Unrolled {
         u.before = g + static_cast<size_t>(1 - parity) * all;
         u.work = work_.as<float>();
         u.sensitivity = sensitivities;
-        step_back(u);
+        step_back(u, run_inside_.as<uint8_t>() + static_cast<size_t>(it - first) * B * N_);
         parity = 1 - parity;
       }
     }
+    last_run_kept_ = false;  // the slots now hold the first run's
     DeviceBuffer partials(sizeof(double) * 2 * ((N_ + 255) / 256));
     const unsigned groups = static_cast<unsigned>((N_ + 255) / 256);
     unrolled_parameters_kernel<<<groups, 256, 0, s>>>(sensitivities + B * N_, sensitivities, map_, alpha1_, B, N_,
@@ -586,7 +660,58 @@ class Unrolled {
     join(s, caller);
   }
 
+  // The seconds and launches since the last forward began of the dual step's adjoint, the
+  // prox's adjoint and the forward dual step, where launches are timed.
+  bool kernel_times(double* out) {
+    if (!timed_) return false;
+    HIP_CHECK(hipStreamSynchronize(stream_.stream));
+    for (int kind = 0; kind < 3; ++kind) {
+      out[2 * kind] = 0.0;
+      out[2 * kind + 1] = 0.0;
+    }
+    for (const Timed& t : timings_) {
+      float ms = 0.0f;
+      HIP_CHECK(hipEventElapsedTime(&ms, t.begin, t.end));
+      out[2 * t.kind] += 1e-3 * ms;
+      out[2 * t.kind + 1] += 1.0;
+    }
+    return true;
+  }
+
+  ~Unrolled() { release_timings(); }
+
  private:
+  // A timed launch: its kind (0 the dual step's adjoint, 1 the prox's, 2 the forward dual
+  // step) and the events around it.
+  struct Timed {
+    int kind;
+    hipEvent_t begin, end;
+  };
+
+  // `launch` on the solve's stream, between two events where launches are timed.
+  template <typename Launch>
+  void timed(int kind, Launch&& launch) {
+    if (!timed_) {
+      launch();
+      return;
+    }
+    Timed t{kind, nullptr, nullptr};
+    HIP_CHECK(hipEventCreate(&t.begin));
+    HIP_CHECK(hipEventCreate(&t.end));
+    timings_.push_back(t);
+    HIP_CHECK(hipEventRecord(t.begin, stream_.stream));
+    launch();
+    HIP_CHECK(hipEventRecord(t.end, stream_.stream));
+  }
+
+  void release_timings() {
+    for (const Timed& t : timings_) {
+      (void)hipEventDestroy(t.begin);
+      (void)hipEventDestroy(t.end);
+    }
+    timings_.clear();
+  }
+
   // Work on `to` waits for what `from` has queued.
   static void join(hipStream_t from, hipStream_t to) {
     if (from == to) return;
@@ -637,41 +762,66 @@ class Unrolled {
     copy(to + 5 * N_, F.r, 3 * N_);
   }
 
+  // Iterations first .. last - 1 from the states at `from`, each iteration's input state
+  // into its own slot of the run, its x- and clip marks beside; the last one's output into
+  // `out`.
+  void run_into_slots(int first, int last, const float* from, float* out) {
+    const int B = frames();
+    const size_t all = static_cast<size_t>(B) * 8 * N_;
+    float* const kept = run_states_.as<float>();
+    copy(kept, from, all);
+    for (int it = first; it < last; ++it) {
+      float* const next = it + 1 < last ? kept + static_cast<size_t>(it + 1 - first) * all : out;
+      step(kept + static_cast<size_t>(it - first) * all, next,
+           run_bars_.as<float>() + static_cast<size_t>(it - first) * B * N_,
+           run_inside_.as<uint8_t>() + static_cast<size_t>(it - first) * B * N_);
+    }
+  }
+
   // One forward iteration of every frame from the states at `now` into those at `next`:
   // the prox (x relaxed, x- into `bars`), then the dual step with the radius per pixel.
-  void step(float* now, float* next, float* bars) {
+  void step(float* now, float* next, float* bars, uint8_t* inside = nullptr) {
     hipStream_t s = stream_.stream;
     UnrolledFrames u = frames_view();
     u.state = now;
     u.next = next;
     u.bars = bars;
     const unsigned B = static_cast<unsigned>(frames());
-    unrolled_prox_kernel<<<dim3(static_cast<unsigned>(solvers_[0]->prox_groups(0)), B), 256, 0, s>>>(u, H_, W_);
+    const dim3 tiles(static_cast<unsigned>(solvers_[0]->prox_groups(0)), B);
+    if (inside != nullptr) {
+      unrolled_prox_kernel<true><<<tiles, 256, 0, s>>>(u, H_, W_, inside);
+    } else {
+      unrolled_prox_kernel<false><<<tiles, 256, 0, s>>>(u, H_, W_, nullptr);
+    }
     HIP_CHECK(hipGetLastError());
     const int strips = (W_ + kStripColumns - 1) / kStripColumns, across = (strips + 3) / 4;
     const int groups = across * ((H_ + dual_rows_ - 1) / dual_rows_);
-    unrolled_dual_kernel<<<dim3(static_cast<unsigned>(groups), B), 256, 0, s>>>(u, radius_.as<float>(), H_, W_,
-                                                                                dual_rows_, across);
-    HIP_CHECK(hipGetLastError());
+    timed(2, [&] {
+      unrolled_dual_kernel<<<dim3(static_cast<unsigned>(groups), B), 256, 0, s>>>(u, radius_.as<float>(), H_, W_,
+                                                                                  dual_rows_, across);
+      HIP_CHECK(hipGetLastError());
+    });
   }
 
   // The adjoint of one iteration of every frame, from its input state and x- (u.state,
-  // u.bars), the gradients after it (u.after) into those before it (u.before).
-  void step_back(const UnrolledFrames& u) {
+  // u.bars), the gradients after it (u.after) into those before it (u.before): the dual
+  // step's adjoint, then the prox's.
+  void step_back(const UnrolledFrames& u, const uint8_t* inside) {
     hipStream_t s = stream_.stream;
     const unsigned B = static_cast<unsigned>(frames());
-    const dim3 pixels((W_ + 63) / 64, (H_ + 3) / 4, B);
-    unrolled_balls_back_kernel<<<pixels, 256, 0, s>>>(u, radius_.as<float>(), H_, W_);
-    HIP_CHECK(hipGetLastError());
-    unrolled_bars_back_kernel<<<pixels, 256, 0, s>>>(u, H_, W_);
-    HIP_CHECK(hipGetLastError());
-    unrolled_r_back_kernel<<<pixels, 256, 0, s>>>(u, H_, W_);
-    HIP_CHECK(hipGetLastError());
-    const unsigned blocks = static_cast<unsigned>(frames_[0]->components[0].blocks());
-    unrolled_prox_back_kernel<<<dim3((blocks + 3) / 4, B), 256, 0, s>>>(u, H_, W_);
-    HIP_CHECK(hipGetLastError());
-    unrolled_div_back_kernel<<<pixels, 256, 0, s>>>(u, H_, W_);
-    HIP_CHECK(hipGetLastError());
+    const int strips = (W_ + kBackColumns - 1) / kBackColumns, runs = (H_ + back_rows_ - 1) / back_rows_;
+    timed(0, [&] {
+      unrolled_dual_back_kernel<<<dim3(static_cast<unsigned>(strips * runs), B), 64, 0, s>>>(
+          u, radius_.as<float>(), H_, W_, back_rows_, strips);
+      HIP_CHECK(hipGetLastError());
+    });
+    const ComponentModel& c = frames_[0]->components[0];
+    const int tile_strips = static_cast<int>((c.columns + 7) / 8), tile_rows = static_cast<int>((c.rows + 3) / 4);
+    timed(1, [&] {
+      unrolled_prox_back_kernel<<<dim3(static_cast<unsigned>(tile_strips * tile_rows), B), 256, 0, s>>>(
+          u, inside, H_, W_, tile_strips);
+      HIP_CHECK(hipGetLastError());
+    });
   }
 
   const std::vector<const FrameModel*> frames_;
@@ -682,11 +832,15 @@ class Unrolled {
   size_t N_ = 0;
   StreamHolder stream_;
   std::vector<std::unique_ptr<Solver<float, float>>> solvers_;
-  DeviceBuffer states_[2], bars_, start_, checkpoints_, run_states_, run_bars_, grads_, work_, radius_;
+  DeviceBuffer states_[2], bars_, start_, checkpoints_, run_states_, run_bars_, run_inside_, grads_, work_, radius_;
   DeviceBuffer components_, scalars_;
   std::vector<UnrolledFrame> scalars_host_;
   int dual_rows_ = 8;
+  int back_rows_ = 16;  // the rows a wavefront of the dual step's adjoint walks
   const float* map_ = nullptr;
   float alpha0_ = 0.0f, alpha1_ = 0.0f;
   bool forwarded_ = false;
+  bool last_run_kept_ = false;  // the run slots hold the last run of the last forward
+  const bool timed_;
+  std::vector<Timed> timings_;
 };

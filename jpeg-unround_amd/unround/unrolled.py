@@ -39,6 +39,7 @@ unround_unrolled_start      i32   handle f64* message
 unround_unrolled_problem    i32   handle u64 f64*
 unround_unrolled_forward    i32   handle address f64 f64 address address message
 unround_unrolled_backward   i32   handle address address f64* address message
+unround_unrolled_kernel_times  i32  handle f64* u64
 """
 
 _declared_library: ctypes.CDLL | None = None
@@ -110,6 +111,16 @@ class UnrolledTGV:
         if status != native.Status.OK:
             raise native.UnroundError(native.Status.ARGUMENT, f"no picture {frame}")
         return out
+
+    def kernel_times(self) -> dict[str, tuple[float, int]] | None:
+        """Seconds and launches since the last forward began of the dual step's adjoint
+        ("dual_back"), the prox's adjoint ("prox_back") and the forward dual step ("dual"),
+        where the handle was made with UNROUND_TIMING=2 in the environment; else None."""
+        values = (ctypes.c_double * 6)()
+        if not self._library.unround_unrolled_kernel_times(self._handle, values, 6):
+            return None
+        names = ("dual_back", "prox_back", "dual")
+        return {name: (float(values[2 * k]), int(values[2 * k + 1])) for k, name in enumerate(names)}
 
     def __call__(self, alpha0: torch.Tensor, alpha1: torch.Tensor, weight_map: torch.Tensor) -> torch.Tensor:
         return _Unroll.apply(alpha0, alpha1, weight_map, self)

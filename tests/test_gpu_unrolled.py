@@ -116,3 +116,38 @@ def test_gradients_do_not_depend_on_the_checkpoints():
         assert np.array_equal(other[0], results[0][0])
         assert other[1] == results[0][1] and other[2] == results[0][2]
         assert np.array_equal(other[3], results[0][3])
+
+
+def test_gradients_do_not_depend_on_the_adjoint_s_runs(monkeypatch):
+    # The dual step's adjoint walks runs of rows, each starting two rows above its first with
+    # the same arithmetic: the map's gradient and alpha1's come out the same bit for bit
+    # whatever the runs; alpha0's sums its sensitivities a run at a time (binary32 per run,
+    # binary64 over them), so it agrees to rounding.
+    images, originals = pictures()
+    results = []
+    for rows in ("5", "16", "64"):
+        monkeypatch.setenv("UNROUND_BACK_ROWS", rows)
+        solve = UnrolledTGV(images, ["--method", "tgv"], iterations=ITERATIONS, checkpoint_every=5)
+        results.append(gpu_gradients(solve, originals))
+        solve.close()
+    for other in results[1:]:
+        assert np.array_equal(other[0], results[0][0])
+        assert other[2] == results[0][2]
+        assert np.array_equal(other[3], results[0][3])
+        assert abs(other[1] - results[0][1]) <= 1e-6 * abs(results[0][1])
+
+
+def test_launches_are_timed_where_asked(monkeypatch):
+    images, originals = pictures()
+    solve = UnrolledTGV(images, ["--method", "tgv"], iterations=ITERATIONS, checkpoint_every=5)
+    gpu_gradients(solve, originals)
+    assert solve.kernel_times() is None
+    monkeypatch.setenv("UNROUND_TIMING", "2")
+    timed = UnrolledTGV(images, ["--method", "tgv"], iterations=ITERATIONS, checkpoint_every=5)
+    gpu_gradients(timed, originals)
+    times = timed.kernel_times()
+    # 20 adjoint iterations; the forward's 20 dual steps and the recomputation of the three
+    # runs before the last (15).
+    assert times["dual_back"][1] == ITERATIONS and times["prox_back"][1] == ITERATIONS
+    assert times["dual"][1] == ITERATIONS + 15
+    assert all(seconds > 0 for seconds, _ in times.values())
