@@ -585,12 +585,54 @@ def _floats(pointer: object, shape: tuple[int, ...], what: str, count: int | Non
     return np.ctypeslib.as_array(pointer, shape=(size,)).astype(np.float64, copy=True).reshape(shape)
 
 
-class _ResultReader:
-    """One result handle of the library, read into this module's types."""
+class _HeldResult:
+    """A result handle of the library, freed once nothing holds it: the arrays read from it
+    view its memory rather than copy it, and each keeps this alive."""
 
     def __init__(self, library: ctypes.CDLL, handle: ctypes.c_void_p) -> None:
         self.lib = library
         self.handle = handle
+
+    def free(self) -> None:
+        handle, self.handle = self.handle, None
+        if handle:
+            self.lib.unround_result_free(handle)
+
+    def __del__(self) -> None:
+        try:
+            self.free()
+        except Exception:  # noqa: BLE001 -- the library may be gone at interpreter exit
+            pass
+
+
+def _viewed(pointer: object, shape: tuple[int, ...], what: str, count: int | None, held: _HeldResult) -> Array:
+    """The doubles at a pointer into a held result as an array of `shape` over that memory
+    (writable, no copy), which keeps the result alive; `count` is how many the library says
+    it holds."""
+    size = int(np.prod(shape, dtype=np.int64))
+    if count is not None and int(count) != size:
+        raise RuntimeError(f"{what}: the library holds {count} values, and shape {shape} needs {size}")
+    if size == 0:
+        return np.zeros(shape, dtype=np.float64)
+    if not pointer:
+        raise RuntimeError(f"{what}: the library returned no pointer for {size} values")
+    memory = (ctypes.c_double * size).from_address(ctypes.cast(pointer, ctypes.c_void_p).value)
+    memory._held = held  # the array's buffer keeps the result, which frees the handle last
+    return np.frombuffer(memory, dtype=np.float64).reshape(shape)
+
+
+class _ResultReader:
+    """One result handle of the library, read into this module's types: the large arrays
+    (picture, canvas, planes, coefficients, the problems, the records) view the result's
+    memory, which lives until the last of them goes."""
+
+    def __init__(self, library: ctypes.CDLL, handle: ctypes.c_void_p) -> None:
+        self.lib = library
+        self.handle = handle
+        self.held = _HeldResult(library, handle)
+
+    def _floats(self, pointer: object, shape: tuple[int, ...], what: str, count: int | None = None) -> Array:
+        return _viewed(pointer, shape, what, count, self.held)
 
     def _counted(self, function: Callable, *arguments: object) -> tuple[object, int]:
         count = ctypes.c_uint64()
@@ -602,15 +644,15 @@ class _ResultReader:
         pointer = self.lib.unround_result_picture(self.handle, *map(ctypes.byref, sizes))
         height, width, channels = (int(size.value) for size in sizes)
         shape = (height, width) if channels == 1 else (height, width, channels)
-        return _floats(pointer, shape, "the picture")
+        return self._floats(pointer, shape, "the picture")
 
     def canvas_and_planes(self, height: int, width: int) -> tuple[Array, Array]:
         sizes = [ctypes.c_uint64() for _ in range(3)]
         pointer = self.lib.unround_result_canvas(self.handle, *map(ctypes.byref, sizes))
         channels, rows, columns = (int(size.value) for size in sizes)
-        canvas = _floats(pointer, (channels, rows, columns), "the canvas")
+        canvas = self._floats(pointer, (channels, rows, columns), "the canvas")
         planes_pointer, planes_count = self._counted(self.lib.unround_result_planes)
-        planes = _floats(planes_pointer, (channels, height, width), "the planes", planes_count)
+        planes = self._floats(planes_pointer, (channels, height, width), "the planes", planes_count)
         return canvas, planes
 
     def component(self, index: int) -> tuple[Array, Problem]:
@@ -620,12 +662,12 @@ class _ResultReader:
         rows, columns, down, across = (int(size.value) for size in sizes)
         blocks = (rows, columns, 8, 8)
         pointer, count = self._counted(self.lib.unround_result_coefficients, index)
-        coefficients = _floats(pointer, blocks, f"coefficients of component {index}", count)
+        coefficients = self._floats(pointer, blocks, f"coefficients of component {index}", count)
         arrays = []
         for which, (what, shape) in enumerate((("lower ends", blocks), ("upper ends", blocks), ("centres", blocks),
                                                ("steps", (8, 8)), ("weights", (8, 8)))):
             pointer, count = self._counted(self.lib.unround_result_problem, index, which)
-            arrays.append(_floats(pointer, shape, f"{what} of component {index}", count))
+            arrays.append(self._floats(pointer, shape, f"{what} of component {index}", count))
         return coefficients, Problem((down, across), *arrays)
 
     def solution(self, alpha: bool = False) -> Solution | None:
@@ -639,7 +681,7 @@ class _ResultReader:
         columns = []
         for which, what in enumerate(("seconds", "primal values", "dual values", "scalings", "partial gaps")):
             values, values_count = self._counted(getattr(self.lib, prefix + "record_values"), which)
-            columns.append(_floats(values, (count,), f"the records' {what}", values_count))
+            columns.append(self._floats(values, (count,), f"the records' {what}", values_count))
         return Solution(int(iterations.value), Stop(int(stop.value)), History(recorded, *columns))
 
     def excess(self, shapes: list[tuple[int, ...]]) -> tuple[Array, ...] | None:
@@ -648,7 +690,7 @@ class _ResultReader:
             pointer, count = self._counted(self.lib.unround_result_excess, index)
             if not pointer:
                 return None
-            found.append(_floats(pointer, shape, f"excess of component {index}", count))
+            found.append(self._floats(pointer, shape, f"excess of component {index}", count))
         return tuple(found) if found else None
 
     def kernel_times(self) -> KernelTimes | None:
@@ -687,12 +729,14 @@ class _ResultReader:
 
 
 def read_result(handle: ctypes.c_void_p) -> Decoded:
-    """The Decoded of a result handle, which is then freed."""
-    library = _lib()
+    """The Decoded of a result handle, which is freed when the last of its arrays goes (at
+    once where reading it fails)."""
+    reader = _ResultReader(_lib(), handle)
     try:
-        return _ResultReader(library, handle).decoded()
-    finally:
-        library.unround_result_free(handle)
+        return reader.decoded()
+    except BaseException:
+        reader.held.free()
+        raise
 
 
 # ---------------------------------------------------------------------------------------
@@ -889,13 +933,13 @@ def solve_batch(images: Sequence[jpegio.Image], settings: Settings | list[str] |
     latencies = np.zeros(max(1, count), dtype=np.float64)
     call = _Call()
     call.check(library.unround_solve_batch(inputs, count, held.handle, handles, _doubles(latencies), *call.message))
-    results: list[Decoded] = []
+    readers = [_ResultReader(library, ctypes.c_void_p(handles[index])) for index in range(count)]
     try:
-        for index in range(count):
-            results.append(_ResultReader(library, ctypes.c_void_p(handles[index])).decoded())
-    finally:
-        for index in range(count):
-            library.unround_result_free(ctypes.c_void_p(handles[index]))
+        results = [reader.decoded() for reader in readers]
+    except BaseException:
+        for reader in readers:
+            reader.held.free()
+        raise
     return results, [float(seconds) for seconds in latencies[:count]]
 
 
