@@ -175,3 +175,19 @@ def test_jfif_conversions_invert_each_other():
     # The exact inverse of docs/math.md:1004-1005 at a grey pixel.
     grey = unround.to_rgb(np.array([[[100.0]], [[128.0]], [[128.0]]]))
     assert np.all(grey == 100.0)
+
+
+@pytest.mark.parametrize("height,width", [(1 << 33, 1 << 33), (1 << 62, 4), ((1 << 64) - 1, (1 << 64) - 1),
+                                          (1 << 31, 1 << 31)])
+def test_jfif_conversions_refuse_sizes_that_overflow(height, width):
+    # 3 x height x width doubles beyond what a size can hold (or a pointer difference can
+    # span) is refused before any sample is read, as the reference's conversions refuse it
+    # (rust/capi/src/lib.rs:1568-1582): the one-double buffers here are never touched.
+    lib = native._lib()
+    room = (ctypes.c_double * 1)()
+    out = (ctypes.c_double * 1)()
+    for name in ("unround_to_rgb", "unround_to_ycbcr"):
+        function = getattr(lib, name)
+        status = function(ctypes.cast(room, ctypes.POINTER(ctypes.c_double)), ctypes.c_uint64(height),
+                          ctypes.c_uint64(width), ctypes.cast(out, ctypes.POINTER(ctypes.c_double)))
+        assert status == native.Status.ARGUMENT, name
