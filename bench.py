@@ -6,7 +6,9 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c4|png|c3|c5
 A step is one iteration of the method over the whole canvas of the workload (all its
 channels). The default workload is the 2048 x 2048 RGB picture of seed 0, JFIF YCbCr
 4:2:0 at quality 75 (IJG tables), TGV with the default alpha1 = 1, alpha0 = 2, in
-binary32. W warm-up iterations run in a solve of their own; then one solve of exactly K
+binary32, K = 500. --config c4 is the 4096 x 4096 grey picture at quality 40 with BF16 p
+and r (binary32 x and w, binary64 records and final step), K = 1000, unless --precision
+or --steps say otherwise. W warm-up iterations run in a solve of their own; then one solve of exactly K
 iterations is timed, its records only at its end (--record-every 0 --tolerance 0),
 bracketed by torch.cuda.synchronize(). value counts the samples of every channel times K
 over the iterations' own time (one pair of HIP events on the solve's stream around the K
@@ -428,13 +430,20 @@ def unrolled_main(args) -> None:
 def main() -> None:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
-    parser.add_argument("--steps", type=int, default=500)
+    parser.add_argument("--steps", type=int, default=None,
+                        help="iterations timed (default: 1000 for c4, as its workload states; 500 otherwise)")
     parser.add_argument("--warmup", type=int, default=20)
     parser.add_argument("--config", default="c2")
-    parser.add_argument("--precision", default="single", choices=["single", "mixed", "double"])
+    parser.add_argument("--precision", default=None, choices=["single", "mixed", "double"],
+                        help="default: mixed for c4 (binary32 x and w, BF16 p and r, as its workload states); "
+                             "single otherwise")
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-batch", action="store_true", help="leave the C3 batch out of the default run")
     args = parser.parse_args()
+    if args.precision is None:
+        args.precision = "mixed" if args.config == "c4" else "single"
+    if args.steps is None:
+        args.steps = 1000 if args.config == "c4" else 500
     if args.gpus != 1:
         raise SystemExit("this benchmark runs on one GPU: the path shards by picture across GPUs with no "
                          "communication, and no figure on several GPUs is measured here (--gpus 1)")
