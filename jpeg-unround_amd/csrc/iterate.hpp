@@ -140,16 +140,32 @@ __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& stagi
   // lane's).
   // Cells two samples wide take each pair of samples of a row in one load of twice the width
   // (the pair is aligned: the first column and the canvas width are even).
-  T up[DH], here[8][DV][DH], across[8][DV][DH], down[8][DV][DH], edge[8][DV];
+  // p stays as stored until it is taken: widening a BF16 sample as it arrives made the
+  // compiler wait for each load before issuing the next.
+  T up[DH], here[8][DV][DH];
+  S across[8][DV][DH], down[8][DV][DH], edge[8][DV];
   const auto load_row = [&](const auto& planes, bool valid, int plane, uint32_t where, auto (&out)[DH]) {
+    using Out = std::remove_cv_t<std::remove_reference_t<decltype(out[0])>>;
     if constexpr (DH == 2) {
       decltype(planes.load(0, 0)) first, second;
       planes.load_pair_if(valid, plane, where, first, second);
-      out[0] = device::widen(first);
-      out[1] = device::widen(second);
+      if constexpr (std::is_same_v<Out, decltype(first)>) {
+        out[0] = first;
+        out[1] = second;
+      } else {
+        out[0] = device::widen(first);
+        out[1] = device::widen(second);
+      }
     } else {
 #pragma unroll
-      for (int bb = 0; bb < DH; ++bb) out[bb] = device::widen(planes.load_if(valid, plane, where + bb));
+      for (int bb = 0; bb < DH; ++bb) {
+        const auto loaded = planes.load_if(valid, plane, where + bb);
+        if constexpr (std::is_same_v<Out, std::remove_cv_t<decltype(loaded)>>) {
+          out[bb] = loaded;
+        } else {
+          out[bb] = device::widen(loaded);
+        }
+      }
     }
   };
   if (edges.up != nullptr) {
@@ -167,9 +183,10 @@ __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& stagi
       load_row(p, live, 0, at(i, first_column), across[m][a]);
       load_row(p, live, 1, at(i, first_column), down[m][a]);
       if (edges.left != nullptr) {
-        edge[m][a] = edges.left[(i - edges.left_top) * edges.left_stride];
+        // Made from a stored sample, so its storage type holds it exactly.
+        edge[m][a] = device::narrow<S>(edges.left[(i - edges.left_top) * edges.left_stride]);
       } else {
-        edge[m][a] = device::widen(p.load_if(live && lane == 0 && first_column > 0, 0, at(i, first_column - 1)));
+        edge[m][a] = p.load_if(live && lane == 0 && first_column > 0, 0, at(i, first_column - 1));
       }
     }
   }
@@ -188,16 +205,17 @@ __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& stagi
       const int i = top + m * DV + a;
       // p_x of the column on the left: the lane's own previous column, or the lane on the
       // left's last one, or beyond the strip, the edge sample.
-      const T shifted = from_left(across[m][a][DH - 1]);
-      const T left = lane == 0 ? edge[m][a] : shifted;
+      const T shifted = from_left(device::widen(across[m][a][DH - 1]));
+      const T left = lane == 0 ? device::widen(edge[m][a]) : shifted;
 #pragma unroll
       for (int bb = 0; bb < DH; ++bb) {
         const int j = first_column + bb;
-        const T before = bb == 0 ? left : across[m][a][bb - 1];
-        T div = backward_down_of(down[m][a][bb], up[bb], i, H);
-        div += backward_across_of(across[m][a][bb], before, j, W);
+        const T before = bb == 0 ? left : device::widen(across[m][a][bb - 1]);
+        const T below = device::widen(down[m][a][bb]);
+        T div = backward_down_of(below, up[bb], i, H);
+        div += backward_across_of(device::widen(across[m][a][bb]), before, j, W);
         const T v = device::fused(descent, div, here[m][a][bb]);
-        up[bb] = down[m][a][bb];
+        up[bb] = below;
         vs[m][a * DH + bb] = v;
         // The cell's sum: the rows of a column first, then the columns.
         partial[bb] = a == 0 ? v : partial[bb] + v;
