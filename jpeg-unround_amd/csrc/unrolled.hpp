@@ -380,23 +380,32 @@ __global__ __launch_bounds__(256) void unrolled_prox_back_kernel(UnrolledFrames 
   const int tx = strip * 8 + b;
   const bool live = ty < K.rows && tx < K.columns;
   const int j = 8 * tx + n, top = 8 * ty;
-  const auto at = [&](int i) { return static_cast<size_t>(i) * W + j; };
-  // grad x~ of the lane's column, to grad zeta (the inverse DCT's transpose is the DCT).
+  const auto at = [&](int i) { return static_cast<uint32_t>(i) * static_cast<uint32_t>(W) + static_cast<uint32_t>(j); };
+  // Every load first (buffer resources: a lane beyond the blocks reads 0, with no branch),
+  // then grad x~ of the lane's column, to grad zeta (the inverse DCT's transpose is the DCT).
+  using Planes = device::BufferPlanesOf<float, device::kStreaming, device::kStreaming>;
+  const Planes bars(views.work.xbar, 1, N), after(views.after.x, 1, N);
   float grad_bar[8], after_x[8], column[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
-    grad_bar[m] = live ? views.work.xbar[at(top + m)] : 0.0f;
-    after_x[m] = live ? views.after.x[at(top + m)] : 0.0f;
-    column[m] = 2.0f * grad_bar[m] + k.rho * after_x[m];
+    grad_bar[m] = bars.load_if(live, 0, at(top + m));
+    after_x[m] = after.load_if(live, 0, at(top + m));
   }
+#pragma unroll
+  for (int m = 0; m < 8; ++m) column[m] = 2.0f * grad_bar[m] + k.rho * after_x[m];
   // A lane's row n of block b: the marks and the slopes.
   const size_t block = live ? static_cast<size_t>(ty) * K.columns + tx : 0;
   const uint64_t marks = live ? *reinterpret_cast<const uint64_t*>(inside + f * N + 64 * block + 8 * n) : 0;
   const bool flat = live && K.flat[block] != 0;
+  // The row's 8 weights loaded together, then the slopes.
+  const double* weights = (flat ? K.flat_weights : K.weights) + 8 * n;
+  double weight[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) weight[c] = weights[c];
   float slope[8];
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
-    const float t = step * static_cast<float>((flat ? K.flat_weights : K.weights)[8 * n + c]);
+    const float t = step * static_cast<float>(weight[c]);
     slope[c] = ((marks >> (8 * c)) & 0xFFu) != 0 ? 1.0f / (1.0f + t) : 0.0f;
   }
   float y[8], row[8];
@@ -420,12 +429,12 @@ __global__ __launch_bounds__(256) void unrolled_prox_back_kernel(UnrolledFrames 
   for (int q = 0; q < 8; ++q) y[q] = room[wave][q][lane];
   float grad_v[8];
   device::inverse8(y, grad_v);
-  if (!live) return;
   const float rest = 1.0f - k.rho;
+  const Planes before(views.before.x, 1, N), tgv(views.work.v, 1, N);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
-    views.before.x[at(top + m)] = grad_v[m] - grad_bar[m] + rest * after_x[m];
-    views.work.v[at(top + m)] = k.descent * grad_v[m];
+    before.store_if(live, 0, at(top + m), grad_v[m] - grad_bar[m] + rest * after_x[m]);
+    tgv.store_if<device::kTemporal>(live, 0, at(top + m), k.descent * grad_v[m]);  // the next launch reads it
   }
 }
 
