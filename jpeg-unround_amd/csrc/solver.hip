@@ -806,6 +806,7 @@ class Solver {
     uint64_t iteration = 0;
     int32_t stop = 0;
     if (solving) {
+      choose_dual_rows();
       const uint64_t most = plan_.iterations, every = plan_.record_every;
       const double samples = static_cast<double>(C_) * H_ * W_;
       bool stopped = false;
@@ -1116,11 +1117,12 @@ class Solver {
     HIP_CHECK(hipStreamSynchronize(stream()));
   }
 
-  // Rows a wavefront of the dual kernel walks down: long enough to spread its first rows'
-  // extra loads thin, short enough for about 2048 wavefronts (two per SIMD), which measured
-  // best at 2048 x 2048; a power of two from 8 to 64. With BF16 duals a row moves fewer
-  // bytes for the same arithmetic, and runs of at most 16 rows (more wavefronts to hide
-  // it) measured best: 2048 x 2048 RGB 139 -> 117 us, 4096 x 4096 grey 181 -> 163 us.
+  // Rows a wavefront of the dual kernel walks down, before a single solve measures its own
+  // (choose_dual_rows), and where nothing is measured (the sweep, the unrolled solve's
+  // pictures): long enough to spread its first rows' extra loads thin, short enough for
+  // about 2048 wavefronts (two per SIMD); a power of two from 8 to 64. With BF16 duals a row
+  // moves fewer bytes for the same arithmetic, and runs of at most 16 rows (more wavefronts
+  // to hide it) measured best: 2048 x 2048 RGB 139 -> 117 us, 4096 x 4096 grey 181 -> 163 us.
   static int dual_rows_for(int H, int W) {
     const long strips = (W + kStripColumns - 1) / kStripColumns;
     const int most = sizeof(S) < sizeof(T) ? 16 : 64;
@@ -1221,16 +1223,8 @@ class Solver {
     return (b.down == 1 && b.across == 1) || (b.down == 2 && b.across == 2);
   }
 
-  // One iteration: its proximal step (unless the last launch made it), then its dual step,
-  // alone or fused with the proximal step of the next iteration (`fuse`, the next keeping
-  // its tilde point where keep_after).
-  void iterate(bool keep, bool fuse, bool keep_after) {
-    hipStream_t s = stream();
-    const T rho = static_cast<T>(plan_.relaxation), rest = static_cast<T>(1.0 - plan_.relaxation);
-    const bool per_kernel = timing_ == Timing::kernels;
-    if (!primal_done_) primal_step(keep);
-    fuse = fuse && fusable_;
-    if (per_kernel) dual_events_.emplace_back();
+  // The dual step's scalars.
+  Steps<T> dual_steps() const {
     Steps<T> steps{};
     for (int c = 0; c < C_; ++c) {
       steps.descent[c] = static_cast<T>(plan_.tau * plan_.gamma[c]);
@@ -1238,8 +1232,81 @@ class Solver {
     }
     steps.radius = static_cast<T>(tgv_ ? settings_.alpha1 : settings_.alpha);
     steps.radius0 = static_cast<T>(settings_.alpha0);
-    steps.rho = rho;
-    steps.rest = rest;
+    steps.rho = static_cast<T>(plan_.relaxation);
+    steps.rest = static_cast<T>(1.0 - plan_.relaxation);
+    return steps;
+  }
+
+  // The dual step of every channel, in the kernel of the frame's kind.
+  void dual_step(const Steps<T>& steps, int keep) {
+    if (C_ == 1) {
+      tgv_ ? launch_dual<1, true, false>(steps, keep) : launch_dual<1, false, false>(steps, keep);
+    } else if (coupled_) {
+      tgv_ ? launch_dual<3, true, true>(steps, keep) : launch_dual<3, false, true>(steps, keep);
+    } else {
+      tgv_ ? launch_dual<3, true, false>(steps, keep) : launch_dual<3, false, false>(steps, keep);
+    }
+    HIP_CHECK(hipGetLastError());
+  }
+
+  // The dual step's rows per wavefront for the frame's shape and kernel, measured once a
+  // process: 8, 16, 32 and 64 rows, each the mean of three launches after an untimed one
+  // (the dual step only reads the state and writes the other buffers, which the first
+  // iteration's dual step writes again, so trying it changes nothing). The measured best
+  // differs from shape to shape and by kernel (2048 x 2048 RGB: 16 or 32 rows within 2 %;
+  // 4096 x 4096 grey binary32: 64, with 32 the slowest; BF16 duals: 16). The environment's
+  // UNROUND_DUAL_ROWS and the fused sweep keep their own.
+  void choose_dual_rows() {
+    if (std::getenv("UNROUND_DUAL_ROWS") != nullptr || fusable_) return;
+    struct Chosen {
+      std::mutex lock;
+      std::map<std::vector<int>, int> rows;
+    };
+    static Chosen chosen;
+    const std::vector<int> key{C_, H_, W_, static_cast<int>(sizeof(T)), static_cast<int>(sizeof(S)), tgv_ ? 1 : 0,
+                               coupled_ ? 1 : 0, narrow_spans() ? 1 : 0};
+    {
+      std::lock_guard<std::mutex> hold(chosen.lock);
+      const auto found = chosen.rows.find(key);
+      if (found != chosen.rows.end()) {
+        dual_rows_ = found->second;
+        return;
+      }
+    }
+    hipStream_t s = stream();
+    const Steps<T> steps = dual_steps();
+    int best = dual_rows_;
+    float best_ms = 0.0f;
+    for (const int rows : {8, 16, 32, 64}) {
+      dual_rows_ = rows;
+      dual_step(steps, 0);
+      EventPair timing;
+      HIP_CHECK(hipEventRecord(timing.begin, s));
+      for (int k = 0; k < 3; ++k) dual_step(steps, 0);
+      HIP_CHECK(hipEventRecord(timing.end, s));
+      HIP_CHECK(hipEventSynchronize(timing.end));
+      float ms = 0.0f;
+      HIP_CHECK(hipEventElapsedTime(&ms, timing.begin, timing.end));
+      if (rows == 8 || ms < best_ms) {
+        best = rows;
+        best_ms = ms;
+      }
+    }
+    dual_rows_ = best;
+    std::lock_guard<std::mutex> hold(chosen.lock);
+    chosen.rows[key] = best;
+  }
+
+  // One iteration: its proximal step (unless the last launch made it), then its dual step,
+  // alone or fused with the proximal step of the next iteration (`fuse`, the next keeping
+  // its tilde point where keep_after).
+  void iterate(bool keep, bool fuse, bool keep_after) {
+    hipStream_t s = stream();
+    const bool per_kernel = timing_ == Timing::kernels;
+    if (!primal_done_) primal_step(keep);
+    fuse = fuse && fusable_;
+    if (per_kernel) dual_events_.emplace_back();
+    const Steps<T> steps = dual_steps();
     // p~ is read by the records of TV and by the partial gap's only.
     const int k = keep ? kKeepTilde | (!tgv_ || plan_.has_partial ? kKeepDualP : 0) : 0;
     if (per_kernel) HIP_CHECK(hipEventRecord(dual_events_.back().begin, s));
@@ -1254,12 +1321,8 @@ class Solver {
           tgv_ ? launch_sweep<3, true, false>(steps, k, kept) : launch_sweep<3, false, false>(steps, k, kept);
         }
       }
-    } else if (C_ == 1) {
-      tgv_ ? launch_dual<1, true, false>(steps, k) : launch_dual<1, false, false>(steps, k);
-    } else if (coupled_) {
-      tgv_ ? launch_dual<3, true, true>(steps, k) : launch_dual<3, false, true>(steps, k);
     } else {
-      tgv_ ? launch_dual<3, true, false>(steps, k) : launch_dual<3, false, false>(steps, k);
+      dual_step(steps, k);
     }
     HIP_CHECK(hipGetLastError());
     if (per_kernel) HIP_CHECK(hipEventRecord(dual_events_.back().end, s));
