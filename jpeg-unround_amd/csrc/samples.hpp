@@ -48,23 +48,28 @@ __global__ __launch_bounds__(256) void samples_prox_kernel(Fields<T, S> F, Sampl
   const S* __restrict__ p = F.p;
   T* __restrict__ x = F.x;
   const uint16_t* __restrict__ levels = term.levels;
-  T across[C], left[C], down[C], up[C], old[C];
+  // p as stored until it is taken (a BF16 sample widened as it arrives makes the compiler
+  // wait for its load before issuing the next).
+  S across[C], left[C], down[C], up[C];
+  T old[C];
   uint16_t level[C];
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     const S* px = p + (2 * c) * N;
     const S* py = p + (2 * c + 1) * N;
-    across[c] = device::widen(px[at]);
-    left[c] = j > 0 ? static_cast<T>(device::widen(px[at - 1])) : T(0);
-    down[c] = device::widen(py[at]);
-    up[c] = i > 0 ? static_cast<T>(device::widen(py[at - W])) : T(0);
+    across[c] = px[at];
+    left[c] = px[j > 0 ? at - 1 : at];
+    down[c] = py[at];
+    up[c] = py[i > 0 ? at - W : at];
     old[c] = x[c * N + at];
     level[c] = levels[c * N + at];
   }
 #pragma unroll
   for (int c = 0; c < C; ++c) {
-    T div = backward_down_of(down[c], up[c], i, H);
-    div += backward_across_of(across[c], left[c], j, W);
+    // backward_across_of and backward_down_of leave out the left and upper samples at the
+    // canvas' edges.
+    T div = backward_down_of(static_cast<T>(device::widen(down[c])), static_cast<T>(device::widen(up[c])), i, H);
+    div += backward_across_of(static_cast<T>(device::widen(across[c])), static_cast<T>(device::widen(left[c])), j, W);
     const T v = old[c] + steps.descent[c] * div;
     const T* entry = term.ends + 4 * static_cast<size_t>(level[c]);
     const T tilde = device::clamp_to(fma(t, entry[2], v) * r, entry[0], entry[1]);
