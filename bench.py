@@ -8,9 +8,11 @@ channels). The default workload is the 2048 x 2048 RGB picture of seed 0, JFIF Y
 4:2:0 at quality 75 (IJG tables), TGV with the default alpha1 = 1, alpha0 = 2, in
 binary32, K = 500. --config c4 is the 4096 x 4096 grey picture at quality 40 with BF16 p
 and r (binary32 x and w, binary64 records and final step), K = 1000, unless --precision
-or --steps say otherwise. W warm-up iterations run in a solve of their own; then one solve of exactly K
-iterations is timed, its records only at its end (--record-every 0 --tolerance 0),
-bracketed by torch.cuda.synchronize(). value counts the samples of every channel times K
+or --steps say otherwise. W warm-up iterations run in a solve of their own (first
+launches, the dual step's rows measured); then the timed solve runs W untimed iterations
+and exactly K timed ones straight after them (so that the K start at the GPU's running
+clocks), its records only at its end (--record-every 0 --tolerance 0), bracketed by
+torch.cuda.synchronize(). value counts the samples of every channel times K
 over the iterations' own time (one pair of HIP events on the solve's stream around the K
 iterations, records and setup excluded, as the reference's throughput does); ms_per_step
 is that time per iteration, and the wall time of the whole timed solve is printed to
@@ -18,8 +20,9 @@ stderr.
 
 roofline: of the dominant kernel, the dual step (dual_kernel, one launch per iteration
 over every channel), timed launch by launch with HIP events in a second solve of
-min(K, 100) iterations. Its algorithmic bytes per channel-sample are one read of x-, p, w
-and r and one write of p, w and r (60 B in binary32 TGV, 40 B with bfloat16 p and r);
+min(K, 100) iterations (after W untimed ones, as in the timed solve). Its algorithmic
+bytes per channel-sample are one read of x-, p, w and r and one write of p, w and r
+(60 B in binary32 TGV, 40 B with bfloat16 p and r);
 achieved = those bytes of one launch over its mean duration, against MI355X's 8 TB/s.
 traffic: the HBM bytes of one launch from rocprofv3 counters (FETCH_SIZE + WRITE_SIZE,
 scaled by a copy kernel's reading), where profiles/traffic.json holds them for this
@@ -472,11 +475,13 @@ def main() -> None:
     base = ["--method", method, "--tolerance", "0", "--precision", args.precision]
     if args.warmup > 0:
         solve(base + ["--iterations", str(args.warmup), "--record-every", "0"])
-    # The timed solve: one pair of events around its iterations, nothing between launches.
+    # The timed solve: W untimed iterations, then one pair of events around the K timed
+    # ones, nothing between launches.
     os.environ["UNROUND_TIMING"] = "1"
+    os.environ["UNROUND_TIMING_WARMUP"] = str(args.warmup)
     torch.cuda.synchronize()
     began = time.perf_counter()
-    decoded = solve(base + ["--iterations", str(args.steps), "--record-every", "0"])
+    decoded = solve(base + ["--iterations", str(args.warmup + args.steps), "--record-every", "0"])
     torch.cuda.synchronize()
     wall = time.perf_counter() - began
     times = decoded.kernel_times
@@ -484,8 +489,9 @@ def main() -> None:
     # The kernels' own durations: events around every launch, in a solve of their own.
     kernel_steps = min(args.steps, 100)
     os.environ["UNROUND_TIMING"] = "2"
-    launches = solve(base + ["--iterations", str(kernel_steps), "--record-every", "0"]).kernel_times
+    launches = solve(base + ["--iterations", str(args.warmup + kernel_steps), "--record-every", "0"]).kernel_times
     os.environ.pop("UNROUND_TIMING", None)
+    os.environ.pop("UNROUND_TIMING_WARMUP", None)
     assert launches is not None and launches.dual_launches == kernel_steps, "the library took no kernel timings"
 
     channels, rows, columns = decoded.canvas.shape
@@ -500,7 +506,7 @@ def main() -> None:
     primal_launch = launches.primal_seconds / max(launches.primal_launches, 1)
     dual_launch_bytes = dual_bytes(method, args.precision) * samples
     achieved = dual_launch_bytes / dual_launch / 1e9
-    log(f"wall {wall:.3f} s for the timed solve ({args.steps} iterations + setup + final projection); "
+    log(f"wall {wall:.3f} s for the timed solve ({args.warmup} + {args.steps} iterations + setup + final projection); "
         f"iterations {seconds:.4f} s; per launch (solve of {kernel_steps} iterations timed launch by launch): "
         f"primal kernels {primal_launch * 1e6:.1f} us x {launches.primal_launches}, dual kernel "
         f"{dual_launch * 1e6:.1f} us x {launches.dual_launches}; whole iterations {iteration_achieved:.1f} GB/s "
@@ -552,7 +558,7 @@ def main() -> None:
                                       "achieved": round(primal_achieved, 1),
                                       "frac": round(primal_achieved / HBM_PEAK_GBS, 4)}
     if args.config in ("c2", "c4") and args.precision != "double":
-        line["accuracy"] = accuracy_against_binary64(args.config, image, decoded, base, args.steps)
+        line["accuracy"] = accuracy_against_binary64(args.config, image, decoded, base, args.warmup + args.steps)
     if args.config == "c2" and not args.no_batch:
         line["batch"] = batch_measure(args.precision)
     if not args.no_cpu_baseline:

@@ -134,8 +134,9 @@ unround_status unround_solve_scratch(const unround_input* input, const unround_s
  * seconds of the iterations (records excluded) and [5] the iterations; at most count
  * values are written. UNROUND_TIMING=1 brackets each run of iterations between two
  * records with one pair of events (launches [1] and [3] are then 0); UNROUND_TIMING=2
- * brackets every iteration and every launch. Returns 1 where timings were taken, 0
- * otherwise. */
+ * brackets every iteration and every launch. UNROUND_TIMING_WARMUP=W leaves the first W
+ * iterations untimed (they run straight before the timed ones). Returns 1 where timings
+ * were taken, 0 otherwise. */
 int32_t unround_result_kernel_times(const unround_result* result, double* values, size_t count);
 
 #ifdef __cplusplus

@@ -208,7 +208,7 @@ unround_status frame_of(const unround_input* input, const unround_settings* sett
 }
 
 // The UNROUND_TIMING environment variable: 1 times the runs of iterations between records,
-// 2 also every launch.
+// 2 also every launch. UNROUND_TIMING_WARMUP=W leaves a solve's first W iterations untimed.
 unround::Timing timing_asked() {
   const char* timing = std::getenv("UNROUND_TIMING");
   if (timing == nullptr) return unround::Timing::none;

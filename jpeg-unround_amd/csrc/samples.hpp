@@ -210,7 +210,7 @@ class SamplesSolver : private SamplesShape, public Solver<T, S> {
 
   void primal_step(bool keep) override {
     hipStream_t s = this->stream();
-    const bool per_kernel = this->timing_ == Timing::kernels;
+    const bool per_kernel = this->timing_kernels();
     if (per_kernel) {
       this->primal_events_.emplace_back();
       HIP_CHECK(hipEventRecord(this->primal_events_.back().begin, s));

@@ -835,12 +835,15 @@ class Solver {
       };
       if (every > 0 || most == 0) take(0);
       bool segment_open = false;
+      const uint64_t untimed = timing_ == Timing::none ? 0 : untimed_iterations();
       while (!stopped && iteration < most) {
         const uint64_t next = iteration + 1;
         const bool keep = (every > 0 && next % every == 0) || next == most;
+        // The first `untimed` iterations run untimed, straight before the timed ones.
+        timed_ = timing_ != Timing::none && iteration >= untimed;
         // The events are read once the iterations between two records are done, so that
         // timing adds no wait between launches.
-        if (timing_ == Timing::kernels || (timing_ == Timing::iterations && !segment_open)) {
+        if (timed_ && (timing_ == Timing::kernels || !segment_open)) {
           launches_.emplace_back();
           HIP_CHECK(hipEventRecord(launches_.back().begin, s));
           segment_open = true;
@@ -850,11 +853,14 @@ class Solver {
         const uint64_t after = next + 1;
         const bool keep_after = (every > 0 && after % every == 0) || after == most;
         iterate(keep, next < most && !keep, keep_after);
-        if (timing_ != Timing::none) launches_.back().iterations += 1;
-        if (timing_ == Timing::kernels) HIP_CHECK(hipEventRecord(launches_.back().end, s));
+        if (timed_) launches_.back().iterations += 1;
+        if (timed_ && timing_ == Timing::kernels) {
+          HIP_CHECK(hipEventRecord(launches_.back().end, s));
+          segment_open = false;
+        }
         iteration = next;
         if ((every > 0 && iteration % every == 0) || iteration == most) {
-          if (timing_ == Timing::iterations) HIP_CHECK(hipEventRecord(launches_.back().end, s));
+          if (timing_ == Timing::iterations && segment_open) HIP_CHECK(hipEventRecord(launches_.back().end, s));
           segment_open = false;
           if (timing_ != Timing::none) collect_kernel_times(out.times);
           take(iteration);
@@ -1165,7 +1171,7 @@ class Solver {
   virtual void primal_step(bool keep) {
     hipStream_t s = stream();
     const T rho = static_cast<T>(plan_.relaxation), rest = static_cast<T>(1.0 - plan_.relaxation);
-    const bool per_kernel = timing_ == Timing::kernels;
+    const bool per_kernel = timing_kernels();
     for (int c = 0; c < C_;) {
       int count = 1;
       while (shares_launch(c + count - 1)) ++count;
@@ -1302,7 +1308,7 @@ class Solver {
   // its tilde point where keep_after).
   void iterate(bool keep, bool fuse, bool keep_after) {
     hipStream_t s = stream();
-    const bool per_kernel = timing_ == Timing::kernels;
+    const bool per_kernel = timing_kernels();
     if (!primal_done_) primal_step(keep);
     fuse = fuse && fusable_;
     if (per_kernel) dual_events_.emplace_back();
@@ -1555,6 +1561,15 @@ class Solver {
   const Settings& settings_;
   const Plan& plan_;
   Timing timing_;
+  bool timed_ = false;  // whether the launches of the iteration under way are timed
+  bool timing_kernels() const { return timing_ == Timing::kernels && timed_; }
+  // Where timing is asked for: how many iterations run untimed first, directly before
+  // the timed ones, so that those start at the GPU's running clocks
+  // (UNROUND_TIMING_WARMUP; none by default).
+  static uint64_t untimed_iterations() {
+    const char* count = std::getenv("UNROUND_TIMING_WARMUP");
+    return count == nullptr ? 0 : static_cast<uint64_t>(std::max(0LL, std::atoll(count)));
+  }
   int C_ = 0, H_ = 0, W_ = 0;
   size_t N_ = 0;
   bool tgv_ = false, coupled_ = false;

@@ -538,7 +538,8 @@ class Solution:
 @dataclass(frozen=True, slots=True)
 class KernelTimes:
     """HIP-event durations of a solve's kernels. UNROUND_TIMING=1 during the solve times the
-    runs of iterations between records; UNROUND_TIMING=2 also every launch."""
+    runs of iterations between records; UNROUND_TIMING=2 also every launch;
+    UNROUND_TIMING_WARMUP=W leaves the first W iterations untimed."""
 
     primal_seconds: float
     primal_launches: int

@@ -89,3 +89,23 @@ def test_runs_repeat_to_the_last_bit():
     assert np.array_equal(first.canvas, second.canvas)
     assert np.array_equal(first.result.history.primal, second.result.history.primal)
     assert np.array_equal(first.result.history.dual, second.result.history.dual)
+
+
+@pytest.mark.parametrize("mode", ["1", "2"])
+def test_timing_leaves_the_warm_up_untimed_and_the_result_unchanged(monkeypatch, mode):
+    picture = synthetic.colour_picture(64, 80, 13)
+    image = synthetic.encode_colour420(picture, 50)
+    options = ["--method", "tgv", "--iterations", "12", "--tolerance", "0", "--record-every", "0"]
+    plain = native.solve(image, options)
+    assert plain.kernel_times is None
+    monkeypatch.setenv("UNROUND_TIMING", mode)
+    monkeypatch.setenv("UNROUND_TIMING_WARMUP", "5")
+    timed = native.solve(image, options)
+    times = timed.kernel_times
+    assert times.iterations == 7 and times.iterations_seconds > 0
+    if mode == "2":
+        # Y, then Cb and Cr in one launch: two proximal launches an iteration.
+        assert times.dual_launches == 7 and times.primal_launches == 14
+    else:
+        assert times.dual_launches == 0 and times.primal_launches == 0
+    assert np.array_equal(timed.canvas, plain.canvas)
