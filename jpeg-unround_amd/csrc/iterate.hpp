@@ -638,10 +638,19 @@ struct DualWalk {
   Planes<T> w, w_next, tw;
   Planes<S> r, r_next, tr;
   Gather edges;
-  // Carried from row to row.
-  T bar[C], bar_next[C], r_here[C][3], r_below[C][3], p_here[C][2], w_here[C][2], w_up[C][2];
-  T edge_t = T(0);
-  S edge_s{};
+  // The rows' loaded values, in three slots that the rows take in turn: row i's values lie
+  // in slot (i - first) % 3 (x- and r from two rows before, p, w and the edge samples from
+  // the row before), so that each value stays in one register from its load to its last
+  // use. A value moved between registers while its load is in flight makes the move wait
+  // for it, and with it for every load and store of the row. S samples stay as stored
+  // until they are taken.
+  T bar_s[3][C];
+  S r_s[3][C][3], p_s[3][C][2];
+  T w_s[3][C][2];
+  T edge_t_s[3] = {T(0), T(0), T(0)};
+  S edge_s_s[3] = {};
+  // w- of the row above, carried from row to row.
+  T w_up[C][2];
 
   __device__ __forceinline__ DualWalk(EdgeRoom<T, C>& room_, SweepRoom<T, C>* sweep_, const Fields<T, S>& F_, int H_,
                                       int W_, int first_, int end_, int group, const Steps<T>& steps_, int keep_,
@@ -716,26 +725,31 @@ struct DualWalk {
     }
     T up_t = T(0);
     S up_s{};
-    edges.load(first, H, W, edge_t, edge_s);
+    // x- and r of rows first and first + 1 into slots 0 and 1, p, w and the edge samples of
+    // row first into slot 0.
+    edges.load(first, H, W, edge_t_s[0], edge_s_s[0]);
     if ((TGV || SWEEP) && first > 0) edges.load(first - 1, H, W, up_t, up_s);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      bar[c] = xbar.load(c, index(first));
-      bar_next[c] = xbar.load(c, index(first + 1));
+      bar_s[0][c] = xbar.load(c, index(first));
+      bar_s[1][c] = xbar.load(c, index(first + 1));
 #pragma unroll
-      for (int e = 0; e < 2; ++e) p_here[c][e] = device::widen(p.load(2 * c + e, index(first)));
+      for (int e = 0; e < 2; ++e) p_s[0][c][e] = p.load(2 * c + e, index(first));
       if constexpr (TGV) {
 #pragma unroll
         for (int e = 0; e < 3; ++e) {
-          r_here[c][e] = device::widen(r.load(3 * c + e, index(first)));
-          r_below[c][e] = device::widen(r.load(3 * c + e, index(first + 1)));
+          r_s[0][c][e] = r.load(3 * c + e, index(first));
+          r_s[1][c][e] = r.load(3 * c + e, index(first + 1));
         }
 #pragma unroll
-        for (int e = 0; e < 2; ++e) w_here[c][e] = w.load(2 * c + e, index(first));
+        for (int e = 0; e < 2; ++e) w_s[0][c][e] = w.load(2 * c + e, index(first));
         w_up[c][0] = w_up[c][1] = T(0);
       }
     }
-    if (!((TGV || SWEEP) && first > 0)) return;
+    if (!((TGV || SWEEP) && first > 0)) {
+      settle();
+      return;
+    }
     // The row above: w- of it (TGV), whose right column's r_xx, r_xy come from the wavefront
     // on the right (its lane 0) through the room's slot 1 (the run's first row takes slot
     // 0); for the sweep also x- of it and of its right column, and p there.
@@ -764,12 +778,14 @@ struct DualWalk {
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       if constexpr (TGV) {
-        T tilde_up[2];
+        T tilde_up[2], r_first[3];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) r_first[e] = device::widen(r_s[0][c][e]);
         const T edge_xx = right_gathered ? lane_value(up_wide, 9 * c + 7) : room.right[1][wave + 1][c][1];
         const T edge_xy = right_gathered ? lane_value(up_wide, 9 * c + 8) : room.right[1][wave + 1][c][2];
         const T shifted_xx = from_right(r_up[c][0]), shifted_xy = from_right(r_up[c][2]);
         const T right_xx = lane == 63 ? edge_xx : shifted_xx, right_xy = lane == 63 ? edge_xy : shifted_xy;
-        w_step(old_up[c], p_up[c], r_up[c], r_here[c], right_xx, right_xy, right, true, steps.descent[c], tilde_up,
+        w_step(old_up[c], p_up[c], r_up[c], r_first, right_xx, right_xy, right, true, steps.descent[c], tilde_up,
                w_up[c]);
       }
     }
@@ -783,7 +799,7 @@ struct DualWalk {
         const T shifted_bar = from_right(bar_up[c]);
         const T bar_right = lane == 63 ? edge_bar : shifted_bar;
         const T g1 = right ? bar_right - bar_up[c] : T(0);
-        const T g2 = bar[c] - bar_up[c];  // row first lies in the canvas
+        const T g2 = bar_s[0][c] - bar_up[c];  // row first lies in the canvas
         const T wx = TGV ? w_up[c][0] : T(0), wy = TGV ? w_up[c][1] : T(0);
         step_p[c][0] = p_ascent(p_up[c][0], steps.ascent[c], g1, wx);
         step_p[c][1] = p_ascent(p_up[c][1], steps.ascent[c], g2, wy);
@@ -797,35 +813,85 @@ struct DualWalk {
         sweep->up[c][wave * kStripColumns + lane] = device::widen(stored);
       }
     }
+    settle();
   }
 
-  // Row i, rows in order from `first`; `band_row` is its row within the sweep's band.
-  __device__ __forceinline__ void step(int i, int band_row = 0) {
+  // The start's loads done before the first row: the compiler's waits in the walk's loop
+  // hold for every way into it, and with the start's loads still in flight at its entry
+  // they would make each row wait for the row before's stores too.
+  __device__ __forceinline__ static void settle() { __builtin_amdgcn_s_waitcnt(0); }
+
+  // Rows first .. end - 1 in order, three a turn, so that each row's slot is known at
+  // compile time; the last one or two after the loop, so that every way round the loop
+  // leaves the slots alike.
+  __device__ __forceinline__ void walk() {
     if (outside) {
-      __syncthreads();
+      // The whole wavefront beyond the canvas keeps the work-group's barriers only, one a
+      // row. Apart from the rows' loop, so that no way round that loop skips a row's work.
+#pragma unroll 1
+      for (int i = first; i < end; ++i) __syncthreads();
       return;
     }
+    // The first row before the loop, so that the loop is entered as it goes round: with a
+    // row's stores behind its loads.
+    step<0>(first);
+    int i = first + 1;
+#pragma unroll 1
+    for (; i + 2 < end; i += 3) {
+      step<1>(i);
+      step<2>(i + 1);
+      step<0>(i + 2);
+    }
+    if (i < end) step<1>(i);
+    if (i + 1 < end) step<2>(i + 1);
+  }
+
+  // Row i, whose slot is PH = (i - first) % 3; for the sweep, row i - first of its band.
+  template <int PH>
+  __device__ __forceinline__ void step(int i) {
+    constexpr int kNextSlot = (PH + 1) % 3, kAfterSlot = (PH + 2) % 3;
+    const int band_row = i - first;
     const bool below = i + 1 < H;
-    // The loads of the next row: x- and r two rows down, p and w one row down, and the edge
-    // samples one row down; none in the run's last row, whose next rows belong to the run
-    // below (an unwanted buffer load reads nothing from memory), but for the sweep's edge
-    // samples, whose x- of the left column one row down the row itself takes.
+    // The loads of the next rows: x- and r two rows down into slot PH + 2, p, w and the
+    // edge samples one row down into slot PH + 1; none in the run's last row, whose next
+    // rows belong to the run below (an unwanted buffer load reads nothing from memory), but
+    // for the sweep's edge samples, whose x- of the left column one row down the row itself
+    // takes.
     const bool ahead = i + 1 < end;
-    T bar_ahead[C], r_ahead[C][3], p_ahead[C][2], w_ahead[C][2], edge_t_ahead = T(0);
-    S edge_s_ahead{};
-    if (ahead || SWEEP) edges.load(i + 1, H, W, edge_t_ahead, edge_s_ahead);
+    if (ahead || SWEEP) edges.load(i + 1, H, W, edge_t_s[kNextSlot], edge_s_s[kNextSlot]);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      bar_ahead[c] = xbar.load_if(ahead, c, index(i + 2));
+      bar_s[kAfterSlot][c] = xbar.load_if(ahead, c, index(i + 2));
 #pragma unroll
-      for (int e = 0; e < 2; ++e) p_ahead[c][e] = device::widen(p.load_if(ahead, 2 * c + e, index(i + 1)));
+      for (int e = 0; e < 2; ++e) p_s[kNextSlot][c][e] = p.load_if(ahead, 2 * c + e, index(i + 1));
       if constexpr (TGV) {
 #pragma unroll
-        for (int e = 0; e < 3; ++e) r_ahead[c][e] = device::widen(r.load_if(ahead, 3 * c + e, index(i + 2)));
+        for (int e = 0; e < 3; ++e) r_s[kAfterSlot][c][e] = r.load_if(ahead, 3 * c + e, index(i + 2));
 #pragma unroll
-        for (int e = 0; e < 2; ++e) w_ahead[c][e] = w.load_if(ahead, 2 * c + e, index(i + 1));
+        for (int e = 0; e < 2; ++e) w_s[kNextSlot][c][e] = w.load_if(ahead, 2 * c + e, index(i + 1));
       }
     }
+    // Row i's values and row i + 1's x- and r, loaded by the rows before (the waits fall
+    // here).
+    T bar[C], bar_next[C], r_here[C][3], r_below[C][3], p_here[C][2], w_here[C][2];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      bar[c] = bar_s[PH][c];
+      bar_next[c] = bar_s[kNextSlot][c];
+#pragma unroll
+      for (int e = 0; e < 2; ++e) p_here[c][e] = device::widen(p_s[PH][c][e]);
+      if constexpr (TGV) {
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          r_here[c][e] = device::widen(r_s[PH][c][e]);
+          r_below[c][e] = device::widen(r_s[kNextSlot][c][e]);
+        }
+#pragma unroll
+        for (int e = 0; e < 2; ++e) w_here[c][e] = w_s[PH][c][e];
+      }
+    }
+    const T edge_t = edge_t_s[PH];
+    const S edge_s = edge_s_s[PH];
 
     // This row's edge samples for the wavefronts beside, then theirs for this one.
     const int slot = (i - first) & 1;
@@ -919,37 +985,32 @@ struct DualWalk {
         const T projected = step_p[c][e] * scale;
         const S stored = device::narrow<S>(relaxed_of(rho, projected, rest, p_here[c][e]));
         if (e == 0) stored_x = stored;
-        if (writes) {
-          if (keep & kKeepDualP) tp.template store<device::kStreaming>(2 * c + e, at, device::narrow<S>(projected));
-          p_next.template store<kNext>(2 * c + e, at, stored);
+        if (keep & kKeepDualP) {
+          tp.template store_if<device::kStreaming>(writes, 2 * c + e, at, device::narrow<S>(projected));
         }
+        p_next.template store_if<kNext>(writes, 2 * c + e, at, stored);
       }
       if constexpr (SWEEP) {
         // p_x of the strip's last column, the left column of the strip on the right.
         if (lane == 63) sweep->left[wave + 1][band_row][c] = device::widen(stored_x);
       }
       if constexpr (TGV) {
-        if (writes) {
 #pragma unroll
-          for (int e = 0; e < 2; ++e) {
-            if (keep) tw.template store<device::kStreaming>(2 * c + e, at, tilde_w[c][e]);
-            w_next.template store<kNext>(2 * c + e, at, relaxed_of(rho, tilde_w[c][e], rest, w_here[c][e]));
-          }
+        for (int e = 0; e < 2; ++e) {
+          if (keep) tw.template store_if<device::kStreaming>(writes, 2 * c + e, at, tilde_w[c][e]);
+          w_next.template store_if<kNext>(writes, 2 * c + e, at, relaxed_of(rho, tilde_w[c][e], rest, w_here[c][e]));
         }
         T tensor_scale = shared_tensor_scale;
         if constexpr (!COUPLED) {
           tensor_scale =
               device::ball_scale(tensor_square_of(step_r[c][0], step_r[c][1], step_r[c][2]), steps.radius0);
         }
-        if (writes) {
 #pragma unroll
-          for (int e = 0; e < 3; ++e) {
-            const T projected = step_r[c][e] * tensor_scale;
-            if (keep) tr.template store<device::kStreaming>(3 * c + e, at, device::narrow<S>(projected));
-            r_next.template store<device::kStreaming>(3 * c + e,
-                                                      at, device::narrow<S>(relaxed_of(rho, projected, rest,
-                                                                                       r_here[c][e])));
-          }
+        for (int e = 0; e < 3; ++e) {
+          const T projected = step_r[c][e] * tensor_scale;
+          if (keep) tr.template store_if<device::kStreaming>(writes, 3 * c + e, at, device::narrow<S>(projected));
+          r_next.template store_if<device::kStreaming>(
+              writes, 3 * c + e, at, device::narrow<S>(relaxed_of(rho, projected, rest, r_here[c][e])));
         }
       }
     }
@@ -960,7 +1021,7 @@ struct DualWalk {
         T left_step[C][2];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-          const T x_left = lane_value(edge_t, kT * c + 3), x_left_below = lane_value(edge_t_ahead, kT * c + 3);
+          const T x_left = lane_value(edge_t, kT * c + 3), x_left_below = lane_value(edge_t_s[kNextSlot], kT * c + 3);
           const T x_right = lane_value(bar[c], 0);
           const T g1 = x_right - x_left;  // the column left of a strip has one on its right
           const T g2 = below ? x_left_below - x_left : T(0);
@@ -978,28 +1039,14 @@ struct DualWalk {
         }
       }
     }
-    // The next row's values become the current ones.
+    // This row's w- is the next one's row above.
+    if constexpr (TGV) {
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
-      bar[c] = bar_next[c];
-      bar_next[c] = bar_ahead[c];
+      for (int c = 0; c < C; ++c) {
 #pragma unroll
-      for (int e = 0; e < 2; ++e) p_here[c][e] = p_ahead[c][e];
-      if constexpr (TGV) {
-#pragma unroll
-        for (int e = 0; e < 3; ++e) {
-          r_here[c][e] = r_below[c][e];
-          r_below[c][e] = r_ahead[c][e];
-        }
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-          w_up[c][e] = bar_w[c][e];
-          w_here[c][e] = w_ahead[c][e];
-        }
+        for (int e = 0; e < 2; ++e) w_up[c][e] = bar_w[c][e];
       }
     }
-    edge_t = edge_t_ahead;
-    edge_s = edge_s_ahead;
   }
 };
 
@@ -1011,8 +1058,7 @@ __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first
   DualWalk<T, S, C, TGV, COUPLED, Planes, MAP> walk(room, nullptr, F, H, W, first, end, group, steps, keep,
                                                     radius_map);
   walk.begin();
-#pragma unroll 1
-  for (int i = first; i < end; ++i) walk.step(i);
+  walk.walk();
 }
 
 // dual_tile is work-group g's part: 4 strips across and `rows` rows down; dual_kernel runs

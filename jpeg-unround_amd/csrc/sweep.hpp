@@ -99,8 +99,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSweepWaves
       DualWalk<T, S, C, TGV, COUPLED, Planes, false, true> walk(room, &sweep, F, H, W, top, bottom, group, steps,
                                                                 keep_dual, nullptr);
       walk.begin_stretch(top == first);
-#pragma unroll 1
-      for (int i = top; i < bottom; ++i) walk.step(i, i - top);
+      walk.walk();
     }
     // The band's p complete in the caches, and its edges in LDS, before any wavefront's
     // prox step reads them.
