@@ -165,6 +165,10 @@ constexpr int kBackColumns = 60;
 // starts two rows above its first so that the rows it carries are whole there.
 __global__ __launch_bounds__(64) void unrolled_dual_back_kernel(UnrolledFrames u, const float* __restrict__ radius,
                                                                 int H, int W, int rows, int strips) {
+  // No multiply-add is fused: the walk's row step is compiled once per slot and again for
+  // the first and last rows, and every copy must round alike, so that a row's gradients do
+  // not depend on which copy makes it (which the run length decides).
+#pragma clang fp contract(off)
   const int f = static_cast<int>(blockIdx.y);
   const int strip = static_cast<int>(blockIdx.x) % strips, run = static_cast<int>(blockIdx.x) / strips;
   const int lane = static_cast<int>(threadIdx.x);
@@ -187,7 +191,7 @@ __global__ __launch_bounds__(64) void unrolled_dual_back_kernel(UnrolledFrames u
   const Planes after_w(v.after.w, 2, N), after_p(v.after.p, 2, N), after_r(v.after.r, 3, N);
   const Planes map(const_cast<float*>(radius), 1, N);
   const Planes before_w(v.before.w, 2, N), before_p(v.before.p, 2, N), before_r(v.before.r, 3, N);
-  const Planes grad_bar(v.work.xbar, 1, N), radius_sensitivity(v.work.radius, 1, N);
+  const Planes grad_bar(v.work.xbar, 1, N);
   const auto index = [&](int i) { return static_cast<uint32_t>(i) * static_cast<uint32_t>(W) + static_cast<uint32_t>(j); };
   const auto inside = [&](int i) { return column && i >= 0 && i < H; };
   // w- of row i from w, p and r of the row and r of the row below.
@@ -209,25 +213,32 @@ __global__ __launch_bounds__(64) void unrolled_dual_back_kernel(UnrolledFrames u
   const int top = first - 2;
   const bool in1 = inside(top + 1), in0 = inside(top);
   const uint32_t at1 = index(top + 1), at0 = index(top);
-  float r1[3] = {r.load_if(in1, 0, at1), r.load_if(in1, 1, at1), r.load_if(in1, 2, at1)};
-  float xb1 = xbar.load_if(in1, 0, at1), v1 = tgv.load_if(in1, 0, at1);
+  // The loads of the step at row i (r, x- and tau g grad v of row i + 2; w, p, grad p, grad
+  // r and the radius of row i + 1; grad w of row i), issued a step ahead of their use into
+  // slot (i + 1 - top) % 3 of three that the rows take in turn; the step at row i reads
+  // r, x- and tau g grad v of row i + 1 from the slot before (the row before's). No loaded
+  // value is moved between registers while its load is in flight (the move would wait for
+  // it, and with it for every access of the row). Beyond the run (`live` false) nothing is
+  // read, with no branch.
+  struct Loads {
+    float r[3], xb, v, w[2], p[2], ap[2], ar[3], R, aw[2];
+  };
+  Loads slots[3];
+  for (int e = 0; e < 3; ++e) slots[2].r[e] = r.load_if(in1, e, at1);
+  slots[2].xb = xbar.load_if(in1, 0, at1);
+  slots[2].v = tgv.load_if(in1, 0, at1);
   float wb0[2];
   {
     const float r0[3] = {r.load_if(in0, 0, at0), r.load_if(in0, 1, at0), r.load_if(in0, 2, at0)};
     w_bar(top, w.load_if(in0, 0, at0), w.load_if(in0, 1, at0), p.load_if(in0, 0, at0), p.load_if(in0, 1, at0), r0,
-          r1, wb0);
+          slots[2].r, wb0);
   }
   float gp0[2] = {0.0f, 0.0f}, gr0[3] = {0.0f, 0.0f, 0.0f}, ap0[2] = {0.0f, 0.0f}, ar0[3] = {0.0f, 0.0f, 0.0f};
   float gp_up = 0.0f, d_up[2] = {0.0f, 0.0f};
   float alpha0_sum = 0.0f;
-  // The loads of the step at row i (r, x- and tau g grad v of row i + 2; w, p, grad p, grad
-  // r and the radius of row i + 1; grad w of row i), issued a step ahead of their use.
-  struct Loads {
-    float r[3], xb, v, w[2], p[2], ap[2], ar[3], R, aw[2];
-  };
-  const auto fetch = [&](int i) {
+  const auto fetch = [&](int i, bool live) {
     Loads l;
-    const bool in2 = inside(i + 2), in1 = inside(i + 1), in0 = inside(i);
+    const bool in2 = live && inside(i + 2), in1 = live && inside(i + 1), in0 = live && inside(i);
     const uint32_t at2 = index(i + 2), at1 = index(i + 1), at0 = index(i);
     for (int e = 0; e < 3; ++e) l.r[e] = r.load_if(in2, e, at2);
     l.xb = xbar.load_if(in2, 0, at2);
@@ -242,12 +253,16 @@ __global__ __launch_bounds__(64) void unrolled_dual_back_kernel(UnrolledFrames u
     l.R = map.load_if(in1, 0, at1);
     return l;
   };
-  Loads next = fetch(top);
-#pragma unroll 1
-  for (int i = top; i < end; ++i) {
+  slots[0] = fetch(top, true);
+  // Row i, whose loads lie in slot PH = (i - top) % 3; it fetches row i + 1's into the next.
+  const auto step = [&](auto phase, int i) {
+    constexpr int PH = decltype(phase)::value;
     const int n1 = i + 1;
-    const Loads l = next;
-    if (i + 1 < end) next = fetch(i + 1);
+    slots[(PH + 1) % 3] = fetch(i + 1, i + 1 < end);
+    const Loads& l = slots[PH];
+    const Loads& before = slots[(PH + 2) % 3];
+    const float(&r1)[3] = before.r;
+    const float xb1 = before.xb, v1 = before.v;
     const float(&r2)[3] = l.r;
     const float xb2 = l.xb, v2 = l.v;
     const float w1x = l.w[0], w1y = l.w[1], p1x = l.p[0], p1y = l.p[1];
@@ -284,7 +299,12 @@ __global__ __launch_bounds__(64) void unrolled_dual_back_kernel(UnrolledFrames u
         const float dot = ph[0] * gp[0] + ph[1] * gp[1];
         const float along = dot * inverse, scale = R * inverse;
         for (int e = 0; e < 2; ++e) gp1[e] = scale * (gp[e] - ph[e] * (along * inverse));
-        if (counted) radius_sensitivity.store(0, index(n1), radius_sensitivity.load(0, index(n1)) + along);
+        // Added where it lies (an atomic add with no return: nothing waits for it). One
+        // lane adds to a sample in a launch, and the launches run in turn, so the sums
+        // come in a fixed order.
+        if (counted) {
+          __hip_atomic_fetch_add(v.work.radius + index(n1), along, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       } else {
         gp1[0] = gp[0];
         gp1[1] = gp[1];
@@ -346,11 +366,23 @@ __global__ __launch_bounds__(64) void unrolled_dual_back_kernel(UnrolledFrames u
     for (int e = 0; e < 3; ++e) {
       gr0[e] = gr1[e];
       ar0[e] = ar1[e];
-      r1[e] = r2[e];
     }
-    xb1 = xb2;
-    v1 = v2;
+  };
+  // The first row before the loop, so that the loop is entered as it goes round (a fetch
+  // in flight), and the rows three a turn.
+  using Zero = std::integral_constant<int, 0>;
+  using One = std::integral_constant<int, 1>;
+  using Two = std::integral_constant<int, 2>;
+  step(Zero{}, top);
+  int i = top + 1;
+#pragma unroll 1
+  for (; i + 2 < end; i += 3) {
+    step(One{}, i);
+    step(Two{}, i + 1);
+    step(Zero{}, i + 2);
   }
+  if (i < end) step(One{}, i);
+  if (i + 1 < end) step(Two{}, i + 1);
   // alpha0's sensitivity of the lane's column over the run's rows, into the run's slot.
   if (own) v.work.alpha0[static_cast<size_t>(run) * W + j] += alpha0_sum;
 }
