@@ -109,18 +109,30 @@ struct Staged {
 };
 
 // The work-group's 256 threads copy the constants of K into `into` (two loads each at
-// most, with no branch); a barrier must follow before any thread reads them.
-__device__ __forceinline__ Staged stage(const Component& K, StagedArrays& into) {
+// most, with no branch); a barrier must follow before any thread reads them. In two
+// halves, so that the loads can be issued before others and the copy made after them:
+// stage_load() loads a thread's two constants, stage_store() puts them in LDS.
+struct StageLoaded {
+  double a, b;
+};
+__device__ __forceinline__ StageLoaded stage_load(const Component& K) {
   static_assert(5 * 64 <= 2 * 256);
   const double* from = K.steps;
+  const int e = threadIdx.x;
+  const int other = e + 256 < 5 * 64 ? e + 256 : e;
+  return StageLoaded{from[e], from[other]};
+}
+__device__ __forceinline__ Staged stage_store(const Component& K, StagedArrays& into, const StageLoaded& loaded) {
   double* to = into.steps;
   const int e = threadIdx.x;
   const bool second = e + 256 < 5 * 64;
   const int other = second ? e + 256 : e;
-  const double a = from[e], b = from[other];
-  to[e] = a;
-  to[other] = second ? b : a;
+  to[e] = loaded.a;
+  to[other] = second ? loaded.b : loaded.a;
   return Staged{into.steps, into.weights, into.flat_weights, into.shrink, into.costs, K.slack, K.mmse, K.costed};
+}
+__device__ __forceinline__ Staged stage(const Component& K, StagedArrays& into) {
+  return stage_store(K, into, stage_load(K));
 }
 
 // A pointer into device memory seen as one into the global address space, so that loads

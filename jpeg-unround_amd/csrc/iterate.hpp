@@ -10,26 +10,38 @@
 // The value of the lane on the left (lane 0: its own) and on the right (lane 63: its own):
 // DPP's whole-wavefront shifts by one lane (wave_shr:1, wave_shl:1), a VALU move of each
 // 32-bit word with no trip through LDS; a lane with no source keeps its own word.
+// lane_shifted_or gives a lane with no source `old` instead.
 template <int CONTROL, typename T>
-__device__ __forceinline__ T lane_shifted(T value) {
+__device__ __forceinline__ T lane_shifted_or(T value, T old) {
   static_assert(sizeof(T) == 4 || sizeof(T) == 8);
   if constexpr (sizeof(T) == 4) {
-    const int word = __builtin_bit_cast(int, value);
-    return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(word, word, CONTROL, 0xF, 0xF, false));
+    const int word = __builtin_bit_cast(int, value), kept = __builtin_bit_cast(int, old);
+    return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(kept, word, CONTROL, 0xF, 0xF, false));
   } else {
     struct Words {
       int low, high;
     };
     Words w = __builtin_bit_cast(Words, value);
-    w.low = __builtin_amdgcn_update_dpp(w.low, w.low, CONTROL, 0xF, 0xF, false);
-    w.high = __builtin_amdgcn_update_dpp(w.high, w.high, CONTROL, 0xF, 0xF, false);
+    const Words k = __builtin_bit_cast(Words, old);
+    w.low = __builtin_amdgcn_update_dpp(k.low, w.low, CONTROL, 0xF, 0xF, false);
+    w.high = __builtin_amdgcn_update_dpp(k.high, w.high, CONTROL, 0xF, 0xF, false);
     return __builtin_bit_cast(T, w);
   }
+}
+template <int CONTROL, typename T>
+__device__ __forceinline__ T lane_shifted(T value) {
+  return lane_shifted_or<CONTROL>(value, value);
 }
 constexpr int kWaveShiftRight = 0x138, kWaveShiftLeft = 0x130;  // DPP controls wave_shr:1, wave_shl:1
 template <typename T>
 __device__ __forceinline__ T from_left(T value) {
   return lane_shifted<kWaveShiftRight>(value);
+}
+// The value of the lane on the left, lane 0 taking `edge`: with no select on the lane, so
+// that a load of `edge` is not moved into a branch of lane 0's own and waited for there.
+template <typename T>
+__device__ __forceinline__ T from_left_or(T value, T edge) {
+  return lane_shifted_or<kWaveShiftRight>(value, edge);
 }
 template <typename T>
 __device__ __forceinline__ T from_right(T value) {
@@ -56,12 +68,19 @@ __device__ __forceinline__ T backward_down_of(T here, T up, int i, int H) {
 }
 
 // The levels of a lane's row of a block in the coefficient phase, row k of block (ty, bx)
-// (one 16-byte load), and the block's flat mark; zeros beyond the blocks.
+// (one 16-byte load), and the block's flat mark; zeros beyond the blocks. level_words()
+// loads them, level_row() takes them apart where they are used, so that the loads can be
+// issued before others without anything waiting for them early.
 struct LevelRow {
   int16_t q[8];
   bool flat;
 };
-__device__ __forceinline__ LevelRow level_row(const Component& K, int ty, int bx, int k, bool inside) {
+struct LevelWords {
+  unsigned int pairs[4];
+  uint8_t flat;
+  bool inside;
+};
+__device__ __forceinline__ LevelWords level_words(const Component& K, int ty, int bx, int k, bool inside) {
   // Beyond the blocks the lane reads block 0 and keeps zeros, so that no branch parts the
   // loads from the others.
   const size_t block = static_cast<size_t>(inside ? ty : 0) * K.columns + (inside ? bx : 0);
@@ -69,16 +88,42 @@ __device__ __forceinline__ LevelRow level_row(const Component& K, int ty, int bx
   const Words words =
       *reinterpret_cast<const __attribute__((address_space(1))) Words*>(device::global_view(K.levels) + 64 * block + 8 * k);
   const uint8_t flat = device::global_view(K.flat)[block];
-  const uint32_t pairs[4] = {words.x, words.y, words.z, words.w};
+  // The words go through plain variables: a bit cast of a vector element is lowered by
+  // clang as a load from the vector's first element.
+  const unsigned int x = words[0], y = words[1], z = words[2], w = words[3];
+  return LevelWords{{x, y, z, w}, flat, inside};
+}
+__device__ __forceinline__ LevelRow level_row(const LevelWords& words) {
   LevelRow row;
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
-    row.q[2 * m] = inside ? static_cast<int16_t>(pairs[m] & 0xffffu) : int16_t{0};
-    row.q[2 * m + 1] = inside ? static_cast<int16_t>(pairs[m] >> 16) : int16_t{0};
+    row.q[2 * m] = words.inside ? static_cast<int16_t>(words.pairs[m] & 0xffffu) : int16_t{0};
+    row.q[2 * m + 1] = words.inside ? static_cast<int16_t>(words.pairs[m] >> 16) : int16_t{0};
   }
-  row.flat = inside && flat != 0;
+  row.flat = words.inside && words.flat != 0;
   return row;
 }
+__device__ __forceinline__ LevelRow level_row(const Component& K, int ty, int bx, int k, bool inside) {
+  return level_row(level_words(K, ty, bx, k, inside));
+}
+
+// Where a prox tile takes its component's constants: OwnStaging loads them and copies them
+// into the work-group's LDS (load() before the tile's samples are loaded, store() after);
+// PreStaged hands over constants already in LDS.
+struct OwnStaging {
+  const Component& K;
+  device::StagedArrays& into;
+  __device__ __forceinline__ device::StageLoaded load() const { return device::stage_load(K); }
+  __device__ __forceinline__ device::Staged store(const device::StageLoaded& loaded) const {
+    return device::stage_store(K, into, loaded);
+  }
+};
+struct PreStaged {
+  struct Nothing {};
+  device::Staged staged;
+  __device__ __forceinline__ Nothing load() const { return {}; }
+  __device__ __forceinline__ device::Staged store(Nothing) const { return staged; }
+};
 
 // ---------------------------------------------------------------------------------------
 // The proximal step of one component whose cells are DV x DH samples (1 or 2 each)
@@ -115,8 +160,9 @@ struct ProxEdges {
 };
 
 // The tile of this wavefront in tile row ty and strip `strip` (8 tiles across) of
-// component c. staging() gives the component's constants in LDS: called once the tile's
-// loads are issued, and read after the tile's first barrier.
+// component c. `staging` gives the component's constants in LDS (OwnStaging or PreStaged),
+// read after the tile's first barrier. The tile's first loads are the constants and its
+// levels, which it takes first; then its samples.
 template <typename T, typename S, int DV, int DH, template <typename> class Planes, typename Staging>
 __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& staging, ProxRooms<T>& rooms,
                                              Fields<T, S> F, int c, T* __restrict__ zeta_kept, T* x_out, int H,
@@ -142,6 +188,10 @@ __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& stagi
   // (the pair is aligned: the first column and the canvas width are even).
   // p stays as stored until it is taken: widening a BF16 sample as it arrives made the
   // compiler wait for each load before issuing the next.
+  // In the coefficient phase a lane takes row n of block b of the strip: its levels.
+  const bool inside = (ty < K.rows) & (tx < K.columns);
+  const auto staging_loaded = staging.load();
+  const LevelWords own_words = level_words(K, ty, tx, n, inside);
   T up[DH], here[8][DV][DH];
   S across[8][DV][DH], down[8][DV][DH], edge[8][DV];
   const auto load_row = [&](const auto& planes, bool valid, int plane, uint32_t where, auto (&out)[DH]) {
@@ -190,11 +240,8 @@ __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& stagi
       }
     }
   }
-  // In the coefficient phase a lane takes row n of block b of the strip: its levels, loaded
-  // while the samples are in flight.
-  const bool inside = (ty < K.rows) & (tx < K.columns);
-  const LevelRow own = level_row(K, ty, tx, n, inside);
-  const device::Staged staged = staging();
+  const device::Staged staged = staging.store(staging_loaded);
+  const LevelRow own = level_row(own_words);
 
   T vs[8][DV * DH], column[8];
 #pragma unroll
@@ -205,8 +252,7 @@ __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& stagi
       const int i = top + m * DV + a;
       // p_x of the column on the left: the lane's own previous column, or the lane on the
       // left's last one, or beyond the strip, the edge sample.
-      const T shifted = from_left(device::widen(across[m][a][DH - 1]));
-      const T left = lane == 0 ? device::widen(edge[m][a]) : shifted;
+      const T left = from_left_or(device::widen(across[m][a][DH - 1]), device::widen(edge[m][a]));
 #pragma unroll
       for (int bb = 0; bb < DH; ++bb) {
         const int j = first_column + bb;
@@ -309,7 +355,7 @@ __device__ __forceinline__ void prox_tile(const Component* __restrict__ componen
   ProxEdges<T> edges;
   edges.inside = inside;
   prox_tile_at<T, S, DV, DH, Planes>(
-      K, [&]() { return device::stage(K, staged_arrays); }, rooms, F, c, zeta_kept, x_out, H, W, descent, step, rho,
+      K, OwnStaging{K, staged_arrays}, rooms, F, c, zeta_kept, x_out, H, W, descent, step, rho,
       rest, keep, (g / strips) * 4 + wave, g % strips, edges);
 }
 

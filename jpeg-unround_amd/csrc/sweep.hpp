@@ -81,10 +81,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSweepWaves
       edges.up_first = wg_first_column;
     }
     const Component& K = components[c];
-    const auto staged = [&]() {
-      const device::StagedArrays& a = staged_arrays[c];
-      return device::Staged{a.steps, a.weights, a.flat_weights, a.shrink, a.costs, K.slack, K.mmse, K.costed};
-    };
+    const device::StagedArrays& a = staged_arrays[c];
+    const PreStaged staged{
+        device::Staged{a.steps, a.weights, a.flat_weights, a.shrink, a.costs, K.slack, K.mmse, K.costed}};
     // Entries chosen by comparison, not by a run-time index: an array indexed at run time
     // would be copied to private memory.
     const auto pick = [c](const auto& three) { return c == 0 ? three[0] : (c == 1 ? three[1] : three[2]); };
