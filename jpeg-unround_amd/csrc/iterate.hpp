@@ -37,11 +37,16 @@ template <typename T>
 __device__ __forceinline__ T from_left(T value) {
   return lane_shifted<kWaveShiftRight>(value);
 }
-// The value of the lane on the left, lane 0 taking `edge`: with no select on the lane, so
-// that a load of `edge` is not moved into a branch of lane 0's own and waited for there.
+// The value of the lane on the left, lane 0 taking `edge` (and on the right, lane 63
+// taking `edge`): with no select on the lane, so that a load of `edge` is not moved into a
+// branch of lane 0's own and waited for there.
 template <typename T>
 __device__ __forceinline__ T from_left_or(T value, T edge) {
   return lane_shifted_or<kWaveShiftRight>(value, edge);
+}
+template <typename T>
+__device__ __forceinline__ T from_right_or(T value, T edge) {
+  return lane_shifted_or<kWaveShiftLeft>(value, edge);
 }
 template <typename T>
 __device__ __forceinline__ T from_right(T value) {
@@ -617,15 +622,23 @@ struct EdgeGather {
 // the records read), kKeepDualP p~ too (which only TV's records and the partial gap read).
 constexpr int kKeepTilde = 1, kKeepDualP = 2;
 
-// What each wavefront of a work-group hands to its neighbours every row: lane 63 the
-// samples that the wavefront on its right takes as its left column's (w_x, w_y, p_x, p_y,
-// r_xx, r_yy, r_xy of the row, r_yy, r_xy of the row below), lane 0 those that the one on
-// its left takes as its right column's (x-, r_xx, r_xy). Two rows' worth, taken in turn,
-// so that one barrier a row parts a row's writes from the reads of the row before.
+// The samples beside each strip of a work-group, row by row: left(entry w) the column left
+// of strip w (w_x, w_y, p_x, p_y, r_xx, r_yy, r_xy of the row, r_yy, r_xy of the row
+// below), right(entry w + 1) the column right of it (x-, r_xx, r_xy). Lane 63 of strip w
+// writes left(w + 1) and lane 0 right(w); the first strip's gather lanes write left(0) and
+// the last's right(4) from memory. Every strip then reads its own two entries, the same
+// address in every lane. Two rows' worth, taken in turn, so that one barrier a row parts a
+// row's writes from the reads of the row before.
 template <typename T, int C>
 struct EdgeRoom {
-  T left[2][4][C][9];
-  T right[2][4][C][3];
+  static constexpr int kLeft = 5 * C * 9, kWords = kLeft + 5 * C * 3;
+  T words[2][kWords];
+  __device__ __forceinline__ static constexpr int left_at(int entry, int c, int e) { return (entry * C + c) * 9 + e; }
+  __device__ __forceinline__ static constexpr int right_at(int entry, int c, int e) {
+    return kLeft + (entry * C + c) * 3 + e;
+  }
+  __device__ __forceinline__ T& left(int slot, int entry, int c, int e) { return words[slot][left_at(entry, c, e)]; }
+  __device__ __forceinline__ T& right(int slot, int entry, int c, int e) { return words[slot][right_at(entry, c, e)]; }
 };
 
 // What the fused sweep takes from the dual step of a band for its prox step: p_x of the
@@ -697,6 +710,8 @@ struct DualWalk {
   S edge_s_s[3] = {};
   // w- of the row above, carried from row to row.
   T w_up[C][2];
+  // Where this lane's gathered samples go in the room (-1: nowhere), within one slot.
+  int t_into = -1, s_into = -1;
 
   __device__ __forceinline__ DualWalk(EdgeRoom<T, C>& room_, SweepRoom<T, C>* sweep_, const Fields<T, S>& F_, int H_,
                                       int W_, int first_, int end_, int group, const Steps<T>& steps_, int keep_,
@@ -731,7 +746,29 @@ struct DualWalk {
         r(F_.r, 3 * C, static_cast<size_t>(H_) * W_),
         r_next(F_.r_next, 3 * C, static_cast<size_t>(H_) * W_),
         tr(F_.tr, 3 * C, static_cast<size_t>(H_) * W_),
-        edges(F_, threadIdx.x & 63, H_, W_, j0, wave == 0, wave == 3) {}
+        edges(F_, threadIdx.x & 63, H_, W_, j0, wave == 0, wave == 3) {
+    // The gather's lanes (EdgeGather's order) and the room's entries: the first strip's
+    // left column into left(0), the last strip's right column into right(4). x- on the
+    // left (the sweep's) stays in the gather.
+    if (lane < kT * C) {
+      const int c = lane / kT, e = lane % kT;
+      if (TGV && e < 2 && left_gathered) t_into = EdgeRoom<T, C>::left_at(0, c, e);
+      if (e == 2 && right_gathered) t_into = EdgeRoom<T, C>::right_at(4, c, 0);
+    }
+    if (Gather::kS && lane < 9 * C) {
+      const int c = lane / 9, e = lane % 9;
+      if ((TGV || e < 2) && e < 7 && left_gathered) s_into = EdgeRoom<T, C>::left_at(0, c, e + 2);
+      if (TGV && e >= 7 && right_gathered) s_into = EdgeRoom<T, C>::right_at(4, c, e - 6);
+    }
+  }
+
+  // The gathered samples of a row (t, s) into slot `slot` of the room.
+  __device__ __forceinline__ void hand_gathered(int slot, T t, S s) {
+    if (t_into >= 0) room.words[slot][t_into] = t;
+    if constexpr (Gather::kS) {
+      if (s_into >= 0) room.words[slot][s_into] = device::widen(s);
+    }
+  }
 
   // The index of the lane's sample in row i, i clamped into the canvas (a clamped load only
   // feeds differences that the canvas' edges take as zero).
@@ -812,25 +849,23 @@ struct DualWalk {
       }
       if constexpr (SWEEP) bar_up[c] = xbar.load(c, index(first - 1));
       if (lane == 0) {
-        if constexpr (SWEEP) room.right[1][wave][c][0] = bar_up[c];
+        if constexpr (SWEEP) room.right(1, wave, c, 0) = bar_up[c];
         if constexpr (TGV) {
-          room.right[1][wave][c][1] = r_up[c][0];
-          room.right[1][wave][c][2] = r_up[c][2];
+          room.right(1, wave, c, 1) = r_up[c][0];
+          room.right(1, wave, c, 2) = r_up[c][2];
         }
       }
     }
+    hand_gathered(1, up_t, up_s);
     __syncthreads();
-    const T up_wide = device::widen(up_s);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       if constexpr (TGV) {
         T tilde_up[2], r_first[3];
 #pragma unroll
         for (int e = 0; e < 3; ++e) r_first[e] = device::widen(r_s[0][c][e]);
-        const T edge_xx = right_gathered ? lane_value(up_wide, 9 * c + 7) : room.right[1][wave + 1][c][1];
-        const T edge_xy = right_gathered ? lane_value(up_wide, 9 * c + 8) : room.right[1][wave + 1][c][2];
-        const T shifted_xx = from_right(r_up[c][0]), shifted_xy = from_right(r_up[c][2]);
-        const T right_xx = lane == 63 ? edge_xx : shifted_xx, right_xy = lane == 63 ? edge_xy : shifted_xy;
+        const T right_xx = from_right_or(r_up[c][0], room.right(1, wave + 1, c, 1));
+        const T right_xy = from_right_or(r_up[c][2], room.right(1, wave + 1, c, 2));
         w_step(old_up[c], p_up[c], r_up[c], r_first, right_xx, right_xy, right, true, steps.descent[c], tilde_up,
                w_up[c]);
       }
@@ -841,9 +876,7 @@ struct DualWalk {
       T step_p[C][2];
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        const T edge_bar = right_gathered ? lane_value(up_t, kT * c + 2) : room.right[1][wave + 1][c][0];
-        const T shifted_bar = from_right(bar_up[c]);
-        const T bar_right = lane == 63 ? edge_bar : shifted_bar;
+        const T bar_right = from_right_or(bar_up[c], room.right(1, wave + 1, c, 0));
         const T g1 = right ? bar_right - bar_up[c] : T(0);
         const T g2 = bar_s[0][c] - bar_up[c];  // row first lies in the canvas
         const T wx = TGV ? w_up[c][0] : T(0), wy = TGV ? w_up[c][1] : T(0);
@@ -939,48 +972,41 @@ struct DualWalk {
     const T edge_t = edge_t_s[PH];
     const S edge_s = edge_s_s[PH];
 
-    // This row's edge samples for the wavefronts beside, then theirs for this one.
+    // This row's edge samples for the wavefronts beside (and the gathered ones), then
+    // theirs for this one.
     const int slot = (i - first) & 1;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       if (lane == 0) {
-        room.right[slot][wave][c][0] = bar[c];
+        room.right(slot, wave, c, 0) = bar[c];
         if constexpr (TGV) {
-          room.right[slot][wave][c][1] = r_here[c][0];
-          room.right[slot][wave][c][2] = r_here[c][2];
+          room.right(slot, wave, c, 1) = r_here[c][0];
+          room.right(slot, wave, c, 2) = r_here[c][2];
         }
       }
       if (TGV && lane == 63) {
         const T handed[9] = {w_here[c][0], w_here[c][1], p_here[c][0], p_here[c][1], r_here[c][0],
                              r_here[c][1], r_here[c][2], r_below[c][1], r_below[c][2]};
 #pragma unroll
-        for (int e = 0; e < 9; ++e) room.left[slot][wave][c][e] = handed[e];
+        for (int e = 0; e < 9; ++e) room.left(slot, wave + 1, c, e) = handed[e];
       }
     }
+    hand_gathered(slot, edge_t, edge_s);
     __syncthreads();
-    // The left column's sample e of channel c (the S gather's order, w_x and w_y at 7, 8)
-    // and the right column's (x-, r_xx, r_xy).
-    const T edge_wide = device::widen(edge_s);
-    const auto left_of = [&](int c, int e) {
-      if (left_gathered) return e >= 7 ? lane_value(edge_t, kT * c + (e - 7)) : lane_value(edge_wide, 9 * c + e);
-      return room.left[slot][wave - 1][c][e < 7 ? e + 2 : e - 7];
-    };
-    const auto right_of = [&](int c, int e) {
-      if (right_gathered) return e == 0 ? lane_value(edge_t, kT * c + 2) : lane_value(edge_wide, 9 * c + 6 + e);
-      return room.right[slot][wave + 1][c][e];
-    };
+    // The left column's sample e of channel c (p_x, p_y, r_xx, r_yy, r_xy, r_yy and r_xy
+    // below, then w_x and w_y at 7, 8) and the right column's (x-, r_xx, r_xy).
+    const auto left_of = [&](int c, int e) { return room.left(slot, wave, c, e < 7 ? e + 2 : e - 7); };
+    const auto right_of = [&](int c, int e) { return room.right(slot, wave + 1, c, e); };
     T step_p[C][2], step_r[C][3], tilde_w[C][2], bar_w[C][2], bar_left[C][2];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const T a = steps.ascent[c];
-      const T edge_bar = right_of(c, 0), shifted_bar = from_right(bar[c]);
-      const T bar_right = lane == 63 ? edge_bar : shifted_bar;
+      const T bar_right = from_right_or(bar[c], right_of(c, 0));
       const T g1 = right ? bar_right - bar[c] : T(0);
       const T g2 = below ? bar_next[c] - bar[c] : T(0);
       if constexpr (TGV) {
-        const T edge_xx = right_of(c, 1), edge_xy = right_of(c, 2);
-        const T shifted_xx = from_right(r_here[c][0]), shifted_xy = from_right(r_here[c][2]);
-        const T right_xx = lane == 63 ? edge_xx : shifted_xx, right_xy = lane == 63 ? edge_xy : shifted_xy;
+        const T right_xx = from_right_or(r_here[c][0], right_of(c, 1));
+        const T right_xy = from_right_or(r_here[c][2], right_of(c, 2));
         w_step(w_here[c], p_here[c], r_here[c], r_below[c], right_xx, right_xy, right, below, steps.descent[c],
                tilde_w[c], bar_w[c]);
         // w- of the column on the left: the lane on the left's, or for lane 0, made from the
@@ -992,8 +1018,8 @@ struct DualWalk {
         T tilde_left[2];
         w_step(old_left, p_left, r_left, r_left_below, r_here[c][0], r_here[c][2], true, below, steps.descent[c],
                tilde_left, bar_left[c]);
-        const T shifted_x = from_left(bar_w[c][0]), shifted_y = from_left(bar_w[c][1]);
-        const T left_x = lane == 0 ? bar_left[c][0] : shifted_x, left_y = lane == 0 ? bar_left[c][1] : shifted_y;
+        const T left_x = from_left_or(bar_w[c][0], bar_left[c][0]);
+        const T left_y = from_left_or(bar_w[c][1], bar_left[c][1]);
         // E w- (docs/math.md:355): backward differences.
         const T across_x = backward_across_of(bar_w[c][0], left_x, j, W);
         const T down_y = backward_down_of(bar_w[c][1], w_up[c][1], i, H);
