@@ -180,18 +180,19 @@ __global__ __launch_bounds__(64) void unrolled_dual_back_kernel(UnrolledFrames u
   const UnrolledBack v = back_of(u, f, N);
   const UnrolledSteps k = v.k;
   const float rest = 1.0f - k.rho;
-  // Every kind of field through a buffer resource of its planes (at most 2^31 bytes each,
-  // as the solve's checks hold), addressed by one offset a row for all of them; a load
-  // beyond the frame reads 0 and a store there is dropped. The default cache policy
-  // throughout: the neighbouring strips' and runs' walks read the edge columns and rows
-  // again.
+  // The fields through buffer resources, addressed by one offset a row for all of them; a
+  // load beyond the frame reads 0 and a store there is dropped. A frame's state, and its
+  // gradients after and before the iteration, are each one resource of 8 planes (x, w_x,
+  // w_y, p_x, p_y, r_xx, r_yy, r_xy: at most 2^31 bytes, as the solve's checks hold), and
+  // the two work planes one more, so that the kernel's resources fit its scalar registers.
+  // The default cache policy throughout: the neighbouring strips' and runs' walks read the
+  // edge columns and rows again.
   using Planes = device::BufferPlanesOf<float, device::kTemporal, device::kTemporal>;
-  const Planes w(v.s.w, 2, N), p(v.s.p, 2, N), r(v.s.r, 3, N), xbar(const_cast<float*>(v.xbar), 1, N);
-  const Planes tgv(v.work.v, 1, N);
-  const Planes after_w(v.after.w, 2, N), after_p(v.after.p, 2, N), after_r(v.after.r, 3, N);
-  const Planes map(const_cast<float*>(radius), 1, N);
-  const Planes before_w(v.before.w, 2, N), before_p(v.before.p, 2, N), before_r(v.before.r, 3, N);
-  const Planes grad_bar(v.work.xbar, 1, N);
+  constexpr int kW = 1, kP = 3, kR = 5;  // the first plane of w, p and r in a state
+  const Planes state(v.s.x, 8, N), after(v.after.x, 8, N), before(v.before.x, 8, N);
+  const Planes xbar(const_cast<float*>(v.xbar), 1, N), map(const_cast<float*>(radius), 1, N);
+  const Planes work(v.work.xbar, 2, N);  // grad x- (plane 0), tau g grad v (plane 1)
+  constexpr int kGradBar = 0, kTgv = 1;
   const auto index = [&](int i) { return static_cast<uint32_t>(i) * static_cast<uint32_t>(W) + static_cast<uint32_t>(j); };
   const auto inside = [&](int i) { return column && i >= 0 && i < H; };
   // w- of row i from w, p and r of the row and r of the row below.
@@ -224,13 +225,14 @@ __global__ __launch_bounds__(64) void unrolled_dual_back_kernel(UnrolledFrames u
     float r[3], xb, v, w[2], p[2], ap[2], ar[3], R, aw[2];
   };
   Loads slots[3];
-  for (int e = 0; e < 3; ++e) slots[2].r[e] = r.load_if(in1, e, at1);
+  for (int e = 0; e < 3; ++e) slots[2].r[e] = state.load_if(in1, kR + e, at1);
   slots[2].xb = xbar.load_if(in1, 0, at1);
-  slots[2].v = tgv.load_if(in1, 0, at1);
+  slots[2].v = work.load_if(in1, kTgv, at1);
   float wb0[2];
   {
-    const float r0[3] = {r.load_if(in0, 0, at0), r.load_if(in0, 1, at0), r.load_if(in0, 2, at0)};
-    w_bar(top, w.load_if(in0, 0, at0), w.load_if(in0, 1, at0), p.load_if(in0, 0, at0), p.load_if(in0, 1, at0), r0,
+    const float r0[3] = {state.load_if(in0, kR, at0), state.load_if(in0, kR + 1, at0), state.load_if(in0, kR + 2, at0)};
+    w_bar(top, state.load_if(in0, kW, at0), state.load_if(in0, kW + 1, at0), state.load_if(in0, kP, at0),
+          state.load_if(in0, kP + 1, at0), r0,
           slots[2].r, wb0);
   }
   float gp0[2] = {0.0f, 0.0f}, gr0[3] = {0.0f, 0.0f, 0.0f}, ap0[2] = {0.0f, 0.0f}, ar0[3] = {0.0f, 0.0f, 0.0f};
@@ -240,16 +242,16 @@ __global__ __launch_bounds__(64) void unrolled_dual_back_kernel(UnrolledFrames u
     Loads l;
     const bool in2 = live && inside(i + 2), in1 = live && inside(i + 1), in0 = live && inside(i);
     const uint32_t at2 = index(i + 2), at1 = index(i + 1), at0 = index(i);
-    for (int e = 0; e < 3; ++e) l.r[e] = r.load_if(in2, e, at2);
+    for (int e = 0; e < 3; ++e) l.r[e] = state.load_if(in2, kR + e, at2);
     l.xb = xbar.load_if(in2, 0, at2);
-    l.v = tgv.load_if(in2, 0, at2);
+    l.v = work.load_if(in2, kTgv, at2);
     for (int e = 0; e < 2; ++e) {
-      l.w[e] = w.load_if(in1, e, at1);
-      l.p[e] = p.load_if(in1, e, at1);
-      l.ap[e] = after_p.load_if(in1, e, at1);
-      l.aw[e] = after_w.load_if(in0, e, at0);
+      l.w[e] = state.load_if(in1, kW + e, at1);
+      l.p[e] = state.load_if(in1, kP + e, at1);
+      l.ap[e] = after.load_if(in1, kP + e, at1);
+      l.aw[e] = after.load_if(in0, kW + e, at0);
     }
-    for (int e = 0; e < 3; ++e) l.ar[e] = after_r.load_if(in1, e, at1);
+    for (int e = 0; e < 3; ++e) l.ar[e] = after.load_if(in1, kR + e, at1);
     l.R = map.load_if(in1, 0, at1);
     return l;
   };
@@ -260,9 +262,9 @@ __global__ __launch_bounds__(64) void unrolled_dual_back_kernel(UnrolledFrames u
     const int n1 = i + 1;
     slots[(PH + 1) % 3] = fetch(i + 1, i + 1 < end);
     const Loads& l = slots[PH];
-    const Loads& before = slots[(PH + 2) % 3];
-    const float(&r1)[3] = before.r;
-    const float xb1 = before.xb, v1 = before.v;
+    const Loads& prior = slots[(PH + 2) % 3];
+    const float(&r1)[3] = prior.r;
+    const float xb1 = prior.xb, v1 = prior.v;
     const float(&r2)[3] = l.r;
     const float xb2 = l.xb, v2 = l.v;
     const float w1x = l.w[0], w1y = l.w[1], p1x = l.p[0], p1y = l.p[1];
@@ -347,14 +349,14 @@ __global__ __launch_bounds__(64) void unrolled_dual_back_kernel(UnrolledFrames u
     const float txy = forward_transpose(d_up[0], d0[0], i, H) + forward_transpose(dy_left, d0[1], j, W);
     const bool writes = own && i >= first;
     const uint32_t at = index(i);
-    grad_bar.store_if<Planes::kNext>(writes, 0, at, gxbar);  // the prox adjoint reads it next
+    work.store_if<Planes::kNext>(writes, kGradBar, at, gxbar);  // the prox adjoint reads it next
     for (int e = 0; e < 2; ++e) {
-      before_w.store_if(writes, e, at, bw[e]);
-      before_p.store_if(writes, e, at, bp[e]);
+      before.store_if(writes, kW + e, at, bw[e]);
+      before.store_if(writes, kP + e, at, bp[e]);
     }
-    before_r.store_if(writes, 0, at, rest * ar0[0] + gr0[0] + xx);
-    before_r.store_if(writes, 1, at, rest * ar0[1] + gr0[1] + yy);
-    before_r.store_if(writes, 2, at, rest * ar0[2] + gr0[2] + txy);
+    before.store_if(writes, kR, at, rest * ar0[0] + gr0[0] + xx);
+    before.store_if(writes, kR + 1, at, rest * ar0[1] + gr0[1] + yy);
+    before.store_if(writes, kR + 2, at, rest * ar0[2] + gr0[2] + txy);
     gp_up = gp0[1];
     d_up[0] = d0[0];
     d_up[1] = d0[1];
@@ -553,8 +555,8 @@ class Unrolled {
       // Its kernels address a frame's planes through buffer resources only.
       const size_t samples = static_cast<size_t>(f->rows) * f->columns;
       if (!planes::dual_fits(1, samples, sizeof(float), sizeof(float)) ||
-          !planes::prox_fits(samples, sizeof(float), sizeof(float)))
-        throw Refused("the unrolled solve takes frames of at most 2^31 / 12 samples");
+          !planes::prox_fits(samples, sizeof(float), sizeof(float)) || !planes::fits(8 * samples * sizeof(float)))
+        throw Refused("the unrolled solve takes frames of at most 2^31 / 32 samples");
     }
     if (settings_.method != Method::tgv) throw Refused("the unrolled solve is TGV's");
     const int B = static_cast<int>(frames_.size());
