@@ -674,8 +674,11 @@ struct SweepRoom {
 // work-group's own (SweepRoom): begin() makes p_y of row first - 1 once more (the run
 // above writes it in the same launch), and step() gives p_x of the column left of each
 // strip, made once more by the first wavefront for the work-group's left edge.
+//
+// KEEPS false leaves the tilde point's stores out of the walk altogether (the iterations
+// that keep none: fewer scalar registers held through the rows).
 template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes, bool MAP = false,
-          bool SWEEP = false>
+          bool SWEEP = false, bool KEEPS = true>
 struct DualWalk {
   static_assert(!(MAP && SWEEP), "the fused sweep takes one radius");
   using Gather = EdgeGather<T, S, C, TGV, SWEEP>;
@@ -1057,7 +1060,7 @@ struct DualWalk {
         const T projected = step_p[c][e] * scale;
         const S stored = device::narrow<S>(relaxed_of(rho, projected, rest, p_here[c][e]));
         if (e == 0) stored_x = stored;
-        if (keep & kKeepDualP) {
+        if (KEEPS && (keep & kKeepDualP)) {
           tp.template store_if<device::kStreaming>(writes, 2 * c + e, at, device::narrow<S>(projected));
         }
         p_next.template store_if<kNext>(writes, 2 * c + e, at, stored);
@@ -1069,7 +1072,7 @@ struct DualWalk {
       if constexpr (TGV) {
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-          if (keep) tw.template store_if<device::kStreaming>(writes, 2 * c + e, at, tilde_w[c][e]);
+          if (KEEPS && keep) tw.template store_if<device::kStreaming>(writes, 2 * c + e, at, tilde_w[c][e]);
           w_next.template store_if<kNext>(writes, 2 * c + e, at, relaxed_of(rho, tilde_w[c][e], rest, w_here[c][e]));
         }
         T tensor_scale = shared_tensor_scale;
@@ -1080,7 +1083,9 @@ struct DualWalk {
 #pragma unroll
         for (int e = 0; e < 3; ++e) {
           const T projected = step_r[c][e] * tensor_scale;
-          if (keep) tr.template store_if<device::kStreaming>(writes, 3 * c + e, at, device::narrow<S>(projected));
+          if (KEEPS && keep) {
+            tr.template store_if<device::kStreaming>(writes, 3 * c + e, at, device::narrow<S>(projected));
+          }
           r_next.template store_if<device::kStreaming>(
               writes, 3 * c + e, at, device::narrow<S>(relaxed_of(rho, projected, rest, r_here[c][e])));
         }
@@ -1123,29 +1128,32 @@ struct DualWalk {
 };
 
 // The work of 4 strips across (strip group `group`) from row `first` to `end`.
-template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes, bool MAP = false>
+template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes, bool MAP = false,
+          bool KEEPS = true>
 __device__ __forceinline__ void dual_run(Fields<T, S> F, int H, int W, int first, int end, int group, Steps<T> steps,
                                          int keep, const T* __restrict__ radius_map = nullptr) {
   __shared__ EdgeRoom<T, C> room;
-  DualWalk<T, S, C, TGV, COUPLED, Planes, MAP> walk(room, nullptr, F, H, W, first, end, group, steps, keep,
-                                                    radius_map);
+  DualWalk<T, S, C, TGV, COUPLED, Planes, MAP, false, KEEPS> walk(room, nullptr, F, H, W, first, end, group, steps,
+                                                                  keep, radius_map);
   walk.begin();
   walk.walk();
 }
 
 // dual_tile is work-group g's part: 4 strips across and `rows` rows down; dual_kernel runs
 // a frame (work-groups in a run on each XCD), and the batch's kernel runs tiles of many.
-template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes, bool MAP = false>
+template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes, bool MAP = false,
+          bool KEEPS = true>
 __device__ __forceinline__ void dual_tile(Fields<T, S> F, int H, int W, int rows, int groups_across,
                                           Steps<T> steps, int keep, int g, const T* __restrict__ radius_map = nullptr) {
   const int first = (g / groups_across) * rows;
-  dual_run<T, S, C, TGV, COUPLED, Planes, MAP>(F, H, W, first, min(first + rows, H), g % groups_across, steps, keep,
-                                               radius_map);
+  dual_run<T, S, C, TGV, COUPLED, Planes, MAP, KEEPS>(F, H, W, first, min(first + rows, H), g % groups_across, steps,
+                                                      keep, radius_map);
 }
 
-template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes>
+// KEEPS false: for the iterations that keep no tilde point (keep 0).
+template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes, bool KEEPS = true>
 __global__ __launch_bounds__(256) void dual_kernel(Fields<T, S> F, int H, int W, int rows, int groups_across,
                                                    Steps<T> steps, int keep) {
-  dual_tile<T, S, C, TGV, COUPLED, Planes>(F, H, W, rows, groups_across, steps, keep,
-                                           device::ordered(blockIdx.x, gridDim.x));
+  dual_tile<T, S, C, TGV, COUPLED, Planes, false, KEEPS>(F, H, W, rows, groups_across, steps, keep,
+                                                         device::ordered(blockIdx.x, gridDim.x));
 }

@@ -1148,7 +1148,10 @@ class Solver {
     const int strips = (W_ + kStripColumns - 1) / kStripColumns;
     const int across = (strips + 3) / 4, down = (H_ + dual_rows_ - 1) / dual_rows_;
     const dim3 grid(across * down);
-    if (narrow_spans()) {
+    if (narrow_spans() && keep == 0) {
+      dual_kernel<T, S, C, TGV, COUPLED, device::BufferPlanes, false>
+          <<<grid, 256, 0, stream()>>>(fields_, H_, W_, dual_rows_, across, steps, 0);
+    } else if (narrow_spans()) {
       dual_kernel<T, S, C, TGV, COUPLED, device::BufferPlanes>
           <<<grid, 256, 0, stream()>>>(fields_, H_, W_, dual_rows_, across, steps, keep);
     } else {
