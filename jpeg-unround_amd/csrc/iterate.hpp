@@ -56,20 +56,19 @@ __device__ __forceinline__ T from_right(T value) {
 // delta_x f at column j from f there and at j - 1 (docs/math.md:337).
 template <typename T>
 __device__ __forceinline__ T backward_across_of(T here, T left, int j, int W) {
-  if (W == 1) return T(0);
-  if (j == 0) return here;
-  if (j == W - 1) return -left;
-  return here - left;
+  // Every case made and one chosen by selects: no branch between the lanes.
+  const T inner = here - left, last = -left;
+  const T value = j == 0 ? here : (j == W - 1 ? last : inner);
+  return W == 1 ? T(0) : value;
 }
 
 // delta_y f at row i from f there and at i - 1.
 template <typename T>
 __device__ __forceinline__ T backward_down_of(T here, T up, int i, int H) {
   const bool first = i == 0, last = i == H - 1;
-  if (!first && !last) return here - up;
-  if (!last) return here;
-  if (!first) return -up;
-  return T(0);
+  const T inner = here - up, bottom = -up;
+  const T value = first ? here : (last ? bottom : inner);
+  return (first && last) ? T(0) : value;
 }
 
 // The levels of a lane's row of a block in the coefficient phase, row k of block (ty, bx)
@@ -978,16 +977,19 @@ struct DualWalk {
     // This row's edge samples for the wavefronts beside (and the gathered ones), then
     // theirs for this one.
     const int slot = (i - first) & 1;
+    if (lane == 0) {
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
-      if (lane == 0) {
+      for (int c = 0; c < C; ++c) {
         room.right(slot, wave, c, 0) = bar[c];
         if constexpr (TGV) {
           room.right(slot, wave, c, 1) = r_here[c][0];
           room.right(slot, wave, c, 2) = r_here[c][2];
         }
       }
-      if (TGV && lane == 63) {
+    }
+    if (TGV && lane == 63) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
         const T handed[9] = {w_here[c][0], w_here[c][1], p_here[c][0], p_here[c][1], r_here[c][0],
                              r_here[c][1], r_here[c][2], r_below[c][1], r_below[c][2]};
 #pragma unroll
