@@ -85,15 +85,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kProxWaves<
 
 // The table is laid out in the order the hardware hands out work-groups (each XCD's share
 // its own run of places, long runs of rows first), so the tiles are taken as they stand.
-template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes>
+// KEEPS false: for the iterations that keep no tilde point (keep 0).
+template <typename T, typename S, int C, bool TGV, bool COUPLED, template <typename> class Planes, bool KEEPS = true>
 __global__ __launch_bounds__(256) void dual_batch_kernel(const DualTile* __restrict__ tiles,
                                                          const ImageSlot<T, S>* __restrict__ slots,
                                                          const int* __restrict__ active, int keep) {
   const DualTile t = tiles[blockIdx.x];
   if (t.image < 0 || active[t.image] == 0) return;
   const ImageSlot<T, S>& m = slots[t.image];
-  dual_run<T, S, C, TGV, COUPLED, Planes>(m.F, m.H, m.W, t.first, min(t.first + t.rows, m.H), t.group, m.steps,
-                                          keep);
+  dual_run<T, S, C, TGV, COUPLED, Planes, false, KEEPS>(m.F, m.H, m.W, t.first, min(t.first + t.rows, m.H), t.group,
+                                                        m.steps, keep);
 }
 
 template <typename T, typename S, bool TGV>
@@ -644,7 +645,10 @@ class Batch {
   template <int C, bool TGV, bool COUPLED>
   void launch_dual(const Launch& launch, const ImageSlot<T, S>* slots, int keep) {
     const auto* tiles = launch.tiles.template as<DualTile>();
-    if (launch.key[2]) {
+    if (launch.key[2] && keep == 0) {
+      dual_batch_kernel<T, S, C, TGV, COUPLED, device::CachedBufferPlanes, false>
+          <<<launch.count, 256, 0, stream_.stream>>>(tiles, slots, state_.active, 0);
+    } else if (launch.key[2]) {
       dual_batch_kernel<T, S, C, TGV, COUPLED, device::CachedBufferPlanes>
           <<<launch.count, 256, 0, stream_.stream>>>(tiles, slots, state_.active, keep);
     } else {
