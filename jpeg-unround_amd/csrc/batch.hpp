@@ -70,7 +70,8 @@ struct BatchState {
   unsigned long long* counter;
 };
 
-template <typename T, typename S, int DV, int DH, template <typename> class Planes>
+// PLAIN: every component of the batch has no slack cost and midpoint centres.
+template <typename T, typename S, int DV, int DH, template <typename> class Planes, bool PLAIN = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kProxWaves<T, DV, DH>))) void prox_batch_kernel(
     const BatchTile* __restrict__ tiles, const ImageSlot<T, S>* __restrict__ slots, const int* __restrict__ active,
     T rho, T rest, int keep) {
@@ -79,8 +80,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kProxWaves<
   const ImageSlot<T, S>& m = slots[t.image];
   // zeta's pointer read from the slot at its index: an index into a copy of the fields
   // would place the copy in private memory, written by every lane.
-  prox_tile<T, S, DV, DH, Planes>(m.components + t.part, m.F, t.part, m.F.zeta[t.part], m.F.x, m.H, m.W,
-                                  m.descent[t.part], m.step[t.part], rho, rest, keep, t.group);
+  prox_tile<T, S, DV, DH, Planes, PLAIN>(m.components + t.part, m.F, t.part, m.F.zeta[t.part], m.F.x, m.H, m.W,
+                                         m.descent[t.part], m.step[t.part], rho, rest, keep, t.group);
 }
 
 // The table is laid out in the order the hardware hands out work-groups (each XCD's share
@@ -236,6 +237,10 @@ class Batch {
     const int M = static_cast<int>(frames_.size());
     const bool solving = settings_.method == Method::tv || settings_.method == Method::tgv;
     tgv_ = settings_.method == Method::tgv;
+    plain_ = true;
+    for (const auto& frame : frames_) {
+      for (const ComponentModel& k : frame->components) plain_ = plain_ && !k.costed && !k.mmse;
+    }
     using Clock = std::chrono::steady_clock;
     const auto began = Clock::now();
     for (int m = 0; m < M; ++m) {
@@ -625,7 +630,11 @@ class Batch {
       const auto go = [&](auto kernel) {
         kernel<<<launch.count, 256, 0, s>>>(tiles, slots, state_.active, rho_, rest_, keep);
       };
-      if (dv == 1 && dh == 1) {
+      if (dv == 1 && dh == 1 && buffers && plain_) {
+        go(prox_batch_kernel<T, S, 1, 1, device::StreamedBufferPlanes, true>);
+      } else if (dv == 2 && dh == 2 && buffers && plain_) {
+        go(prox_batch_kernel<T, S, 2, 2, device::StreamedBufferPlanes, true>);
+      } else if (dv == 1 && dh == 1) {
         buffers ? go(prox_batch_kernel<T, S, 1, 1, device::StreamedBufferPlanes>)
                 : go(prox_batch_kernel<T, S, 1, 1, device::WidePlanes>);
       } else if (dv == 2 && dh == 2) {
@@ -835,6 +844,7 @@ class Batch {
   const std::vector<const Plan*>& plans_;
   const Settings& settings_;
   bool tgv_ = false;
+  bool plain_ = false;  // every component: no slack cost, midpoint centres
   uint64_t most_ = 0, every_ = 0, reached_ = 0;
   T rho_ = T(0), rest_ = T(0);
   StreamHolder stream_, records_stream_;

@@ -176,12 +176,28 @@ __device__ __forceinline__ T clamp_to(T value, T low, T high) {
 // cpp/src/model.cpp:62-75): the quadratic's minimizer, clipped to the interval. The
 // frequency's constants are read first and every case is computed and selected, so that
 // a row of coefficients runs without branches. `within`: whether the quadratic's
-// minimizer (with no slack cost) lay inside the interval.
-template <typename Constants, typename T>
+// minimizer (with no slack cost) lay inside the interval. PLAIN: for a component with no
+// slack cost and midpoint centres (the defaults), which the caller has checked; the same
+// values with the other cases left out.
+template <bool PLAIN = false, typename Constants, typename T>
 __device__ __forceinline__ T prox_of(const Constants& K, int16_t q, bool flat, int k, T e, T step, bool& within) {
 #pragma clang fp contract(off)
   const double frequency_step = K.steps[k];
   const double weight_of = (flat ? K.flat_weights : K.weights)[k];
+  if constexpr (PLAIN) {
+    // ends_from's lower, upper and midpoint centre.
+    const double level = static_cast<double>(q);
+    const double shift = k == 0 ? 1024.0 : 0.0;
+    const double lower = ((level - 0.5) - K.slack) * frequency_step + shift;
+    const double upper = ((level + 0.5) + K.slack) * frequency_step + shift;
+    const double centre = level * frequency_step + shift;
+    const T weight = static_cast<T>(weight_of);
+    const T t = step * weight;
+    const T r = T(1) / (T(1) + t);
+    const T z = fused(t, static_cast<T>(centre), e) * r;
+    within = z >= static_cast<T>(lower) && z <= static_cast<T>(upper);
+    return clamp_to(z, static_cast<T>(lower), static_cast<T>(upper));
+  }
   const double shrink = K.shrink[k];
   const double cost = K.costs[k];
   double lower, upper, centre, own_lower, own_upper;
@@ -199,10 +215,10 @@ __device__ __forceinline__ T prox_of(const Constants& K, int16_t q, bool flat, i
   within = z >= static_cast<T>(lower) && z <= static_cast<T>(upper);
   return clamp_to(K.costed ? costed : z, static_cast<T>(lower), static_cast<T>(upper));
 }
-template <typename Constants, typename T>
+template <bool PLAIN = false, typename Constants, typename T>
 __device__ __forceinline__ T prox_of(const Constants& K, int16_t q, bool flat, int k, T e, T step) {
   bool within;
-  return prox_of(K, q, flat, k, e, step, within);
+  return prox_of<PLAIN>(K, q, flat, k, e, step, within);
 }
 
 // g*(s) of one coefficient, in binary64 (cpp/src/model.cpp:80-109).

@@ -166,8 +166,10 @@ struct ProxEdges {
 // The tile of this wavefront in tile row ty and strip `strip` (8 tiles across) of
 // component c. `staging` gives the component's constants in LDS (OwnStaging or PreStaged),
 // read after the tile's first barrier. The tile's first loads are the constants and its
-// levels, which it takes first; then its samples.
-template <typename T, typename S, int DV, int DH, template <typename> class Planes, typename Staging>
+// levels, which it takes first; then its samples. PLAIN: the component has no slack cost
+// and midpoint centres (device::prox_of's PLAIN).
+template <typename T, typename S, int DV, int DH, template <typename> class Planes, bool PLAIN = false,
+          typename Staging>
 __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& staging, ProxRooms<T>& rooms,
                                              Fields<T, S> F, int c, T* __restrict__ zeta_kept, T* x_out, int H,
                                              int W, T descent, T step, T rho, T rest, int keep, int ty, int strip,
@@ -291,7 +293,7 @@ __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& stagi
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
     bool within = false;
-    const T z = device::prox_of(staged, own.q[u], own.flat, 8 * n + u, coefficient[u], step, within);
+    const T z = device::prox_of<PLAIN>(staged, own.q[u], own.flat, 8 * n + u, coefficient[u], step, within);
     zeta[u] = inside ? z : T(0);
     if (marks != nullptr) marks[u] = within;
   }
@@ -347,7 +349,7 @@ __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& stagi
 
 // Work-group g's part of one component's step: 4 tile rows of one strip, its constants
 // staged here.
-template <typename T, typename S, int DV, int DH, template <typename> class Planes>
+template <typename T, typename S, int DV, int DH, template <typename> class Planes, bool PLAIN = false>
 __device__ __forceinline__ void prox_tile(const Component* __restrict__ component, Fields<T, S> F, int c,
                                           T* __restrict__ zeta_kept, T* x_out, int H, int W, T descent, T step, T rho,
                                           T rest, int keep, int g, uint8_t* inside = nullptr) {
@@ -358,7 +360,7 @@ __device__ __forceinline__ void prox_tile(const Component* __restrict__ componen
   const int strips = (K.tiles_x + 7) >> 3;
   ProxEdges<T> edges;
   edges.inside = inside;
-  prox_tile_at<T, S, DV, DH, Planes>(
+  prox_tile_at<T, S, DV, DH, Planes, PLAIN>(
       K, OwnStaging{K, staged_arrays}, rooms, F, c, zeta_kept, x_out, H, W, descent, step, rho,
       rest, keep, (g / strips) * 4 + wave, g % strips, edges);
 }
@@ -378,14 +380,15 @@ struct ProxSteps {
 };
 
 // Components c .. c + gridDim.y - 1 of one cell shape and size, blockIdx.y the component: the
-// chroma of 4:2:0 in one launch, so that its work-groups fill the GPU together.
-template <typename T, typename S, int DV, int DH, template <typename> class Planes>
+// chroma of 4:2:0 in one launch, so that its work-groups fill the GPU together. PLAIN: every
+// one of them has no slack cost and midpoint centres.
+template <typename T, typename S, int DV, int DH, template <typename> class Planes, bool PLAIN = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kProxWaves<T, DV, DH>))) void prox_kernel(
     const Component* __restrict__ component, Fields<T, S> F, int c, int H, int W, ProxSteps<T> steps, T rho, T rest,
     int keep) {
   const int k = static_cast<int>(blockIdx.y);
-  prox_tile<T, S, DV, DH, Planes>(component + k, F, c + k, F.zeta[c + k], F.x, H, W, steps.descent[k], steps.step[k],
-                                  rho, rest, keep, device::ordered(blockIdx.x, gridDim.x));
+  prox_tile<T, S, DV, DH, Planes, PLAIN>(component + k, F, c + k, F.zeta[c + k], F.x, H, W, steps.descent[k],
+                                         steps.step[k], rho, rest, keep, device::ordered(blockIdx.x, gridDim.x));
 }
 
 // The same for any cell up to 4 x 4, its samples read again for the write-back.

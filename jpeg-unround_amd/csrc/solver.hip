@@ -1196,7 +1196,15 @@ class Solver {
       // Through buffer resources where a component's planes span at most 2^31 bytes, else
       // with 64-bit addresses.
       const bool buffers = prox_buffers();
-      if (k.down == 1 && k.across == 1) {
+      // The two common cell shapes with the defaults' data term (no slack cost, midpoint
+      // centres) take the kernel that leaves the other cases out.
+      bool plain = true;
+      for (int m = 0; m < count; ++m) plain = plain && !frame_.components[c + m].costed && !frame_.components[c + m].mmse;
+      if (k.down == 1 && k.across == 1 && buffers && plain) {
+        prox(prox_kernel<T, S, 1, 1, device::BufferPlanes, true>);
+      } else if (k.down == 2 && k.across == 2 && buffers && plain) {
+        prox(prox_kernel<T, S, 2, 2, device::BufferPlanes, true>);
+      } else if (k.down == 1 && k.across == 1) {
         buffers ? prox(prox_kernel<T, S, 1, 1, device::BufferPlanes>) : prox(prox_kernel<T, S, 1, 1, device::WidePlanes>);
       } else if (k.down == 2 && k.across == 2) {
         buffers ? prox(prox_kernel<T, S, 2, 2, device::BufferPlanes>) : prox(prox_kernel<T, S, 2, 2, device::WidePlanes>);

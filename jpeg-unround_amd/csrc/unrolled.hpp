@@ -475,8 +475,9 @@ __global__ __launch_bounds__(256) void unrolled_prox_back_kernel(UnrolledFrames 
 // The forward prox of every frame (frame blockIdx.y): x of the state relaxed into the next
 // state, x- into `bars`; no tilde copies. MARKS: the clip's marks of every frame's
 // coefficients also go to `inside` ([frame][N] bytes) for the adjoint (an instance of its
-// own, with the registers the marks take).
-template <bool MARKS>
+// own, with the registers the marks take). PLAIN: every frame's component has midpoint
+// centres (the unrolled solve takes no slack cost).
+template <bool MARKS, bool PLAIN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MARKS ? 3 : kProxWaves<float, 1, 1>))) void
 unrolled_prox_kernel(UnrolledFrames u, int H, int W, uint8_t* inside) {
   const int f = static_cast<int>(blockIdx.y);
@@ -488,10 +489,9 @@ unrolled_prox_kernel(UnrolledFrames u, int H, int W, uint8_t* inside) {
   F.xbar = u.bars + f * N;
   F.tx = F.xbar;  // not written: no tilde copies here
   F.p = now + 3 * N;
-  prox_tile<float, float, 1, 1, device::BufferPlanes>(u.components + f, F, 0, nullptr, frame_state(u.next, f, N), H,
-                                                      W, scalars.k.descent, scalars.step, scalars.k.rho,
-                                                      1.0f - scalars.k.rho, 0, device::ordered(blockIdx.x, gridDim.x),
-                                                      MARKS ? inside + f * N : nullptr);
+  prox_tile<float, float, 1, 1, device::BufferPlanes, PLAIN>(
+      u.components + f, F, 0, nullptr, frame_state(u.next, f, N), H, W, scalars.k.descent, scalars.step,
+      scalars.k.rho, 1.0f - scalars.k.rho, 0, device::ordered(blockIdx.x, gridDim.x), MARKS ? inside + f * N : nullptr);
 }
 
 __global__ void scale_kernel(const float* __restrict__ map, float factor, float* __restrict__ out, size_t count) {
@@ -831,10 +831,14 @@ class Unrolled {
     u.bars = bars;
     const unsigned B = static_cast<unsigned>(frames());
     const dim3 tiles(static_cast<unsigned>(solvers_[0]->prox_groups(0)), B);
+    bool plain = true;
+    for (const FrameModel* f : frames_) plain = plain && !f->components[0].costed && !f->components[0].mmse;
     if (inside != nullptr) {
-      unrolled_prox_kernel<true><<<tiles, 256, 0, s>>>(u, H_, W_, inside);
+      plain ? unrolled_prox_kernel<true, true><<<tiles, 256, 0, s>>>(u, H_, W_, inside)
+            : unrolled_prox_kernel<true, false><<<tiles, 256, 0, s>>>(u, H_, W_, inside);
     } else {
-      unrolled_prox_kernel<false><<<tiles, 256, 0, s>>>(u, H_, W_, nullptr);
+      plain ? unrolled_prox_kernel<false, true><<<tiles, 256, 0, s>>>(u, H_, W_, nullptr)
+            : unrolled_prox_kernel<false, false><<<tiles, 256, 0, s>>>(u, H_, W_, nullptr);
     }
     HIP_CHECK(hipGetLastError());
     const int strips = (W_ + kStripColumns - 1) / kStripColumns, across = (strips + 3) / 4;
