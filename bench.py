@@ -26,7 +26,8 @@ bytes per channel-sample are one read of x-, p, w and r and one write of p, w an
 achieved = those bytes of one launch over its mean duration, against MI355X's 8 TB/s.
 traffic: the HBM bytes of one launch from rocprofv3 counters (FETCH_SIZE + WRITE_SIZE,
 scaled by a copy kernel's reading), where profiles/traffic.json holds them for this
-workload. iteration_achieved is the same for a whole iteration (2 x the state's bytes +
+workload and the dual step's rows per wavefront that the solve measured best and took
+(dual_rows). iteration_achieved is the same for a whole iteration (2 x the state's bytes +
 2 x the coefficients per sample, each int16 level read once).
 
 accuracy: the timed solve's result against the same solve in binary64 on the GPU (largest
@@ -187,13 +188,15 @@ def dual_bytes(method: str, precision: str) -> float:
     return (iterate + 2 * dual + 2 * iterate + 3 * dual) + (2 * dual + 2 * iterate + 3 * dual)
 
 
-def measured_traffic(kernel: str, workload_name: str, precision: str) -> float | None:
-    """HBM bytes per launch of `kernel` from the committed counter summary, if it covers this case."""
+def measured_traffic(kernel: str, workload_name: str, precision: str, rows: int) -> float | None:
+    """HBM bytes per launch of `kernel` from the committed counter summary, if it covers this case
+    at the dual step's rows per wavefront that the solve took."""
     path = ROOT / "profiles" / "traffic.json"
     if not path.exists():
         return None
     for entry in json.loads(path.read_text()).get("entries", []):
-        if (entry.get("kernel"), entry.get("config"), entry.get("precision")) == (kernel, workload_name, precision):
+        if (entry.get("kernel"), entry.get("config"), entry.get("precision"), entry.get("rows")) == (
+                kernel, workload_name, precision, rows):
             return float(entry["bytes_per_launch"])
     return None
 
@@ -541,7 +544,8 @@ def main() -> None:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic("dual_kernel", args.config, args.precision),
+            "traffic": measured_traffic("dual_kernel", args.config, args.precision, launches.dual_rows),
+            "dual_rows": launches.dual_rows,
             "iteration_achieved": round(iteration_achieved, 1),
             "iteration_frac": round(iteration_achieved / HBM_PEAK_GBS, 4),
         },

@@ -131,8 +131,8 @@ unround_status unround_solve_scratch(const unround_input* input, const unround_s
 /* The durations of the solve's kernels, where the environment variable UNROUND_TIMING was
  * set during the solve (HIP events on the solve's stream): values[0] the seconds of the
  * primal kernels and [1] their launches, [2] and [3] those of the dual kernels, [4] the
- * seconds of the iterations (records excluded) and [5] the iterations; at most count
- * values are written. UNROUND_TIMING=1 brackets each run of iterations between two
+ * seconds of the iterations (records excluded), [5] the iterations and [6] the dual step's
+ * rows per wavefront; at most count values are written. UNROUND_TIMING=1 brackets each run of iterations between two
  * records with one pair of events (launches [1] and [3] are then 0); UNROUND_TIMING=2
  * brackets every iteration and every launch. UNROUND_TIMING_WARMUP=W leaves the first W
  * iterations untimed (they run straight before the timed ones). Returns 1 where timings

@@ -1031,9 +1031,11 @@ const double* unround_result_excess(const unround_result* result, uint64_t compo
 int32_t unround_result_kernel_times(const unround_result* result, double* values, size_t count) {
   if (result == nullptr || values == nullptr) return 0;
   const unround::KernelTimes& t = result->out.times;
-  const double all[6] = {t.primal_seconds, static_cast<double>(t.primal_launches), t.dual_seconds,
-                         static_cast<double>(t.dual_launches), t.iterations_seconds, static_cast<double>(t.iterations)};
-  for (size_t k = 0; k < count && k < 6; ++k) values[k] = all[k];
+  const double all[7] = {t.primal_seconds,     static_cast<double>(t.primal_launches),
+                         t.dual_seconds,       static_cast<double>(t.dual_launches),
+                         t.iterations_seconds, static_cast<double>(t.iterations),
+                         static_cast<double>(t.dual_rows)};
+  for (size_t k = 0; k < count && k < 7; ++k) values[k] = all[k];
   return t.iterations > 0 ? 1 : 0;
 }
 

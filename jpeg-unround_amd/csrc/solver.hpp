@@ -51,6 +51,7 @@ struct KernelTimes {
   uint64_t dual_launches = 0;
   double iterations_seconds = 0.0;  // the iterations alone, records excluded
   uint64_t iterations = 0;
+  int dual_rows = 0;  // the dual step's rows per wavefront (a frame's solve)
 };
 
 struct SolveOutput {

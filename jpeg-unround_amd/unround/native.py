@@ -539,7 +539,8 @@ class Solution:
 class KernelTimes:
     """HIP-event durations of a solve's kernels. UNROUND_TIMING=1 during the solve times the
     runs of iterations between records; UNROUND_TIMING=2 also every launch;
-    UNROUND_TIMING_WARMUP=W leaves the first W iterations untimed."""
+    UNROUND_TIMING_WARMUP=W leaves the first W iterations untimed. dual_rows: the dual
+    step's rows per wavefront."""
 
     primal_seconds: float
     primal_launches: int
@@ -547,6 +548,7 @@ class KernelTimes:
     dual_launches: int
     iterations_seconds: float
     iterations: int
+    dual_rows: int = 0
 
 
 @dataclass(frozen=True, slots=True, eq=False)
@@ -695,10 +697,11 @@ class _ResultReader:
         return tuple(found) if found else None
 
     def kernel_times(self) -> KernelTimes | None:
-        values = (ctypes.c_double * 6)()
-        if not self.lib.unround_result_kernel_times(self.handle, values, 6):
+        values = (ctypes.c_double * 7)()
+        if not self.lib.unround_result_kernel_times(self.handle, values, 7):
             return None
-        return KernelTimes(values[0], int(values[1]), values[2], int(values[3]), values[4], int(values[5]))
+        return KernelTimes(values[0], int(values[1]), values[2], int(values[3]), values[4], int(values[5]),
+                           int(values[6]))
 
     def icc_profile(self) -> bytes | None:
         pointer, count = self._counted(self.lib.unround_result_icc_profile)
