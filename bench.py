@@ -349,7 +349,7 @@ def unrolled_main(args) -> None:
         middle = time.perf_counter()
         loss.backward()
         torch.cuda.synchronize()
-        return float(loss), middle
+        return float(loss.detach()), middle
 
     once()  # warm-up
     alpha0.grad = alpha1.grad = weight_map.grad = None

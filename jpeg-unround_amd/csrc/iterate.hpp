@@ -52,6 +52,22 @@ template <typename T>
 __device__ __forceinline__ T from_right(T value) {
   return lane_shifted<kWaveShiftLeft>(value);
 }
+// The value of `value` in lane `source` (a constant), as a scalar.
+template <typename T>
+__device__ __forceinline__ T lane_value(T value, int source) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8);
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, value), source));
+  } else {
+    struct Words {
+      int low, high;
+    };
+    Words w = __builtin_bit_cast(Words, value);
+    w.low = __builtin_amdgcn_readlane(w.low, source);
+    w.high = __builtin_amdgcn_readlane(w.high, source);
+    return __builtin_bit_cast(T, w);
+  }
+}
 
 // delta_x f at column j from f there and at j - 1 (docs/math.md:337).
 template <typename T>
@@ -200,6 +216,14 @@ __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& stagi
   const LevelWords own_words = level_words(K, ty, tx, n, inside);
   T up[DH], here[8][DV][DH];
   S across[8][DV][DH], down[8][DV][DH], edge[8][DV];
+  // p_x of the column left of the strip, for lane 0: read from memory, lane k holds canvas
+  // row top + k's (one register, not one a row); from edges.left, one a row.
+  S edge_column{};
+  if (edges.left == nullptr) {
+    const int strip_column = 64 * strip * DH;  // lane 0's first column
+    const bool edge_live = (ty < K.tiles_y) & (8 * strip < K.tiles_x) & (lane < 8 * DV) & (strip_column > 0);
+    edge_column = p.load_if(edge_live, 0, at(top + min(lane, 8 * DV - 1), strip_column - 1));
+  }
   const auto load_row = [&](const auto& planes, bool valid, int plane, uint32_t where, auto (&out)[DH]) {
     using Out = std::remove_cv_t<std::remove_reference_t<decltype(out[0])>>;
     if constexpr (DH == 2) {
@@ -241,8 +265,6 @@ __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& stagi
       if (edges.left != nullptr) {
         // Made from a stored sample, so its storage type holds it exactly.
         edge[m][a] = device::narrow<S>(edges.left[(i - edges.left_top) * edges.left_stride]);
-      } else {
-        edge[m][a] = p.load_if(live && lane == 0 && first_column > 0, 0, at(i, first_column - 1));
       }
     }
   }
@@ -258,7 +280,9 @@ __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& stagi
       const int i = top + m * DV + a;
       // p_x of the column on the left: the lane's own previous column, or the lane on the
       // left's last one, or beyond the strip, the edge sample.
-      const T left = from_left_or(device::widen(across[m][a][DH - 1]), device::widen(edge[m][a]));
+      const T edge_here = edges.left != nullptr ? device::widen(edge[m][a])
+                                                : lane_value(device::widen(edge_column), m * DV + a);
+      const T left = from_left_or(device::widen(across[m][a][DH - 1]), edge_here);
 #pragma unroll
       for (int bb = 0; bb < DH; ++bb) {
         const int j = first_column + bb;
@@ -369,7 +393,7 @@ __device__ __forceinline__ void prox_tile(const Component* __restrict__ componen
 // without spilling (the scheduler otherwise trades them for latency it does not need).
 template <typename T, int DV, int DH>
 constexpr int kProxWaves =
-    sizeof(T) == 4 ? (DV * DH == 1 ? 5 : 3) : (DV * DH == 1 ? 4 : (DV * DH == 2 ? 2 : 1));
+    sizeof(T) == 4 ? (DV * DH == 1 ? 8 : 3) : (DV * DH == 1 ? 4 : (DV * DH == 2 ? 2 : 1));
 
 // The steps of the components one prox launch takes, from its first: component c + k has
 // descent[k] (tau gamma_c) and step[k] (tau over its cell's samples).
@@ -502,22 +526,6 @@ __global__ __launch_bounds__(256) void prox_any_cell_kernel(const Component* __r
 // and r on the left, from which lane 0 makes w- of the column on its left) come from one
 // gather per row, a lane per sample, read back as scalars.
 constexpr int kStripColumns = 64;
-
-// The value of `v` in lane `source` (a constant), as a scalar.
-template <typename T>
-__device__ __forceinline__ T lane_value(T v, int source) {
-  if constexpr (sizeof(T) == 4) {
-    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), source));
-  } else {
-    struct Halves {
-      int low, high;
-    };
-    Halves h = __builtin_bit_cast(Halves, v);
-    h.low = __builtin_amdgcn_readlane(h.low, source);
-    h.high = __builtin_amdgcn_readlane(h.high, source);
-    return __builtin_bit_cast(T, h);
-  }
-}
 
 // The arithmetic of the dual step at one sample, shared by every path that makes it (the
 // row walk, and the fused sweep's making of a neighbour's p once more), with its fused
