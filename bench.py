@@ -58,6 +58,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 from pathlib import Path
@@ -352,12 +353,18 @@ def unrolled_main(args) -> None:
         return float(loss.detach()), middle
 
     once()  # warm-up
-    alpha0.grad = alpha1.grad = weight_map.grad = None
-    torch.cuda.synchronize()
-    began = time.perf_counter()
-    loss, middle = once()
-    ended = time.perf_counter()
-    forward, backward = middle - began, ended - middle
+    # Five timed forward and backward pairs, each of the two times their median; the
+    # gradients reported are the last pair's (one backward each).
+    pairs = []
+    for _ in range(5):
+        alpha0.grad = alpha1.grad = weight_map.grad = None
+        torch.cuda.synchronize()
+        began = time.perf_counter()
+        loss, middle = once()
+        ended = time.perf_counter()
+        pairs.append((middle - began, ended - middle))
+    forward = statistics.median(pair[0] for pair in pairs)
+    backward = statistics.median(pair[1] for pair in pairs)
     samples = count * size * size
     # The launches' own durations: a handle made with UNROUND_TIMING=2 (events around every
     # launch on its stream), one forward and backward.

@@ -482,6 +482,18 @@ __device__ __forceinline__ int ordered(int g, int total) {
   return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
 }
 
+// ordered() over a grid of frames (blockIdx.y) of gridDim.x tiles each: the hardware counts
+// work-groups x first, so each XCD takes one contiguous run of (frame, tile) pairs whatever
+// the tiles a frame.
+struct FrameTile {
+  int frame, tile;
+};
+__device__ __forceinline__ FrameTile ordered_frames() {
+  const int tiles = static_cast<int>(gridDim.x);
+  const int t = ordered(static_cast<int>(blockIdx.y) * tiles + static_cast<int>(blockIdx.x), tiles * static_cast<int>(gridDim.y));
+  return FrameTile{t / tiles, t % tiles};
+}
+
 // ---------------------------------------------------------------------------------------
 // Planes of one kind of field: `count` planes of N samples each from `base`, a sample
 // addressed by its index in its plane.

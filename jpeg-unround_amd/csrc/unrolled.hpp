@@ -152,25 +152,29 @@ __device__ __forceinline__ UnrolledBack back_of(const UnrolledFrames& u, int f, 
 // again what the neighbouring strips make, the two columns each side its own need.
 constexpr int kBackColumns = 60;
 
-// 1. The adjoint of the dual step of one iteration, for one frame (blockIdx.y): a wavefront
-// walks rows first .. first + rows - 1 of its strip, a lane per column. Per row i it makes
-// the forward's w-, p^ and r^ of row i + 1 again (from the iteration's input state and x-,
-// loading rows i + 1 and i + 2), the balls' Jacobians there (grad p^, grad r^ from the
-// gradients after the iteration, the radius and alpha0 sensitivities added up), and then
-// row i's gradients: grad x- = sigma g grad^T grad p^, grad w- = sigma g (E^T grad r^ -
-// grad p^), grad w~ = rho grad w+ + 2 grad w-, and grad w, grad p (less div's part) and
-// grad r = (1 - rho) grad r+ + grad r^ + div_2^T (tau g grad w~) before the iteration. The
-// gradient of p after the iteration is completed on the way in with div^T of the last prox
-// adjoint's tau g grad v. Neighbours in the row come from the neighbouring lanes; the walk
-// starts two rows above its first so that the rows it carries are whole there.
+// 1. The adjoint of the dual step of one iteration, for one frame (grid y; the work-groups
+// taken in ordered_frames' order): a wavefront walks rows first .. first + rows - 1 of its
+// strip, a lane per column. Per row i it makes the forward's w-, p^ and r^ of row i + 1
+// again (from the iteration's input state and x-, loading rows i + 1 and i + 2), the
+// balls' Jacobians there (grad p^, grad r^ from the gradients after the iteration, the
+// radius and alpha0 sensitivities added up), and then row i's gradients: grad x- = sigma g
+// grad^T grad p^, grad w- = sigma g (E^T grad r^ - grad p^), grad w~ = rho grad w+ + 2
+// grad w-, and grad w, grad p (less div's part) and grad r = (1 - rho) grad r+ + grad r^ +
+// div_2^T (tau g grad w~) before the iteration. The gradient of p after the iteration is
+// completed on the way in with div^T of the last prox adjoint's tau g grad v. Neighbours
+// in the row come from the neighbouring lanes; the walk starts two rows above its first so
+// that the rows it carries are whole there.
 __global__ __launch_bounds__(64) void unrolled_dual_back_kernel(UnrolledFrames u, const float* __restrict__ radius,
                                                                 int H, int W, int rows, int strips) {
   // No multiply-add is fused: the walk's row step is compiled once per slot and again for
   // the first and last rows, and every copy must round alike, so that a row's gradients do
   // not depend on which copy makes it (which the run length decides).
 #pragma clang fp contract(off)
-  const int f = static_cast<int>(blockIdx.y);
-  const int strip = static_cast<int>(blockIdx.x) % strips, run = static_cast<int>(blockIdx.x) / strips;
+  // Neighbouring strips and runs, which read each other's edge columns and rows again,
+  // share an XCD's L2.
+  const device::FrameTile place = device::ordered_frames();
+  const int f = place.frame;
+  const int strip = place.tile % strips, run = place.tile / strips;
   const int lane = static_cast<int>(threadIdx.x);
   const int j = strip * kBackColumns - 2 + lane;
   const bool column = j >= 0 && j < W;
