@@ -579,7 +579,8 @@ class Unrolled {
     bars_ = DeviceBuffer(sizeof(float) * B * N_);
     start_ = DeviceBuffer(sizeof(float) * B * state);
     checkpoints_ = DeviceBuffer(sizeof(float) * B * state * (iterations_ / every_ + 1));
-    run_states_ = DeviceBuffer(sizeof(float) * B * state * every_);
+    // A run's input states after its first (whose is its checkpoint's).
+    run_states_ = DeviceBuffer(sizeof(float) * B * state * std::max(every_ - 1, 1));
     run_bars_ = DeviceBuffer(sizeof(float) * B * N_ * every_);
     run_inside_ = DeviceBuffer(static_cast<size_t>(B) * N_ * every_);
     grads_ = DeviceBuffer(sizeof(float) * B * state * 2);
@@ -636,7 +637,8 @@ class Unrolled {
     for (int it = 0; it < iterations_; ++it) {
       if (it % every_ == 0) copy(checkpoint(it / every_), states_[live].as<float>(), all);
       if (it == last_run * every_) {
-        run_into_slots(it, iterations_, states_[live].as<float>(), states_[1 - live].as<float>());
+        // From the checkpoint just kept: the backward takes the run's first input state there.
+        run_into_slots(it, iterations_, checkpoint(last_run), states_[1 - live].as<float>());
         live = 1 - live;
         break;
       }
@@ -678,7 +680,7 @@ class Unrolled {
       if (run != runs - 1 || !last_run_kept_) run_into_slots(first, last, checkpoint(run), states_[0].as<float>());
       for (int it = last - 1; it >= first; --it) {
         UnrolledFrames u = frames_view();
-        u.state = run_states_.as<float>() + static_cast<size_t>(it - first) * all;
+        u.state = run_state(it - first, checkpoint(run));
         u.bars = run_bars_.as<float>() + static_cast<size_t>(it - first) * B * N_;
         u.after = g + static_cast<size_t>(parity) * all;
         u.before = g + static_cast<size_t>(1 - parity) * all;
@@ -809,18 +811,20 @@ class Unrolled {
     copy(to + 5 * N_, F.r, 3 * N_);
   }
 
-  // Iterations first .. last - 1 from the states at `from`, each iteration's input state
-  // into its own slot of the run, its x- and clip marks beside; the last one's output into
-  // `out`.
-  void run_into_slots(int first, int last, const float* from, float* out) {
+  // The input states of iteration k of a run (k from 0): the run's checkpoint `start` for
+  // the first, the run's slots for the others.
+  float* run_state(int k, float* start) {
+    return k == 0 ? start : run_states_.as<float>() + static_cast<size_t>(k - 1) * frames() * 8 * N_;
+  }
+
+  // Iterations first .. last - 1 from the checkpoint `from` (which the steps only read),
+  // each later iteration's input state into its own slot of the run, every iteration's x-
+  // and clip marks beside; the last one's output into `out`.
+  void run_into_slots(int first, int last, float* from, float* out) {
     const int B = frames();
-    const size_t all = static_cast<size_t>(B) * 8 * N_;
-    float* const kept = run_states_.as<float>();
-    copy(kept, from, all);
     for (int it = first; it < last; ++it) {
-      float* const next = it + 1 < last ? kept + static_cast<size_t>(it + 1 - first) * all : out;
-      step(kept + static_cast<size_t>(it - first) * all, next,
-           run_bars_.as<float>() + static_cast<size_t>(it - first) * B * N_,
+      float* const next = it + 1 < last ? run_state(it + 1 - first, from) : out;
+      step(run_state(it - first, from), next, run_bars_.as<float>() + static_cast<size_t>(it - first) * B * N_,
            run_inside_.as<uint8_t>() + static_cast<size_t>(it - first) * B * N_);
     }
   }
