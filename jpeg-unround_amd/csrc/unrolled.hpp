@@ -578,7 +578,8 @@ class Unrolled {
     states_[1] = DeviceBuffer(sizeof(float) * B * state);
     bars_ = DeviceBuffer(sizeof(float) * B * N_);
     start_ = DeviceBuffer(sizeof(float) * B * state);
-    checkpoints_ = DeviceBuffer(sizeof(float) * B * state * (iterations_ / every_ + 1));
+    // The checkpoints after the start's (which is the start itself): runs 1 .. the last.
+    checkpoints_ = DeviceBuffer(sizeof(float) * B * state * std::max((iterations_ - 1) / every_, 1));
     // A run's input states after its first (whose is its checkpoint's).
     run_states_ = DeviceBuffer(sizeof(float) * B * state * std::max(every_ - 1, 1));
     run_bars_ = DeviceBuffer(sizeof(float) * B * N_ * every_);
@@ -628,27 +629,34 @@ class Unrolled {
     HIP_CHECK(hipGetLastError());
     upload_scalars();
     const int B = frames();
-    const size_t all = static_cast<size_t>(B) * 8 * N_;
-    int live = 0;
-    copy(states_[live].as<float>(), start_.as<float>(), all);
-    // The runs before the last, keeping their first states; the last run as the backward
-    // recomputes a run, its states, x- and clip marks kept for it.
+    // The runs before the last, each iteration into the next state: the first state of each
+    // run into its checkpoint, the others into the two states in turn (the steps only read
+    // their input state, so the first run reads the kept start itself). The last run as the
+    // backward recomputes a run, its states, x- and clip marks kept for it.
     const int last_run = (iterations_ - 1) / every_;
+    float* now = checkpoint(0);
+    float* last = now;  // with no iterations, the start
+    int spare = 0;
     for (int it = 0; it < iterations_; ++it) {
-      if (it % every_ == 0) copy(checkpoint(it / every_), states_[live].as<float>(), all);
       if (it == last_run * every_) {
-        // From the checkpoint just kept: the backward takes the run's first input state there.
-        run_into_slots(it, iterations_, checkpoint(last_run), states_[1 - live].as<float>());
-        live = 1 - live;
+        last = states_[spare].as<float>();
+        run_into_slots(it, iterations_, now, last);
         break;
       }
-      step(states_[live].as<float>(), states_[1 - live].as<float>(), bars_.as<float>());
-      live = 1 - live;
+      float* next = nullptr;
+      if ((it + 1) % every_ == 0) {
+        next = checkpoint((it + 1) / every_);
+      } else {
+        next = states_[spare].as<float>();
+        spare = 1 - spare;
+      }
+      step(now, next, bars_.as<float>());
+      now = next;
     }
     last_run_kept_ = true;
     // x of every frame: the first of its 8 planes.
-    HIP_CHECK(hipMemcpy2DAsync(canvas, sizeof(float) * N_, states_[live].pointer, sizeof(float) * 8 * N_,
-                               sizeof(float) * N_, static_cast<size_t>(B), hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipMemcpy2DAsync(canvas, sizeof(float) * N_, last, sizeof(float) * 8 * N_, sizeof(float) * N_,
+                               static_cast<size_t>(B), hipMemcpyDeviceToDevice, s));
     forwarded_ = true;
     join(s, caller);
   }
@@ -771,8 +779,11 @@ class Unrolled {
     HIP_CHECK(hipEventDestroy(event));
   }
 
-  // The states of every frame at checkpoint `index`.
-  float* checkpoint(int index) { return checkpoints_.as<float>() + static_cast<size_t>(index) * frames() * 8 * N_; }
+  // The states of every frame at the start of run `index`: the kept start for the first.
+  float* checkpoint(int index) {
+    return index == 0 ? start_.as<float>()
+                      : checkpoints_.as<float>() + static_cast<size_t>(index - 1) * frames() * 8 * N_;
+  }
 
   void copy(float* to, const float* from, size_t count) {
     HIP_CHECK(hipMemcpyAsync(to, from, sizeof(float) * count, hipMemcpyDeviceToDevice, stream_.stream));
