@@ -135,6 +135,16 @@ __device__ __forceinline__ Staged stage(const Component& K, StagedArrays& into) 
   return stage_store(K, into, stage_load(K));
 }
 
+// Orders a wavefront's own LDS stores before its lanes' loads of them, with no other
+// wavefront taking part: a wave barrier between wavefront-scope release and acquire fences
+// (a wavefront's LDS operations are performed in order, so no wait is needed; the fences
+// keep the compiler from moving the accesses across it).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // A pointer into device memory seen as one into the global address space, so that loads
 // through it are global loads rather than generic ones.
 template <typename X>

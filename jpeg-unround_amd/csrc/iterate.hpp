@@ -128,14 +128,17 @@ __device__ __forceinline__ LevelRow level_row(const Component& K, int ty, int bx
 }
 
 // Where a prox tile takes its component's constants: OwnStaging loads them and copies them
-// into the work-group's LDS (load() before the tile's samples are loaded, store() after);
+// into the work-group's LDS (load() before the tile's samples are loaded, store() after,
+// with the work-group's barrier: every wavefront of the work-group takes part);
 // PreStaged hands over constants already in LDS.
 struct OwnStaging {
   const Component& K;
   device::StagedArrays& into;
   __device__ __forceinline__ device::StageLoaded load() const { return device::stage_load(K); }
   __device__ __forceinline__ device::Staged store(const device::StageLoaded& loaded) const {
-    return device::stage_store(K, into, loaded);
+    const device::Staged staged = device::stage_store(K, into, loaded);
+    __syncthreads();
+    return staged;
   }
 };
 struct PreStaged {
@@ -180,10 +183,12 @@ struct ProxEdges {
 };
 
 // The tile of this wavefront in tile row ty and strip `strip` (8 tiles across) of
-// component c. `staging` gives the component's constants in LDS (OwnStaging or PreStaged),
-// read after the tile's first barrier. The tile's first loads are the constants and its
-// levels, which it takes first; then its samples. PLAIN: the component has no slack cost
-// and midpoint centres (device::prox_of's PLAIN).
+// component c. `staging` gives the component's constants in LDS (OwnStaging, whose store
+// ends with the work-group's barrier, or PreStaged, staged before). The tile's first loads
+// are the constants and its levels, which it takes first; then its samples. Its transposes
+// go through the wavefront's own part of `rooms`, so past the staging no wavefront waits
+// for another's loads. PLAIN: the component has no slack cost and midpoint centres
+// (device::prox_of's PLAIN).
 template <typename T, typename S, int DV, int DH, template <typename> class Planes, bool PLAIN = false,
           typename Staging>
 __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& staging, ProxRooms<T>& rooms,
@@ -306,7 +311,8 @@ __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& stagi
   device::forward8(column, y);
 #pragma unroll
   for (int k = 0; k < 8; ++k) rooms.forward[wave][k][lane] = y[k];
-  __syncthreads();
+  // The wavefront's own part of the room: no other wavefront waits for this one's loads.
+  device::wave_sync();
   // A lane now takes row n of block b of the strip.
   T row[8], coefficient[8], zeta[8];
 #pragma unroll
@@ -330,7 +336,7 @@ __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& stagi
   device::inverse8(zeta, back_row);
 #pragma unroll
   for (int v = 0; v < 8; ++v) rooms.inverse[wave][n][8 * b + v] = back_row[v];
-  __syncthreads();
+  device::wave_sync();
   T back_column[8], back[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) back_column[m] = rooms.inverse[wave][m][lane];

@@ -547,7 +547,7 @@ __device__ __forceinline__ double conjugate_terms(const Component& K, const Fiel
   device::forward8(column, y);
 #pragma unroll
   for (int k = 0; k < 8; ++k) room[k][lane] = y[k];
-  __syncthreads();
+  device::wave_sync();  // the wavefront's own room
   double row[8], coefficient[8];
 #pragma unroll
   for (int u = 0; u < 8; ++u) row[u] = room[n][8 * b + u];

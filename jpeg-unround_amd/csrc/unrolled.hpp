@@ -448,12 +448,13 @@ __global__ __launch_bounds__(256) void unrolled_prox_back_kernel(UnrolledFrames 
   }
   float y[8], row[8];
   device::forward8(column, y);
+  // Each wavefront transposes through its own part of the room: no work-group barriers.
 #pragma unroll
   for (int q = 0; q < 8; ++q) room[wave][q][lane] = y[q];
-  __syncthreads();
+  device::wave_sync();
 #pragma unroll
   for (int c = 0; c < 8; ++c) row[c] = room[wave][n][8 * b + c];
-  __syncthreads();
+  device::wave_sync();
   float grad_zeta[8], grad_e[8];
   device::forward8(row, grad_zeta);
 #pragma unroll
@@ -462,7 +463,7 @@ __global__ __launch_bounds__(256) void unrolled_prox_back_kernel(UnrolledFrames 
   device::inverse8(grad_e, row);
 #pragma unroll
   for (int c = 0; c < 8; ++c) room[wave][n][8 * b + c] = row[c];
-  __syncthreads();
+  device::wave_sync();
 #pragma unroll
   for (int q = 0; q < 8; ++q) y[q] = room[wave][q][lane];
   float grad_v[8];
