@@ -120,7 +120,8 @@ __global__ __launch_bounds__(256) void value_batch_kernel(const BatchTile* __res
 }
 
 // from_r as in conjugate_kernel: 1 into the conjugates (TGV's xi), 0 into `partials` when
-// `partial` (the partial gap's) or into the conjugates (TV's).
+// `partial` (the partial gap's) or into the conjugates (TV's). The batch's cells are 1 or 2
+// samples each way: the fixed shapes' instance.
 template <typename T, typename S>
 __global__ __launch_bounds__(256) void conjugate_batch_kernel(const BatchTile* __restrict__ tiles,
                                                               const RecordSlot<T, S>* __restrict__ slots,
@@ -130,8 +131,8 @@ __global__ __launch_bounds__(256) void conjugate_batch_kernel(const BatchTile* _
   if (active[t.image] == 0) return;
   const RecordSlot<T, S>& m = slots[t.image];
   double* into = (partial ? m.partials : m.conjugates) + m.conjugate_offset[t.part];
-  conjugate_tile<T, S>(m.components[t.part], m.F, t.part, m.H, m.W, m.gamma[t.part], from_r, m.results + 2, m.alpha1,
-                       m.free_radius, into, t.group);
+  conjugate_tile<T, S, false>(m.components[t.part], m.F, t.part, m.H, m.W, m.gamma[t.part], from_r, m.results + 2,
+                              m.alpha1, m.free_radius, into, t.group);
 }
 
 struct ReduceJob {

@@ -109,3 +109,35 @@ def test_timing_leaves_the_warm_up_untimed_and_the_result_unchanged(monkeypatch,
     else:
         assert times.dual_launches == 0 and times.primal_launches == 0
     assert np.array_equal(timed.canvas, plain.canvas)
+
+
+def encode_colour411(picture: np.ndarray, quality: int):
+    """An RGB picture as YCbCr 4:1:1 (luma sampled 4 across and 1 down, chroma cells 4
+    samples wide and 1 high), quantized at a quality with the scaled tables."""
+    from unround import jpegio
+
+    height, width, _ = picture.shape
+    planes = synthetic._ycbcr_planes(picture)
+    rows, columns = -(-height // 8) * 8, -(-width // 32) * 32
+    luma = synthetic._padded(planes[0], rows, columns)
+    chroma = [np.floor(synthetic._padded(p, rows, columns).reshape(rows, columns // 4, 4).mean(axis=2) + 0.5)
+              for p in planes[1:]]
+    luma_table = synthetic.ijg_table(synthetic.LUMA, quality)
+    chroma_table = synthetic.ijg_table(synthetic.CHROMA, quality)
+    components = (jpegio.make_component(synthetic._levels(luma, luma_table), luma_table, 4, 1),
+                  jpegio.make_component(synthetic._levels(chroma[0], chroma_table), chroma_table, 1, 1),
+                  jpegio.make_component(synthetic._levels(chroma[1], chroma_table), chroma_table, 1, 1))
+    return jpegio.make_image(height, width, jpegio.ColorSpace.YCBCR, components)
+
+
+def test_tgv_colour411_cells_of_any_shape():
+    # Chroma cells 4 x 1 take the prox and the conjugates' instances for any cell shape;
+    # the dual values of the records (the conjugates' sums) match the oracle's too.
+    picture = synthetic.colour_picture(72, 136, 14)
+    image = encode_colour411(picture, 70)
+    options = ["--method", "tgv", "--iterations", "40", "--tolerance", "0", "--record-every", "8"]
+    decoded, expected = check(image, picture, options)
+    history = decoded.result.history
+    relative = np.abs(history.dual - expected.dual) / np.abs(expected.primal)
+    print(f"  records: dual rel {np.max(relative):.3g}")
+    assert np.max(relative) < 1e-4
