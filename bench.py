@@ -427,7 +427,10 @@ def unrolled_main(args) -> None:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            # The counted bytes of a launch at the adjoint's 16 rows (none where the
+            # environment sets other rows).
+            "traffic": None if os.environ.get("UNROUND_BACK_ROWS") else measured_traffic(
+                "unrolled_dual_back_kernel", "c5", "single", 16),
             "prox_back_us": round(1e6 * prox_back_s / prox_back_n, 2),
             "prox_back_frac": round(prox_back_bytes * samples / (prox_back_s / prox_back_n) / 1e9 / HBM_PEAK_GBS, 4),
             "backward_bytes_per_sample_iteration": round(backward_bytes / samples / iterations, 1),
@@ -567,7 +570,8 @@ def main() -> None:
         line["roofline"]["primal"] = {"kernel": "samples_prox_kernel", "bytes_per_launch": int(primal_bytes),
                                       "launch_us": round(primal_launch * 1e6, 2),
                                       "achieved": round(primal_achieved, 1),
-                                      "frac": round(primal_achieved / HBM_PEAK_GBS, 4)}
+                                      "frac": round(primal_achieved / HBM_PEAK_GBS, 4),
+                                      "traffic": measured_traffic("samples_prox_kernel", "png", args.precision, 0)}
     if args.config in ("c2", "c4") and args.precision != "double":
         line["accuracy"] = accuracy_against_binary64(args.config, image, decoded, base, args.warmup + args.steps)
     if args.config == "c2" and not args.no_batch:
