@@ -326,6 +326,9 @@ __device__ __forceinline__ void prox_tile_at(const Component& K, Staging&& stagi
     const T z = device::prox_of<PLAIN>(staged, own.q[u], own.flat, 8 * n + u, coefficient[u], step, within);
     zeta[u] = inside ? z : T(0);
     if (marks != nullptr) marks[u] = within;
+    // One sample a cell: each coefficient's map before the next one's, not interleaved
+    // (fewer registers held at once; the 1 x 1 tile 2 % faster, the 2 x 2 one no faster).
+    if constexpr (DV * DH == 1) __builtin_amdgcn_sched_barrier(0);
   }
   if (keep && inside) {
     T* kept = zeta_kept + 64 * (static_cast<size_t>(ty) * K.columns + tx) + 8 * n;
